@@ -1,0 +1,262 @@
+"""ctypes binding of the C ABI in include/hc.h (libhochunk3d_amd.so).
+
+This is the same binding a Fortran driver makes with ISO_C_BINDING: plain
+pointers and sizes, arrays in the reference's column-major layout.  Loading
+fails loudly when the library is missing: there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+LIBNAME = "libhochunk3d_amd.so"
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+HC_FP64, HC_FP32 = 0, 1
+NO = 8
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+# Every symbol include/hc.h declares.
+EXPORTS = (
+    "hc_create", "hc_set_grid", "hc_set_temperature", "hc_set_dust", "hc_set_sources",
+    "hc_run_iteration", "hc_temperature", "hc_destroy", "hc_last_error", "hc_abi_version",
+)
+
+
+class HcDims(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "nrg", "ntg", "npg", "ndg", "nfreq", "nmu", "nph", "nap", "npeel", "nxhst",
+        "nlammax", "nT", "nnu", "precision", "imcube")]
+
+
+class HcSources(C.Structure):
+    _fields_ = [
+        ("rstar", C.c_double), ("tstar", C.c_double), ("tshock", C.c_double),
+        ("limb", C.c_int32), ("ips", C.c_int32), ("ispot", C.c_int32), ("nspot", C.c_int32),
+    ] + [(n, C.c_double) for n in (
+        "fspot", "csprad", "cspot1", "sspot1", "scspot1", "ssspot1", "cspot2", "sspot2",
+        "scspot2", "ssspot2", "accfrac", "accfrac2", "numin", "lnurat", "rmaxi", "ltot")] + [
+        ("iintmax", C.c_int32), ("pad", C.c_int32),
+        ("u", _dp), ("aperture2", _dp), ("coste", _dp), ("sinte", _dp), ("cospe", _dp),
+        ("sinpe", _dp),
+    ]
+
+
+class HcTallies(C.Structure):
+    _fields_ = [
+        ("dtauabs", _dp), ("nabs", _dp), ("jmean", _dp),
+        ("killed", _ip), ("dbig", _ip), ("dsg", _ip),
+        ("sed", _dp), ("peel", _dp),
+        ("img_i", _dp), ("img_q", _dp), ("img_u", _dp), ("img_v", _dp), ("si_init", _dp),
+    ] + [(n, C.c_double) for n in (
+        "tot", "flux", "sflux", "nscat", "scount", "abflux", "aflux", "sumsub", "sumall",
+        "hit", "htot")] + [
+        ("n_err_nowall", C.c_int64), ("n_err_dust", C.c_int64), ("n_warn", C.c_int64),
+        ("kernel_ms", C.c_double),
+    ]
+
+
+TRACE_DTYPE = np.dtype([
+    ("x", "f8"), ("y", "f8"), ("z", "f8"), ("cost", "f8"), ("sint", "f8"), ("phi", "f8"),
+    ("I", "f8"), ("Q", "f8"), ("U", "f8"), ("V", "f8"), ("nub", "f8"), ("path", "f8"),
+    ("tau_peel", "f8"), ("ncross", "i8"), ("npeelcross", "i8"), ("nscat", "i8"), ("nabs", "i8"),
+    ("nstar", "i8"), ("ntrial", "i8"), ("cellhash", "u8"), ("status", "i4"), ("ir", "i4"),
+    ("it", "i4"), ("ip", "i4"), ("pad", "i4"),
+], align=True)
+
+SCALARS = ("tot", "flux", "sflux", "nscat", "scount", "abflux", "aflux", "sumsub", "sumall",
+           "hit", "htot", "n_err_nowall", "n_err_dust", "n_warn")
+
+
+def library_path() -> str:
+    return os.path.join(_HERE, LIBNAME)
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load the library (raises OSError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = library_path()
+    if not os.path.exists(path):
+        raise OSError(f"{path} is not built; run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    lib.hc_create.argtypes = [C.POINTER(HcDims), C.c_int, C.POINTER(C.c_void_p)]
+    lib.hc_set_grid.argtypes = [C.c_void_p] + [_dp] * 8 + [C.c_double]
+    lib.hc_set_temperature.argtypes = [C.c_void_p, _dp]
+    lib.hc_set_dust.argtypes = [C.c_void_p, _ip] + [_dp] * 10
+    lib.hc_set_sources.argtypes = [C.c_void_p, C.POINTER(HcSources)]
+    lib.hc_run_iteration.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
+                                     C.c_int32, C.c_uint64, C.POINTER(HcTallies), C.c_void_p]
+    lib.hc_temperature.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, _dp,
+                                   C.POINTER(C.c_int64)]
+    lib.hc_destroy.argtypes = [C.c_void_p]
+    lib.hc_last_error.argtypes = [C.c_void_p]
+    lib.hc_last_error.restype = C.c_char_p
+    lib.hc_abi_version.argtypes = []
+    for name in EXPORTS:
+        if name not in ("hc_last_error",):
+            getattr(lib, name).restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def _ptr(a: Optional[np.ndarray], ct=C.c_double):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _f64(a) -> np.ndarray:
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+@dataclass
+class PassResult:
+    """Tallies of one transport pass (reference layouts, FP64)."""
+    dtauabs: np.ndarray
+    nabs: np.ndarray
+    jmean: np.ndarray
+    killed: np.ndarray
+    dbig: np.ndarray
+    sed: np.ndarray        # (9, nfreq, nmu, nph, nap, no) I,Q,U,V,I2,Q2,U2,V2,N
+    peel: Optional[np.ndarray]
+    img_i: Optional[np.ndarray]
+    img_q: Optional[np.ndarray]
+    img_u: Optional[np.ndarray]
+    img_v: Optional[np.ndarray]
+    si_init: np.ndarray
+    scalars: dict
+    trace: Optional[np.ndarray]
+    kernel_ms: float
+
+
+class HcError(RuntimeError):
+    pass
+
+
+class Transport:
+    """One GPU context holding a model's grid, dust and sources."""
+
+    def __init__(self, model, device: int = 0, precision: int = HC_FP64, imcube=None):
+        self.lib = load()
+        m = model
+        self.model = m
+        cfg = m.cfg
+        self.imcube = bool(cfg.imcube if imcube is None else imcube)
+        self.dims = HcDims(m.nrg, m.ntg, m.npg, 8, cfg.nfreq, cfg.nmu, cfg.nph, cfg.nap,
+                           cfg.npeel, cfg.nximg, m.dust.nudust.shape[0], m.dust.lTint.size,
+                           m.dust.nuint.size, precision, int(self.imcube))
+        h = C.c_void_p()
+        rc = self.lib.hc_create(C.byref(self.dims), device, C.byref(h))
+        if rc != 0:
+            raise HcError(f"hc_create failed ({rc})")
+        self.h = h
+        self._keep = []
+        self._check(self.lib.hc_set_grid(
+            h, _ptr(_f64(m.r2arr)), _ptr(_f64(m.tan2arr)), _ptr(_f64(m.thetarr)),
+            _ptr(_f64(m.aarr)), _ptr(_f64(m.barr)), _ptr(_f64(m.phiarr)),
+            _ptr(_f64(m.densarr)), _ptr(_f64(m.tdust)), float(cfg.tsub)), "hc_set_grid")
+        d = m.dust
+        nl = np.ascontiguousarray(d.nlambda, dtype=np.int32)
+        self._check(self.lib.hc_set_dust(
+            h, _ptr(nl, C.c_int32), _ptr(_f64(d.nudust)), _ptr(_f64(d.kappad)),
+            _ptr(_f64(d.adust)), _ptr(_f64(d.gdust)), _ptr(_f64(d.pldust)),
+            _ptr(_f64(d.kappav)), _ptr(_f64(d.lTint)), _ptr(_f64(d.nuint)),
+            _ptr(_f64(d.kapBint)), _ptr(_f64(d.kappap))), "hc_set_dust")
+        self.set_sources()
+
+    def set_sources(self, **over):
+        m = self.model
+        arrs = [_f64(a) for a in (m.u, m.aperture2, m.coste, m.sinte, m.cospe, m.sinpe)]
+        self._keep = arrs
+        s = HcSources()
+        s.rstar = m.rstar
+        s.tstar = m.tstar_ev
+        s.tshock = m.tshock_ev
+        s.accfrac2 = m.cfg.accfrac2
+        s.numin = m.numin
+        s.lnurat = m.lnurat
+        s.rmaxi = m.rmaxi
+        s.ltot = m.ltot
+        s.iintmax = 10000
+        s.nspot = 1
+        for k, v in over.items():
+            setattr(s, k, v)
+        s.u, s.aperture2, s.coste, s.sinte, s.cospe, s.sinpe = [_ptr(a) for a in arrs]
+        self._check(self.lib.hc_set_sources(self.h, C.byref(s)), "hc_set_sources")
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.hc_last_error(self.h)
+            raise HcError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def set_temperature(self, tdust: np.ndarray):
+        self._check(self.lib.hc_set_temperature(self.h, _ptr(_f64(tdust))), "hc_set_temperature")
+
+    def run(self, np_: int, ipeel: int = 0, seed1: int = 12345678, seed2: int = 87654321,
+            gen: int = 1, seg0: int = 0, trace: bool = False, images: bool = True) -> PassResult:
+        m = self.model
+        cfg = m.cfg
+        shp4 = (m.nrg, m.ntg, m.npg, 8)
+        shp3 = (m.nrg, m.ntg, m.npg)
+        dtauabs = np.zeros(shp4, order="F")
+        nabs = np.zeros(shp4, order="F")
+        jmean = np.zeros(shp3, order="F")
+        killed = np.zeros(shp3, np.int32, order="F")
+        dbig = np.zeros(shp3, np.int32, order="F")
+        sed = np.zeros((cfg.nfreq, cfg.nmu, cfg.nph, cfg.nap, NO, 9), order="F")
+        peel = np.zeros((cfg.nfreq, max(cfg.npeel, 1), cfg.nap, NO, 9), order="F") if ipeel else None
+        cube = bool(ipeel and self.imcube and images)
+        nimg = (max(cfg.npeel, 1), cfg.nximg, cfg.nximg, cfg.nfreq)
+        img_i = np.zeros(nimg + (NO,), order="F") if cube else None
+        img_q = np.zeros(nimg, order="F") if cube else None
+        img_u = np.zeros(nimg, order="F") if cube else None
+        img_v = np.zeros(nimg, order="F") if cube else None
+        si_init = np.zeros((cfg.nfreq, 5), order="F")
+        t = HcTallies()
+        t.dtauabs, t.nabs, t.jmean = _ptr(dtauabs), _ptr(nabs), _ptr(jmean)
+        t.killed, t.dbig = _ptr(killed, C.c_int32), _ptr(dbig, C.c_int32)
+        t.sed = _ptr(sed)
+        t.peel = _ptr(peel)
+        t.img_i, t.img_q, t.img_u, t.img_v = (_ptr(a) for a in (img_i, img_q, img_u, img_v))
+        t.si_init = _ptr(si_init)
+        tr = np.zeros(np_, TRACE_DTYPE) if trace else None
+        rc = self.lib.hc_run_iteration(self.h, int(np_), int(ipeel), int(seed1), int(seed2),
+                                       int(gen), int(seg0), C.byref(t),
+                                       tr.ctypes.data if tr is not None else None)
+        self._check(rc, "hc_run_iteration")
+        scal = {k: getattr(t, k) for k in SCALARS}
+        sed = np.moveaxis(sed, -1, 0)
+        if peel is not None:
+            peel = np.moveaxis(peel, -1, 0)
+        return PassResult(dtauabs, nabs, jmean, killed, dbig, sed, peel, img_i, img_q, img_u,
+                          img_v, si_init, scal, tr, t.kernel_ms)
+
+    def temperature(self, np_: int):
+        m = self.model
+        t2 = np.zeros((m.nrg, m.ntg, m.npg, 8), order="F")
+        n = C.c_int64(0)
+        self._check(self.lib.hc_temperature(self.h, _ptr(_f64(m.massarr)), float(m.ltot),
+                                            float(np_), _ptr(t2), C.byref(n)), "hc_temperature")
+        return t2, int(n.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.hc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,463 @@
+// hc_device.h -- device-side building blocks of the photon-packet transport:
+// the L'Ecuyer-Cote generator, wall roots, opacity interpolation, CS-HG
+// Stokes scattering and peel-off weighting, Lucy re-emission frequency and the
+// radiative-equilibrium temperature.  The FP64 routines evaluate the
+// reference's Fortran expressions term by term in the same order (the
+// translation unit is compiled with -ffp-contract=off), so that the plain FP64
+// path rounds like the reference and its CPU restatement.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hc {
+
+constexpr double kPi = 3.1415926535897932384626433;  // constants.f90:5
+constexpr double kR2P = 2.0 * kPi;
+constexpr double kRsol = 6.9598e10;                   // constants.f90:11
+constexpr double kSigT = 5.670400e-5;                 // constants.f90:21
+constexpr double kHuge = 1.7976931348623157e308;      // huge(1d0)
+constexpr int kMaxSpecies = 8;                        // grid_mod.f90:13
+constexpr int kNo = 8;                                // SED output slots
+constexpr double kTauMax = 25.0;                      // peeloff_3d.f90:40
+
+// ---------------------------------------------------------------------------
+// RNG: combined MRG of ecuyer_cote_mod.f90:252-293 (Schrage steps), one
+// segment per packet.  The segment start is seeded on the host side of the
+// launch from a table of a^(2^w 2^k) mod m.
+// ---------------------------------------------------------------------------
+struct Mrg {
+  int32_t s1, s2;
+};
+
+__device__ __forceinline__ double ran(Mrg &r) {
+  int32_t k = r.s1 / 53668;
+  int32_t s1 = 40014 * (r.s1 - k * 53668) - k * 12211;
+  if (s1 < 0) s1 += 2147483563;
+  k = r.s2 / 52774;
+  int32_t s2 = 40692 * (r.s2 - k * 52774) - k * 3791;
+  if (s2 < 0) s2 += 2147483399;
+  r.s1 = s1;
+  r.s2 = s2;
+  int32_t z = s1 - s2;
+  if (z < 1) z += 2147483563 - 1;
+  return (double)z * 4.656613057e-10;  // random_standard_uniform_mod.f90:43
+}
+
+__device__ __forceinline__ int32_t mulmod(int32_t a, int32_t b, int32_t m) {
+  return (int32_t)(((uint64_t)(uint32_t)a * (uint64_t)(uint32_t)b) % (uint64_t)(uint32_t)m);
+}
+
+// pow1[k] = (a1^(2^w))^(2^k) mod m1, pow2 likewise; seg selects the product.
+__device__ __forceinline__ Mrg mrg_segment(int32_t g1, int32_t g2, uint64_t seg,
+                                           const int32_t *__restrict__ pow1,
+                                           const int32_t *__restrict__ pow2) {
+  Mrg r{g1, g2};
+  int k = 0;
+  while (seg) {
+    if (seg & 1u) {
+      r.s1 = mulmod(r.s1, pow1[k], 2147483563);
+      r.s2 = mulmod(r.s2, pow2[k], 2147483399);
+    }
+    seg >>= 1;
+    ++k;
+  }
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Wall roots: nearest-wall bookkeeping of closest_wall.f90 and the three
+// surface families.
+// ---------------------------------------------------------------------------
+struct Walls {
+  double tn1, tp1, tp2;
+  int in1, ip1, ip2;
+  __device__ __forceinline__ void reset() {
+    tn1 = -kHuge;
+    tp1 = kHuge;
+    tp2 = kHuge;
+    in1 = ip1 = ip2 = 0;
+  }
+  __device__ __forceinline__ void insert(double t, int i) {
+    if (t < 0.0) {
+      if (t > tn1) {
+        tn1 = t;
+        in1 = i;
+      }
+    } else if (t < tp1) {
+      tp2 = tp1;
+      ip2 = ip1;
+      tp1 = t;
+      ip1 = i;
+    } else if (t < tp2) {
+      tp2 = t;
+      ip2 = i;
+    }
+  }
+  // find_next_wall: on a wall, skip the root that is that wall.
+  __device__ __forceinline__ bool next(bool on_wall, double &t, int &face) const {
+    if (on_wall && !(fabs(tn1) < fabs(tp1))) {
+      t = tp2;
+      face = ip2;
+    } else {
+      t = tp1;
+      face = ip1;
+    }
+    return t != kHuge;
+  }
+};
+
+__device__ __forceinline__ void radface(Walls &w, double ux, double uy, double uz, double x,
+                                        double y, double z, double r2a, double r2b) {
+  const double bb = 2.0 * (x * ux + y * uy + uz * z);
+  double cc = x * x + y * y + z * z - r2a;
+  double descr = bb * bb - 4.0 * cc;
+  if (descr > 0.0) {
+    descr = sqrt(descr);
+    w.insert((-bb + descr) * 0.5, 1);
+    w.insert((-bb - descr) * 0.5, 1);
+  }
+  cc = x * x + y * y + z * z - r2b;
+  descr = bb * bb - 4.0 * cc;
+  if (descr > 0.0) {
+    descr = sqrt(descr);
+    w.insert((-bb + descr) * 0.5, 2);
+    w.insert((-bb - descr) * 0.5, 2);
+  }
+}
+
+// One theta wall: cone x^2+y^2 = t2 z^2 (t2 > 0), the midplane (t2 = -1) or
+// a pole (t2 = 0, no wall).  A cell whose two walls carry the same t2 lies
+// across the midplane (even wall counts); its two walls are the two nappes of
+// one cone and each root goes to the nappe its z says (split = true).
+__device__ __forceinline__ void cone_roots(Walls &w, double ux, double uy, double uz, double x,
+                                           double y, double z, double t2, int face,
+                                           bool split) {
+  if (t2 == 0.0) return;
+  if (t2 == -1.0) {
+    if (uz != 0.0) w.insert(-z / uz, face);
+    return;
+  }
+  const double a = ux * ux + uy * uy - t2 * uz * uz;
+  const double b = 2.0 * (x * ux + y * uy - t2 * z * uz);
+  const double c = x * x + y * y - t2 * z * z;
+  if (a == 0.0) {
+    if (b != 0.0) {
+      const double t = -c / b;
+      w.insert(t, split ? ((z + t * uz) > 0.0 ? 3 : 4) : face);
+    }
+    return;
+  }
+  double descr = b * b - 4.0 * a * c;
+  if (descr > 0.0) {
+    descr = sqrt(descr);
+    const double t1 = (-b + descr) / (2.0 * a);
+    const double t2r = (-b - descr) / (2.0 * a);
+    if (split) {
+      w.insert(t1, (z + t1 * uz) > 0.0 ? 3 : 4);
+      w.insert(t2r, (z + t2r * uz) > 0.0 ? 3 : 4);
+    } else {
+      w.insert(t1, face);
+      w.insert(t2r, face);
+    }
+  }
+}
+
+__device__ __forceinline__ void thetaface(Walls &w, double ux, double uy, double uz, double x,
+                                          double y, double z, double t2a, double t2b) {
+  if (t2a == t2b && t2a > 0.0) {
+    cone_roots(w, ux, uy, uz, x, y, z, t2a, 3, true);
+  } else {
+    cone_roots(w, ux, uy, uz, x, y, z, t2a, 3, false);
+    cone_roots(w, ux, uy, uz, x, y, z, t2b, 4, false);
+  }
+}
+
+__device__ __forceinline__ void phiface(Walls &w, double a1, double b1, double a2, double b2,
+                                        double ux, double uy, double x, double y) {
+  const double d1 = (a1 * ux + b1 * uy);
+  if (d1 != 0.0) w.insert(-(a1 * x + b1 * y) / d1, 5);
+  const double d2 = (a2 * ux + b2 * uy);
+  if (d2 != 0.0) w.insert(-(a2 * x + b2 * y) / d2, 6);
+}
+
+// nr.f:131-153 bisection.
+__device__ __forceinline__ int locate(const double *__restrict__ xx, int n, double x) {
+  int jl = 0, ju = n + 1;
+  const bool incr = xx[n - 1] > xx[0];
+  while (ju - jl > 1) {
+    const int jm = (ju + jl) >> 1;
+    if (incr == (x > xx[jm - 1]))
+      jl = jm;
+    else
+      ju = jm;
+  }
+  if (jl == 0 && x == xx[0]) jl = 1;
+  return jl;
+}
+
+// ---------------------------------------------------------------------------
+// Dust: one species of opacset (dust.f90:13-76).
+// ---------------------------------------------------------------------------
+struct Opac1 {
+  double kappa, albedo, g, pl;
+};
+
+__device__ __forceinline__ Opac1 opacset1(const double *__restrict__ nud,
+                                          const double *__restrict__ kd,
+                                          const double *__restrict__ ad,
+                                          const double *__restrict__ gd,
+                                          const double *__restrict__ pd, int nl, double nu) {
+  Opac1 o;
+  if (nu >= nud[0]) {
+    o.g = gd[0];
+    o.pl = pd[0];
+    o.kappa = kd[0];
+    o.albedo = ad[0];
+  } else if (nu <= nud[nl - 1]) {
+    o.g = gd[nl - 1];
+    o.pl = pd[nl - 1];
+    o.kappa = kd[nl - 1];
+    o.albedo = ad[nl - 1];
+  } else {
+    int i0 = 1, i1 = nl;
+    while (i1 - i0 > 1) {
+      const int i = (i1 + i0) >> 1;
+      if (nu <= nud[i - 1])
+        i0 = i;
+      else
+        i1 = i;
+    }
+    const double f = (nu - nud[i1 - 1]) / (nud[i0 - 1] - nud[i1 - 1]);
+    const double fm1 = 1.0 - f;
+    o.g = fm1 * gd[i1 - 1] + f * gd[i0 - 1];
+    o.pl = fm1 * pd[i1 - 1] + f * pd[i0 - 1];
+    o.kappa = fm1 * kd[i1 - 1] + f * kd[i0 - 1];
+    o.albedo = fm1 * ad[i1 - 1] + f * ad[i0 - 1];
+  }
+  return o;
+}
+
+// Derived HG terms of a species (dust.f90:68-72).
+struct HG {
+  double g, g2, g2p1, onemg2, twog, pl;
+  __device__ __forceinline__ HG(double g_, double pl_) : g(g_), pl(pl_) {
+    g2 = g * g;
+    g2p1 = g2 + 1.0;
+    onemg2 = 1.0 - g2;
+    twog = 2.0 * g;
+  }
+};
+
+// dustmat.f90:39-66 (p4 = 0).
+__device__ __forceinline__ void dustmat(const HG &h, double cost, double cost2, double &p1,
+                                        double &p2, double &p3) {
+  const double temp = h.g2p1 - h.twog * cost;
+  p1 = 1.5 * h.onemg2 / (2.0 + h.g2) * (1.0 + cost2) / (temp * sqrt(temp));
+  p2 = -h.pl * p1 * (1.0 - cost2) / (1.0 + cost2);
+  p3 = p1 * 2.0 * cost / (1.0 + cost2);
+}
+
+__device__ __forceinline__ double cs_peak(const HG &h) {  // stokes.f90:54-62
+  const double temp = (h.g > 0.0) ? h.g2p1 - h.twog : h.g2p1 + h.twog;
+  return 1.5 * (h.onemg2) / (2.0 + h.g2) * 2.0 / temp / sqrt(temp) * 1.01;
+}
+
+// Packet direction and Stokes vector that scattering rotates.
+struct Dir {
+  double cost, sint, cosp, sinp, phi;
+};
+struct Stokes4 {
+  double I, Q, U, V;
+};
+
+// Mueller rows for the scattering frame (stokes.f90:206-240 / 337-369),
+// shared by stokes and stokespeel.  upper selects the ri1 > pi branch.
+struct Mueller {
+  double a11, a12, a13, a21, a22, a23, a31, a32, a33, a44;
+};
+
+__device__ __forceinline__ Mueller mueller(bool upper, double p1, double p2, double p3,
+                                           double sini2, double cosi2, double s2a, double c2a) {
+  // s2a/c2a are sin(2 i1), cos(2 i1) (lower) or sin(2 i3), cos(2 i3) (upper)
+  Mueller m;
+  const double sin2i2 = 2.0 * sini2 * cosi2;
+  const double cos2i2 = 2.0 * (cosi2 * cosi2) - 1.0;
+  const double sin2 = sin2i2 * s2a;
+  const double cos2 = cos2i2 * c2a;
+  const double sin2cos1 = sin2i2 * c2a;
+  const double cos2sin1 = cos2i2 * s2a;
+  m.a11 = p1;
+  m.a12 = p2 * c2a;
+  m.a21 = p2 * cos2i2;
+  m.a22 = p1 * cos2 - p3 * sin2;
+  m.a33 = -p1 * sin2 + p3 * cos2;
+  m.a44 = p3;
+  if (upper) {
+    m.a13 = p2 * s2a;
+    m.a23 = p1 * cos2sin1 + p3 * sin2cos1;
+    m.a31 = -p2 * sin2i2;
+    m.a32 = -p1 * sin2cos1 - p3 * cos2sin1;
+  } else {
+    m.a13 = -p2 * s2a;
+    m.a23 = -p1 * cos2sin1 - p3 * sin2cos1;
+    m.a31 = p2 * sin2i2;
+    m.a32 = p1 * sin2cos1 + p3 * cos2sin1;
+  }
+  return m;
+}
+
+// stokes.f90:1-392 (idust = 1): rejection sampling of (mu, i1), new direction
+// by spherical trigonometry, Stokes rotation normalised by the accepted
+// probability.  ntrial/nhit count trials and acceptances (htot, hit).
+__device__ __forceinline__ void stokes(Dir &d, Stokes4 &s, const HG &h, Mrg &rng,
+                                       double &ntrial, double &nhit) {
+  const double costp = d.cost, sintp = d.sint, phip = d.phi;
+  double peak = cs_peak(h);
+  double bmu, cosb2, p1, p2, p3, sinbt, ri, c2a, s2a, ci, si, rprob;
+  bool upper;
+  for (;;) {
+    ntrial += 1.0;
+    bmu = 1.0 - 2.0 * ran(rng);
+    cosb2 = bmu * bmu;
+    dustmat(h, bmu, cosb2, p1, p2, p3);
+    sinbt = sqrt(1.0 - cosb2);
+    const double ri1 = kR2P * ran(rng);
+    upper = ri1 > kPi;
+    ri = upper ? kR2P - ri1 : ri1;
+    ci = cos(ri);
+    si = sin(ri);
+    s2a = 2.0 * si * ci;
+    c2a = 2.0 * (ci * ci) - 1.0;
+    const double a12 = p2 * c2a;
+    const double a13 = upper ? p2 * s2a : -p2 * s2a;
+    rprob = (p1 * s.I + a12 * s.Q + a13 * s.U) / s.I;
+    if (rprob > peak) peak = rprob;
+    if (peak * ran(rng) > rprob) continue;
+    nhit += 1.0;
+    break;
+  }
+  if (bmu == 1.0) return;  // stokes.f90:102-105
+  double cost, sint, sini2, cosi2;
+  if (bmu == -1.0) {
+    si = 1.0;
+    sini2 = 1.0;
+    ci = 0.0;
+    cosi2 = 0.0;
+    cost = -costp;
+    sint = sintp;
+  } else {
+    cost = costp * bmu + sintp * sinbt * ci;
+    if (fabs(cost) < 1.0) {
+      sint = fabs(sqrt(1.0 - cost * cost));
+      sini2 = si * sintp / sint;
+      const double bott = sint * sinbt;
+      cosi2 = costp / bott - cost * bmu / bott;
+      if (cosi2 >= 1.0) {
+        cosi2 = 1.0;
+        sini2 = 0.0;
+      }
+      if (cosi2 <= -1.0) {
+        cosi2 = -1.0;
+        sini2 = 0.0;
+      }
+    } else {
+      cosi2 = 0.0;
+      if (cost >= 1.0) {
+        cost = 1.0;
+        cosi2 = -1.0;
+      }
+      if (cost <= -1.0) {
+        cost = -1.0;
+        cosi2 = 1.0;
+      }
+      sint = 0.0;
+      sini2 = 0.0;
+    }
+  }
+  double cosdph = upper ? -cosi2 * ci + sini2 * si * bmu : -ci * cosi2 + si * sini2 * bmu;
+  if (fabs(cosdph) > 1.0) cosdph = (cosdph > 1.0) ? 1.0 : -1.0;
+  double phi = upper ? phip + acos(cosdph) : phip - acos(cosdph);
+  if (phi > kR2P) phi = phi - kR2P;
+  if (phi < 0.0) phi = phi + kR2P;
+  const Mueller m = mueller(upper, p1, p2, p3, sini2, cosi2, s2a, c2a);
+  const double a = rprob;
+  const double sI = (m.a11 * s.I + m.a12 * s.Q + m.a13 * s.U) / a;
+  const double sQ = (m.a21 * s.I + m.a22 * s.Q + m.a23 * s.U + 0.0 * s.V) / a;
+  const double sU = (m.a31 * s.I + m.a32 * s.Q + m.a33 * s.U + 0.0 * s.V) / a;
+  const double sV = (0.0 * s.Q + 0.0 * s.U + m.a44 * s.V) / a;
+  s.I = sI;
+  s.Q = sQ;
+  s.U = sU;
+  s.V = sV;
+  d.cost = cost;
+  d.sint = sint;
+  d.phi = phi;
+  d.cosp = cos(phi);
+  d.sinp = sin(phi);
+}
+
+// findangle3.f90 (observer azimuth phi2 = 0 as peeloff_3d.f90:168 passes
+// it) followed by stokespeel.f90 for one observer direction (ce, se).
+__device__ __forceinline__ void peel_weight(const Dir &d, double ce, double se, double cpe,
+                                            double spe, const HG &h, Stokes4 &s) {
+  double bmu = d.cost * ce + (d.sint * se * (d.cosp * cpe + d.sinp * spe));
+  if (bmu > 1.0) bmu = 1.0;
+  if (bmu < -1.0) bmu = -1.0;
+  const double sintnew = sqrt(1.0 - bmu * bmu);
+  double ri1 = 0.0;
+  if (d.sint > 1.e-8 && sintnew > 1.e-8) {
+    double cosx = (ce - (d.cost * bmu)) / (d.sint * sintnew);
+    if (cosx > 1.0) cosx = 1.0;
+    if (cosx < -1.0) cosx = -1.0;
+    const double x = acos(cosx);
+    const double phi2 = 0.0;
+    if (phi2 > d.phi)
+      ri1 = (phi2 - d.phi > kPi) ? x : kR2P - x;
+    else
+      ri1 = (d.phi - phi2 > kPi) ? kR2P - x : x;
+  }
+  const double cosb2 = bmu * bmu;
+  double p1, p2, p3;
+  dustmat(h, bmu, cosb2, p1, p2, p3);
+  const double sinbt = sqrt(1.0 - cosb2);
+  const bool upper = ri1 > kPi;
+  const double ri = upper ? kR2P - ri1 : ri1;
+  double ci = cos(ri), si = sin(ri);
+  const double s2a = 2.0 * si * ci;
+  const double c2a = 2.0 * (ci * ci) - 1.0;
+  if (bmu == 1.0) return;  // stokespeel.f90:116-119
+  double sini2, cosi2;
+  if (bmu == -1.0) {
+    si = 1.0;
+    sini2 = 1.0;
+    ci = 0.0;
+    cosi2 = 0.0;
+  } else if (fabs(ce) < 1.0) {
+    sini2 = si * d.sint / se;
+    const double bott = se * sinbt;
+    cosi2 = d.cost / bott - ce * bmu / bott;
+    if (cosi2 >= 1.0) {
+      cosi2 = 1.0;
+      sini2 = 0.0;
+    }
+    if (cosi2 <= -1.0) {
+      cosi2 = -1.0;
+      sini2 = 0.0;
+    }
+  } else {
+    cosi2 = (ce >= 1.0) ? -1.0 : 1.0;
+    sini2 = 0.0;
+  }
+  const Mueller m = mueller(upper, p1, p2, p3, sini2, cosi2, s2a, c2a);
+  const double sI = (m.a11 * s.I + m.a12 * s.Q + m.a13 * s.U);
+  const double sQ = (m.a21 * s.I + m.a22 * s.Q + m.a23 * s.U + 0.0 * s.V);
+  const double sU = (m.a31 * s.I + m.a32 * s.Q + m.a33 * s.U + 0.0 * s.V);
+  const double sV = (0.0 * s.Q + 0.0 * s.U + m.a44 * s.V);
+  s.I = sI;
+  s.Q = sQ;
+  s.U = sU;
+  s.V = sV;
+}
+
+}  // namespace hc

@@ -1,0 +1,1423 @@
+// hc_transport.hip -- the photon-packet transport pass on the GPU and the C ABI
+// around it (include/hc.h).
+//
+// Execution model: one persistent launch per pass.  Every lane pulls packet
+// indices from a device counter and carries each packet from launch to
+// escape, star re-emission or kill (the body of propagate.f90 around the
+// reconstructed tauint), so lanes stay busy until the pass drains regardless
+// of how unevenly the packets random-walk.  Packet p draws from its own
+// segment of the L'Ecuyer-Cote generator, so results do not depend on which
+// lane ran it: the deterministic-trace mode compares packet by packet with
+// the CPU restatement.
+//
+// Device layout: per-cell data are cell-major with the active species
+// innermost (dens[cell*nact + k]), cell = (ir-1) + nrg*((it-1) + ntg*(ip-1))
+// as in the reference's arrays, so one crossing reads one contiguous run of
+// nact doubles.  densm holds the same densities with species at or above the
+// sublimation temperature zeroed (peeloff_3d.f90:103-110), so a crossing needs
+// no temperature read.  Tallies are FP64 device atomics.
+#include "hc_device.h"
+#include "../../include/hc.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace hc {
+
+enum Scalar {
+  S_TOT = 0, S_FLUX, S_SFLUX, S_NSCAT, S_SCOUNT, S_ABFLUX, S_AFLUX, S_SUMSUB, S_SUMALL,
+  S_HIT, S_HTOT, S_ERR_WALL, S_ERR_DUST, S_WARN, S_COUNT
+};
+
+struct Params {
+  // grid
+  int nrg, ntg, npg, nact;
+  int act[kMaxSpecies];      // reference species id (1-based) of slot k
+  int slot[kMaxSpecies];     // slot of species id-1, or -1
+  const double *r2arr, *tan2arr, *thetarr, *aarr, *barr, *phiarr;
+  const double *dens, *densm, *tdust;
+  // dust
+  int nlammax, nT, nnu;
+  int nlambda[kMaxSpecies];
+  const double *nudust, *kappad, *adust, *gdust, *pldust;
+  double kappav[kMaxSpecies];
+  const double *lTint, *nuint, *kapBint;
+  // sources
+  double rstar, rmincgs, tstar, tshock;
+  int ips, ispot, nspot, ipeel;
+  double fspot, csprad, cspot1, sspot1, scspot1, ssspot1, cspot2, sspot2, scspot2, ssspot2;
+  long long np, npacc, npaccstar;
+  // binning
+  int nfreq, nmu, nph, nap, npeel, nxhst, imcube, iintmax;
+  double numin, lnurat, rmaxi, fractxh;
+  const double *u, *sini, *aperture2, *coste, *sinte, *cospe, *sinpe;
+  // rng
+  int32_t g1, g2;
+  unsigned long long seg0;
+  const int32_t *pow1, *pow2;
+  // tallies
+  double *dtauabs, *nabs, *jmean;
+  int32_t *killed, *dbig;
+  double *sed, *peel, *img_i, *img_q, *img_u, *img_v, *si_init;
+  double *scal;
+  unsigned long long *next_packet;
+  hc_trace *trace;
+};
+
+// ---------------------------------------------------------------------------
+// packet state
+// ---------------------------------------------------------------------------
+struct Pkt {
+  double x, y, z, ux, uy, uz;
+  Dir d;
+  Stokes4 s;
+  double nub, rsq, rtot;
+  int ir, it, ip;
+  bool on_wall;
+  int iflag, i_orig;
+  double kapd[kMaxSpecies];
+};
+
+struct Acc {  // per-lane scalar tallies, flushed once per lane
+  double v[S_COUNT];
+};
+
+struct Trace {
+  double path, tau_peel;
+  long long ncross, npeelcross, nscat, nabs, nstar;
+  unsigned long long hash;
+};
+
+__device__ __forceinline__ size_t cell_of(const Params &P, int ir, int it, int ip) {
+  return (size_t)(ir - 1) + (size_t)P.nrg * ((size_t)(it - 1) + (size_t)P.ntg * (size_t)(ip - 1));
+}
+
+__device__ __forceinline__ void hash_cell(Trace &t, int ir, int it, int ip) {
+  t.hash ^= (unsigned long long)(unsigned)ir | ((unsigned long long)(unsigned)it << 20) |
+            ((unsigned long long)(unsigned)ip << 40);
+  t.hash *= 1099511628211ull;
+}
+
+__device__ __forceinline__ void atomic_add(double *p, double v) { unsafeAtomicAdd(p, v); }
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) {
+  return v < lo ? lo : (v > hi ? hi : v);
+}
+
+// Masked opacity of a cell: sum over active species in increasing id.
+__device__ __forceinline__ double cell_opac(const Params &P, const Pkt &k, size_t cell) {
+  const double *dm = P.densm + cell * (size_t)P.nact;
+  double opac = 0.0;
+#pragma unroll
+  for (int s = 0; s < kMaxSpecies; ++s)
+    if (s < P.nact) opac = opac + k.kapd[s] * dm[s];
+  return opac;
+}
+
+__device__ __forceinline__ bool next_wall(const Params &P, double ux, double uy, double uz,
+                                          double x, double y, double z, int ir, int it, int ip,
+                                          bool on_wall, double &dt, int &face) {
+  Walls w;
+  w.reset();
+  radface(w, ux, uy, uz, x, y, z, P.r2arr[ir - 1], P.r2arr[ir]);
+  if (P.ntg > 2) thetaface(w, ux, uy, uz, x, y, z, P.tan2arr[it - 1], P.tan2arr[it]);
+  if (P.npg > 2 && P.ntg > 2)
+    phiface(w, P.aarr[ip - 1], P.barr[ip - 1], P.aarr[ip], P.barr[ip], ux, uy, x, y);
+  return w.next(on_wall, dt, face);
+}
+
+__device__ __forceinline__ void face_step(const Params &P, int face, int &ir, int &it, int &ip) {
+  switch (face) {
+    case 1: ir -= 1; break;
+    case 2: ir += 1; break;
+    case 3: it -= 1; break;
+    case 4: it += 1; break;
+    case 5: ip -= 1; break;
+    default: ip += 1; break;
+  }
+  if (ip == 0) ip = P.npg - 1;
+  if (ip == P.npg) ip = 1;
+}
+
+enum { ST_INTERACT = 0, ST_EXIT = 1, ST_STAR = 2, ST_ERROR = 3 };
+
+// Lucy tally of one chord: dtauabs += kapd (1-albedo) rho ds per active
+// species, jmean += ds R*.
+__device__ __forceinline__ void lucy_tally(const Params &P, const Pkt &k, const double *alb,
+                                           size_t cell, double ds) {
+  if (P.dtauabs) {
+    const double *dm = P.densm + cell * (size_t)P.nact;
+    double *ta = P.dtauabs + cell * (size_t)P.nact;
+#pragma unroll
+    for (int s = 0; s < kMaxSpecies; ++s)
+      if (s < P.nact && dm[s] != 0.0)
+        atomic_add(&ta[s], k.kapd[s] * (1.0 - alb[s]) * dm[s] * ds);
+  }
+  if (P.jmean) atomic_add(&P.jmean[cell], ds * P.rmincgs);
+}
+
+// The cell walk to optical depth tau (tauint reconstruction).
+__device__ int tauint(const Params &P, Pkt &k, const double *alb, double tau, Acc &acc,
+                      Trace &tr, bool tracing) {
+  if (k.on_wall) {
+    const bool inner = fabs(k.rsq - P.r2arr[k.ir - 1]) < fabs(k.rsq - P.r2arr[k.ir]);
+    const bool outwards = k.ux * k.x + k.uy * k.y + k.uz * k.z > 0.0;
+    if (inner && !outwards) {
+      k.ir -= 1;
+      if (k.ir == 0) return ST_STAR;
+    }
+    if (!inner && outwards) k.ir += 1;
+    if (k.ir == P.nrg) return ST_EXIT;
+  }
+  double tsum = 0.0;
+  for (;;) {
+    const size_t cell = cell_of(P, k.ir, k.it, k.ip);
+    double dt;
+    int face;
+    if (!next_wall(P, k.ux, k.uy, k.uz, k.x, k.y, k.z, k.ir, k.it, k.ip, k.on_wall, dt, face)) {
+      acc.v[S_ERR_WALL] += 1.0;
+      return ST_ERROR;
+    }
+    const double opac = cell_opac(P, k, cell);
+    const double dtau = opac * dt;
+    if (tsum + dtau >= tau) {
+      const double ds = (tau - tsum) / opac;
+      lucy_tally(P, k, alb, cell, ds);
+      k.x = k.x + ds * k.ux;
+      k.y = k.y + ds * k.uy;
+      k.z = k.z + ds * k.uz;
+      k.on_wall = false;
+      k.rsq = k.x * k.x + k.y * k.y + k.z * k.z;
+      k.rtot = sqrt(k.rsq);
+      if (tracing) tr.path += ds;
+      return ST_INTERACT;
+    }
+    lucy_tally(P, k, alb, cell, dt);
+    tsum = tsum + dtau;
+    face_step(P, face, k.ir, k.it, k.ip);
+    k.x = k.x + dt * k.ux;
+    k.y = k.y + dt * k.uy;
+    k.z = k.z + dt * k.uz;
+    k.on_wall = true;
+    k.rsq = k.x * k.x + k.y * k.y + k.z * k.z;
+    k.rtot = sqrt(k.rsq);
+    if (tracing) {
+      tr.path += dt;
+      tr.ncross++;
+      hash_cell(tr, k.ir, k.it, k.ip);
+    }
+    if (k.ir == P.nrg) return ST_EXIT;
+    if (k.ir == 0) return ST_STAR;
+  }
+}
+
+__device__ __forceinline__ HG hg_of(const Params &P, int id, double nu) {
+  const size_t o = (size_t)P.nlammax * id;
+  const Opac1 op = opacset1(P.nudust + o, P.kappad + o, P.adust + o, P.gdust + o, P.pldust + o,
+                            P.nlambda[id], nu);
+  return HG(op.g, op.pl);
+}
+
+// peeloff_3d.f90:5-288 for observer `peelid`.  sip is the peel weight; the
+// remaining Stokes components come from the packet.
+__device__ __attribute__((noinline)) void peeloff(const Params &P, const Pkt &k, double sip,
+                                                  int idust2, int peelid, Acc &acc, Trace &tr,
+                                                  bool tracing) {
+  Stokes4 s{sip, k.s.Q, k.s.U, k.s.V};
+  double x1 = k.x, y1 = k.y, z1 = k.z;
+  int ir = k.ir, it = k.it, ip = k.ip;
+  bool on_wall = k.on_wall;
+  const double cospe = P.cospe[peelid], sinpe = P.sinpe[peelid];
+  const double coste = P.coste[peelid], sinte = P.sinte[peelid];
+  const double ux1 = sinte * cospe, uy1 = sinte * sinpe, uz1 = coste;
+  if (on_wall) {
+    const bool inner = fabs(k.rsq - P.r2arr[ir - 1]) < fabs(k.rsq - P.r2arr[ir]);
+    const bool outwards = ux1 * x1 + uy1 * y1 + uz1 * z1 > 0.0;
+    if (inner && !outwards) {
+      if (ir == 1) return;
+      ir = ir - 1;
+    }
+    if (!inner && outwards) ir = ir + 1;
+  }
+  double tsum = 0.0;
+  while (ir < P.nrg) {
+    double dt;
+    int face;
+    if (!next_wall(P, ux1, uy1, uz1, x1, y1, z1, ir, it, ip, on_wall, dt, face)) {
+      acc.v[S_ERR_WALL] += 1.0;
+      return;
+    }
+    tsum = tsum + cell_opac(P, k, cell_of(P, ir, it, ip)) * dt;
+    if (tsum > kTauMax) {
+      if (tracing) tr.tau_peel += tsum;
+      return;
+    }
+    face_step(P, face, ir, it, ip);
+    x1 = x1 + dt * ux1;
+    y1 = y1 + dt * uy1;
+    z1 = z1 + dt * uz1;
+    on_wall = true;
+    if (tracing) tr.npeelcross++;
+    if (ir == 0) {
+      if (tracing) tr.tau_peel += tsum;
+      return;
+    }
+  }
+  if (tracing) tr.tau_peel += tsum;
+  const double factor = exp(-tsum);
+  s.I = s.I * factor;
+  s.Q = s.Q * factor;
+  s.U = s.U * factor;
+  s.V = s.V * factor;
+  if (k.iflag == 1) peel_weight(k.d, coste, sinte, cospe, sinpe, hg_of(P, idust2, k.nub), s);
+  const double yimage = z1 * sinte - y1 * coste * sinpe - x1 * coste * cospe;
+  const double ximage = y1 * cospe - x1 * sinpe;
+  const int ix = (int)((ximage + P.rmaxi) / P.fractxh) + 1;
+  const int iy = (int)((yimage + P.rmaxi) / P.fractxh) + 1;
+  const double rho2 = ximage * ximage + yimage * yimage;
+  const int inub =
+      clampi((int)((double)P.nfreq * log(k.nub / P.numin) / P.lnurat) + 1, 1, P.nfreq);
+  const int i_inter = (k.iflag == 1) ? 2 : ((k.i_orig == 1) ? 1 : 3);
+  const int indx[3] = {1, 1 + k.i_orig, 5 + i_inter};
+  if (P.peel) {
+    const size_t n = (size_t)P.nfreq * P.npeel * P.nap * kNo;
+    for (int ia = 0; ia < P.nap; ++ia) {
+      if (rho2 < P.aperture2[ia]) {
+        for (int j = 0; j < 3; ++j) {
+          const size_t q = (size_t)(inub - 1) +
+                           (size_t)P.nfreq * ((size_t)peelid +
+                                              (size_t)P.npeel * ((size_t)ia + (size_t)P.nap *
+                                                                                  (size_t)(indx[j] - 1)));
+          atomic_add(&P.peel[q], s.I);
+          atomic_add(&P.peel[q + n], s.Q);
+          atomic_add(&P.peel[q + 2 * n], s.U);
+          atomic_add(&P.peel[q + 3 * n], s.V);
+          atomic_add(&P.peel[q + 4 * n], s.I * s.I);
+          atomic_add(&P.peel[q + 5 * n], s.Q * s.Q);
+          atomic_add(&P.peel[q + 6 * n], s.U * s.U);
+          atomic_add(&P.peel[q + 7 * n], s.V * s.V);
+          atomic_add(&P.peel[q + 8 * n], 1.0);
+        }
+      }
+    }
+  }
+  if (P.imcube && P.img_i && ix <= P.nxhst && iy <= P.nxhst && ix > 0 && iy > 0) {
+    const size_t nx = (size_t)P.nxhst;
+    const size_t q = (size_t)peelid +
+                     (size_t)P.npeel * ((size_t)(ix - 1) + nx * ((size_t)(iy - 1) +
+                                                                  nx * (size_t)(inub - 1)));
+    const size_t n4 = (size_t)P.npeel * nx * nx * P.nfreq;
+    for (int j = 0; j < 3; ++j) atomic_add(&P.img_i[q + n4 * (size_t)(indx[j] - 1)], s.I);
+    atomic_add(&P.img_q[q], s.Q);
+    atomic_add(&P.img_u[q], s.U);
+    atomic_add(&P.img_v[q], s.V);
+  }
+}
+
+__device__ __forceinline__ double inczeta_at(int i) {
+  // inczeta(i) = sum_{j=i..1} 1/j^4 accumulated from the largest j down, as
+  // zetainit (random.f90:79-97) builds it; evaluated on the fly.
+  double z = 0.0;
+  for (int j = i; j >= 1; --j) z = z + 1.0 / (double)(j * j * j * j);
+  return z;
+}
+
+__constant__ double c_inczeta[146];
+
+__device__ __forceinline__ double bbfreq(Mrg &rng, double teff) {  // random.f90:49-77
+  const double z0 = c_inczeta[145] * ran(rng);
+  const double z1 = ran(rng), z2 = ran(rng), z3 = ran(rng), z4 = ran(rng);
+  int i = 1;
+  while (z0 > c_inczeta[i]) ++i;
+  return -log(z1 * z2 * z3 * z4) * teff / (double)i;
+}
+
+// lucydustfreq.f90:19-157 with its guard chain; j clamped to [1,nT-1].
+__device__ double lucydustfreq(const Params &P, Mrg &rng, double T0, int id) {
+  const int nT = P.nT, nnu = P.nnu;
+  const double *lT = P.lTint, *nuint = P.nuint;
+  const double *kb = P.kapBint + (size_t)nT * nnu * id;
+  auto KB = [&](int jj, int ii) { return kb[(size_t)(jj - 1) + (size_t)nT * (size_t)(ii - 1)]; };
+  const double lT0 = log(T0);
+  const double xi = ran(rng);
+  int j = (int)((double)(nT - 1) * (lT0 - lT[0]) / (lT[nT - 1] - lT[0])) + 1;
+  j = clampi(j, 1, nT - 1);
+  double f = (lT0 - lT[j - 1]) / (lT[j] - lT[j - 1]);
+  if (f < 0.0) f = 0.0;
+  if (f > 1.0) f = 1.0;
+  int i0 = 1, i1 = nnu;
+  double P0 = (1.0 - f) * KB(j, i0) + f * KB(j + 1, i0);
+  if (xi < P0) return (P0 > 0.0) ? T0 * xi * nuint[0] / P0 : T0 * nuint[0];
+  double P1 = (1.0 - f) * KB(j, i1) + f * KB(j + 1, i1);
+  if (P1 > 1.0) P1 = 1.0;
+  if (xi >= P1) {
+    if (P1 < 1.0 && xi < 1.0) return T0 * nuint[nnu - 1] * (1.0 - P1) / (1.0 - xi);
+    return T0 * nuint[nnu - 1];
+  }
+  while ((i1 - i0) > 1) {
+    const int i = (i0 + i1) / 2;
+    double Pm = (1.0 - f) * KB(j, i) + f * KB(j + 1, i);
+    if (Pm > 1.0) Pm = 1.0;
+    if (Pm < P0) {
+      P0 = Pm;
+      i0 = i;
+    }
+    if (Pm > P1) {
+      P1 = Pm;
+      i1 = 1;  // lucydustfreq.f90:109 as written
+    }
+    if (Pm > xi) {
+      i1 = i;
+      P1 = Pm;
+    } else {
+      i0 = i;
+      P0 = Pm;
+    }
+  }
+  if (xi < P0)
+    f = 0.0;
+  else if (xi > P1)
+    f = 1.0;
+  else if (P0 <= 0.0)
+    f = 0.5;
+  else if (P1 <= 0.0 || xi <= 0.0) {
+  } else
+    f = log(xi / P0) / log(P1 / P0);
+  return T0 * nuint[i0 - 1] * pow(nuint[i1 - 1] / nuint[i0 - 1], f);
+}
+
+// opacset for all active species: kapd = kappa * R*(cm) * kappa_V and the
+// albedo.  launch_order picks newdisktherm's (kappa*rmincgs)*kappav or
+// propagate's ((kappa*rstar)*rsol)*kappav grouping.
+__device__ __forceinline__ void set_opacity(const Params &P, Pkt &k, double *alb,
+                                            bool launch_order) {
+#pragma unroll
+  for (int s = 0; s < kMaxSpecies; ++s) {
+    if (s < P.nact) {
+      const int id = P.act[s] - 1;
+      const size_t o = (size_t)P.nlammax * id;
+      const Opac1 op = opacset1(P.nudust + o, P.kappad + o, P.adust + o, P.gdust + o,
+                                P.pldust + o, P.nlambda[id], k.nub);
+      k.kapd[s] = launch_order ? op.kappa * P.rmincgs * P.kappav[id]
+                               : op.kappa * P.rstar * kRsol * P.kappav[id];
+      alb[s] = op.albedo;
+    }
+  }
+}
+
+__device__ __forceinline__ void initp_sample(Pkt &k, Mrg &rng) {  // initp_common.f90:1-29
+  k.d.cost = sqrt(ran(rng));
+  k.d.sint = sqrt(1.0 - k.d.cost * k.d.cost);
+  k.d.phi = kR2P * ran(rng);
+  k.d.cosp = cos(k.d.phi);
+  k.d.sinp = sin(k.d.phi);
+}
+
+__device__ __forceinline__ void initp_transform(Pkt &k, double cosb, double sinb, double lp) {
+  if (fabs(cosb) < 0.9999999) {  // initp_common.f90:31-94
+    double costi = k.d.cost * cosb - k.d.sint * sinb * k.d.cosp;
+    if (fabs(costi) > 1.0) {
+      if (costi > 1.0) costi = 1.0;
+      if (k.d.cost < -1.0) costi = -1.0;
+    }
+    const double sinti = sqrt(1.0 - costi * costi);
+    double phinew;
+    if (sinti > 0.0000001) {
+      double cospn = (k.d.cost - costi * cosb) / (sinti * sinb);
+      if (fabs(cospn) > 1.0) cospn = (cospn < 0.0) ? -1.0 : 1.0;
+      if (k.d.phi < kPi) {
+        phinew = acos(cospn);
+        if (phinew > kPi) phinew = kPi;
+      } else {
+        phinew = kR2P - acos(cospn);
+        if (phinew < kPi) phinew = kPi;
+      }
+    } else {
+      phinew = 0.0;
+    }
+    k.d.phi = phinew;
+    k.d.cost = costi;
+    k.d.sint = sinti;
+  } else if (cosb < 0.0) {
+    k.d.cost = -k.d.cost;
+  }
+  k.d.phi = k.d.phi + lp;
+  if (k.d.phi > kR2P) k.d.phi = k.d.phi - kR2P;
+  if (k.d.phi < 0.0) k.d.phi = k.d.phi + kR2P;
+  k.d.sinp = sin(k.d.phi);
+  k.d.cosp = cos(k.d.phi);
+}
+
+__device__ __forceinline__ double clamp1(double x) { return x > 1.0 ? 1.0 : (x < -1.0 ? -1.0 : x); }
+
+__device__ __forceinline__ void bin_init(const Params &P, double nub, int i_orig_init) {
+  if (!P.si_init) return;
+  const int inub = clampi((int)((double)P.nfreq * log(nub / P.numin) / P.lnurat) + 1, 1, P.nfreq);
+  atomic_add(&P.si_init[(size_t)(inub - 1) + (size_t)P.nfreq * (size_t)(i_orig_init - 1)], 1.0);
+  atomic_add(&P.si_init[(size_t)(inub - 1)], 1.0);
+}
+
+__device__ void peel_direct(const Params &P, const Pkt &k, double cosb, double sinb,
+                            double coslp, double sinlp, Acc &acc, Trace &tr, bool tracing) {
+  for (int q = 0; q < P.npeel; ++q) {
+    if (!P.ips) {
+      const double cosnorm =
+          cosb * P.coste[q] + (sinb * P.sinte[q] * (P.cospe[q] * coslp + P.sinpe[q] * sinlp));
+      const double sipnew = 4.0 * k.s.I * cosnorm;
+      if (cosnorm > 0.0) peeloff(P, k, sipnew, 0, q, acc, tr, tracing);
+    } else {
+      peeloff(P, k, k.s.I * 2.0, 0, q, acc, tr, tracing);
+    }
+  }
+}
+
+// Star / hot-spot launch (initpspot.f90:3-175 or initp_ps.f90:3-79, then
+// newdisktherm.F90:256-377).
+__device__ void launch_star(const Params &P, Pkt &k, double *alb, Mrg &rng, bool hotspot,
+                            Acc &acc, Trace &tr, bool tracing) {
+  double cosb, sinb, lp;
+  int ispotph = hotspot ? 1 : 0;
+  k.s = Stokes4{1.0, 0.0, 0.0, 0.0};
+  if (!P.ips) {
+    for (;;) {
+      for (;;) {
+        cosb = 1.0 - 2.0 * ran(rng);
+        sinb = sqrt(1.0 - cosb * cosb);
+        if (cosb != 0.0 && fabs(cosb) < 1.0) break;
+      }
+      lp = kR2P * ran(rng);
+      if (P.ispot == 1 && ispotph == 1) {
+        const double cphi = cos(lp), sphi = sin(lp);
+        const double d1 = P.scspot1 * sinb * cphi + P.ssspot1 * sinb * sphi + P.cspot1 * cosb;
+        const double d2 = P.scspot2 * sinb * cphi + P.ssspot2 * sinb * sphi + P.cspot2 * cosb;
+        if (P.nspot == 1 ? (d1 < P.csprad) : (d1 < P.csprad && d2 < P.csprad)) continue;
+      }
+      break;
+    }
+    if (P.ispot == 1 && P.fspot < 1.0) {
+      double dellon = (lp < kPi) ? lp : kR2P - lp;
+      if (P.cspot1 * cosb + P.sspot1 * sinb * cos(dellon) > P.csprad) ispotph = 1;
+      if (P.nspot == 2) {
+        dellon = (lp < kPi) ? kPi - lp : lp - kPi;
+        if (P.cspot2 * cosb + P.sspot2 * sinb * cos(dellon) > P.csprad) ispotph = 1;
+      }
+    }
+    initp_sample(k, rng);
+    initp_transform(k, cosb, sinb, lp);
+  } else {
+    for (;;) {
+      cosb = clamp1(1.0 - 2.0 * ran(rng));
+      sinb = clamp1(sqrt(1.0 - cosb * cosb));
+      if (cosb != 0.0) break;
+    }
+    k.d.cost = cosb;
+    k.d.sint = sinb;
+    lp = kR2P * ran(rng);
+    k.d.phi = lp;
+    k.d.sinp = clamp1(sin(k.d.phi));
+    k.d.cosp = clamp1(cos(k.d.phi));
+  }
+  k.nub = bbfreq(rng, ispotph ? P.tshock : P.tstar);
+  if (hotspot) {
+    if (ran(rng) < 0.5) k.nub = 1.2398 / 0.06 + ran(rng) * (1.2398 / 0.015 - 1.2398 / 0.06);
+    bin_init(P, k.nub, 3);
+  } else {
+    bin_init(P, k.nub, 2);
+  }
+  k.i_orig = 1;
+  set_opacity(P, k, alb, true);
+  const double coslp = cos(lp), sinlp = sin(lp);
+  k.z = cosb * 1.0;
+  const double rp = sinb * 1.0;
+  k.x = rp * coslp;
+  k.y = rp * sinlp;
+  k.rsq = k.z * k.z + rp * rp;
+  k.rtot = sqrt(k.rsq);
+  k.ux = k.d.sint * k.d.cosp;
+  k.uy = k.d.sint * k.d.sinp;
+  k.uz = k.d.cost;
+  k.iflag = 0;
+  k.ir = 1;
+  k.on_wall = true;
+  k.it = locate(P.thetarr, P.ntg, acos(cosb));
+  k.ip = locate(P.phiarr, P.npg, lp);
+  if (P.ipeel) peel_direct(P, k, cosb, sinb, coslp, sinlp, acc, tr, tracing);
+}
+
+// Star re-emission of a packet that reached the stellar surface.
+__device__ void star_reemit(const Params &P, Pkt &k, double *alb, Mrg &rng, Acc &acc,
+                            Trace &tr, bool tracing) {
+  acc.v[S_AFLUX] += 1.0;
+  const double cosb = clamp1(k.z / k.rtot);
+  const double sinb = sqrt(1.0 - cosb * cosb);
+  double lp = atan2(k.y, k.x);
+  if (lp < 0.0) lp = lp + kR2P;
+  k.s = Stokes4{1.0, 0.0, 0.0, 0.0};
+  initp_sample(k, rng);
+  initp_transform(k, cosb, sinb, lp);
+  k.nub = bbfreq(rng, P.tstar);
+  k.i_orig = 1;
+  k.iflag = 0;
+  set_opacity(P, k, alb, true);
+  k.ux = k.d.sint * k.d.cosp;
+  k.uy = k.d.sint * k.d.sinp;
+  k.uz = k.d.cost;
+  k.ir = 1;
+  k.on_wall = true;
+  if (P.ipeel) peel_direct(P, k, cosb, sinb, cos(lp), sin(lp), acc, tr, tracing);
+  if (tracing) tr.nstar++;
+}
+
+// select_dust.f90 over the active species (zero-density slots add nothing).
+__device__ __forceinline__ int select_dust(const Params &P, const Pkt &k, size_t cell, Mrg &rng) {
+  const double xran = ran(rng);
+  const double *dn = P.dens + cell * (size_t)P.nact;
+  double opac = 0.0, frac = 0.0;
+#pragma unroll
+  for (int s = 0; s < kMaxSpecies; ++s)
+    if (s < P.nact) opac = opac + k.kapd[s] * dn[s];
+  if (opac == 0.0) return -1;
+  const double slack = (double)1.e-10f;
+#pragma unroll
+  for (int s = 0; s < kMaxSpecies; ++s) {
+    if (s < P.nact) {
+      frac = frac + k.kapd[s] * dn[s] / opac;
+      if (xran <= frac + slack) return s;
+    }
+  }
+  return -1;
+}
+
+__device__ void bin_exit(const Params &P, Pkt &k, double xpold, double ypold, double zpold,
+                         int iscat, Acc &acc) {
+  int it = (int)((double)P.nmu * (1.0 - k.d.cost) / 2.0) + 1;
+  it = clampi(it, 1, P.nmu);
+  if (k.d.phi < 0.0) k.d.phi = k.d.phi + kR2P;
+  const int ip = clampi((int)((double)P.nph * k.d.phi / kR2P) + 1, 1, P.nph);
+  const double ut = P.u[it - 1];
+  const double x = ypold * k.d.cosp - xpold * k.d.sinp;
+  const double y = zpold * P.sini[it - 1] - ypold * ut * k.d.sinp - xpold * ut * k.d.cosp;
+  const double rho2 = (x * x + y * y);
+  if (rho2 < P.aperture2[P.nap - 1] * 1.0001) {
+    acc.v[S_FLUX] += 1.0;
+    if (k.iflag == 1) {
+      acc.v[S_SFLUX] += 1.0;
+      acc.v[S_NSCAT] += (double)iscat;
+    }
+  }
+  const int inub =
+      clampi((int)((double)P.nfreq * log(k.nub / P.numin) / P.lnurat) + 1, 1, P.nfreq);
+  const int i_inter = (k.iflag == 1) ? 2 : ((k.i_orig == 1) ? 1 : 3);
+  const int indx[3] = {1, 1 + k.i_orig, 5 + i_inter};
+  const bool counted = !(k.iflag == 0 && k.i_orig == 4);
+  if (P.sed && counted) {
+    const size_t n = (size_t)P.nfreq * P.nmu * P.nph * P.nap * kNo;
+    for (int ia = 0; ia < P.nap; ++ia) {
+      if (rho2 < P.aperture2[ia]) {
+        for (int j = 0; j < 3; ++j) {
+          const size_t q =
+              (size_t)(inub - 1) +
+              (size_t)P.nfreq *
+                  ((size_t)(it - 1) +
+                   (size_t)P.nmu * ((size_t)(ip - 1) +
+                                    (size_t)P.nph * ((size_t)ia + (size_t)P.nap *
+                                                                      (size_t)(indx[j] - 1))));
+          atomic_add(&P.sed[q], k.s.I);
+          atomic_add(&P.sed[q + n], k.s.Q);
+          atomic_add(&P.sed[q + 2 * n], k.s.U);
+          atomic_add(&P.sed[q + 3 * n], k.s.V);
+          atomic_add(&P.sed[q + 4 * n], k.s.I * k.s.I);
+          atomic_add(&P.sed[q + 5 * n], k.s.Q * k.s.Q);
+          atomic_add(&P.sed[q + 6 * n], k.s.U * k.s.U);
+          atomic_add(&P.sed[q + 7 * n], k.s.V * k.s.V);
+          atomic_add(&P.sed[q + 8 * n], 1.0);
+        }
+      }
+    }
+  }
+  if (counted) acc.v[S_SUMSUB] += k.s.I;
+  acc.v[S_SUMALL] += k.s.I;
+  if (k.iflag == 1) acc.v[S_SCOUNT] += 1.0;
+}
+
+// propagate.f90:1-325 for one packet; returns 0 escaped, 1 killed, 2 error.
+__device__ int propagate(const Params &P, Pkt &k, double *alb, Mrg &rng, Acc &acc, Trace &tr,
+                         bool tracing) {
+  double xpold = k.x, ypold = k.y, zpold = k.z;
+  int iscat = 0, iint = 0, pir = 0, pit = 0, pip = 0;
+  bool first = true;
+  int st = tauint(P, k, alb, -log(ran(rng)), acc, tr, tracing);
+  for (;;) {
+    while (st == ST_STAR) {
+      star_reemit(P, k, alb, rng, acc, tr, tracing);
+      const double tau = -log(ran(rng));
+      xpold = k.x;
+      ypold = k.y;
+      zpold = k.z;
+      st = tauint(P, k, alb, tau, acc, tr, tracing);
+    }
+    if (st == ST_ERROR) return 2;
+    if (st == ST_EXIT) break;
+    const size_t cell = cell_of(P, k.ir, k.it, k.ip);
+    if (!first) {
+      if (k.ir != pir || k.it != pit || k.ip != pip) iint = 0;
+      if (iint > P.iintmax) {
+        if (P.killed) atomicAdd(&P.killed[cell], 1);
+        acc.v[S_ABFLUX] += 1.0;
+        return 1;
+      }
+    }
+    const int slot = select_dust(P, k, cell, rng);
+    if (slot < 0) {
+      acc.v[S_ERR_DUST] += 1.0;
+      return 2;
+    }
+    const int idust2 = P.act[slot] - 1;
+    if (first) acc.v[S_TOT] += 1.0;
+    first = false;
+    const double albedo = alb[slot];
+    if (ran(rng) <= albedo) {
+      k.iflag = 1;
+      ++iscat;
+      ++iint;
+      const HG h = hg_of(P, idust2, k.nub);
+      if (P.ipeel)
+        for (int q = 0; q < P.npeel; ++q) peeloff(P, k, k.s.I, idust2, q, acc, tr, tracing);
+      stokes(k.d, k.s, h, rng, acc.v[S_HTOT], acc.v[S_HIT]);
+      if (tracing) tr.nscat++;
+    } else {
+      if (P.dbig) atomicAdd(&P.dbig[cell], 1);
+      k.i_orig = (idust2 == 0 || idust2 == 1 || idust2 == 4 || idust2 == 5) ? 2 : 3;
+      if (P.nabs) atomic_add(&P.nabs[cell * (size_t)P.nact + slot], 1.0);
+      k.s = Stokes4{1.0, 0.0, 0.0, 0.0};
+      k.d.cost = 1.0 - 2.0 * ran(rng);
+      k.d.sint = sqrt(1.0 - pow(k.d.cost, 2.0));
+      k.d.phi = kR2P * ran(rng);
+      k.d.cosp = cos(k.d.phi);
+      k.d.sinp = sin(k.d.phi);
+      k.nub = lucydustfreq(P, rng, P.tdust[cell * (size_t)P.nact + slot], idust2);
+      k.uy = k.d.sint * k.d.sinp;
+      k.ux = k.d.sint * k.d.cosp;
+      k.uz = k.d.cost;
+      k.iflag = 0;
+      set_opacity(P, k, alb, false);
+      if (P.ipeel)
+        for (int q = 0; q < P.npeel; ++q) peeloff(P, k, k.s.I, idust2, q, acc, tr, tracing);
+      ++iint;
+      if (tracing) tr.nabs++;
+    }
+    const double tau = -log(ran(rng));
+    xpold = k.x;
+    ypold = k.y;
+    zpold = k.z;
+    k.ux = k.d.sint * k.d.cosp;
+    k.uy = k.d.sint * k.d.sinp;
+    k.uz = k.d.cost;
+    pir = k.ir;
+    pit = k.it;
+    pip = k.ip;
+    st = tauint(P, k, alb, tau, acc, tr, tracing);
+  }
+  bin_exit(P, k, xpold, ypold, zpold, iscat, acc);
+  return 0;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void transport_fp64(Params P) {
+  Acc acc;
+#pragma unroll
+  for (int i = 0; i < S_COUNT; ++i) acc.v[i] = 0.0;
+  const bool tracing = P.trace != nullptr;
+  for (;;) {
+    const unsigned long long p = atomicAdd(P.next_packet, 1ull);
+    if (p >= (unsigned long long)P.np) break;
+    Mrg rng = mrg_segment(P.g1, P.g2, P.seg0 + p, P.pow1, P.pow2);
+    Pkt k;
+    double alb[kMaxSpecies];
+    Trace tr{0.0, 0.0, 0, 0, 0, 0, 0, 14695981039346656037ull};
+    const long long iphot = (long long)p + 1;
+    launch_star(P, k, alb, rng, iphot < P.npacc + P.npaccstar, acc, tr, tracing);
+    if (tracing) hash_cell(tr, k.ir, k.it, k.ip);
+    const int status = propagate(P, k, alb, rng, acc, tr, tracing);
+    if (tracing) {
+      hc_trace &o = P.trace[p];
+      o.x = k.x;
+      o.y = k.y;
+      o.z = k.z;
+      o.cost = k.d.cost;
+      o.sint = k.d.sint;
+      o.phi = k.d.phi;
+      o.I = k.s.I;
+      o.Q = k.s.Q;
+      o.U = k.s.U;
+      o.V = k.s.V;
+      o.nub = k.nub;
+      o.path = tr.path;
+      o.tau_peel = tr.tau_peel;
+      o.ncross = tr.ncross;
+      o.npeelcross = tr.npeelcross;
+      o.nscat = tr.nscat;
+      o.nabs = tr.nabs;
+      o.nstar = tr.nstar;
+      o.ntrial = 0;
+      o.cellhash = tr.hash;
+      o.status = status;
+      o.ir = k.ir;
+      o.it = k.it;
+      o.ip = k.ip;
+      o.pad = 0;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < S_COUNT; ++i) {
+    const double v = wave_sum(acc.v[i]);
+    if ((threadIdx.x & 63) == 0 && v != 0.0) atomic_add(&P.scal[i], v);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Lucy temperature (tfinal reconstruction with dusttemp.f90:6-58).
+// ---------------------------------------------------------------------------
+struct TParams {
+  int nrg, ntg, npg, nact, nT;
+  int act[kMaxSpecies];
+  double kappav[kMaxSpecies];
+  const double *lTint, *kappap, *dtauabs, *mass4, *tdust;
+  double *tdust2;
+  double ltot, nphot;
+  unsigned long long *nchanged;
+};
+
+__device__ __forceinline__ double kapp(const TParams &T, double t, int id) {  // dust.f90:114-142
+  const double *kp = T.kappap + (size_t)T.nT * id;
+  const double lt = log(t);
+  const int k = (int)((double)(T.nT - 1) * (lt - T.lTint[0]) / (T.lTint[T.nT - 1] - T.lTint[0])) + 1;
+  if (k < 1 || k > T.nT - 1) return k < 1 ? kp[0] : kp[T.nT - 1];
+  const double f = (lt - T.lTint[k - 1]) / (T.lTint[k] - T.lTint[k - 1]);
+  return (1.0 - f) * kp[k - 1] + f * kp[k];
+}
+
+__global__ __launch_bounds__(256) void lucy_temperature(TParams T) {
+  const size_t ncellw = (size_t)T.nrg * T.ntg * T.npg;
+  const size_t n = ncellw * (size_t)T.nact;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long changed = 0;
+  for (; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t cell = i / (size_t)T.nact;
+    const int s = (int)(i - cell * (size_t)T.nact);
+    const int id = T.act[s] - 1;
+    const int ir = (int)(cell % (size_t)T.nrg);
+    const int it = (int)((cell / (size_t)T.nrg) % (size_t)T.ntg);
+    const int ip = (int)(cell / ((size_t)T.nrg * T.ntg));
+    const double m4 = T.mass4[i];
+    const double told = T.tdust[i];
+    double tnew = 0.1 / 11605.0;
+    if (ir < T.nrg - 1 && it < T.ntg - 1 && ip < T.npg - 1 && m4 > 0.0) {
+      const double na = T.dtauabs[i];
+      double Td = (told != 0.0) ? told : 0.1 / 11605.0;
+      double Tprev = 0.5 * Td;
+      int iter = 0;
+      while (fabs(Td - Tprev) > 1.e-5 * Td && iter < 1000) {
+        Tprev = Td;
+        Td = pow((na * T.ltot) / (4.0 * kSigT * T.nphot * T.kappav[id] * kapp(T, Td, id)), 0.25) /
+             11605.0;
+        Td = Td / m4;
+        if (Td > 1600.0 / 11605.0) Td = 1600.0 / 11605.0;
+        Td = fmax(Td, 0.1 / 11605.0);
+        ++iter;
+      }
+      tnew = Td;
+    }
+    if (fabs(tnew - told) > 0.01 * told) ++changed;
+    T.tdust2[i] = tnew;
+  }
+  for (int off = 32; off > 0; off >>= 1) changed += __shfl_xor(changed, off, 64);
+  if ((threadIdx.x & 63) == 0 && changed) atomicAdd(T.nchanged, changed);
+}
+
+// Masked density: zero where the species is at or above Tsub.
+__global__ void mask_density(const double *dens, const double *tdust, double tsub_ev,
+                             double *densm, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i < n; i += (size_t)gridDim.x * blockDim.x) densm[i] = tdust[i] < tsub_ev ? dens[i] : 0.0;
+}
+
+}  // namespace hc
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using namespace hc;
+
+struct hc_ctx {
+  hc_dims dims;
+  int device = 0;
+  std::string err;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool have_grid = false, have_dust = false, have_src = false;
+  int nact = 0;
+  int act[kMaxSpecies] = {0};
+  int slot[kMaxSpecies];
+  size_t ncellw = 0;
+  double tsub = 1600.0;
+  // device buffers
+  double *r2arr = nullptr, *tan2arr = nullptr, *thetarr = nullptr, *aarr = nullptr,
+         *barr = nullptr, *phiarr = nullptr;
+  double *dens = nullptr, *densm = nullptr, *tdust = nullptr;
+  double *nudust = nullptr, *kappad = nullptr, *adust = nullptr, *gdust = nullptr,
+         *pldust = nullptr, *lTint = nullptr, *nuint = nullptr, *kapBint = nullptr,
+         *kappap = nullptr;
+  double *u = nullptr, *sini = nullptr, *aperture2 = nullptr, *coste = nullptr,
+         *sinte = nullptr, *cospe = nullptr, *sinpe = nullptr;
+  int32_t *pow1 = nullptr, *pow2 = nullptr;
+  double *dtauabs = nullptr, *nabs = nullptr, *jmean = nullptr;
+  int32_t *killed = nullptr, *dbig = nullptr;
+  double *sed = nullptr, *peel = nullptr, *img_i = nullptr, *img_q = nullptr, *img_u = nullptr,
+         *img_v = nullptr, *si_init = nullptr, *scal = nullptr, *mass4 = nullptr;
+  unsigned long long *counter = nullptr;
+  hc_trace *trace = nullptr;
+  size_t trace_cap = 0;
+  // host copies of scalars
+  int32_t nlambda[kMaxSpecies] = {0};
+  double kappav[kMaxSpecies] = {0};
+  hc_sources src;
+};
+
+namespace {
+
+int fail(hc_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HCK(expr)                                                                    \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail(ctx, HC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+int dalloc(hc_ctx *ctx, T **p, size_t n) {
+  if (*p) return HC_OK;
+  HCK(hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T)));
+  return HC_OK;
+}
+
+template <class T>
+int upload(hc_ctx *ctx, T **p, const T *h, size_t n) {
+  int rc = dalloc(ctx, p, n);
+  if (rc) return rc;
+  HCK(hipMemcpy(*p, h, n * sizeof(T), hipMemcpyHostToDevice));
+  return HC_OK;
+}
+
+int64_t powmod_host(int64_t a, uint64_t e, int64_t m) {
+  int64_t r = 1, b = a % m;
+  while (e) {
+    if (e & 1u) r = (r * b) % m;
+    b = (b * b) % m;
+    e >>= 1;
+  }
+  return r;
+}
+
+// Cell-major, active-species-inner copy of a (nrg,ntg,npg,ndg) array.
+void to_device_layout(const hc_ctx *ctx, const double *ref, std::vector<double> &out) {
+  out.assign(ctx->ncellw * (size_t)ctx->nact, 0.0);
+  for (int s = 0; s < ctx->nact; ++s) {
+    const double *src = ref + ctx->ncellw * (size_t)(ctx->act[s] - 1);
+    for (size_t c = 0; c < ctx->ncellw; ++c) out[c * (size_t)ctx->nact + s] = src[c];
+  }
+}
+
+int refresh_mask(hc_ctx *ctx) {
+  const size_t n = ctx->ncellw * (size_t)ctx->nact;
+  hipLaunchKernelGGL(mask_density, dim3(1024), dim3(256), 0, ctx->stream, ctx->dens, ctx->tdust,
+                     ctx->tsub / 11605.0, ctx->densm, n);
+  HCK(hipGetLastError());
+  HCK(hipStreamSynchronize(ctx->stream));
+  return HC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hc_abi_version(void) { return HC_ABI_VERSION; }
+
+const char *hc_last_error(const hc_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int hc_create(const hc_dims *dims, int device, hc_ctx **out) {
+  if (!dims || !out) return HC_E_ARG;
+  *out = nullptr;
+  if (dims->nrg < 2 || dims->ntg < 2 || dims->npg < 2 || dims->ndg < 1 ||
+      dims->ndg > kMaxSpecies || dims->nfreq < 1 || dims->nmu < 1 || dims->nph < 1 ||
+      dims->nap < 1 || dims->npeel < 0 || dims->nxhst < 1 || dims->nlammax < 2 || dims->nT < 2 ||
+      dims->nnu < 2)
+    return HC_E_ARG;
+  if (dims->precision != HC_FP64) return HC_E_UNSUPPORTED;
+  hc_ctx *ctx = new hc_ctx();
+  ctx->dims = *dims;
+  ctx->device = device;
+  ctx->ncellw = (size_t)dims->nrg * dims->ntg * dims->npg;
+  for (int i = 0; i < kMaxSpecies; ++i) ctx->slot[i] = -1;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreate(&ctx->stream);
+  if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
+  if (e != hipSuccess) {
+    delete ctx;
+    return HC_E_HIP;
+  }
+  // RNG jump tables: (a^(2^30))^(2^k) mod m for k = 0..63
+  std::vector<int32_t> p1(64), p2(64);
+  int64_t b1 = 1033780774, b2 = 1494757890;  // a1w, a2w (ecuyer_cote_mod.f90:14-15)
+  for (int k = 0; k < 64; ++k) {
+    p1[k] = (int32_t)b1;
+    p2[k] = (int32_t)b2;
+    b1 = (b1 * b1) % 2147483563;
+    b2 = (b2 * b2) % 2147483399;
+  }
+  int rc = upload(ctx, &ctx->pow1, p1.data(), 64);
+  if (!rc) rc = upload(ctx, &ctx->pow2, p2.data(), 64);
+  if (!rc) rc = dalloc(ctx, &ctx->scal, (size_t)S_COUNT);
+  if (!rc) rc = dalloc(ctx, &ctx->counter, 1);
+  double z[146];
+  z[0] = 0.0;
+  for (int i = 1; i <= 145; ++i) z[i] = 0.0;
+  for (int j = 145; j >= 1; --j) {
+    const double t = 1.0 / (double)(j * j * j * j);
+    for (int i = 145; i >= j; --i) z[i] = z[i] + t;
+  }
+  if (!rc && hipMemcpyToSymbol(HIP_SYMBOL(c_inczeta), z, sizeof z) != hipSuccess) rc = HC_E_HIP;
+  if (rc) {
+    hc_destroy(ctx);
+    return rc;
+  }
+  *out = ctx;
+  return HC_OK;
+}
+
+int hc_set_grid(hc_ctx *ctx, const double *r2arr, const double *tan2arr, const double *thetarr,
+                const double *aarr, const double *barr, const double *phiarr,
+                const double *densarr, const double *tdust, double tsub) {
+  if (!ctx) return HC_E_ARG;
+  if (!r2arr || !tan2arr || !thetarr || !aarr || !barr || !phiarr || !densarr || !tdust)
+    return fail(ctx, HC_E_ARG, "hc_set_grid: null array");
+  const hc_dims &d = ctx->dims;
+  HCK(hipSetDevice(ctx->device));
+  // active species: any non-zero density (fixed at the first call)
+  if (!ctx->have_grid) {
+    ctx->nact = 0;
+    for (int id = 0; id < d.ndg; ++id) {
+      const double *p = densarr + ctx->ncellw * (size_t)id;
+      bool any = false;
+      for (size_t c = 0; c < ctx->ncellw && !any; ++c) any = p[c] != 0.0;
+      if (any) {
+        ctx->slot[id] = ctx->nact;
+        ctx->act[ctx->nact++] = id + 1;
+      }
+    }
+    if (ctx->nact == 0) return fail(ctx, HC_E_ARG, "hc_set_grid: all densities are zero");
+  }
+  int rc = upload(ctx, &ctx->r2arr, r2arr, (size_t)d.nrg);
+  if (!rc) rc = upload(ctx, &ctx->tan2arr, tan2arr, (size_t)d.ntg);
+  if (!rc) rc = upload(ctx, &ctx->thetarr, thetarr, (size_t)d.ntg);
+  if (!rc) rc = upload(ctx, &ctx->aarr, aarr, (size_t)d.npg);
+  if (!rc) rc = upload(ctx, &ctx->barr, barr, (size_t)d.npg);
+  if (!rc) rc = upload(ctx, &ctx->phiarr, phiarr, (size_t)d.npg);
+  if (rc) return rc;
+  std::vector<double> tmp;
+  to_device_layout(ctx, densarr, tmp);
+  rc = upload(ctx, &ctx->dens, tmp.data(), tmp.size());
+  if (!rc) rc = dalloc(ctx, &ctx->densm, tmp.size());
+  to_device_layout(ctx, tdust, tmp);
+  if (!rc) rc = upload(ctx, &ctx->tdust, tmp.data(), tmp.size());
+  if (rc) return rc;
+  ctx->tsub = tsub;
+  ctx->have_grid = true;
+  return refresh_mask(ctx);
+}
+
+int hc_set_temperature(hc_ctx *ctx, const double *tdust) {
+  if (!ctx || !tdust) return HC_E_ARG;
+  if (!ctx->have_grid) return fail(ctx, HC_E_STATE, "hc_set_temperature before hc_set_grid");
+  HCK(hipSetDevice(ctx->device));
+  std::vector<double> tmp;
+  to_device_layout(ctx, tdust, tmp);
+  HCK(hipMemcpy(ctx->tdust, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice));
+  return refresh_mask(ctx);
+}
+
+int hc_set_dust(hc_ctx *ctx, const int32_t *nlambda, const double *nudust, const double *kappad,
+                const double *adust, const double *gdust, const double *pldust,
+                const double *kappav, const double *lTint, const double *nuint,
+                const double *kapBint, const double *kappap) {
+  if (!ctx) return HC_E_ARG;
+  if (!nlambda || !nudust || !kappad || !adust || !gdust || !pldust || !kappav || !lTint ||
+      !nuint || !kapBint || !kappap)
+    return fail(ctx, HC_E_ARG, "hc_set_dust: null array");
+  const hc_dims &d = ctx->dims;
+  HCK(hipSetDevice(ctx->device));
+  for (int id = 0; id < d.ndg; ++id) {
+    if (nlambda[id] < 1 || nlambda[id] > d.nlammax)
+      return fail(ctx, HC_E_ARG, "hc_set_dust: nlambda out of range");
+    ctx->nlambda[id] = nlambda[id];
+    ctx->kappav[id] = kappav[id];
+  }
+  const size_t nl = (size_t)d.nlammax * d.ndg;
+  int rc = upload(ctx, &ctx->nudust, nudust, nl);
+  if (!rc) rc = upload(ctx, &ctx->kappad, kappad, nl);
+  if (!rc) rc = upload(ctx, &ctx->adust, adust, nl);
+  if (!rc) rc = upload(ctx, &ctx->gdust, gdust, nl);
+  if (!rc) rc = upload(ctx, &ctx->pldust, pldust, nl);
+  if (!rc) rc = upload(ctx, &ctx->lTint, lTint, (size_t)d.nT);
+  if (!rc) rc = upload(ctx, &ctx->nuint, nuint, (size_t)d.nnu);
+  if (!rc) rc = upload(ctx, &ctx->kapBint, kapBint, (size_t)d.nT * d.nnu * d.ndg);
+  if (!rc) rc = upload(ctx, &ctx->kappap, kappap, (size_t)d.nT * d.ndg);
+  if (rc) return rc;
+  ctx->have_dust = true;
+  return HC_OK;
+}
+
+int hc_set_sources(hc_ctx *ctx, const hc_sources *src) {
+  if (!ctx || !src) return HC_E_ARG;
+  const hc_dims &d = ctx->dims;
+  if (!src->u || !src->aperture2 || (d.npeel > 0 && (!src->coste || !src->sinte || !src->cospe ||
+                                                     !src->sinpe)))
+    return fail(ctx, HC_E_ARG, "hc_set_sources: null array");
+  if (src->limb != 0) return fail(ctx, HC_E_UNSUPPORTED, "limb-darkened launch not supported");
+  if (src->accfrac != 0.0)
+    return fail(ctx, HC_E_UNSUPPORTED, "disk-accretion launch not supported");
+  HCK(hipSetDevice(ctx->device));
+  std::vector<double> sini(d.nmu);
+  for (int i = 0; i < d.nmu; ++i) sini[i] = std::sqrt(1.0 - src->u[i] * src->u[i]);
+  int rc = upload(ctx, &ctx->u, src->u, (size_t)d.nmu);
+  if (!rc) rc = upload(ctx, &ctx->sini, sini.data(), (size_t)d.nmu);
+  if (!rc) rc = upload(ctx, &ctx->aperture2, src->aperture2, (size_t)d.nap);
+  if (d.npeel > 0) {
+    if (!rc) rc = upload(ctx, &ctx->coste, src->coste, (size_t)d.npeel);
+    if (!rc) rc = upload(ctx, &ctx->sinte, src->sinte, (size_t)d.npeel);
+    if (!rc) rc = upload(ctx, &ctx->cospe, src->cospe, (size_t)d.npeel);
+    if (!rc) rc = upload(ctx, &ctx->sinpe, src->sinpe, (size_t)d.npeel);
+  }
+  if (rc) return rc;
+  ctx->src = *src;
+  ctx->have_src = true;
+  return HC_OK;
+}
+
+int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int32_t seed2,
+                     int32_t gen, uint64_t seg0, hc_tallies *out, hc_trace *trace) {
+  if (!ctx || !out) return HC_E_ARG;
+  if (!ctx->have_grid || !ctx->have_dust || !ctx->have_src)
+    return fail(ctx, HC_E_STATE, "hc_run_iteration: grid, dust and sources must be set");
+  if (np < 0 || gen < 1 || gen > 32 || seed1 <= 0 || seed1 >= 2147483563 || seed2 <= 0 ||
+      seed2 >= 2147483399)
+    return fail(ctx, HC_E_ARG, "hc_run_iteration: bad np or seeds");
+  const hc_dims &d = ctx->dims;
+  HCK(hipSetDevice(ctx->device));
+  const size_t nc = ctx->ncellw * (size_t)ctx->nact;
+  const size_t nsed = (size_t)9 * d.nfreq * d.nmu * d.nph * d.nap * kNo;
+  const size_t npeelsed = (size_t)9 * d.nfreq * std::max(d.npeel, 1) * d.nap * kNo;
+  const size_t nimg = (size_t)std::max(d.npeel, 1) * d.nxhst * d.nxhst * d.nfreq;
+  int rc = 0;
+  rc = rc ? rc : dalloc(ctx, &ctx->dtauabs, nc);
+  rc = rc ? rc : dalloc(ctx, &ctx->nabs, nc);
+  rc = rc ? rc : dalloc(ctx, &ctx->jmean, ctx->ncellw);
+  rc = rc ? rc : dalloc(ctx, &ctx->killed, ctx->ncellw);
+  rc = rc ? rc : dalloc(ctx, &ctx->dbig, ctx->ncellw);
+  rc = rc ? rc : dalloc(ctx, &ctx->sed, nsed);
+  rc = rc ? rc : dalloc(ctx, &ctx->peel, npeelsed);
+  rc = rc ? rc : dalloc(ctx, &ctx->si_init, (size_t)d.nfreq * 5);
+  if (d.imcube && ipeel) {
+    rc = rc ? rc : dalloc(ctx, &ctx->img_i, nimg * kNo);
+    rc = rc ? rc : dalloc(ctx, &ctx->img_q, nimg);
+    rc = rc ? rc : dalloc(ctx, &ctx->img_u, nimg);
+    rc = rc ? rc : dalloc(ctx, &ctx->img_v, nimg);
+  }
+  if (rc) return rc;
+  if (trace && (size_t)np > ctx->trace_cap) {
+    if (ctx->trace) HCK(hipFree(ctx->trace));
+    ctx->trace = nullptr;
+    rc = dalloc(ctx, &ctx->trace, (size_t)np);
+    if (rc) return rc;
+    ctx->trace_cap = (size_t)np;
+  }
+  hipStream_t s = ctx->stream;
+  HCK(hipMemsetAsync(ctx->dtauabs, 0, nc * sizeof(double), s));
+  HCK(hipMemsetAsync(ctx->nabs, 0, nc * sizeof(double), s));
+  HCK(hipMemsetAsync(ctx->jmean, 0, ctx->ncellw * sizeof(double), s));
+  HCK(hipMemsetAsync(ctx->killed, 0, ctx->ncellw * sizeof(int32_t), s));
+  HCK(hipMemsetAsync(ctx->dbig, 0, ctx->ncellw * sizeof(int32_t), s));
+  HCK(hipMemsetAsync(ctx->sed, 0, nsed * sizeof(double), s));
+  HCK(hipMemsetAsync(ctx->peel, 0, npeelsed * sizeof(double), s));
+  HCK(hipMemsetAsync(ctx->si_init, 0, (size_t)d.nfreq * 5 * sizeof(double), s));
+  HCK(hipMemsetAsync(ctx->scal, 0, S_COUNT * sizeof(double), s));
+  HCK(hipMemsetAsync(ctx->counter, 0, sizeof(unsigned long long), s));
+  const bool cube = d.imcube && ipeel && ctx->img_i;
+  if (cube) {
+    HCK(hipMemsetAsync(ctx->img_i, 0, nimg * kNo * sizeof(double), s));
+    HCK(hipMemsetAsync(ctx->img_q, 0, nimg * sizeof(double), s));
+    HCK(hipMemsetAsync(ctx->img_u, 0, nimg * sizeof(double), s));
+    HCK(hipMemsetAsync(ctx->img_v, 0, nimg * sizeof(double), s));
+  }
+
+  Params P;
+  std::memset(&P, 0, sizeof P);
+  P.nrg = d.nrg;
+  P.ntg = d.ntg;
+  P.npg = d.npg;
+  P.nact = ctx->nact;
+  for (int i = 0; i < kMaxSpecies; ++i) {
+    P.act[i] = ctx->act[i];
+    P.slot[i] = ctx->slot[i];
+    P.nlambda[i] = ctx->nlambda[i];
+    P.kappav[i] = ctx->kappav[i];
+  }
+  P.r2arr = ctx->r2arr;
+  P.tan2arr = ctx->tan2arr;
+  P.thetarr = ctx->thetarr;
+  P.aarr = ctx->aarr;
+  P.barr = ctx->barr;
+  P.phiarr = ctx->phiarr;
+  P.dens = ctx->dens;
+  P.densm = ctx->densm;
+  P.tdust = ctx->tdust;
+  P.nlammax = d.nlammax;
+  P.nT = d.nT;
+  P.nnu = d.nnu;
+  P.nudust = ctx->nudust;
+  P.kappad = ctx->kappad;
+  P.adust = ctx->adust;
+  P.gdust = ctx->gdust;
+  P.pldust = ctx->pldust;
+  P.lTint = ctx->lTint;
+  P.nuint = ctx->nuint;
+  P.kapBint = ctx->kapBint;
+  const hc_sources &S = ctx->src;
+  P.rstar = S.rstar;
+  P.rmincgs = S.rstar * kRsol;
+  P.tstar = S.tstar;
+  P.tshock = S.tshock;
+  P.ips = S.ips;
+  P.ispot = S.ispot;
+  P.nspot = S.nspot;
+  P.ipeel = ipeel && d.npeel > 0;
+  P.fspot = S.fspot;
+  P.csprad = S.csprad;
+  P.cspot1 = S.cspot1;
+  P.sspot1 = S.sspot1;
+  P.scspot1 = S.scspot1;
+  P.ssspot1 = S.ssspot1;
+  P.cspot2 = S.cspot2;
+  P.sspot2 = S.sspot2;
+  P.scspot2 = S.scspot2;
+  P.ssspot2 = S.ssspot2;
+  // source split (newdisktherm.F90:143-176; no external illumination)
+  long long ns = (long long)((double)np * (1.0 - S.accfrac));
+  const double rn = (double)(float)np * (1.0 - S.accfrac);
+  if (rn - (double)ns > 0.5) ns = ns + 1;
+  P.npacc = (long long)((double)np * S.accfrac);
+  P.npaccstar = (long long)(S.accfrac2 * (double)np);
+  P.np = np;
+  P.nfreq = d.nfreq;
+  P.nmu = d.nmu;
+  P.nph = d.nph;
+  P.nap = d.nap;
+  P.npeel = d.npeel;
+  P.nxhst = d.nxhst;
+  P.imcube = cube ? 1 : 0;
+  P.iintmax = S.iintmax > 0 ? S.iintmax : 10000;
+  P.numin = S.numin;
+  P.lnurat = S.lnurat;
+  P.rmaxi = S.rmaxi;
+  P.fractxh = 2.0 * S.rmaxi / (double)d.nxhst;
+  P.u = ctx->u;
+  P.sini = ctx->sini;
+  P.aperture2 = ctx->aperture2;
+  P.coste = ctx->coste;
+  P.sinte = ctx->sinte;
+  P.cospe = ctx->cospe;
+  P.sinpe = ctx->sinpe;
+  // generator gen's initial seeds (set_all_seeds, ecuyer_cote_mod.f90:345-362)
+  P.g1 = (int32_t)((powmod_host(2082007225, (uint64_t)(gen - 1), 2147483563) * seed1) % 2147483563);
+  P.g2 = (int32_t)((powmod_host(784306273, (uint64_t)(gen - 1), 2147483399) * seed2) % 2147483399);
+  P.seg0 = seg0;
+  P.pow1 = ctx->pow1;
+  P.pow2 = ctx->pow2;
+  P.dtauabs = out->dtauabs ? ctx->dtauabs : nullptr;
+  P.nabs = ctx->nabs;
+  P.jmean = out->jmean ? ctx->jmean : nullptr;
+  P.killed = ctx->killed;
+  P.dbig = ctx->dbig;
+  P.sed = ctx->sed;
+  P.peel = P.ipeel ? ctx->peel : nullptr;
+  P.img_i = cube ? ctx->img_i : nullptr;
+  P.img_q = ctx->img_q;
+  P.img_u = ctx->img_u;
+  P.img_v = ctx->img_v;
+  P.si_init = ctx->si_init;
+  P.scal = ctx->scal;
+  P.next_packet = ctx->counter;
+  P.trace = trace ? ctx->trace : nullptr;
+
+  int dev_cus = 0, bpm = 0;
+  HCK(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  HCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpm, transport_fp64, 256, 0));
+  long long blocks = (long long)std::max(dev_cus, 1) * std::max(bpm, 1);
+  blocks = std::min<long long>(blocks, std::max<long long>((np + 255) / 256, 1));
+  HCK(hipEventRecord(ctx->ev0, s));
+  if (np > 0) hipLaunchKernelGGL(transport_fp64, dim3((unsigned)blocks), dim3(256), 0, s, P);
+  HCK(hipGetLastError());
+  HCK(hipEventRecord(ctx->ev1, s));
+  HCK(hipStreamSynchronize(s));
+  float ms = 0.f;
+  HCK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  out->kernel_ms = ms;
+
+  // download
+  double scal[S_COUNT];
+  HCK(hipMemcpy(scal, ctx->scal, sizeof scal, hipMemcpyDeviceToHost));
+  out->tot = scal[S_TOT];
+  out->flux = scal[S_FLUX];
+  out->sflux = scal[S_SFLUX];
+  out->nscat = scal[S_NSCAT];
+  out->scount = scal[S_SCOUNT];
+  out->abflux = scal[S_ABFLUX];
+  out->aflux = scal[S_AFLUX];
+  out->sumsub = scal[S_SUMSUB];
+  out->sumall = scal[S_SUMALL];
+  out->hit = scal[S_HIT];
+  out->htot = scal[S_HTOT];
+  out->n_err_nowall = (int64_t)scal[S_ERR_WALL];
+  out->n_err_dust = (int64_t)scal[S_ERR_DUST];
+  out->n_warn = (int64_t)scal[S_WARN];
+  std::vector<double> tmp;
+  auto per_species = [&](double *dev, double *host) -> int {
+    if (!host) return HC_OK;
+    tmp.resize(nc);
+    HCK(hipMemcpy(tmp.data(), dev, nc * sizeof(double), hipMemcpyDeviceToHost));
+    std::memset(host, 0, ctx->ncellw * (size_t)d.ndg * sizeof(double));
+    for (int k = 0; k < ctx->nact; ++k) {
+      double *dst = host + ctx->ncellw * (size_t)(ctx->act[k] - 1);
+      for (size_t c = 0; c < ctx->ncellw; ++c) dst[c] = tmp[c * (size_t)ctx->nact + k];
+    }
+    return HC_OK;
+  };
+  if ((rc = per_species(ctx->dtauabs, out->dtauabs))) return rc;
+  if ((rc = per_species(ctx->nabs, out->nabs))) return rc;
+  if (out->jmean)
+    HCK(hipMemcpy(out->jmean, ctx->jmean, ctx->ncellw * sizeof(double), hipMemcpyDeviceToHost));
+  if (out->killed)
+    HCK(hipMemcpy(out->killed, ctx->killed, ctx->ncellw * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (out->dbig)
+    HCK(hipMemcpy(out->dbig, ctx->dbig, ctx->ncellw * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (out->dsg) std::memset(out->dsg, 0, ctx->ncellw * sizeof(int32_t));
+  if (out->sed) HCK(hipMemcpy(out->sed, ctx->sed, nsed * sizeof(double), hipMemcpyDeviceToHost));
+  if (out->peel && d.npeel > 0)
+    HCK(hipMemcpy(out->peel, ctx->peel, npeelsed * sizeof(double), hipMemcpyDeviceToHost));
+  if (out->si_init)
+    HCK(hipMemcpy(out->si_init, ctx->si_init, (size_t)d.nfreq * 5 * sizeof(double),
+                  hipMemcpyDeviceToHost));
+  if (cube) {
+    if (out->img_i)
+      HCK(hipMemcpy(out->img_i, ctx->img_i, nimg * kNo * sizeof(double), hipMemcpyDeviceToHost));
+    if (out->img_q)
+      HCK(hipMemcpy(out->img_q, ctx->img_q, nimg * sizeof(double), hipMemcpyDeviceToHost));
+    if (out->img_u)
+      HCK(hipMemcpy(out->img_u, ctx->img_u, nimg * sizeof(double), hipMemcpyDeviceToHost));
+    if (out->img_v)
+      HCK(hipMemcpy(out->img_v, ctx->img_v, nimg * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  if (trace && np > 0)
+    HCK(hipMemcpy(trace, ctx->trace, (size_t)np * sizeof(hc_trace), hipMemcpyDeviceToHost));
+  return HC_OK;
+}
+
+int hc_temperature(hc_ctx *ctx, const double *massarr, double ltot, double nphot, double *tdust2,
+                   int64_t *nchanged) {
+  if (!ctx || !massarr || !tdust2) return HC_E_ARG;
+  if (!ctx->have_grid || !ctx->have_dust || !ctx->dtauabs)
+    return fail(ctx, HC_E_STATE, "hc_temperature needs a completed pass");
+  const hc_dims &d = ctx->dims;
+  HCK(hipSetDevice(ctx->device));
+  const size_t nc = ctx->ncellw * (size_t)ctx->nact;
+  std::vector<double> tmp;
+  to_device_layout(ctx, massarr, tmp);
+  int rc = upload(ctx, &ctx->mass4, tmp.data(), nc);
+  if (rc) return rc;
+  double *t2 = nullptr;
+  HCK(hipMalloc((void **)&t2, nc * sizeof(double)));
+  TParams T;
+  std::memset(&T, 0, sizeof T);
+  T.nrg = d.nrg;
+  T.ntg = d.ntg;
+  T.npg = d.npg;
+  T.nact = ctx->nact;
+  T.nT = d.nT;
+  for (int i = 0; i < kMaxSpecies; ++i) {
+    T.act[i] = ctx->act[i];
+    T.kappav[i] = ctx->kappav[i];
+  }
+  T.lTint = ctx->lTint;
+  T.kappap = ctx->kappap;
+  T.dtauabs = ctx->dtauabs;
+  T.mass4 = ctx->mass4;
+  T.tdust = ctx->tdust;
+  T.tdust2 = t2;
+  T.ltot = ltot;
+  T.nphot = nphot;
+  T.nchanged = ctx->counter;
+  HCK(hipMemsetAsync(ctx->counter, 0, sizeof(unsigned long long), ctx->stream));
+  const unsigned blocks = (unsigned)std::min<size_t>((nc + 255) / 256, 4096);
+  hipLaunchKernelGGL(lucy_temperature, dim3(std::max(blocks, 1u)), dim3(256), 0, ctx->stream, T);
+  HCK(hipGetLastError());
+  HCK(hipStreamSynchronize(ctx->stream));
+  tmp.resize(nc);
+  HCK(hipMemcpy(tmp.data(), t2, nc * sizeof(double), hipMemcpyDeviceToHost));
+  unsigned long long nch = 0;
+  HCK(hipMemcpy(&nch, ctx->counter, sizeof nch, hipMemcpyDeviceToHost));
+  HCK(hipFree(t2));
+  for (size_t i = 0; i < ctx->ncellw * (size_t)d.ndg; ++i) tdust2[i] = 0.1 / 11605.0;
+  for (int k = 0; k < ctx->nact; ++k) {
+    double *dst = tdust2 + ctx->ncellw * (size_t)(ctx->act[k] - 1);
+    for (size_t c = 0; c < ctx->ncellw; ++c) dst[c] = tmp[c * (size_t)ctx->nact + k];
+  }
+  if (nchanged) *nchanged = (int64_t)nch;
+  return HC_OK;
+}
+
+int hc_destroy(hc_ctx *ctx) {
+  if (!ctx) return HC_OK;
+  (void)hipSetDevice(ctx->device);
+  void *bufs[] = {ctx->r2arr,  ctx->tan2arr, ctx->thetarr, ctx->aarr,    ctx->barr,   ctx->phiarr,
+                  ctx->dens,   ctx->densm,   ctx->tdust,   ctx->nudust,  ctx->kappad, ctx->adust,
+                  ctx->gdust,  ctx->pldust,  ctx->lTint,   ctx->nuint,   ctx->kapBint, ctx->kappap,
+                  ctx->u,      ctx->sini,    ctx->aperture2, ctx->coste, ctx->sinte,  ctx->cospe,
+                  ctx->sinpe,  ctx->pow1,    ctx->pow2,    ctx->dtauabs, ctx->nabs,   ctx->jmean,
+                  ctx->killed, ctx->dbig,    ctx->sed,     ctx->peel,    ctx->img_i,  ctx->img_q,
+                  ctx->img_u,  ctx->img_v,   ctx->si_init, ctx->scal,    ctx->mass4,  ctx->counter,
+                  ctx->trace};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return HC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,317 @@
+"""Model setup for the transport path: spherical-polar grid, dust densities,
+sources and binning.
+
+This is the host-side step before the photon loop (gridset.f90:411-1098 and
+setup.f90) that produces the boundary arrays of the C ABI, in the reference's
+Fortran layout (wall counts nrg/ntg/npg, column-major):
+
+* radial walls: ``rarr(1) = rmin = 1`` (the stellar surface) then log spacing
+  from the inner dust radius to ``rmax`` (gridset.f90:439-451), with the
+  envelope / disk inner radius pinned to a wall (gridset.f90:457-470);
+* theta walls: power-law (exponent 2) spacing away from the midplane, poles
+  at 0 and pi with ``tan2arr = 0`` and the midplane wall with ``tan2arr = -1``
+  (gridset.f90:529-601).  The reference builder needs an odd wall count; an
+  even count (a cell across the midplane) is built with the same
+  symmetric spacing and no midplane wall;
+* phi walls: uniform, ``aarr = sin(phi)``, ``barr = -cos(phi)``
+  (gridset.f90:641-650);
+* densities at cell centres (gridset.f90:1002-1083): two flared disks
+  (densdiskg.f90), an Ulrich infalling envelope (densenv.f90 with the
+  streamline cubic), and a polynomial bipolar cavity; ``massarr`` is
+  ``(rho V)^(1/4)``.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from .dust import NDG, DustTables, dustinit, synthetic_dust_table
+
+PI = 3.1415926535897932384626433
+R2P = 2.0 * PI
+RSOL = 6.9598e10
+MSOL = 1.989e33
+LSOL = 3.839e33
+SIGT = 5.670400e-5
+GN = 6.67428e-8
+AU2CM = 1.49598e13
+KB = 1.3806503e-16
+MH = 1.67262158e-24
+
+
+@dataclass
+class ModelConfig:
+    """Parameters of a disk + envelope + star model (names follow the .par
+    keys of reapar.f90 where one exists)."""
+    nr: int = 64                   # cells; NRG = nr + 1
+    ntheta: int = 64               # NTG = ntheta + 1
+    nphi: int = 32                 # NPG = nphi + 1
+    tstar: float = 4000.0          # K
+    rstar: float = 2.0             # Rsun
+    massc: float = 0.5             # Msun
+    massd: float = 0.01            # Msun
+    rmind_rsub: float = 7.0        # RMIND in units of the sublimation radius
+    rmaxd_au: float = 300.0
+    h100_au: float = 10.0          # scale height at 100 AU (CZMIN = R100)
+    a_exp: float = 2.25            # density exponent (A)
+    b_exp: float = 1.25            # flaring exponent (B)
+    fmass: tuple = (0.7, 0.3)      # mass fractions of disk species 1, 2
+    zscale_rel: tuple = (0.6, 1.0)  # scale height of species 1, 2 relative to h
+    rate: float = 5.0e-6           # envelope infall rate, Msun/yr
+    rmax_au: float = 5000.0
+    rc_au: float = 300.0           # centrifugal radius
+    cavity_deg: float = 20.0       # THET1 = THET2
+    cavity_exp: float = 1.5        # EX1
+    rho_cavity: float = 1.0e-20    # g/cm^3 at 1000 AU, falls as r^-2
+    rhoamb: float = 0.0
+    tsub: float = 1600.0
+    # binning / imaging
+    nfreq: int = 200
+    nmu: int = 20
+    nph: int = 1
+    nap: int = 1
+    aperture_au: float = 1.0e5
+    npeel: int = 10
+    nximg: int = 101
+    rmaxi_au: float = 1000.0
+    imcube: bool = True
+    thete_deg: Optional[tuple] = None
+    phie_deg: float = 0.0
+    # sources
+    accfrac2: float = 0.0          # hot-spot luminosity fraction
+    tshock: float = 8000.0
+    fractal: Optional[dict] = None
+    dust_table: Optional[np.ndarray] = field(default=None, repr=False)
+
+
+@dataclass
+class Model:
+    cfg: ModelConfig
+    nrg: int
+    ntg: int
+    npg: int
+    rarr: np.ndarray
+    r2arr: np.ndarray
+    thetarr: np.ndarray
+    tan2arr: np.ndarray
+    phiarr: np.ndarray
+    aarr: np.ndarray
+    barr: np.ndarray
+    densarr: np.ndarray            # (nrg,ntg,npg,ndg) F-order, g/cm^3
+    vcell: np.ndarray              # (nrg,ntg,npg) cm^3
+    massarr: np.ndarray            # (rho V)^(1/4)
+    tdust: np.ndarray              # eV
+    dust: DustTables
+    # sources / binning
+    rstar: float
+    tstar_ev: float
+    tshock_ev: float
+    ltot: float
+    numin: float
+    lnurat: float
+    u: np.ndarray
+    aperture2: np.ndarray
+    coste: np.ndarray
+    sinte: np.ndarray
+    cospe: np.ndarray
+    sinpe: np.ndarray
+    rmaxi: float
+    autors: float
+
+    @property
+    def ncells(self) -> int:
+        return (self.nrg - 1) * (self.ntg - 1) * (self.npg - 1)
+
+
+def _locate(xx: np.ndarray, x: float) -> int:
+    """nr.f:131-153 (1-based)."""
+    j = int(np.searchsorted(xx, x, side="left"))
+    if j == 0 and x == xx[0]:
+        j = 1
+    return j
+
+
+def _zerod(xmu: np.ndarray, rx: np.ndarray) -> np.ndarray:
+    """Streamline of the Ulrich flow: root mu0 in [0,1] of
+    mu0^3 + (rx - 1) mu0 - rx mu = 0 (the cubic densenv.f90:41 solves)."""
+    lo = np.zeros_like(xmu)
+    hi = np.ones_like(xmu)
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        f = mid ** 3 + (rx - 1.0) * mid - rx * xmu
+        up = f > 0.0
+        hi = np.where(up, mid, hi)
+        lo = np.where(up, lo, mid)
+    return 0.5 * (lo + hi)
+
+
+def theta_walls(ntg: int) -> np.ndarray:
+    """Power-law theta walls symmetric about the midplane (gridset.f90:529-559)."""
+    ncell = ntg - 1
+    th = np.zeros(ntg)
+    if ntg % 2 == 1:
+        nttmp = ntg // 2
+        m = PI / 2.0 / float(nttmp) ** 2
+        tmp = m * np.arange(1, nttmp + 1, dtype=float) ** 2
+        th[nttmp] = PI / 2
+        th[nttmp + 1:] = tmp + PI / 2
+        for it in range(nttmp):
+            th[it] = PI - th[ntg - 1 - it]
+    else:
+        # even wall count: a midplane-straddling cell of half-width d0
+        nh = ntg // 2          # walls per hemisphere
+        # walls at pi/2 + d0 + m k^2, k = 0..nh-1, with pi/2 + d0 + m (nh-1)^2 = pi
+        d0 = 0.5 * (PI / 2.0) / (nh - 0.5) ** 2 if nh > 1 else PI / 2
+        m = (PI / 2.0 - d0) / max((nh - 1) ** 2, 1)
+        up = PI / 2 + d0 + m * np.arange(nh, dtype=float) ** 2
+        th[nh:] = up
+        th[:nh] = PI - up[::-1]
+    th[0] = 0.0
+    th[-1] = PI
+    assert np.all(np.diff(th) > 0.0), "theta walls not increasing"
+    assert ncell == len(th) - 1
+    return th
+
+
+def build_model(cfg: ModelConfig) -> Model:
+    nrg, ntg, npg = cfg.nr + 1, cfg.ntheta + 1, cfg.nphi + 1
+    rmincgs = cfg.rstar * RSOL
+    autors = AU2CM / RSOL / cfg.rstar
+    tstar_ev = cfg.tstar / 11605.0
+    rminsub = (cfg.tsub / cfg.tstar) ** (-2.10)      # setup.f90:62
+    rmind = rminsub * cfg.rmind_rsub
+    rmine = rmind
+    rmax = cfg.rmax_au * autors
+    rmaxd = cfg.rmaxd_au * autors
+
+    # radial walls (gridset.f90:423-470)
+    rarr = np.zeros(nrg)
+    rarr[0] = 1.0
+    rdduste = min(rmind, rmine)
+    rfac = (rmax / rdduste) ** (1.0 / float(nrg - 2))
+    rarr[1:] = rdduste * rfac ** np.arange(nrg - 1, dtype=float)
+    rarr[-1] = rmax
+    r2arr = rarr ** 2
+
+    thetarr = theta_walls(ntg)
+    tan2arr = np.tan(thetarr) ** 2
+    tan2arr[0] = 0.0
+    tan2arr[-1] = 0.0
+    if ntg % 2 == 1:
+        tan2arr[ntg // 2] = -1.0
+    else:
+        nh = ntg // 2
+        tan2arr[nh] = tan2arr[nh - 1] = np.tan(thetarr[nh]) ** 2
+
+    dp = R2P / float(npg - 1)
+    phiarr = dp * np.arange(npg, dtype=float)
+    aarr = np.sin(phiarr)
+    barr = -np.cos(phiarr)
+
+    # cell centres (gridset.f90:494-499, 609-620, 653-655)
+    rave = 10.0 ** (0.5 * (np.log10(rarr[:-1]) + np.log10(rarr[1:])))
+    thave = np.sqrt(0.5 * (thetarr[:-1] ** 2 + thetarr[1:] ** 2))
+    phave = 0.5 * (phiarr[:-1] + phiarr[1:])
+    R, TH, PH = np.meshgrid(rave, thave, phave, indexing="ij")
+    dr3 = (rarr[1:] ** 3 - rarr[:-1] ** 3)[:, None, None]
+    dcost = (np.cos(thetarr[1:]) - np.cos(thetarr[:-1]))[None, :, None]
+    dph = np.diff(phiarr)[None, None, :]
+    vol = -dr3 * dcost * dph / 3.0 * rmincgs ** 3
+    cost = np.cos(TH)
+    sint = np.sin(TH)
+
+    nc = (nrg - 1, ntg - 1, npg - 1)
+    dens = np.zeros(nc + (NDG,))
+
+    # flared disks (densdiskg.f90 with CZMIN = R100: z1 = h100 / 100AU^b)
+    rp = R * sint
+    zp = R * cost
+    for id_, (fm, zr) in enumerate(zip(cfg.fmass, cfg.zscale_rel)):
+        if fm <= 0.0:
+            continue
+        z1 = zr * cfg.h100_au * autors / (100.0 * autors) ** cfg.b_exp
+        zmr = z1 * rp ** cfg.b_exp
+        gexp = (zp / zmr) ** 2
+        shape = rp ** (-cfg.a_exp) * np.where(gexp < 120.0, np.exp(-0.5 * gexp), 0.0)
+        shape = np.where((rp >= rmind) & (rp <= rmaxd), shape, 0.0)
+        m = np.sum(shape * vol)
+        if m > 0:
+            dens[..., id_] = shape * (fm * cfg.massd * MSOL / m)
+
+    # Ulrich envelope (envset.f90, densenv.f90) with a polynomial cavity
+    rc = cfg.rc_au * autors
+    rdcgs = rc * rmincgs
+    const = 1.0 / 3.15576e7 * np.sqrt(MSOL) / (4.0 * PI) / np.sqrt(GN)
+    rhoe0 = const * cfg.rate / np.sqrt(cfg.massc) / rdcgs ** 1.5
+    cosa = np.abs(cost)
+    rx = R / rc
+    xmu = cosa
+    mu0 = np.maximum(_zerod(xmu, rx), 1e-12)
+    factor = 1.0 / ((xmu / mu0) + 2.0 * (mu0 ** 2) / rx)
+    env = rhoe0 * rx ** (-1.5) * factor / np.sqrt(1.0 + xmu / mu0)
+    env = np.where(R > rmine, np.maximum(env, cfg.rhoamb), 0.0)
+    env = np.where(R < rmax, env, 0.0)
+    # polynomial cavity z > c1e rp^ex1 (envset.f90:25-33, z01 = 0)
+    r1max = rmax * np.tan(np.deg2rad(cfg.cavity_deg))
+    c1e = rmax / r1max ** cfg.cavity_exp
+    zup = c1e * (R * sint) ** cfg.cavity_exp
+    inhole = (R * cosa) > zup
+    cav = cfg.rho_cavity * (R / (1000.0 * autors)) ** (-2.0)
+    cav = np.where((R > rmine) & (R < rmax), cav, 0.0)
+    dens[..., 2] = np.where(inhole, 0.0, env)
+    dens[..., 3] = np.where(inhole, cav, 0.0)
+
+    if cfg.fractal:
+        from .fractal import fractal_multiplier
+        mult = fractal_multiplier(rarr, thetarr, phiarr, **cfg.fractal)
+        dens[..., 2] *= mult
+
+    densarr = np.zeros((nrg, ntg, npg, NDG), order="F")
+    densarr[:-1, :-1, :-1, :] = dens
+    vcell = np.zeros((nrg, ntg, npg), order="F")
+    vcell[:-1, :-1, :-1] = vol
+    massarr = np.zeros_like(densarr)
+    massarr[:-1, :-1, :-1, :] = (dens * vol[..., None]) ** 0.25
+    tdust = np.full((nrg, ntg, npg, NDG), 0.1 / 11605.0, order="F")
+
+    table = cfg.dust_table if cfg.dust_table is not None else synthetic_dust_table()
+    dust = dustinit(table)
+
+    # luminosity and binning (gridset.f90 lstar, setup.f90:36-40, initarr.f90:24-28)
+    lstar = 4.0 * PI * SIGT * cfg.tstar ** 4 * rmincgs ** 2
+    ltot = lstar
+    numin = 1.2398 / 5000.0
+    numax = 1.2398 / 0.01
+    lnurat = np.log(numax / numin)
+    dmu = 2.0 / cfg.nmu
+    u = 1.0 - dmu * (np.arange(1, cfg.nmu + 1) - 0.5)
+    if cfg.nap == 1:
+        ap = np.array([cfg.aperture_au])
+    else:
+        ap = np.logspace(0, np.log10(cfg.aperture_au), cfg.nap)
+    aperture2 = (ap * autors) ** 2
+    if cfg.thete_deg is None:
+        # observers at the centres of npeel equal-cos(i) bins
+        uu = 1.0 - (np.arange(1, cfg.npeel + 1) - 0.5) / cfg.npeel
+        thete = np.arccos(uu)
+    else:
+        thete = np.deg2rad(np.asarray(cfg.thete_deg, float))
+    if np.any(np.min(np.abs(thete[:, None] - thetarr[None, :]), axis=1) == 0.0):
+        raise ValueError("a peel-off angle equals a theta wall (gridset.f90:569-577)")
+    phie = np.full_like(thete, np.deg2rad(cfg.phie_deg))
+    return Model(cfg, nrg, ntg, npg, rarr, r2arr, thetarr, tan2arr, phiarr, aarr, barr, densarr,
+                 vcell, massarr, tdust, dust, cfg.rstar, tstar_ev, cfg.tshock / 11605.0, ltot,
+                 numin, float(lnurat), u, aperture2, np.cos(thete), np.sin(thete), np.cos(phie),
+                 np.sin(phie), cfg.rmaxi_au * autors, autors)
+
+
+def config1(**over) -> ModelConfig:
+    """The plain-method, CPU-runnable disk + envelope + star model:
+    64 x 64 x 32 cells, 200 wavelengths, 10 peel-off inclinations, 101^2
+    IQUV images."""
+    c = ModelConfig()
+    for k, v in over.items():
+        setattr(c, k, v)
+    return c

@@ -1,0 +1,143 @@
+/*
+ * hc.h -- C ABI of the MI355X photon-packet transport library
+ * (libhochunk3d_amd.so).  One context per GPU, one host thread per context,
+ * synchronous calls.  Every array crosses the boundary in the reference's
+ * Fortran column-major layout and dimensions (wall counts nrg/ntg/npg, ndg
+ * species), so a Fortran caller passes its module arrays unchanged.
+ *
+ * The boundary replaces the photon loop and tally reduction of Hochunk3d's
+ * disk() (newdisktherm.F90:194-547), called once per Lucy iteration
+ * (newtts.F90:312) and once for the final peeled pass (newtts.F90:349), and
+ * the temperature update tfinal (newdisktherm.F90:557).
+ *
+ * Status: every entry point returns 0 on success and a negative HC_E* code on
+ * failure; hc_last_error() gives the message.  Conditions on which the
+ * reference stops (no wall found, zero-opacity species draw) are counted in
+ * hc_tallies instead of aborting.
+ */
+#ifndef HC_H
+#define HC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HC_ABI_VERSION 1
+
+enum {
+  HC_OK = 0,
+  HC_E_ARG = -1,      /* invalid argument or size */
+  HC_E_HIP = -2,      /* HIP runtime failure */
+  HC_E_STATE = -3,    /* call order (grid/dust/sources not set) */
+  HC_E_UNSUPPORTED = -4
+};
+
+enum { HC_FP64 = 0, HC_FP32 = 1 };
+
+typedef struct hc_ctx hc_ctx;
+
+/* Sizes fixed for the life of a context (grid_mod.f90:10-51,
+ * dust_mod.f90:7-17, the missing tts_mod's nfreq..nxhst). */
+typedef struct {
+  int32_t nrg, ntg, npg;   /* wall counts (cells: nrg-1, ntg-1, npg-1) */
+  int32_t ndg;             /* species, 8 in the reference */
+  int32_t nfreq, nmu, nph, nap, npeel, nxhst;
+  int32_t nlammax, nT, nnu;
+  int32_t precision;       /* HC_FP64 or HC_FP32 packet state */
+  int32_t imcube;          /* allocate the (npeel,nx,nx,nfreq,no) image cube */
+} hc_dims;
+
+/* Source and binning scalars: the tts_mod / spot_mod / stokesvar_mod values
+ * that the photon loop reads (newdisktherm.F90:60-180, propagate.f90,
+ * peeloff_3d.f90).  Pointers are host arrays copied at hc_set_sources. */
+typedef struct {
+  double rstar;            /* solar radii */
+  double tstar, tshock;    /* eV (K / 11605) */
+  int32_t limb, ips, ispot, nspot;
+  double fspot, csprad, cspot1, sspot1, scspot1, ssspot1, cspot2, sspot2, scspot2, ssspot2;
+  double accfrac, accfrac2;
+  double numin, lnurat;    /* SED frequency binning (setup.f90:36-40) */
+  double rmaxi;            /* image half-size, stellar radii */
+  double ltot;             /* erg/s */
+  int32_t iintmax;         /* 10000 (propagate.f90:27) */
+  int32_t pad;
+  const double *u;         /* [nmu] cos(theta) bin centres (initarr.f90:24-28) */
+  const double *aperture2; /* [nap] squared aperture radii, stellar radii^2 */
+  const double *coste, *sinte, *cospe, *sinpe; /* [npeel] observer directions */
+} hc_sources;
+
+/* Caller-owned host outputs written by hc_run_iteration.  Any pointer may be
+ * NULL to skip that tally.  Per-cell arrays are (nrg,ntg,npg[,ndg]);
+ * sed is 9 stacked (nfreq,nmu,nph,nap,8) arrays in the order
+ * I,Q,U,V,I^2,Q^2,U^2,V^2,N (propagate.f90:276-301); peel is 9 stacked
+ * (nfreq,npeel,nap,8) arrays (peeloff_3d.f90:247-262); img_i is
+ * (npeel,nx,nx,nfreq,8), img_q/u/v are (npeel,nx,nx,nfreq)
+ * (peeloff_3d.f90:270-277).  All accumulators are FP64 (the reference's
+ * SED/image arrays are FP32; the caller may narrow them). */
+typedef struct {
+  double *dtauabs, *nabs, *jmean;
+  int32_t *killed, *dbig, *dsg;
+  double *sed, *peel;
+  double *img_i, *img_q, *img_u, *img_v;
+  double *si_init;         /* (nfreq,5) */
+  double tot, flux, sflux, nscat, scount, abflux, aflux, sumsub, sumall, hit, htot;
+  int64_t n_err_nowall, n_err_dust, n_warn;
+  double kernel_ms;        /* device time of the transport launch(es) */
+} hc_tallies;
+
+/* Per-packet record of the deterministic-trace mode. */
+typedef struct {
+  double x, y, z, cost, sint, phi, I, Q, U, V, nub, path, tau_peel;
+  int64_t ncross, npeelcross, nscat, nabs, nstar, ntrial;
+  uint64_t cellhash;
+  int32_t status, ir, it, ip, pad;
+} hc_trace;
+
+/* Replaces grid_allocate + the module-global setup of the reference
+ * (grid_mod.f90:55-92); selects the device. */
+int hc_create(const hc_dims *dims, int device, hc_ctx **out);
+
+/* Walls and densities built by gridset (gridset.f90:411-1098):
+ * r2arr[nrg], tan2arr/thetarr[ntg], aarr/barr/phiarr[npg],
+ * densarr/tdust (nrg,ntg,npg,ndg) with tdust in eV, Tsub in K. */
+int hc_set_grid(hc_ctx *ctx, const double *r2arr, const double *tan2arr, const double *thetarr,
+                const double *aarr, const double *barr, const double *phiarr,
+                const double *densarr, const double *tdust, double tsub);
+
+/* New dust temperatures between Lucy iterations (initarr.f90:78-84). */
+int hc_set_temperature(hc_ctx *ctx, const double *tdust);
+
+/* Dust tables of dustinit/setemiss (dust.f90:314-474): nlambda[ndg];
+ * nudust..pldust (nlammax,ndg); kappav[ndg]; lTint[nT]; nuint[nnu];
+ * kapBint (nT,nnu,ndg) normalised cumulative; kappap (nT,ndg). */
+int hc_set_dust(hc_ctx *ctx, const int32_t *nlambda, const double *nudust, const double *kappad,
+                const double *adust, const double *gdust, const double *pldust,
+                const double *kappav, const double *lTint, const double *nuint,
+                const double *kapBint, const double *kappap);
+
+int hc_set_sources(hc_ctx *ctx, const hc_sources *src);
+
+/* One transport pass (the body of disk(), newdisktherm.F90:194-547): np
+ * packets, peel-off when ipeel=1.  Packet p (0-based) draws from segment
+ * seg0+p of L'Ecuyer-Cote generator gen seeded with (seed1,seed2)
+ * (ecuyer_cote_mod.f90).  trace, when non-NULL, receives np records. */
+int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int32_t seed2,
+                     int32_t gen, uint64_t seg0, hc_tallies *out, hc_trace *trace);
+
+/* Lucy temperature update on the device from the last pass's dtauabs
+ * (reconstructed tfinal, newdisktherm.F90:557 with dusttemp.f90:6-58).
+ * massarr (nrg,ntg,npg,ndg) = (rho V)^(1/4); writes tdust2 and the count of
+ * (cell,species) whose T moved by more than 1 %. */
+int hc_temperature(hc_ctx *ctx, const double *massarr, double ltot, double nphot,
+                   double *tdust2, int64_t *nchanged);
+
+int hc_destroy(hc_ctx *ctx);
+const char *hc_last_error(const hc_ctx *ctx);
+int hc_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

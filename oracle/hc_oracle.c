@@ -1,0 +1,1568 @@
+/*
+ * hc_oracle.c -- FP64 CPU restatement of the Hochunk3d photon-packet
+ * transport path.  TEST INFRASTRUCTURE: only the tests, the smoke check and
+ * the benchmark's CPU-baseline leg load this library, and only as the checker.
+ *
+ * Arithmetic follows the reference's Fortran expressions term by term, in
+ * left-to-right order, so that the restatement rounds like the reference
+ * (build with -ffp-contract=off).  Each routine cites the file:line it
+ * restates.  Three routines whose sources are absent from the reference tree
+ * are reconstructed from their call sites (tauint, thetaface, tfinal); their
+ * contracts are stated where they are defined.
+ */
+#include "hc_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define PI 3.1415926535897932384626433  /* constants.f90:5 */
+#define R2P (2.0 * PI)
+#define RSOL 6.9598e10                   /* constants.f90:11 */
+#define SIGT 5.670400e-5                 /* constants.f90:21 */
+#define HUGE_T DBL_MAX
+
+/* ------------------------------------------------------------------------ */
+/* RNG: ecuyer_cote_mod.f90 (L'Ecuyer & Cote 1991)                           */
+/* ------------------------------------------------------------------------ */
+enum {
+  A1 = 40014, A2 = 40692, M1 = 2147483563, M2 = 2147483399,
+  A1VW = 2082007225, A2VW = 784306273, A1W = 1033780774, A2W = 1494757890
+};  /* ecuyer_cote_mod.f90:14-17 */
+
+int32_t hco_mulmod(int32_t a, int32_t s, int32_t m) {
+  /* a*s mod m for 0<a,s<m: the value multiply_modulo
+   * (ecuyer_cote_mod.f90:131-230) computes; here via a 64-bit product. */
+  return (int32_t)(((int64_t)a * (int64_t)s) % (int64_t)m);
+}
+
+static int32_t powmod(int32_t a, uint64_t e, int32_t m) {
+  int64_t r = 1, b = a;
+  while (e) {
+    if (e & 1u) r = (r * b) % m;
+    b = (b * b) % m;
+    e >>= 1;
+  }
+  return (int32_t)r;
+}
+
+/* Packet p of a pass draws from segment `segment` of generator `gen`:
+ * generator g starts at seed * a^(2^(v+w) (g-1)) (set_all_seeds,
+ * ecuyer_cote_mod.f90:345-362) and each segment is 2^w draws further on
+ * (reinitialize_current_generator(1), ecuyer_cote_mod.f90:318-320).  The
+ * segment start is therefore seed_g * (a^(2^w))^segment mod m. */
+void hco_rng_packet(hco_rng *r, int32_t seed1, int32_t seed2, int gen, uint64_t segment) {
+  int32_t g1 = seed1, g2 = seed2;
+  if (gen > 1) {
+    g1 = hco_mulmod(powmod(A1VW, (uint64_t)(gen - 1), M1), g1, M1);
+    g2 = hco_mulmod(powmod(A2VW, (uint64_t)(gen - 1), M2), g2, M2);
+  }
+  r->s1 = hco_mulmod(powmod(A1W, segment, M1), g1, M1);
+  r->s2 = hco_mulmod(powmod(A2W, segment, M2), g2, M2);
+}
+
+/* random_large_integer (ecuyer_cote_mod.f90:252-293) scaled by
+ * random_standard_uniform_mod.f90:43. */
+double hco_ran(hco_rng *r) {
+  int32_t s1 = r->s1, s2 = r->s2, k, z;
+  k = s1 / 53668;
+  s1 = A1 * (s1 - k * 53668) - k * 12211;
+  if (s1 < 0) s1 = s1 + M1;
+  k = s2 / 52774;
+  s2 = A2 * (s2 - k * 52774) - k * 3791;
+  if (s2 < 0) s2 = s2 + M2;
+  r->s1 = s1;
+  r->s2 = s2;
+  z = s1 - s2;
+  if (z < 1) z = z + M1 - 1;
+  return (double)z * 4.656613057e-10;
+}
+
+/* ------------------------------------------------------------------------ */
+/* wall geometry                                                             */
+/* ------------------------------------------------------------------------ */
+void hco_reset_t(hco_walls *w) {  /* closest_wall.f90:13-22 */
+  w->tn1 = -HUGE_T;
+  w->tp1 = HUGE_T;
+  w->tp2 = HUGE_T;
+  w->in1 = w->ip1 = w->ip2 = 0;
+}
+
+void hco_insert_t(hco_walls *w, double t, int i) {  /* closest_wall.f90:24-45 */
+  if (t < 0.0) {
+    if (t > w->tn1) {
+      w->tn1 = t;
+      w->in1 = i;
+    }
+  } else {
+    if (t < w->tp1) {
+      w->tp2 = w->tp1;
+      w->ip2 = w->ip1;
+      w->tp1 = t;
+      w->ip1 = i;
+    } else if (t < w->tp2) {
+      w->tp2 = t;
+      w->ip2 = i;
+    }
+  }
+}
+
+/* closest_wall.f90:47-66; returns 0 when no wall exists (the reference
+ * stops there). */
+int hco_find_next_wall(const hco_walls *w, int on_wall, double *t, int *face) {
+  if (on_wall) {
+    if (fabs(w->tn1) < fabs(w->tp1)) {
+      *t = w->tp1;
+      *face = w->ip1;
+    } else {
+      *t = w->tp2;
+      *face = w->ip2;
+    }
+  } else {
+    *t = w->tp1;
+    *face = w->ip1;
+  }
+  return *t != HUGE_T;
+}
+
+void hco_radface(hco_walls *w, double ux, double uy, double uz, double x1, double y1, double z1,
+                 double r2_1, double r2_2) {  /* radface.f90:34-61 */
+  double bb, cc, descr;
+  bb = 2.0 * (x1 * ux + y1 * uy + uz * z1);
+  cc = x1 * x1 + y1 * y1 + z1 * z1 - r2_1;
+  descr = bb * bb - 4.0 * cc;
+  if (descr > 0.0) {
+    descr = sqrt(descr);
+    hco_insert_t(w, (-bb + descr) * 0.5, 1);
+    hco_insert_t(w, (-bb - descr) * 0.5, 1);
+  }
+  cc = x1 * x1 + y1 * y1 + z1 * z1 - r2_2;
+  descr = bb * bb - 4.0 * cc;
+  if (descr > 0.0) {
+    descr = sqrt(descr);
+    hco_insert_t(w, (-bb + descr) * 0.5, 2);
+    hco_insert_t(w, (-bb - descr) * 0.5, 2);
+  }
+}
+
+/* thetaface: source absent from the reference tree (Makefile.in lists
+ * thetaface.o).  Reconstructed from its call (peeloff_3d.f90:98, arguments
+ * ux,uy,uz,x,y,z,tan2arr(it),tan2arr(it+1)) and the wall conventions of
+ * gridset.f90:581-601:
+ *   tan2 > 0 : cone x^2+y^2 = tan2*z^2; both real roots of
+ *              (ux^2+uy^2-tan2*uz^2) t^2 + 2(x ux+y uy-tan2 z uz) t
+ *              + (x^2+y^2-tan2 z^2) = 0 are inserted (a linear root when the
+ *              quadratic coefficient is zero);
+ *   tan2 = -1: the midplane z = 0, root -z/uz when uz /= 0;
+ *   tan2 = 0 : a pole (theta = 0 or pi), no wall.
+ * Face ids 3 (wall it) and 4 (wall it+1) follow peeloff_3d.f90:126-129.
+ * Roots on the mirror nappe (pi - theta) are inserted unfiltered: from inside
+ * a cell the mirror nappe can only be reached after a nearer crossing of the
+ * cell's own walls, so it never becomes the selected wall. */
+static void thetaface_one(hco_walls *w, double ux, double uy, double uz, double x1, double y1,
+                          double z1, double t2, int face) {
+  double a, b, c, descr;
+  if (t2 == 0.0) return;
+  if (t2 == -1.0) {
+    if (uz != 0.0) hco_insert_t(w, -z1 / uz, face);
+    return;
+  }
+  a = ux * ux + uy * uy - t2 * uz * uz;
+  b = 2.0 * (x1 * ux + y1 * uy - t2 * z1 * uz);
+  c = x1 * x1 + y1 * y1 - t2 * z1 * z1;
+  if (a == 0.0) {
+    if (b != 0.0) hco_insert_t(w, -c / b, face);
+    return;
+  }
+  descr = b * b - 4.0 * a * c;
+  if (descr > 0.0) {
+    descr = sqrt(descr);
+    hco_insert_t(w, (-b + descr) / (2.0 * a), face);
+    hco_insert_t(w, (-b - descr) / (2.0 * a), face);
+  }
+}
+
+/* Grids with an even wall count (which gridset.f90:529-559 cannot build)
+ * have a cell across the midplane whose two walls are the two nappes of one
+ * cone (equal tan2).  There each root is assigned to the nappe given by the
+ * sign of its z: face 3 above the midplane, face 4 below. */
+static void straddle_roots(hco_walls *w, double ux, double uy, double uz, double x1, double y1,
+                           double z1, double t2) {
+  double a, b, c, descr, t;
+  a = ux * ux + uy * uy - t2 * uz * uz;
+  b = 2.0 * (x1 * ux + y1 * uy - t2 * z1 * uz);
+  c = x1 * x1 + y1 * y1 - t2 * z1 * z1;
+  if (a == 0.0) {
+    if (b != 0.0) {
+      t = -c / b;
+      hco_insert_t(w, t, (z1 + t * uz) > 0.0 ? 3 : 4);
+    }
+    return;
+  }
+  descr = b * b - 4.0 * a * c;
+  if (descr > 0.0) {
+    descr = sqrt(descr);
+    t = (-b + descr) / (2.0 * a);
+    hco_insert_t(w, t, (z1 + t * uz) > 0.0 ? 3 : 4);
+    t = (-b - descr) / (2.0 * a);
+    hco_insert_t(w, t, (z1 + t * uz) > 0.0 ? 3 : 4);
+  }
+}
+
+void hco_thetaface(hco_walls *w, double ux, double uy, double uz, double x1, double y1, double z1,
+                   double t2a, double t2b) {
+  if (t2a == t2b && t2a > 0.0) {
+    straddle_roots(w, ux, uy, uz, x1, y1, z1, t2a);
+    return;
+  }
+  thetaface_one(w, ux, uy, uz, x1, y1, z1, t2a, 3);
+  thetaface_one(w, ux, uy, uz, x1, y1, z1, t2b, 4);
+}
+
+/* phiface.f90:43-57 with C=D=0 (the planes contain the z axis). */
+void hco_phiface(hco_walls *w, double a1, double b1, double a2, double b2, double ux, double uy,
+                 double x1, double y1) {
+  double denom1, denom2;
+  denom1 = (a1 * ux + b1 * uy);
+  if (denom1 != 0.0) hco_insert_t(w, -(a1 * x1 + b1 * y1) / denom1, 5);
+  denom2 = (a2 * ux + b2 * uy);
+  if (denom2 != 0.0) hco_insert_t(w, -(a2 * x1 + b2 * y1) / denom2, 6);
+}
+
+/* nr.f:131-153 locate: 1-based j with xx(j) < x <= xx(j+1) for an
+ * increasing table, 0 or n outside; x == xx(1) gives 1. */
+int hco_locate(const double *xx, int n, double x) {
+  int jl = 0, ju = n + 1, jm;
+  int incr = xx[n - 1] > xx[0];
+  while (ju - jl > 1) {
+    jm = (ju + jl) / 2;
+    if (incr == (x > xx[jm - 1]))
+      jl = jm;
+    else
+      ju = jm;
+  }
+  if (jl == 0 && x == xx[0]) jl = 1;
+  return jl;
+}
+
+/* ------------------------------------------------------------------------ */
+/* dust                                                                      */
+/* ------------------------------------------------------------------------ */
+#define IDX2(i, j, n1) ((size_t)(i) + (size_t)(n1) * (size_t)(j))
+
+void hco_opacset(const hco_dust *d, double nu, hco_opac *o) {  /* dust.f90:9-80 */
+  int id;
+  for (id = 0; id < HCO_NDG; id++) {
+    const int nl = d->nlambda[id];
+    const double *nud = d->nudust + (size_t)d->nlammax * id;
+    const double *kd = d->kappad + (size_t)d->nlammax * id;
+    const double *ad = d->adust + (size_t)d->nlammax * id;
+    const double *gd = d->gdust + (size_t)d->nlammax * id;
+    const double *pd = d->pldust + (size_t)d->nlammax * id;
+    if (nu >= nud[0]) {
+      o->g[id] = gd[0];
+      o->pl[id] = pd[0];
+      o->kappa[id] = kd[0];
+      o->albedo[id] = ad[0];
+    } else if (nu <= nud[nl - 1]) {
+      o->g[id] = gd[nl - 1];
+      o->pl[id] = pd[nl - 1];
+      o->kappa[id] = kd[nl - 1];
+      o->albedo[id] = ad[nl - 1];
+    } else {
+      int i0 = 1, i1 = nl, i;
+      double f, fm1;
+      while (i1 - i0 > 1) {
+        i = (i1 + i0) / 2;
+        if (nu <= nud[i - 1])
+          i0 = i;
+        else
+          i1 = i;
+      }
+      f = (nu - nud[i1 - 1]) / (nud[i0 - 1] - nud[i1 - 1]);
+      fm1 = 1.0 - f;
+      o->g[id] = fm1 * gd[i1 - 1] + f * gd[i0 - 1];
+      o->pl[id] = fm1 * pd[i1 - 1] + f * pd[i0 - 1];
+      o->kappa[id] = fm1 * kd[i1 - 1] + f * kd[i0 - 1];
+      o->albedo[id] = fm1 * ad[i1 - 1] + f * ad[i0 - 1];
+    }
+    o->g2[id] = o->g[id] * o->g[id];
+    o->g2p1[id] = o->g2[id] + 1.0;
+    o->onemg2[id] = 1.0 - o->g2[id];
+    o->twog[id] = 2.0 * o->g[id];
+  }
+}
+
+static double inczeta[146];
+static int inczeta_ready = 0;
+
+static void zetainit(void) {  /* random.f90:79-97 */
+  int i, j;
+  double temp;
+  double z[146];
+  for (i = 1; i <= 145; i++) z[i] = 0.0;
+  for (j = 145; j >= 1; j--) {
+    temp = 1.0 / (double)(j * j * j * j);
+    for (i = 145; i >= j; i--) z[i] = z[i] + temp;
+  }
+  memcpy(inczeta, z, sizeof z);
+  inczeta_ready = 1;
+}
+
+double hco_bbfreq(hco_rng *r, double teff) {  /* random.f90:49-77 */
+  double z0, z1, z2, z3, z4;
+  int i;
+  if (!inczeta_ready) zetainit();
+  z0 = inczeta[145] * hco_ran(r);
+  z1 = hco_ran(r);
+  z2 = hco_ran(r);
+  z3 = hco_ran(r);
+  z4 = hco_ran(r);
+  i = 1;
+  while (z0 > inczeta[i]) i = i + 1;
+  return -log(z1 * z2 * z3 * z4) * teff / (double)i;
+}
+
+/* lucydustfreq.f90:19-157, including its guard chain.  The row index j is
+ * clamped to [1,nT-1] where the reference would index out of bounds. */
+double hco_lucydustfreq(const hco_dust *d, hco_rng *r, double T0, int id) {
+  const int nT = d->nT, nnu = d->nnu;
+  const double *lTint = d->lTint, *nuint = d->nuint;
+  const double *kb = d->kapBint + (size_t)nT * nnu * id;
+  double lT0, xi, f, P0, P1, P;
+  int j, i0, i1, i;
+#define KB(jj, ii) kb[IDX2((jj) - 1, (ii) - 1, nT)]
+  lT0 = log(T0);
+  xi = hco_ran(r);
+  j = (int)((double)(nT - 1) * (lT0 - lTint[0]) / (lTint[nT - 1] - lTint[0])) + 1;
+  if (j < 1) j = 1;
+  if (j > nT - 1) j = nT - 1;
+  f = (lT0 - lTint[j - 1]) / (lTint[j] - lTint[j - 1]);
+  if (f < 0.0) f = 0.0;
+  if (f > 1.0) f = 1.0;
+  i0 = 1;
+  i1 = nnu;
+  P0 = (1.0 - f) * KB(j, i0) + f * KB(j + 1, i0);
+  if (xi < P0) {
+    if (P0 > 0.0) return T0 * xi * nuint[0] / P0;
+    return T0 * nuint[0];
+  }
+  P1 = (1.0 - f) * KB(j, i1) + f * KB(j + 1, i1);
+  if (P1 > 1.0) P1 = 1.0;
+  if (xi >= P1) {
+    if (P1 < 1.0) {
+      if (xi < 1.0) return T0 * nuint[nnu - 1] * (1.0 - P1) / (1.0 - xi);
+      return T0 * nuint[nnu - 1];
+    }
+    return T0 * nuint[nnu - 1];
+  }
+  while ((i1 - i0) > 1) {
+    i = (i0 + i1) / 2;
+    P = (1.0 - f) * KB(j, i) + f * KB(j + 1, i);
+    if (P > 1.0) P = 1.0;
+    if (P < P0) {
+      P0 = P;
+      i0 = i;
+    }
+    if (P > P1) {
+      P1 = P;
+      i1 = 1; /* lucydustfreq.f90:109 as written */
+    }
+    if (P > xi) {
+      i1 = i;
+      P1 = P;
+    } else {
+      i0 = i;
+      P0 = P;
+    }
+  }
+  if (xi < P0)
+    f = 0.0;
+  else if (xi > P1)
+    f = 1.0;
+  else if (P0 <= 0.0)
+    f = 0.5;
+  else if (P1 <= 0.0) {
+    /* f keeps its interpolation weight (lucydustfreq.f90:138-139) */
+  } else if (xi <= 0.0) {
+  } else
+    f = log(xi / P0) / log(P1 / P0);
+#undef KB
+  return T0 * nuint[i0 - 1] * pow(nuint[i1 - 1] / nuint[i0 - 1], f);
+}
+
+double hco_kapp(const hco_dust *d, double T, int id) {  /* dust.f90:114-142 */
+  const int nT = d->nT;
+  const double *kp = d->kappap + (size_t)nT * id;
+  double lT = log(T), f;
+  int k = (int)((double)(nT - 1) * (lT - d->lTint[0]) / (d->lTint[nT - 1] - d->lTint[0])) + 1;
+  if (k < 1 || k > nT - 1) return k < 1 ? kp[0] : kp[nT - 1];
+  f = (lT - d->lTint[k - 1]) / (d->lTint[k] - d->lTint[k - 1]);
+  return (1.0 - f) * kp[k - 1] + f * kp[k];
+}
+
+/* dusttemp.f90:6-58 (mass4 = massarr = (rho V)^(1/4)); the fixed-point
+ * loop is capped at 1000 sweeps where the reference would spin. */
+double hco_dusttemp(const hco_dust *d, double na, double T0, double ltot, double nphot,
+                    double mass4, int id) {
+  const double eps = 1.e-5;
+  double Td, Told;
+  int n = 0;
+  Td = (T0 != 0.0) ? T0 : 0.1 / 11605.0;
+  Told = 0.5 * Td;
+  while (fabs(Td - Told) > eps * Td && n < 1000) {
+    Told = Td;
+    Td = pow((na * ltot) / (4.0 * SIGT * nphot * d->kappav[id] * hco_kapp(d, Td, id)), 0.25) /
+         11605.0;
+    if (mass4 > 0.0)
+      Td = Td / mass4;
+    else
+      Td = 0.1 / 11605.0;
+    if (Td > 1600.0 / 11605.0) Td = 1600.0 / 11605.0;
+    Td = fmax(Td, 0.1 / 11605.0);
+    n++;
+  }
+  return Td;
+}
+
+/* tfinal: source absent (Makefile.in lists tfinal.o; called at
+ * newdisktherm.F90:557).  Reconstructed as the Lucy (1999) update the
+ * reference's own pieces imply: dtauabs holds the absorbed optical depth
+ * sum_k kapd*(1-albedo)*rho*ds per cell and species, so the cell absorbs
+ * (L/N)*dtauabs, and dusttemp's balance with na -> dtauabs gives T.  Cells
+ * outside the (nrg-1,ntg-1,npg-1) block and species with zero mass keep
+ * 0.1 K, the initarr.f90:53 initial value. */
+int64_t hco_tfinal(const hco_grid *g, const hco_dust *d, const double *dtauabs, double ltot,
+                   double nphot, double *tdust2) {
+  const size_t ncell = (size_t)g->nrg * g->ntg * g->npg;
+  int64_t nchanged = 0;
+  int ir, it, ip, id;
+  for (id = 0; id < HCO_NDG; id++)
+    for (ip = 0; ip < g->npg; ip++)
+      for (it = 0; it < g->ntg; it++)
+        for (ir = 0; ir < g->nrg; ir++) {
+          size_t c = (size_t)ir + (size_t)g->nrg * ((size_t)it + (size_t)g->ntg * ip);
+          size_t k = c + ncell * id;
+          double tnew = 0.1 / 11605.0;
+          if (ir < g->nrg - 1 && it < g->ntg - 1 && ip < g->npg - 1 && g->massarr[k] > 0.0)
+            tnew = hco_dusttemp(d, dtauabs[k], g->tdust[k], ltot, nphot, g->massarr[k], id);
+          if (fabs(tnew - g->tdust[k]) > 0.01 * g->tdust[k]) nchanged++;
+          tdust2[k] = tnew;
+        }
+  return nchanged;
+}
+
+/* ------------------------------------------------------------------------ */
+/* scattering                                                                */
+/* ------------------------------------------------------------------------ */
+static void dustmat(double *p1, double *p2, double *p3, double *p4, double cost, double cost2,
+                    const hco_opac *o, int id) {  /* dustmat.f90:39-66 */
+  double temp = o->g2p1[id] - o->twog[id] * cost;
+  *p1 = 1.5 * o->onemg2[id] / (2.0 + o->g2[id]) * (1.0 + cost2) / (temp * sqrt(temp));
+  *p2 = -o->pl[id] * *p1 * (1.0 - cost2) / (1.0 + cost2);
+  *p3 = *p1 * 2.0 * cost / (1.0 + cost2);
+  *p4 = 0.0;
+}
+
+static double cs_peak(const hco_opac *o, int id) {  /* stokes.f90:54-62 */
+  double temp;
+  if (o->g[id] > 0.0)
+    temp = o->g2p1[id] - o->twog[id];
+  else
+    temp = o->g2p1[id] + o->twog[id];
+  return 1.5 * (o->onemg2[id]) / (2.0 + o->g2[id]) * 2.0 / temp / sqrt(temp) * 1.01;
+}
+
+/* stokes.f90:1-392: rejection-sample the scattering angle and azimuth, then
+ * rotate the Stokes vector and the direction. */
+void hco_stokes(hco_scat *s, const hco_opac *o, int id, hco_rng *r) {
+  const double costp = s->cost, sintp = s->sint, phip = s->phi;
+  const double sip = s->I, sqp = s->Q, sup = s->U, svp = s->V;
+  double peak = cs_peak(o, id);
+  double bmu, cosb2, b, p1, p2, p3, p4, sinbt, ri1, ri3, xran, rprob, a;
+  double cosi1 = 0, sini1 = 0, cosi2 = 0, sini2 = 0, cosi3 = 0, sini3 = 0;
+  double sin2i1, cos2i1, sin2i2, cos2i2, sin2i3, cos2i3, bott, cosdph;
+  double sin2, cos2, sin2cos1, cos2sin1;
+  double a11, a12, a13, a21, a22, a23, a24, a31, a32, a33, a34, a42, a43, a44;
+  double cost = s->cost, sint = s->sint, phi = s->phi;
+  for (;;) {
+    s->htot = s->htot + 1.0;
+    xran = hco_ran(r);
+    bmu = 1.0 - 2.0 * xran;
+    cosb2 = bmu * bmu;
+    b = cosb2 - 1.0;
+    (void)b;
+    dustmat(&p1, &p2, &p3, &p4, bmu, cosb2, o, id);
+    if (fabs(bmu) > 1.0) {
+      if (bmu > 1.0) {
+        bmu = 1.0;
+        cosb2 = 1.0;
+      } else {
+        bmu = -1.0;
+        cosb2 = 1.0;
+      }
+    }
+    sinbt = sqrt(1.0 - cosb2);
+    xran = hco_ran(r);
+    ri1 = R2P * xran;
+    if (ri1 > PI) {
+      ri3 = R2P - ri1;
+      cosi3 = cos(ri3);
+      sini3 = sin(ri3);
+      sin2i3 = 2.0 * sini3 * cosi3;
+      cos2i3 = 2.0 * (cosi3 * cosi3) - 1.0;
+      a11 = p1;
+      a12 = p2 * cos2i3;
+      a13 = p2 * sin2i3;
+      rprob = (a11 * sip + a12 * sqp + a13 * sup) / sip;
+      if (rprob > peak) peak = rprob;
+      xran = hco_ran(r);
+      if (peak * xran > rprob) continue;
+      s->hit = s->hit + 1.0;
+      a = rprob;
+      if (bmu == 1.0) return; /* stokes.f90:102-105: leave the packet as is */
+      if (bmu == -1.0) {
+        sini3 = 1.0;
+        sini2 = 1.0;
+        cosi3 = 0.0;
+        cosi2 = 0.0;
+        cost = -costp;
+        sint = sintp;
+      } else {
+        cost = costp * bmu + sintp * sinbt * cosi3;
+        if (fabs(cost) < 1.0) {
+          sint = fabs(sqrt(1.0 - cost * cost));
+          sini2 = sini3 * sintp / sint;
+          bott = sint * sinbt;
+          cosi2 = costp / bott - cost * bmu / bott;
+          if (cosi2 >= 1.0) {
+            cosi2 = 1.0;
+            sini2 = 0.0;
+          }
+          if (cosi2 <= -1.0) {
+            cosi2 = -1.0;
+            sini2 = 0.0;
+          }
+        } else {
+          if (cost >= 1.0) {
+            cost = 1.0;
+            cosi2 = -1.0;
+          }
+          if (cost <= -1.0) {
+            cost = -1.0;
+            cosi2 = 1.0;
+          }
+          sint = 0.0;
+          sini2 = 0.0;
+        }
+      }
+      cosdph = -cosi2 * cosi3 + sini2 * sini3 * bmu;
+      if (fabs(cosdph) > 1.0) cosdph = (cosdph > 1.0) ? 1.0 : -1.0;
+      phi = phip + acos(cosdph);
+      if (phi > R2P) phi = phi - R2P;
+      if (phi < 0.0) phi = phi + R2P;
+      sin2i2 = 2.0 * sini2 * cosi2;
+      cos2i2 = 2.0 * (cosi2 * cosi2) - 1.0;
+      sin2 = sin2i2 * sin2i3;
+      cos2 = cos2i2 * cos2i3;
+      sin2cos1 = sin2i2 * cos2i3;
+      cos2sin1 = cos2i2 * sin2i3;
+      a21 = p2 * cos2i2;
+      a22 = p1 * cos2 - p3 * sin2;
+      a23 = p1 * cos2sin1 + p3 * sin2cos1;
+      a24 = -p4 * sin2i2;
+      a31 = -p2 * sin2i2;
+      a32 = -p1 * sin2cos1 - p3 * cos2sin1;
+      a33 = -p1 * sin2 + p3 * cos2;
+      a34 = -p4 * cos2i2;
+      a42 = -p4 * sin2i3;
+      a43 = p4 * cos2i3;
+      a44 = p3;
+    } else {
+      cosi1 = cos(ri1);
+      sini1 = sin(ri1);
+      sin2i1 = 2.0 * sini1 * cosi1;
+      cos2i1 = 2.0 * (cosi1 * cosi1) - 1.0;
+      a11 = p1;
+      a12 = p2 * cos2i1;
+      a13 = -p2 * sin2i1;
+      rprob = (a11 * sip + a12 * sqp + a13 * sup) / sip;
+      if (rprob > peak) peak = rprob;
+      xran = hco_ran(r);
+      if (peak * xran > rprob) continue;
+      s->hit = s->hit + 1.0;
+      a = rprob;
+      if (bmu == 1.0) return;
+      if (bmu == -1.0) {
+        sini1 = 1.0;
+        sini2 = 1.0;
+        cosi1 = 0.0;
+        cosi2 = 0.0;
+        cost = -costp;
+        sint = sintp;
+      } else {
+        cost = costp * bmu + sintp * sinbt * cosi1;
+        if (fabs(cost) < 1.0) {
+          sint = fabs(sqrt(1.0 - cost * cost));
+          sini2 = sini1 * sintp / sint;
+          bott = sint * sinbt;
+          cosi2 = costp / bott - cost * bmu / bott;
+          if (cosi2 >= 1.0) {
+            cosi2 = 1.0;
+            sini2 = 0.0;
+          }
+          if (cosi2 <= -1.0) {
+            cosi2 = -1.0;
+            sini2 = 0.0;
+          }
+        } else {
+          if (cost >= 1.0) {
+            cost = 1.0;
+            cosi2 = -1.0;
+          }
+          if (cost <= -1.0) {
+            cost = -1.0;
+            cosi2 = 1.0;
+          }
+          sint = 0.0;
+          sini2 = 0.0;
+        }
+      }
+      cosdph = -cosi1 * cosi2 + sini1 * sini2 * bmu;
+      if (fabs(cosdph) > 1.0) cosdph = (cosdph > 1.0) ? 1.0 : -1.0;
+      phi = phip - acos(cosdph);
+      if (phi > R2P) phi = phi - R2P;
+      if (phi < 0.0) phi = phi + R2P;
+      sin2i2 = 2.0 * sini2 * cosi2;
+      cos2i2 = 2.0 * (cosi2 * cosi2) - 1.0;
+      sin2 = sin2i2 * sin2i1;
+      cos2 = cos2i2 * cos2i1;
+      sin2cos1 = sin2i2 * cos2i1;
+      cos2sin1 = cos2i2 * sin2i1;
+      a21 = p2 * cos2i2;
+      a22 = p1 * cos2 - p3 * sin2;
+      a23 = -p1 * cos2sin1 - p3 * sin2cos1;
+      a24 = p4 * sin2i2;
+      a31 = p2 * sin2i2;
+      a32 = p1 * sin2cos1 + p3 * cos2sin1;
+      a33 = -p1 * sin2 + p3 * cos2;
+      a34 = -p4 * cos2i2;
+      a42 = p4 * sin2i1;
+      a43 = p4 * cos2i1;
+      a44 = p3;
+    }
+    break;
+  }
+  s->I = (a11 * sip + a12 * sqp + a13 * sup) / a;
+  s->Q = (a21 * sip + a22 * sqp + a23 * sup + a24 * svp) / a;
+  s->U = (a31 * sip + a32 * sqp + a33 * sup + a34 * svp) / a;
+  s->V = (a42 * sqp + a43 * sup + a44 * svp) / a;
+  s->cost = cost;
+  s->sint = sint;
+  s->phi = phi;
+  s->cosp = cos(phi);
+  s->sinp = sin(phi);
+}
+
+/* stokespeel.f90:1-382: the un-normalised phase-matrix weighting of a peeled
+ * packet for the given scattering cosine bmu and rotation angle ri1.  cost
+ * and sint are the peel direction (clamped in place like the reference). */
+void hco_stokespeel(double *I, double *Q, double *U, double *V, double *cost, double *sint,
+                    double ri1, double bmu, double costp, double sintp, const hco_opac *o, int id) {
+  const double sip = *I, sqp = *Q, sup = *U, svp = *V;
+  double cosb2, p1, p2, p3, p4, sinbt;
+  double cosi1 = 0, sini1 = 0, cosi2 = 0, sini2 = 0, cosi3 = 0, sini3 = 0, ri3;
+  double sin2i1, cos2i1, sin2i2, cos2i2, sin2i3, cos2i3, bott;
+  double sin2, cos2, sin2cos1, cos2sin1;
+  double a11, a12, a13, a21, a22, a23, a24, a31, a32, a33, a34, a42, a43, a44;
+  cosb2 = bmu * bmu;
+  dustmat(&p1, &p2, &p3, &p4, bmu, cosb2, o, id);
+  if (fabs(bmu) > 1.0) {
+    bmu = (bmu > 1.0) ? 1.0 : -1.0;
+    cosb2 = 1.0;
+  }
+  sinbt = sqrt(1.0 - cosb2);
+  if (ri1 > PI) {
+    ri3 = R2P - ri1;
+    cosi3 = cos(ri3);
+    sini3 = sin(ri3);
+    sin2i3 = 2.0 * sini3 * cosi3;
+    cos2i3 = 2.0 * (cosi3 * cosi3) - 1.0;
+    a11 = p1;
+    a12 = p2 * cos2i3;
+    a13 = p2 * sin2i3;
+    if (bmu == 1.0) return; /* stokespeel.f90:116-119 */
+    if (bmu == -1.0) {
+      sini3 = 1.0;
+      sini2 = 1.0;
+      cosi3 = 0.0;
+      cosi2 = 0.0;
+    } else if (fabs(*cost) < 1.0) {
+      sini2 = sini3 * sintp / *sint;
+      bott = *sint * sinbt;
+      cosi2 = costp / bott - *cost * bmu / bott;
+      if (cosi2 >= 1.0) {
+        cosi2 = 1.0;
+        sini2 = 0.0;
+      }
+      if (cosi2 <= -1.0) {
+        cosi2 = -1.0;
+        sini2 = 0.0;
+      }
+    } else {
+      if (*cost >= 1.0) {
+        *cost = 1.0;
+        cosi2 = -1.0;
+      }
+      if (*cost <= -1.0) {
+        *cost = -1.0;
+        cosi2 = 1.0;
+      }
+      *sint = 0.0;
+      sini2 = 0.0;
+    }
+    sin2i2 = 2.0 * sini2 * cosi2;
+    cos2i2 = 2.0 * (cosi2 * cosi2) - 1.0;
+    sin2 = sin2i2 * sin2i3;
+    cos2 = cos2i2 * cos2i3;
+    sin2cos1 = sin2i2 * cos2i3;
+    cos2sin1 = cos2i2 * sin2i3;
+    a21 = p2 * cos2i2;
+    a22 = p1 * cos2 - p3 * sin2;
+    a23 = p1 * cos2sin1 + p3 * sin2cos1;
+    a24 = -p4 * sin2i2;
+    a31 = -p2 * sin2i2;
+    a32 = -p1 * sin2cos1 - p3 * cos2sin1;
+    a33 = -p1 * sin2 + p3 * cos2;
+    a34 = -p4 * cos2i2;
+    a42 = -p4 * sin2i3;
+    a43 = p4 * cos2i3;
+    a44 = p3;
+  } else {
+    cosi1 = cos(ri1);
+    sini1 = sin(ri1);
+    sin2i1 = 2.0 * sini1 * cosi1;
+    cos2i1 = 2.0 * (cosi1 * cosi1) - 1.0;
+    a11 = p1;
+    a12 = p2 * cos2i1;
+    a13 = -p2 * sin2i1;
+    if (bmu == 1.0) return;
+    if (bmu == -1.0) {
+      sini1 = 1.0;
+      sini2 = 1.0;
+      cosi1 = 0.0;
+      cosi2 = 0.0;
+    } else if (fabs(*cost) < 1.0) {
+      sini2 = sini1 * sintp / *sint;
+      bott = *sint * sinbt;
+      cosi2 = costp / bott - *cost * bmu / bott;
+      if (cosi2 >= 1.0) {
+        cosi2 = 1.0;
+        sini2 = 0.0;
+      }
+      if (cosi2 <= -1.0) {
+        cosi2 = -1.0;
+        sini2 = 0.0;
+      }
+    } else {
+      if (*cost >= 1.0) {
+        *cost = 1.0;
+        cosi2 = -1.0;
+      }
+      if (*cost <= -1.0) {
+        *cost = -1.0;
+        cosi2 = 1.0;
+      }
+      *sint = 0.0;
+      sini2 = 0.0;
+    }
+    sin2i2 = 2.0 * sini2 * cosi2;
+    cos2i2 = 2.0 * (cosi2 * cosi2) - 1.0;
+    sin2 = sin2i2 * sin2i1;
+    cos2 = cos2i2 * cos2i1;
+    sin2cos1 = sin2i2 * cos2i1;
+    cos2sin1 = cos2i2 * sin2i1;
+    a21 = p2 * cos2i2;
+    a22 = p1 * cos2 - p3 * sin2;
+    a23 = -p1 * cos2sin1 - p3 * sin2cos1;
+    a24 = p4 * sin2i2;
+    a31 = p2 * sin2i2;
+    a32 = p1 * sin2cos1 + p3 * cos2sin1;
+    a33 = -p1 * sin2 + p3 * cos2;
+    a34 = -p4 * cos2i2;
+    a42 = p4 * sin2i1;
+    a43 = p4 * cos2i1;
+    a44 = p3;
+  }
+  *I = (a11 * sip + a12 * sqp + a13 * sup);
+  *Q = (a21 * sip + a22 * sqp + a23 * sup + a24 * svp);
+  *U = (a31 * sip + a32 * sqp + a33 * sup + a34 * svp);
+  *V = (a42 * sqp + a43 * sup + a44 * svp);
+}
+
+static double clamp1(double x) {  /* check.f90:1-30 */
+  if (x > 1.0) return 1.0;
+  if (x < -1.0) return -1.0;
+  return x;
+}
+
+/* findangle3.f90:1-70.  ri1 is left at 0 in the near-pole branches where the
+ * reference leaves it unset. */
+void hco_findangle3(double cost1, double sint1, double phi1, double cosp1, double sinp1,
+                    double cost2, double sint2, double phi2, double cosp2, double sinp2,
+                    double *costnew, double *sintnew, double *phinew, double *ri1) {
+  double cosx, x;
+  *ri1 = 0.0;
+  *costnew = cost1 * cost2 + (sint1 * sint2 * (cosp1 * cosp2 + sinp1 * sinp2));
+  *costnew = clamp1(*costnew);
+  *sintnew = sqrt(1.0 - *costnew * *costnew);
+  if (sint1 > 1.e-8) {
+    if (*sintnew > 1.e-8) {
+      cosx = (cost2 - (cost1 * *costnew)) / (sint1 * *sintnew);
+      cosx = clamp1(cosx);
+      x = acos(cosx);
+      if (phi2 > phi1) {
+        if (phi2 - phi1 > PI) {
+          *ri1 = x;
+          *phinew = PI + x;
+        } else {
+          *phinew = PI - x;
+          *ri1 = R2P - x;
+        }
+      } else {
+        if (phi1 - phi2 > PI) {
+          *phinew = PI - x;
+          *ri1 = R2P - x;
+        } else {
+          *phinew = PI + x;
+          *ri1 = x;
+        }
+      }
+    } else {
+      *phinew = 0.0;
+    }
+  } else {
+    *phinew = phi2;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* packet transport                                                          */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  double xp, yp, zp, ux, uy, uz;
+  double cost, sint, cosp, sinp, phi;
+  double sip, sqp, sup, svp, nub;
+  double rsq, rtot;
+  int ir, it, ip, on_wall;
+  int iflag, i_orig;
+  double kapd[HCO_NDG];
+  hco_opac op;
+} pkt_t;
+
+typedef struct {
+  const hco_grid *g;
+  const hco_dust *d;
+  const hco_setup *s;
+  hco_tallies *t;
+  hco_rng rng;
+  hco_trace *tr;
+  int ipeel;
+  double rmincgs;
+  double tsub_ev;
+  size_t ncell;
+  int nact;               /* species with any density */
+  int act[HCO_NDG];
+} ctx_t;
+
+static inline size_t cell_index(const hco_grid *g, int ir, int it, int ip) {
+  return (size_t)(ir - 1) + (size_t)g->nrg * ((size_t)(it - 1) + (size_t)g->ntg * (size_t)(ip - 1));
+}
+
+static inline void hash_cell(hco_trace *tr, int ir, int it, int ip) {
+  if (!tr) return;
+  tr->cellhash ^= (uint64_t)(uint32_t)ir | ((uint64_t)(uint32_t)it << 20) |
+                  ((uint64_t)(uint32_t)ip << 40);
+  tr->cellhash *= 1099511628211ull;
+}
+
+/* Masked opacity of a cell (peeloff_3d.f90:103-110). */
+static inline double cell_opac(const ctx_t *c, const pkt_t *P, size_t cell) {
+  double opac = 0.0;
+  int id;
+  for (id = 0; id < HCO_NDG; id++) {
+    size_t k = cell + c->ncell * id;
+    if (c->g->tdust[k] < c->tsub_ev) opac = opac + P->kapd[id] * c->g->densarr[k];
+  }
+  return opac;
+}
+
+/* Walls of the current cell and the next one along (ux,uy,uz)
+ * (peeloff_3d.f90:92-103). */
+static inline int next_wall(const ctx_t *c, double ux, double uy, double uz, double x, double y,
+                            double z, int ir, int it, int ip, int on_wall, double *dt, int *face) {
+  const hco_grid *g = c->g;
+  hco_walls w;
+  hco_reset_t(&w);
+  hco_radface(&w, ux, uy, uz, x, y, z, g->r2arr[ir - 1], g->r2arr[ir]);
+  if (g->ntg > 2) hco_thetaface(&w, ux, uy, uz, x, y, z, g->tan2arr[it - 1], g->tan2arr[it]);
+  if (g->npg > 2 && g->ntg > 2)
+    hco_phiface(&w, g->aarr[ip - 1], g->barr[ip - 1], g->aarr[ip], g->barr[ip], ux, uy, x, y);
+  return hco_find_next_wall(&w, on_wall, dt, face);
+}
+
+static inline void face_step(const hco_grid *g, int face, int *ir, int *it, int *ip) {
+  switch (face) {  /* peeloff_3d.f90:120-139 */
+    case 1: *ir -= 1; break;
+    case 2: *ir += 1; break;
+    case 3: *it -= 1; break;
+    case 4: *it += 1; break;
+    case 5: *ip -= 1; break;
+    default: *ip += 1; break;
+  }
+  if (*ip == 0) *ip = g->npg - 1;
+  if (*ip == g->npg) *ip = 1;
+}
+
+enum { ST_INTERACT = 0, ST_EXIT = 1, ST_STAR = 2, ST_ERROR = 3 };
+
+/* tauint: source absent (Makefile.in lists tauint_3d.o; called at
+ * propagate.f90:50,185).  Reconstructed on the peel-off walk of
+ * peeloff_3d.f90:74-154: walk cells from (xp,yp,zp) along (ux,uy,uz) summing
+ * the masked opacity times the chord until the sum reaches tau.
+ *  - Interaction inside a cell: advance by (tau-tsum)/opac, on_wall=false.
+ *  - ir reaches nrg: escape (exitflag).  ir reaches 0: the packet hit the
+ *    star (the caller re-emits it from the stellar surface).
+ *  - Lucy tallies per traversed chord ds (stellar radii):
+ *    dtauabs(cell,id) += kapd(id)*(1-albedo(id))*rho(cell,id)*ds for the
+ *    unmasked species, jmean(cell) += ds*R*(cm).
+ *  - The radial on_wall re-orientation at entry is the one of
+ *    peeloff_3d.f90:58-72. */
+static int tauint(ctx_t *c, pkt_t *P, double tau) {
+  const hco_grid *g = c->g;
+  hco_tallies *t = c->t;
+  double tsum = 0.0, dt, opac, dtau;
+  int face;
+  if (P->on_wall) {
+    int inner = fabs(P->rsq - g->r2arr[P->ir - 1]) < fabs(P->rsq - g->r2arr[P->ir]);
+    int outwards = P->ux * P->xp + P->uy * P->yp + P->uz * P->zp > 0.0;
+    if (inner && !outwards) {
+      P->ir -= 1;
+      if (P->ir == 0) return ST_STAR;
+    }
+    if (!inner && outwards) P->ir += 1;
+    if (P->ir == g->nrg) return ST_EXIT;
+  }
+  for (;;) {
+    size_t cell = cell_index(g, P->ir, P->it, P->ip);
+    int id;
+    if (!next_wall(c, P->ux, P->uy, P->uz, P->xp, P->yp, P->zp, P->ir, P->it, P->ip, P->on_wall,
+                   &dt, &face)) {
+      t->n_err_nowall++;
+      return ST_ERROR;
+    }
+    opac = cell_opac(c, P, cell);
+    dtau = opac * dt;
+    if (tsum + dtau >= tau) {
+      double ds = (tau - tsum) / opac;
+      if (t->dtauabs)
+        for (id = 0; id < HCO_NDG; id++) {
+          size_t k = cell + c->ncell * id;
+          if (g->tdust[k] < c->tsub_ev)
+            t->dtauabs[k] += P->kapd[id] * (1.0 - P->op.albedo[id]) * g->densarr[k] * ds;
+        }
+      if (t->jmean) t->jmean[cell] += ds * c->rmincgs;
+      P->xp = P->xp + ds * P->ux;
+      P->yp = P->yp + ds * P->uy;
+      P->zp = P->zp + ds * P->uz;
+      P->on_wall = 0;
+      P->rsq = P->xp * P->xp + P->yp * P->yp + P->zp * P->zp;
+      P->rtot = sqrt(P->rsq);
+      if (c->tr) c->tr->path += ds;
+      return ST_INTERACT;
+    }
+    if (t->dtauabs)
+      for (id = 0; id < HCO_NDG; id++) {
+        size_t k = cell + c->ncell * id;
+        if (g->tdust[k] < c->tsub_ev)
+          t->dtauabs[k] += P->kapd[id] * (1.0 - P->op.albedo[id]) * g->densarr[k] * dt;
+      }
+    if (t->jmean) t->jmean[cell] += dt * c->rmincgs;
+    tsum = tsum + dtau;
+    face_step(g, face, &P->ir, &P->it, &P->ip);
+    P->xp = P->xp + dt * P->ux;
+    P->yp = P->yp + dt * P->uy;
+    P->zp = P->zp + dt * P->uz;
+    P->on_wall = 1;
+    P->rsq = P->xp * P->xp + P->yp * P->yp + P->zp * P->zp;
+    P->rtot = sqrt(P->rsq);
+    if (c->tr) {
+      c->tr->path += dt;
+      c->tr->ncross++;
+      hash_cell(c->tr, P->ir, P->it, P->ip);
+    }
+    if (P->ir == g->nrg) return ST_EXIT;
+    if (P->ir == 0) return ST_STAR;
+  }
+}
+
+static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* peeloff_3d.f90:5-288 for one observer direction. */
+static void peeloff(ctx_t *c, const pkt_t *P0, double sip, int idust2, int peelid) {
+  const hco_grid *g = c->g;
+  const hco_setup *s = c->s;
+  hco_tallies *t = c->t;
+  double sqp = P0->sqp, sup = P0->sup, svp = P0->svp;
+  double xp1 = P0->xp, yp1 = P0->yp, zp1 = P0->zp, rsq1 = P0->rsq;
+  int ir = P0->ir, it = P0->it, ip = P0->ip, on_wall = P0->on_wall;
+  double cospe = s->cospe[peelid], sinpe = s->sinpe[peelid];
+  double coste = s->coste[peelid], sinte = s->sinte[peelid];
+  double ux1 = sinte * cospe, uy1 = sinte * sinpe, uz1 = coste;
+  double tsum = 0.0, dt, opac, factor, ximage, yimage, rho2;
+  int face, ix, iy, inub, ia, i_inter, k, indx[3];
+  if (on_wall) {
+    int inner = fabs(rsq1 - g->r2arr[ir - 1]) < fabs(rsq1 - g->r2arr[ir]);
+    int outwards = ux1 * xp1 + uy1 * yp1 + uz1 * zp1 > 0.0;
+    if (inner && !outwards) {
+      if (ir == 1) return;
+      ir = ir - 1;
+    }
+    if (!inner && outwards) ir = ir + 1;
+  }
+  while (ir < g->nrg) {
+    if (!next_wall(c, ux1, uy1, uz1, xp1, yp1, zp1, ir, it, ip, on_wall, &dt, &face)) {
+      t->n_err_nowall++;
+      return;
+    }
+    opac = cell_opac(c, P0, cell_index(g, ir, it, ip));
+    tsum = tsum + opac * dt;
+    if (tsum > HCO_TAUMAX) {
+      if (c->tr) c->tr->tau_peel += tsum;
+      return;
+    }
+    face_step(g, face, &ir, &it, &ip);
+    xp1 = xp1 + dt * ux1;
+    yp1 = yp1 + dt * uy1;
+    zp1 = zp1 + dt * uz1;
+    on_wall = 1;
+    if (c->tr) c->tr->npeelcross++;
+    if (ir == g->nrg) break;
+    if (ir == 0) {
+      if (c->tr) c->tr->tau_peel += tsum;
+      return;
+    }
+  }
+  if (c->tr) c->tr->tau_peel += tsum;
+  factor = exp(-tsum);
+  sip = sip * factor;
+  sqp = sqp * factor;
+  sup = sup * factor;
+  svp = svp * factor;
+  if (P0->iflag == 1) {
+    double bmu, sinbm, phinew, ri1;
+    /* phie = 0 (reapar.f90:494) is what peeloff_3d.f90:168 passes as the
+     * observer azimuth whatever PHIE holds. */
+    hco_findangle3(P0->cost, P0->sint, P0->phi, P0->cosp, P0->sinp, coste, sinte, 0.0, cospe,
+                   sinpe, &bmu, &sinbm, &phinew, &ri1);
+    hco_stokespeel(&sip, &sqp, &sup, &svp, &coste, &sinte, ri1, bmu, P0->cost, P0->sint, &P0->op,
+                   idust2);
+  }
+  yimage = zp1 * sinte - yp1 * coste * sinpe - xp1 * coste * cospe;
+  ximage = yp1 * cospe - xp1 * sinpe;
+  ix = (int)((ximage + s->rmaxi) / (2.0 * s->rmaxi / (double)s->nxhst)) + 1;
+  iy = (int)((yimage + s->rmaxi) / (2.0 * s->rmaxi / (double)s->nxhst)) + 1;
+  rho2 = ximage * ximage + yimage * yimage;
+  inub = clampi((int)((double)s->nfreq * log(P0->nub / s->numin) / s->lnurat) + 1, 1, s->nfreq);
+  if (P0->iflag == 1)
+    i_inter = 2;
+  else
+    i_inter = (P0->i_orig == 1) ? 1 : 3;
+  indx[0] = 1;
+  indx[1] = 1 + P0->i_orig;
+  indx[2] = 5 + i_inter;
+  if (t->peel) {
+    const size_t n = (size_t)s->nfreq * s->npeel * s->nap * HCO_NO;
+    for (ia = 0; ia < s->nap; ia++) {
+      if (rho2 < s->aperture2[ia]) {
+        for (k = 0; k < 3; k++) {
+          size_t q = (size_t)(inub - 1) +
+                     (size_t)s->nfreq * ((size_t)peelid + (size_t)s->npeel *
+                                         ((size_t)ia + (size_t)s->nap * (size_t)(indx[k] - 1)));
+          t->peel[q] += sip;
+          t->peel[q + n] += sqp;
+          t->peel[q + 2 * n] += sup;
+          t->peel[q + 3 * n] += svp;
+          t->peel[q + 4 * n] += sip * sip;
+          t->peel[q + 5 * n] += sqp * sqp;
+          t->peel[q + 6 * n] += sup * sup;
+          t->peel[q + 7 * n] += svp * svp;
+          t->peel[q + 8 * n] += 1.0;
+        }
+      }
+    }
+  }
+  if (s->imcube && t->img_i && ix <= s->nxhst && iy <= s->nxhst && ix > 0 && iy > 0) {
+    const size_t nx = (size_t)s->nxhst;
+    size_t q = (size_t)peelid + (size_t)s->npeel * ((size_t)(ix - 1) + nx * ((size_t)(iy - 1) +
+                                                     nx * (size_t)(inub - 1)));
+    const size_t n4 = (size_t)s->npeel * nx * nx * s->nfreq;
+    for (k = 0; k < 3; k++) t->img_i[q + n4 * (size_t)(indx[k] - 1)] += sip;
+    t->img_q[q] += sqp;
+    t->img_u[q] += sup;
+    t->img_v[q] += svp;
+  }
+}
+
+static void peel_all(ctx_t *c, const pkt_t *P, double sip, int idust2) {
+  int k;
+  for (k = 0; k < c->s->npeel; k++) peeloff(c, P, sip, idust2, k);
+}
+
+static void set_kapd_launch(ctx_t *c, pkt_t *P) {  /* newdisktherm.F90:310-312 */
+  int id;
+  for (id = 0; id < HCO_NDG; id++) P->kapd[id] = P->op.kappa[id] * c->rmincgs * c->d->kappav[id];
+}
+
+static void set_kapd_reemit(ctx_t *c, pkt_t *P) {  /* propagate.f90:149-151 */
+  int id;
+  for (id = 0; id < HCO_NDG; id++)
+    P->kapd[id] = P->op.kappa[id] * c->s->rstar * RSOL * c->d->kappav[id];
+}
+
+/* initp_sample with limb=0 (initp_common.f90:1-29). */
+static void initp_sample(ctx_t *c, pkt_t *P) {
+  double xran = hco_ran(&c->rng);
+  P->cost = sqrt(xran);
+  P->sint = sqrt(1.0 - P->cost * P->cost);
+  xran = hco_ran(&c->rng);
+  P->phi = R2P * xran;
+  P->cosp = cos(P->phi);
+  P->sinp = sin(P->phi);
+}
+
+/* initp_transform (initp_common.f90:31-94), including its clamp of costi
+ * guarded by cost < -1 as written. */
+static void initp_transform(pkt_t *P, double cosb, double sinb, double lp) {
+  if (fabs(cosb) < 0.9999999) {
+    double costi, sinti, cospn, phinew;
+    costi = P->cost * cosb - P->sint * sinb * P->cosp;
+    if (fabs(costi) > 1.0) {
+      if (costi > 1.0) costi = 1.0;
+      if (P->cost < -1.0) costi = -1.0;
+    }
+    sinti = sqrt(1.0 - costi * costi);
+    if (sinti > 0.0000001) {
+      cospn = (P->cost - costi * cosb) / (sinti * sinb);
+      if (fabs(cospn) > 1.0) cospn = (cospn < 0.0) ? -1.0 : 1.0;
+      if (P->phi < PI) {
+        phinew = acos(cospn);
+        if (phinew > PI) phinew = PI;
+      } else {
+        phinew = R2P - acos(cospn);
+        if (phinew < PI) phinew = PI;
+      }
+    } else {
+      phinew = 0.0;
+    }
+    P->phi = phinew;
+    P->cost = costi;
+    P->sint = sinti;
+  } else {
+    if (cosb < 0.0) P->cost = -P->cost;
+  }
+  P->phi = P->phi + lp;
+  if (P->phi > R2P) P->phi = P->phi - R2P;
+  if (P->phi < 0.0) P->phi = P->phi + R2P;
+  P->sinp = sin(P->phi);
+  P->cosp = cos(P->phi);
+}
+
+static void bin_init(ctx_t *c, double nub, int i_orig_init) {
+  const hco_setup *s = c->s;
+  int inub;
+  if (!c->t->si_init) return;
+  inub = clampi((int)((double)s->nfreq * log(nub / s->numin) / s->lnurat) + 1, 1, s->nfreq);
+  c->t->si_init[(size_t)(inub - 1) + (size_t)s->nfreq * (size_t)(i_orig_init - 1)] += 1.0;
+  c->t->si_init[(size_t)(inub - 1)] += 1.0;
+}
+
+/* Direct stellar peel (newdisktherm.F90:342-377). */
+static void peel_direct(ctx_t *c, pkt_t *P, double cosb, double sinb, double coslp,
+                        double sinlp) {
+  const hco_setup *s = c->s;
+  int k;
+  for (k = 0; k < s->npeel; k++) {
+    if (!s->ips) {
+      double cosnorm = cosb * s->coste[k] +
+                       (sinb * s->sinte[k] * (s->cospe[k] * coslp + s->sinpe[k] * sinlp));
+      double sipnew = 4.0 * P->sip * cosnorm;
+      if (cosnorm > 0.0) peeloff(c, P, sipnew, 0, k);
+    } else {
+      peeloff(c, P, P->sip * 2.0, 0, k);
+    }
+  }
+}
+
+/* Star / hot-spot launch: initpspot.f90:3-175 (or initp_ps.f90:3-79 for the
+ * point-source mode), then newdisktherm.F90:256-337.  hotspot selects the
+ * hot-spot branch of the photon loop (newdisktherm.F90:256-273): the packet
+ * starts inside the spot when ispot=1, is emitted at tshock and half of such
+ * packets are X-rays.  Stellar packets take the tstar blackbody. */
+static void launch_star(ctx_t *c, pkt_t *P, int hotspot) {
+  const hco_setup *s = c->s;
+  const hco_grid *g = c->g;
+  double cosb, sinb, lp, coslp, sinlp, rp, xran;
+  int ispotph = hotspot;
+  P->sip = 1.0;
+  P->sqp = P->sup = P->svp = 0.0;
+  if (!s->ips) {
+    for (;;) {
+      for (;;) {
+        xran = hco_ran(&c->rng);
+        cosb = 1.0 - 2.0 * xran;
+        sinb = sqrt(1.0 - cosb * cosb);
+        if (cosb != 0.0 && fabs(cosb) < 1.0) break;
+      }
+      xran = hco_ran(&c->rng);
+      lp = R2P * xran;
+      if (s->ispot == 1 && ispotph == 1) {
+        double cphi = cos(lp), sphi = sin(lp);
+        double dotsp1 = s->scspot1 * sinb * cphi + s->ssspot1 * sinb * sphi + s->cspot1 * cosb;
+        double dotsp2 = s->scspot2 * sinb * cphi + s->ssspot2 * sinb * sphi + s->cspot2 * cosb;
+        if (s->nspot == 1) {
+          if (dotsp1 < s->csprad) continue;
+        } else {
+          if (dotsp1 < s->csprad && dotsp2 < s->csprad) continue;
+        }
+      }
+      break;
+    }
+    if (s->ispot == 1 && s->fspot < 1.0) {
+      double dellon = (lp < PI) ? lp : R2P - lp;
+      double costhp = s->cspot1 * cosb + s->sspot1 * sinb * cos(dellon);
+      if (costhp > s->csprad) ispotph = 1;
+      if (s->nspot == 2) {
+        dellon = (lp < PI) ? PI - lp : lp - PI;
+        costhp = s->cspot2 * cosb + s->sspot2 * sinb * cos(dellon);
+        if (costhp > s->csprad) ispotph = 1;
+      }
+    }
+    initp_sample(c, P);
+    initp_transform(P, cosb, sinb, lp);
+  } else {
+    for (;;) {
+      xran = hco_ran(&c->rng);
+      cosb = clamp1(1.0 - 2.0 * xran);
+      sinb = clamp1(sqrt(1.0 - cosb * cosb));
+      if (cosb != 0.0) break;
+    }
+    P->cost = cosb;
+    P->sint = sinb;
+    xran = hco_ran(&c->rng);
+    lp = R2P * xran;
+    P->phi = lp;
+    P->sinp = clamp1(sin(P->phi));
+    P->cosp = clamp1(cos(P->phi));
+  }
+  P->nub = hco_bbfreq(&c->rng, ispotph ? s->tshock : s->tstar);
+  if (hotspot) {
+    if (hco_ran(&c->rng) < 0.5)
+      P->nub = 1.2398 / 0.06 + hco_ran(&c->rng) * (1.2398 / 0.015 - 1.2398 / 0.06);
+    bin_init(c, P->nub, 3);
+  } else {
+    bin_init(c, P->nub, 2);
+  }
+  P->i_orig = 1;
+  hco_opacset(c->d, P->nub, &P->op);
+  set_kapd_launch(c, P);
+  coslp = cos(lp);
+  sinlp = sin(lp);
+  P->zp = cosb * 1.0;
+  rp = sinb * 1.0;
+  P->xp = rp * coslp;
+  P->yp = rp * sinlp;
+  P->rsq = P->zp * P->zp + rp * rp;
+  P->rtot = sqrt(P->rsq);
+  P->ux = P->sint * P->cosp;
+  P->uy = P->sint * P->sinp;
+  P->uz = P->cost;
+  P->iflag = 0;
+  P->ir = 1;
+  P->on_wall = 1;
+  P->it = hco_locate(g->thetarr, g->ntg, acos(cosb));
+  P->ip = hco_locate(g->phiarr, g->npg, lp);
+  if (c->ipeel) peel_direct(c, P, cosb, sinb, coslp, sinlp);
+}
+
+/* A packet that reaches the stellar surface is absorbed and re-emitted by
+ * the star (output.F90:378 "flux which gets absorbed (and reemitted) by
+ * star").  Reconstruction: re-launch from the impact point with the stellar
+ * launch's direction and spectrum sampling, counted in aflux. */
+static void star_reemit(ctx_t *c, pkt_t *P) {
+  double cosb, sinb, lp;
+  c->t->aflux += 1.0;
+  cosb = clamp1(P->zp / P->rtot);
+  sinb = sqrt(1.0 - cosb * cosb);
+  lp = atan2(P->yp, P->xp);
+  if (lp < 0.0) lp = lp + R2P;
+  P->sip = 1.0;
+  P->sqp = P->sup = P->svp = 0.0;
+  initp_sample(c, P);
+  initp_transform(P, cosb, sinb, lp);
+  P->nub = hco_bbfreq(&c->rng, c->s->tstar);
+  P->i_orig = 1;
+  P->iflag = 0;
+  hco_opacset(c->d, P->nub, &P->op);
+  set_kapd_launch(c, P);
+  P->ux = P->sint * P->cosp;
+  P->uy = P->sint * P->sinp;
+  P->uz = P->cost;
+  P->ir = 1;
+  P->on_wall = 1;
+  if (c->ipeel) peel_direct(c, P, cosb, sinb, cos(lp), sin(lp));
+  if (c->tr) c->tr->nstar++;
+}
+
+/* select_dust.f90:1-67 with mask is_any; 0 when the cell has no opacity. */
+static int select_dust(ctx_t *c, const pkt_t *P, size_t cell) {
+  double xran = hco_ran(&c->rng), opac = 0.0, frac = 0.0;
+  int id;
+  for (id = 0; id < HCO_NDG; id++) opac = opac + P->kapd[id] * c->g->densarr[cell + c->ncell * id];
+  if (opac == 0.0) return 0;
+  for (id = 0; id < HCO_NDG; id++) {
+    frac = frac + P->kapd[id] * c->g->densarr[cell + c->ncell * id] / opac;
+    if (xran <= frac + (double)1.e-10f) return id + 1;
+  }
+  return 0;
+}
+
+static void bin_exit(ctx_t *c, pkt_t *P, double xpold, double ypold, double zpold, int iscat) {
+  const hco_setup *s = c->s;
+  hco_tallies *t = c->t;
+  double x, y, rho2, sini;
+  int it, ip, inub, i_inter, ia, k, indx[3];
+  it = (int)((double)s->nmu * (1.0 - P->cost) / 2.0) + 1;
+  it = clampi(it, 1, s->nmu);
+  if (P->phi < 0.0) P->phi = P->phi + R2P;
+  ip = clampi((int)((double)s->nph * P->phi / R2P) + 1, 1, s->nph);
+  sini = sqrt(1.0 - s->u[it - 1] * s->u[it - 1]);
+  x = ypold * P->cosp - xpold * P->sinp;
+  y = zpold * sini - ypold * s->u[it - 1] * P->sinp - xpold * s->u[it - 1] * P->cosp;
+  rho2 = (x * x + y * y);
+  if (rho2 < s->aperture2[s->nap - 1] * 1.0001) {
+    t->flux = t->flux + 1.0;
+    if (P->iflag == 1) {
+      t->sflux = t->sflux + 1.0;
+      t->nscat = t->nscat + (double)iscat;
+    }
+  }
+  inub = clampi((int)((double)s->nfreq * log(P->nub / s->numin) / s->lnurat) + 1, 1, s->nfreq);
+  if (P->iflag == 1)
+    i_inter = 2;
+  else
+    i_inter = (P->i_orig == 1) ? 1 : 3;
+  indx[0] = 1;
+  indx[1] = 1 + P->i_orig;
+  indx[2] = 5 + i_inter;
+  if (t->sed) {
+    const size_t n = (size_t)s->nfreq * s->nmu * s->nph * s->nap * HCO_NO;
+    for (ia = 0; ia < s->nap; ia++) {
+      if (rho2 < s->aperture2[ia] && !(P->iflag == 0 && P->i_orig == 4)) {
+        for (k = 0; k < 3; k++) {
+          size_t q = (size_t)(inub - 1) +
+                     (size_t)s->nfreq *
+                         ((size_t)(it - 1) +
+                          (size_t)s->nmu * ((size_t)(ip - 1) +
+                                            (size_t)s->nph * ((size_t)ia + (size_t)s->nap *
+                                                                             (size_t)(indx[k] - 1))));
+          t->sed[q] += P->sip;
+          t->sed[q + n] += P->sqp;
+          t->sed[q + 2 * n] += P->sup;
+          t->sed[q + 3 * n] += P->svp;
+          t->sed[q + 4 * n] += P->sip * P->sip;
+          t->sed[q + 5 * n] += P->sqp * P->sqp;
+          t->sed[q + 6 * n] += P->sup * P->sup;
+          t->sed[q + 7 * n] += P->svp * P->svp;
+          t->sed[q + 8 * n] += 1.0;
+        }
+      }
+    }
+  }
+  if (!(P->iflag == 0 && P->i_orig == 4)) t->sumsub = t->sumsub + P->sip;
+  t->sumall = t->sumall + P->sip;
+  if (P->iflag == 1) t->scount = t->scount + 1.0;
+}
+
+static void finish_trace(ctx_t *c, const pkt_t *P, int status) {
+  hco_trace *tr = c->tr;
+  if (!tr) return;
+  tr->x = P->xp;
+  tr->y = P->yp;
+  tr->z = P->zp;
+  tr->cost = P->cost;
+  tr->sint = P->sint;
+  tr->phi = P->phi;
+  tr->I = P->sip;
+  tr->Q = P->sqp;
+  tr->U = P->sup;
+  tr->V = P->svp;
+  tr->nub = P->nub;
+  tr->status = status;
+  tr->ir = P->ir;
+  tr->it = P->it;
+  tr->ip = P->ip;
+}
+
+/* propagate.f90:1-325: the event loop of one packet. */
+static int propagate(ctx_t *c, pkt_t *P) {
+  const hco_setup *s = c->s;
+  const hco_grid *g = c->g;
+  hco_tallies *t = c->t;
+  double xpold = P->xp, ypold = P->yp, zpold = P->zp, tau;
+  int iscat = 0, iint = 0, idust2 = 0, st, first = 1;
+  int pir = 0, pit = 0, pip = 0;
+  tau = -log(hco_ran(&c->rng));
+  st = tauint(c, P, tau);
+  for (;;) {
+    while (st == ST_STAR) {
+      star_reemit(c, P);
+      tau = -log(hco_ran(&c->rng));
+      xpold = P->xp;
+      ypold = P->yp;
+      zpold = P->zp;
+      st = tauint(c, P, tau);
+    }
+    if (st == ST_ERROR) return HCO_ERROR;
+    if (st == ST_EXIT) break;
+    if (!first) {
+      if (P->ir != pir || P->it != pit || P->ip != pip) iint = 0;
+      if (iint > s->iintmax) {
+        size_t cell = cell_index(g, P->ir, P->it, P->ip);
+        if (t->killed) t->killed[cell] += 1;
+        t->abflux = t->abflux + 1.0;
+        return HCO_KILLED;
+      }
+    }
+    {
+      size_t cell = cell_index(g, P->ir, P->it, P->ip);
+      idust2 = select_dust(c, P, cell);
+      if (idust2 == 0) {
+        t->n_err_dust++;
+        return HCO_ERROR;
+      }
+      if (first) t->tot = t->tot + 1.0;
+      first = 0;
+      if (hco_ran(&c->rng) <= P->op.albedo[idust2 - 1]) {
+        hco_scat sc;
+        P->iflag = 1;
+        iscat++;
+        iint++;
+        if (c->ipeel) peel_all(c, P, P->sip, idust2 - 1);
+        sc.cost = P->cost;
+        sc.sint = P->sint;
+        sc.cosp = P->cosp;
+        sc.sinp = P->sinp;
+        sc.phi = P->phi;
+        sc.I = P->sip;
+        sc.Q = P->sqp;
+        sc.U = P->sup;
+        sc.V = P->svp;
+        sc.hit = t->hit;
+        sc.htot = t->htot;
+        hco_stokes(&sc, &P->op, idust2 - 1, &c->rng);
+        t->hit = sc.hit;
+        t->htot = sc.htot;
+        P->cost = sc.cost;
+        P->sint = sc.sint;
+        P->cosp = sc.cosp;
+        P->sinp = sc.sinp;
+        P->phi = sc.phi;
+        P->sip = sc.I;
+        P->sqp = sc.Q;
+        P->sup = sc.U;
+        P->svp = sc.V;
+        if (c->tr) c->tr->nscat++;
+      } else {
+        size_t k = cell + c->ncell * (size_t)(idust2 - 1);
+        if (t->dbig) t->dbig[cell] += 1;
+        P->i_orig = (idust2 == 1 || idust2 == 2 || idust2 == 5 || idust2 == 6) ? 2 : 3;
+        if (t->nabs) t->nabs[k] += 1.0;
+        /* emit_common.f90:57-70 (Lucy branch) -> reemit.f90:15-33 */
+        P->sip = 1.0;
+        P->sqp = P->sup = P->svp = 0.0;
+        P->cost = 1.0 - 2.0 * hco_ran(&c->rng);
+        P->sint = sqrt(1.0 - pow(P->cost, 2.0));
+        P->phi = R2P * hco_ran(&c->rng);
+        P->cosp = cos(P->phi);
+        P->sinp = sin(P->phi);
+        P->nub = hco_lucydustfreq(c->d, &c->rng, g->tdust[k], idust2 - 1);
+        P->uy = P->sint * P->sinp;
+        P->ux = P->sint * P->cosp;
+        P->uz = P->cost;
+        P->iflag = 0;
+        hco_opacset(c->d, P->nub, &P->op);
+        set_kapd_reemit(c, P);
+        if (c->ipeel) peel_all(c, P, P->sip, idust2 - 1);
+        iint++;
+        if (c->tr) c->tr->nabs++;
+      }
+    }
+    tau = -log(hco_ran(&c->rng));
+    xpold = P->xp;
+    ypold = P->yp;
+    zpold = P->zp;
+    P->ux = P->sint * P->cosp;
+    P->uy = P->sint * P->sinp;
+    P->uz = P->cost;
+    pir = P->ir;
+    pit = P->it;
+    pip = P->ip;
+    st = tauint(c, P, tau);
+  }
+  bin_exit(c, P, xpold, ypold, zpold, iscat);
+  return HCO_ESCAPED;
+}
+
+int hco_run_pass(const hco_grid *g, const hco_dust *d, const hco_setup *s, hco_tallies *t,
+                 int64_t np, int64_t p_begin, int64_t p_end, int ipeel, int32_t seed1,
+                 int32_t seed2, int gen, uint64_t seg0, hco_trace *trace) {
+  ctx_t c;
+  int64_t p, ns, npacc, npaccstar;
+  double rn;
+  memset(&c, 0, sizeof c);
+  c.g = g;
+  c.d = d;
+  c.s = s;
+  c.t = t;
+  c.ipeel = ipeel;
+  c.rmincgs = s->rstar * RSOL;
+  c.tsub_ev = g->tsub / 11605.0;
+  c.ncell = (size_t)g->nrg * g->ntg * g->npg;
+  /* source split, newdisktherm.F90:143-176 (no external illumination) */
+  ns = (int64_t)((double)np * (1.0 - s->accfrac));
+  rn = (double)(float)np * (1.0 - s->accfrac);
+  if (rn - (double)ns > 0.5) ns = ns + 1;
+  npacc = (int64_t)((double)np * s->accfrac);
+  npaccstar = (int64_t)(s->accfrac2 * (double)np);
+  if (ns + npacc != np) ns = ns + np - (ns + npacc);
+  if (npacc > 1) return -1; /* disk-accretion launch is not restated */
+  for (p = p_begin; p < p_end; p++) {
+    pkt_t P;
+    int64_t iphot = p + 1;
+    int status;
+    memset(&P, 0, sizeof P);
+    hco_rng_packet(&c.rng, seed1, seed2, gen, seg0 + (uint64_t)p);
+    c.tr = trace ? &trace[p - p_begin] : NULL;
+    if (c.tr) {
+      memset(c.tr, 0, sizeof *c.tr);
+      c.tr->cellhash = 14695981039346656037ull;
+    }
+    launch_star(&c, &P, iphot < npacc + npaccstar ? 1 : 0);
+    hash_cell(c.tr, P.ir, P.it, P.ip);
+    status = propagate(&c, &P);
+    finish_trace(&c, &P, status);
+  }
+  return 0;
+}

@@ -1,0 +1,170 @@
+/*
+ * hc_oracle.h -- FP64 CPU restatement of Hochunk3d's photon-packet transport
+ * path.  TEST INFRASTRUCTURE ONLY: the parity tests, the smoke check and the
+ * CPU-baseline leg of the benchmark link this as the checker.  The product
+ * (hochunk3d_amd/csrc) never includes, links or calls anything here.
+ *
+ * Every array uses the reference's Fortran column-major layout with the
+ * reference's dimensions (wall counts nrg/ntg/npg, ndg species), addressed
+ * 0-based here.  Units follow the reference: lengths in stellar radii,
+ * temperatures and photon energies in eV (T[K]/11605, nu = 1.2398/lambda[um]),
+ * densities in g/cm^3, kapd = kappa/kappa_V * R*(cm) * kappa_V.
+ */
+#ifndef HC_ORACLE_H
+#define HC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HCO_NDG 8          /* grid_mod.f90:13 ndg */
+#define HCO_NO 8           /* SED output slots: all/star/disk/env/ext/direct/scat/thermal */
+#define HCO_NO_INIT 5
+#define HCO_TAUMAX 25.0    /* peeloff_3d.f90:40 */
+
+typedef struct {
+  int nrg, ntg, npg;            /* wall counts; cells are (nrg-1)(ntg-1)(npg-1) */
+  const double *r2arr;          /* [nrg] */
+  const double *tan2arr;        /* [ntg]; 0 at poles, -1 at the midplane wall */
+  const double *thetarr;        /* [ntg] */
+  const double *aarr, *barr;    /* [npg] sin(phi), -cos(phi) */
+  const double *phiarr;         /* [npg] */
+  const double *densarr;        /* (nrg,ntg,npg,ndg) */
+  const double *tdust;          /* (nrg,ntg,npg,ndg) eV */
+  const double *massarr;        /* (nrg,ntg,npg,ndg) (rho*V)^(1/4), for tfinal */
+  double tsub;                  /* K */
+} hco_grid;
+
+typedef struct {
+  int nlammax;
+  int nlambda[HCO_NDG];
+  const double *nudust, *kappad, *adust, *gdust, *pldust;  /* (nlammax,ndg) */
+  double kappav[HCO_NDG];
+  int nT, nnu;
+  const double *lTint;          /* [nT] */
+  const double *nuint;          /* [nnu] */
+  const double *kapBint;        /* (nT,nnu,ndg) normalised cumulative */
+  const double *kappap;         /* (nT,ndg) */
+} hco_dust;
+
+typedef struct {
+  double rstar;                 /* solar radii */
+  double tstar;                 /* eV (already divided by 11605) */
+  double tshock;                /* eV */
+  int limb;                     /* 0: isotropic intensity (only mode restated) */
+  int ips;                      /* point-source stellar launch (initp_ps) */
+  int iplanckst;                /* 1: blackbody star */
+  /* hot spot */
+  int ispot, nspot;
+  double fspot, csprad, cspot1, sspot1, scspot1, ssspot1, cspot2, sspot2, scspot2, ssspot2;
+  /* source split (newdisktherm.F90:143-176) */
+  double accfrac, accfrac2;
+  /* binning */
+  int nfreq, nmu, nph, nap, npeel, nxhst;
+  double numin, lnurat;
+  const double *u;              /* [nmu] */
+  const double *aperture2;      /* [nap] */
+  const double *coste, *sinte, *cospe, *sinpe;  /* [npeel] */
+  double rmaxi;
+  int imcube;
+  int iintmax;                  /* 10000 */
+  double ltot;                  /* erg/s */
+} hco_setup;
+
+typedef struct {
+  /* per-cell Lucy / absorption tallies, (nrg,ntg,npg[,ndg]) */
+  double *dtauabs, *nabs, *jmean;
+  int32_t *killed, *dbig, *dsg;
+  /* SED: 9 arrays (I,Q,U,V,I2,Q2,U2,V2,N) of (nfreq,nmu,nph,nap,no) */
+  double *sed;
+  /* peeled SED: 9 arrays of (nfreq,npeel,nap,no) */
+  double *peel;
+  /* image cube I (npeel,nx,nx,nfreq,no); Q,U,V (npeel,nx,nx,nfreq) */
+  double *img_i, *img_q, *img_u, *img_v;
+  double *si_init;              /* (nfreq,no_init) */
+  /* scalars */
+  double tot, flux, sflux, nscat, scount, abflux, aflux, sumsub, sumall;
+  double hit, htot;
+  int64_t n_err_nowall, n_err_dust, n_warn;
+} hco_tallies;
+
+/* Per-packet record for the deterministic-trace comparison. */
+typedef struct {
+  double x, y, z;               /* last position (escape point / kill point) */
+  double cost, sint, phi;       /* last direction */
+  double I, Q, U, V, nub;
+  double path;                  /* total geometric path on transport rays */
+  double tau_peel;              /* sum of peel optical depths (peeled passes) */
+  int64_t ncross, npeelcross;   /* wall crossings: transport rays, peel rays */
+  int64_t nscat, nabs, nstar, ntrial;
+  uint64_t cellhash;            /* FNV-1a over the (ir,it,ip) sequence */
+  int32_t status;               /* 0 escaped, 1 killed, 2 error */
+  int32_t ir, it, ip, pad;
+} hco_trace;
+
+enum { HCO_ESCAPED = 0, HCO_KILLED = 1, HCO_ERROR = 2 };
+
+/* ---- RNG: L'Ecuyer-Cote combined MRG (ecuyer_cote_mod.f90) ---- */
+typedef struct { int32_t s1, s2; } hco_rng;
+void hco_rng_packet(hco_rng *r, int32_t seed1, int32_t seed2, int gen, uint64_t segment);
+double hco_ran(hco_rng *r);
+int32_t hco_mulmod(int32_t a, int32_t s, int32_t m);
+
+/* ---- leaf routines, exposed for golden-vector tests ---- */
+typedef struct { double tn1, tp1, tp2; int in1, ip1, ip2; } hco_walls;
+void hco_reset_t(hco_walls *w);
+void hco_insert_t(hco_walls *w, double t, int face);
+void hco_radface(hco_walls *w, double ux, double uy, double uz, double x, double y, double z,
+                 double r2a, double r2b);
+void hco_thetaface(hco_walls *w, double ux, double uy, double uz, double x, double y, double z,
+                   double t2a, double t2b);
+void hco_phiface(hco_walls *w, double a1, double b1, double a2, double b2,
+                 double ux, double uy, double x, double y);
+int hco_find_next_wall(const hco_walls *w, int on_wall, double *t, int *face);
+int hco_locate(const double *xx, int n, double x);
+double hco_bbfreq(hco_rng *r, double teff);
+double hco_lucydustfreq(const hco_dust *d, hco_rng *r, double T0, int id);
+double hco_kapp(const hco_dust *d, double T, int id);
+double hco_dusttemp(const hco_dust *d, double na, double T0, double ltot, double nphot,
+                    double mass4, int id);
+typedef struct {
+  double kappa[HCO_NDG], albedo[HCO_NDG], g[HCO_NDG], pl[HCO_NDG];
+  double g2[HCO_NDG], g2p1[HCO_NDG], onemg2[HCO_NDG], twog[HCO_NDG];
+} hco_opac;
+void hco_opacset(const hco_dust *d, double nu, hco_opac *o);
+
+/* Scattering state for the leaf-level tests of stokes/stokespeel. */
+typedef struct {
+  double cost, sint, cosp, sinp, phi;
+  double I, Q, U, V;
+  double hit, htot;
+} hco_scat;
+void hco_stokes(hco_scat *s, const hco_opac *o, int id, hco_rng *r);
+void hco_stokespeel(double *I, double *Q, double *U, double *V, double *cost, double *sint,
+                    double ri1, double bmu, double costp, double sintp, const hco_opac *o, int id);
+void hco_findangle3(double cost1, double sint1, double phi1, double cosp1, double sinp1,
+                    double cost2, double sint2, double phi2, double cosp2, double sinp2,
+                    double *costnew, double *sintnew, double *phinew, double *ri1);
+
+/* ---- one transport pass over packets [p_begin, p_end) ----
+ * ipeel: 0 for the Lucy passes, 1 for the final peeled pass.
+ * np: the pass's total packet count (sets the source split and is the
+ * packet-energy normalisation).  Packet p uses RNG segment seg0 + p of
+ * generator gen.  trace may be NULL, else holds p_end - p_begin records. */
+int hco_run_pass(const hco_grid *g, const hco_dust *d, const hco_setup *s, hco_tallies *t,
+                 int64_t np, int64_t p_begin, int64_t p_end, int ipeel,
+                 int32_t seed1, int32_t seed2, int gen, uint64_t seg0, hco_trace *trace);
+
+/* ---- Lucy temperature update (reconstructed tfinal) ----
+ * tdust2 = radiative-equilibrium T from dtauabs for every cell and species
+ * with non-zero mass; returns the number of (cell,species) whose T changed by
+ * more than 1 % against tdust. */
+int64_t hco_tfinal(const hco_grid *g, const hco_dust *d, const double *dtauabs, double ltot,
+                   double nphot, double *tdust2);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
