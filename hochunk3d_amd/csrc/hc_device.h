@@ -10,6 +10,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#define HCM_FN __host__ __device__ static __forceinline__
+#include "hc_math.h"
+
 namespace hc {
 
 constexpr double kPi = 3.1415926535897932384626433;  // constants.f90:5
@@ -279,7 +282,7 @@ struct Mueller {
 
 __device__ __forceinline__ Mueller mueller(bool upper, double p1, double p2, double p3,
                                            double sini2, double cosi2, double s2a, double c2a) {
-  // s2a/c2a are sin(2 i1), cos(2 i1) (lower) or sin(2 i3), cos(2 i3) (upper)
+  // s2a/c2a are hcm_sin(2 i1), hcm_cos(2 i1) (lower) or hcm_sin(2 i3), hcm_cos(2 i3) (upper)
   Mueller m;
   const double sin2i2 = 2.0 * sini2 * cosi2;
   const double cos2i2 = 2.0 * (cosi2 * cosi2) - 1.0;
@@ -325,8 +328,8 @@ __device__ __forceinline__ void stokes(Dir &d, Stokes4 &s, const HG &h, Mrg &rng
     const double ri1 = kR2P * ran(rng);
     upper = ri1 > kPi;
     ri = upper ? kR2P - ri1 : ri1;
-    ci = cos(ri);
-    si = sin(ri);
+    ci = hcm_cos(ri);
+    si = hcm_sin(ri);
     s2a = 2.0 * si * ci;
     c2a = 2.0 * (ci * ci) - 1.0;
     const double a12 = p2 * c2a;
@@ -377,7 +380,7 @@ __device__ __forceinline__ void stokes(Dir &d, Stokes4 &s, const HG &h, Mrg &rng
   }
   double cosdph = upper ? -cosi2 * ci + sini2 * si * bmu : -ci * cosi2 + si * sini2 * bmu;
   if (fabs(cosdph) > 1.0) cosdph = (cosdph > 1.0) ? 1.0 : -1.0;
-  double phi = upper ? phip + acos(cosdph) : phip - acos(cosdph);
+  double phi = upper ? phip + hcm_acos(cosdph) : phip - hcm_acos(cosdph);
   if (phi > kR2P) phi = phi - kR2P;
   if (phi < 0.0) phi = phi + kR2P;
   const Mueller m = mueller(upper, p1, p2, p3, sini2, cosi2, s2a, c2a);
@@ -393,8 +396,8 @@ __device__ __forceinline__ void stokes(Dir &d, Stokes4 &s, const HG &h, Mrg &rng
   d.cost = cost;
   d.sint = sint;
   d.phi = phi;
-  d.cosp = cos(phi);
-  d.sinp = sin(phi);
+  d.cosp = hcm_cos(phi);
+  d.sinp = hcm_sin(phi);
 }
 
 // findangle3.f90 (observer azimuth phi2 = 0 as peeloff_3d.f90:168 passes
@@ -410,7 +413,7 @@ __device__ __forceinline__ void peel_weight(const Dir &d, double ce, double se, 
     double cosx = (ce - (d.cost * bmu)) / (d.sint * sintnew);
     if (cosx > 1.0) cosx = 1.0;
     if (cosx < -1.0) cosx = -1.0;
-    const double x = acos(cosx);
+    const double x = hcm_acos(cosx);
     const double phi2 = 0.0;
     if (phi2 > d.phi)
       ri1 = (phi2 - d.phi > kPi) ? x : kR2P - x;
@@ -423,7 +426,7 @@ __device__ __forceinline__ void peel_weight(const Dir &d, double ce, double se, 
   const double sinbt = sqrt(1.0 - cosb2);
   const bool upper = ri1 > kPi;
   const double ri = upper ? kR2P - ri1 : ri1;
-  double ci = cos(ri), si = sin(ri);
+  double ci = hcm_cos(ri), si = hcm_sin(ri);
   const double s2a = 2.0 * si * ci;
   const double c2a = 2.0 * (ci * ci) - 1.0;
   if (bmu == 1.0) return;  // stokespeel.f90:116-119

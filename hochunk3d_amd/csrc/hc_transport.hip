@@ -270,7 +270,7 @@ __device__ __attribute__((noinline)) void peeloff(const Params &P, const Pkt &k,
     }
   }
   if (tracing) tr.tau_peel += tsum;
-  const double factor = exp(-tsum);
+  const double factor = hcm_exp(-tsum);
   s.I = s.I * factor;
   s.Q = s.Q * factor;
   s.U = s.U * factor;
@@ -282,7 +282,7 @@ __device__ __attribute__((noinline)) void peeloff(const Params &P, const Pkt &k,
   const int iy = (int)((yimage + P.rmaxi) / P.fractxh) + 1;
   const double rho2 = ximage * ximage + yimage * yimage;
   const int inub =
-      clampi((int)((double)P.nfreq * log(k.nub / P.numin) / P.lnurat) + 1, 1, P.nfreq);
+      clampi((int)((double)P.nfreq * hcm_log(k.nub / P.numin) / P.lnurat) + 1, 1, P.nfreq);
   const int i_inter = (k.iflag == 1) ? 2 : ((k.i_orig == 1) ? 1 : 3);
   const int indx[3] = {1, 1 + k.i_orig, 5 + i_inter};
   if (P.peel) {
@@ -335,7 +335,7 @@ __device__ __forceinline__ double bbfreq(Mrg &rng, double teff) {  // random.f90
   const double z1 = ran(rng), z2 = ran(rng), z3 = ran(rng), z4 = ran(rng);
   int i = 1;
   while (z0 > c_inczeta[i]) ++i;
-  return -log(z1 * z2 * z3 * z4) * teff / (double)i;
+  return -hcm_log(z1 * z2 * z3 * z4) * teff / (double)i;
 }
 
 // lucydustfreq.f90:19-157 with its guard chain; j clamped to [1,nT-1].
@@ -344,7 +344,7 @@ __device__ double lucydustfreq(const Params &P, Mrg &rng, double T0, int id) {
   const double *lT = P.lTint, *nuint = P.nuint;
   const double *kb = P.kapBint + (size_t)nT * nnu * id;
   auto KB = [&](int jj, int ii) { return kb[(size_t)(jj - 1) + (size_t)nT * (size_t)(ii - 1)]; };
-  const double lT0 = log(T0);
+  const double lT0 = hcm_log(T0);
   const double xi = ran(rng);
   int j = (int)((double)(nT - 1) * (lT0 - lT[0]) / (lT[nT - 1] - lT[0])) + 1;
   j = clampi(j, 1, nT - 1);
@@ -388,8 +388,8 @@ __device__ double lucydustfreq(const Params &P, Mrg &rng, double T0, int id) {
     f = 0.5;
   else if (P1 <= 0.0 || xi <= 0.0) {
   } else
-    f = log(xi / P0) / log(P1 / P0);
-  return T0 * nuint[i0 - 1] * pow(nuint[i1 - 1] / nuint[i0 - 1], f);
+    f = hcm_log(xi / P0) / hcm_log(P1 / P0);
+  return T0 * nuint[i0 - 1] * hcm_pow(nuint[i1 - 1] / nuint[i0 - 1], f);
 }
 
 // opacset for all active species: kapd = kappa * R*(cm) * kappa_V and the
@@ -415,8 +415,8 @@ __device__ __forceinline__ void initp_sample(Pkt &k, Mrg &rng) {  // initp_commo
   k.d.cost = sqrt(ran(rng));
   k.d.sint = sqrt(1.0 - k.d.cost * k.d.cost);
   k.d.phi = kR2P * ran(rng);
-  k.d.cosp = cos(k.d.phi);
-  k.d.sinp = sin(k.d.phi);
+  k.d.cosp = hcm_cos(k.d.phi);
+  k.d.sinp = hcm_sin(k.d.phi);
 }
 
 __device__ __forceinline__ void initp_transform(Pkt &k, double cosb, double sinb, double lp) {
@@ -432,10 +432,10 @@ __device__ __forceinline__ void initp_transform(Pkt &k, double cosb, double sinb
       double cospn = (k.d.cost - costi * cosb) / (sinti * sinb);
       if (fabs(cospn) > 1.0) cospn = (cospn < 0.0) ? -1.0 : 1.0;
       if (k.d.phi < kPi) {
-        phinew = acos(cospn);
+        phinew = hcm_acos(cospn);
         if (phinew > kPi) phinew = kPi;
       } else {
-        phinew = kR2P - acos(cospn);
+        phinew = kR2P - hcm_acos(cospn);
         if (phinew < kPi) phinew = kPi;
       }
     } else {
@@ -450,15 +450,15 @@ __device__ __forceinline__ void initp_transform(Pkt &k, double cosb, double sinb
   k.d.phi = k.d.phi + lp;
   if (k.d.phi > kR2P) k.d.phi = k.d.phi - kR2P;
   if (k.d.phi < 0.0) k.d.phi = k.d.phi + kR2P;
-  k.d.sinp = sin(k.d.phi);
-  k.d.cosp = cos(k.d.phi);
+  k.d.sinp = hcm_sin(k.d.phi);
+  k.d.cosp = hcm_cos(k.d.phi);
 }
 
 __device__ __forceinline__ double clamp1(double x) { return x > 1.0 ? 1.0 : (x < -1.0 ? -1.0 : x); }
 
 __device__ __forceinline__ void bin_init(const Params &P, double nub, int i_orig_init) {
   if (!P.si_init) return;
-  const int inub = clampi((int)((double)P.nfreq * log(nub / P.numin) / P.lnurat) + 1, 1, P.nfreq);
+  const int inub = clampi((int)((double)P.nfreq * hcm_log(nub / P.numin) / P.lnurat) + 1, 1, P.nfreq);
   atomic_add(&P.si_init[(size_t)(inub - 1) + (size_t)P.nfreq * (size_t)(i_orig_init - 1)], 1.0);
   atomic_add(&P.si_init[(size_t)(inub - 1)], 1.0);
 }
@@ -493,7 +493,7 @@ __device__ void launch_star(const Params &P, Pkt &k, double *alb, Mrg &rng, bool
       }
       lp = kR2P * ran(rng);
       if (P.ispot == 1 && ispotph == 1) {
-        const double cphi = cos(lp), sphi = sin(lp);
+        const double cphi = hcm_cos(lp), sphi = hcm_sin(lp);
         const double d1 = P.scspot1 * sinb * cphi + P.ssspot1 * sinb * sphi + P.cspot1 * cosb;
         const double d2 = P.scspot2 * sinb * cphi + P.ssspot2 * sinb * sphi + P.cspot2 * cosb;
         if (P.nspot == 1 ? (d1 < P.csprad) : (d1 < P.csprad && d2 < P.csprad)) continue;
@@ -502,10 +502,10 @@ __device__ void launch_star(const Params &P, Pkt &k, double *alb, Mrg &rng, bool
     }
     if (P.ispot == 1 && P.fspot < 1.0) {
       double dellon = (lp < kPi) ? lp : kR2P - lp;
-      if (P.cspot1 * cosb + P.sspot1 * sinb * cos(dellon) > P.csprad) ispotph = 1;
+      if (P.cspot1 * cosb + P.sspot1 * sinb * hcm_cos(dellon) > P.csprad) ispotph = 1;
       if (P.nspot == 2) {
         dellon = (lp < kPi) ? kPi - lp : lp - kPi;
-        if (P.cspot2 * cosb + P.sspot2 * sinb * cos(dellon) > P.csprad) ispotph = 1;
+        if (P.cspot2 * cosb + P.sspot2 * sinb * hcm_cos(dellon) > P.csprad) ispotph = 1;
       }
     }
     initp_sample(k, rng);
@@ -520,8 +520,8 @@ __device__ void launch_star(const Params &P, Pkt &k, double *alb, Mrg &rng, bool
     k.d.sint = sinb;
     lp = kR2P * ran(rng);
     k.d.phi = lp;
-    k.d.sinp = clamp1(sin(k.d.phi));
-    k.d.cosp = clamp1(cos(k.d.phi));
+    k.d.sinp = clamp1(hcm_sin(k.d.phi));
+    k.d.cosp = clamp1(hcm_cos(k.d.phi));
   }
   k.nub = bbfreq(rng, ispotph ? P.tshock : P.tstar);
   if (hotspot) {
@@ -532,7 +532,7 @@ __device__ void launch_star(const Params &P, Pkt &k, double *alb, Mrg &rng, bool
   }
   k.i_orig = 1;
   set_opacity(P, k, alb, true);
-  const double coslp = cos(lp), sinlp = sin(lp);
+  const double coslp = hcm_cos(lp), sinlp = hcm_sin(lp);
   k.z = cosb * 1.0;
   const double rp = sinb * 1.0;
   k.x = rp * coslp;
@@ -545,7 +545,7 @@ __device__ void launch_star(const Params &P, Pkt &k, double *alb, Mrg &rng, bool
   k.iflag = 0;
   k.ir = 1;
   k.on_wall = true;
-  k.it = locate(P.thetarr, P.ntg, acos(cosb));
+  k.it = locate(P.thetarr, P.ntg, hcm_acos(cosb));
   k.ip = locate(P.phiarr, P.npg, lp);
   if (P.ipeel) peel_direct(P, k, cosb, sinb, coslp, sinlp, acc, tr, tracing);
 }
@@ -556,7 +556,7 @@ __device__ void star_reemit(const Params &P, Pkt &k, double *alb, Mrg &rng, Acc 
   acc.v[S_AFLUX] += 1.0;
   const double cosb = clamp1(k.z / k.rtot);
   const double sinb = sqrt(1.0 - cosb * cosb);
-  double lp = atan2(k.y, k.x);
+  double lp = hcm_atan2(k.y, k.x);
   if (lp < 0.0) lp = lp + kR2P;
   k.s = Stokes4{1.0, 0.0, 0.0, 0.0};
   initp_sample(k, rng);
@@ -570,7 +570,7 @@ __device__ void star_reemit(const Params &P, Pkt &k, double *alb, Mrg &rng, Acc 
   k.uz = k.d.cost;
   k.ir = 1;
   k.on_wall = true;
-  if (P.ipeel) peel_direct(P, k, cosb, sinb, cos(lp), sin(lp), acc, tr, tracing);
+  if (P.ipeel) peel_direct(P, k, cosb, sinb, hcm_cos(lp), hcm_sin(lp), acc, tr, tracing);
   if (tracing) tr.nstar++;
 }
 
@@ -612,7 +612,7 @@ __device__ void bin_exit(const Params &P, Pkt &k, double xpold, double ypold, do
     }
   }
   const int inub =
-      clampi((int)((double)P.nfreq * log(k.nub / P.numin) / P.lnurat) + 1, 1, P.nfreq);
+      clampi((int)((double)P.nfreq * hcm_log(k.nub / P.numin) / P.lnurat) + 1, 1, P.nfreq);
   const int i_inter = (k.iflag == 1) ? 2 : ((k.i_orig == 1) ? 1 : 3);
   const int indx[3] = {1, 1 + k.i_orig, 5 + i_inter};
   const bool counted = !(k.iflag == 0 && k.i_orig == 4);
@@ -652,11 +652,11 @@ __device__ int propagate(const Params &P, Pkt &k, double *alb, Mrg &rng, Acc &ac
   double xpold = k.x, ypold = k.y, zpold = k.z;
   int iscat = 0, iint = 0, pir = 0, pit = 0, pip = 0;
   bool first = true;
-  int st = tauint(P, k, alb, -log(ran(rng)), acc, tr, tracing);
+  int st = tauint(P, k, alb, -hcm_log(ran(rng)), acc, tr, tracing);
   for (;;) {
     while (st == ST_STAR) {
       star_reemit(P, k, alb, rng, acc, tr, tracing);
-      const double tau = -log(ran(rng));
+      const double tau = -hcm_log(ran(rng));
       xpold = k.x;
       ypold = k.y;
       zpold = k.z;
@@ -697,10 +697,10 @@ __device__ int propagate(const Params &P, Pkt &k, double *alb, Mrg &rng, Acc &ac
       if (P.nabs) atomic_add(&P.nabs[cell * (size_t)P.nact + slot], 1.0);
       k.s = Stokes4{1.0, 0.0, 0.0, 0.0};
       k.d.cost = 1.0 - 2.0 * ran(rng);
-      k.d.sint = sqrt(1.0 - pow(k.d.cost, 2.0));
+      k.d.sint = sqrt(1.0 - hcm_pow(k.d.cost, 2.0));
       k.d.phi = kR2P * ran(rng);
-      k.d.cosp = cos(k.d.phi);
-      k.d.sinp = sin(k.d.phi);
+      k.d.cosp = hcm_cos(k.d.phi);
+      k.d.sinp = hcm_sin(k.d.phi);
       k.nub = lucydustfreq(P, rng, P.tdust[cell * (size_t)P.nact + slot], idust2);
       k.uy = k.d.sint * k.d.sinp;
       k.ux = k.d.sint * k.d.cosp;
@@ -712,7 +712,7 @@ __device__ int propagate(const Params &P, Pkt &k, double *alb, Mrg &rng, Acc &ac
       ++iint;
       if (tracing) tr.nabs++;
     }
-    const double tau = -log(ran(rng));
+    const double tau = -hcm_log(ran(rng));
     xpold = k.x;
     ypold = k.y;
     zpold = k.z;
@@ -800,7 +800,7 @@ struct TParams {
 
 __device__ __forceinline__ double kapp(const TParams &T, double t, int id) {  // dust.f90:114-142
   const double *kp = T.kappap + (size_t)T.nT * id;
-  const double lt = log(t);
+  const double lt = hcm_log(t);
   const int k = (int)((double)(T.nT - 1) * (lt - T.lTint[0]) / (T.lTint[T.nT - 1] - T.lTint[0])) + 1;
   if (k < 1 || k > T.nT - 1) return k < 1 ? kp[0] : kp[T.nT - 1];
   const double f = (lt - T.lTint[k - 1]) / (T.lTint[k] - T.lTint[k - 1]);
@@ -829,7 +829,7 @@ __global__ __launch_bounds__(256) void lucy_temperature(TParams T) {
       int iter = 0;
       while (fabs(Td - Tprev) > 1.e-5 * Td && iter < 1000) {
         Tprev = Td;
-        Td = pow((na * T.ltot) / (4.0 * kSigT * T.nphot * T.kappav[id] * kapp(T, Td, id)), 0.25) /
+        Td = hcm_pow((na * T.ltot) / (4.0 * kSigT * T.nphot * T.kappav[id] * kapp(T, Td, id)), 0.25) /
              11605.0;
         Td = Td / m4;
         if (Td > 1600.0 / 11605.0) Td = 1600.0 / 11605.0;

@@ -87,7 +87,7 @@ def _species_arrays(table: np.ndarray):
     ad = table[:, 2] / table[:, 1]
     gd = table[:, 4].copy()
     pd = table[:, 5].copy()
-    kd[0] = 0.00001 * kd[0]  # dust.f90:353 (first wavelength scaled down)
+    kd[0] = float(np.float32(0.00001)) * kd[0]  # dust.f90:353, a default-real literal
     iv = int(np.nonzero(lam <= 0.55)[0][-1])  # 0-based index of the last lambda <= 0.55
     f = (5500.0 - lamdust[iv]) / (lamdust[iv + 1] - lamdust[iv])
     kv = (1.0 - f) * kd[iv] + f * kd[iv + 1]

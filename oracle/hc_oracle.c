@@ -17,6 +17,28 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Elementary functions: the C library by default; with HCO_DETMATH the
+ * deterministic FP64 set the GPU's plain FP64 path uses (hc_math.h), so a
+ * packet's arithmetic is bit-identical on both sides. */
+#ifdef HCO_DETMATH
+#include "../hochunk3d_amd/csrc/hc_math.h"
+#define HCO_LOG hcm_log
+#define HCO_EXP hcm_exp
+#define HCO_SIN hcm_sin
+#define HCO_COS hcm_cos
+#define HCO_ACOS hcm_acos
+#define HCO_POW hcm_pow
+#define HCO_ATAN2 hcm_atan2
+#else
+#define HCO_LOG log
+#define HCO_EXP exp
+#define HCO_SIN sin
+#define HCO_COS cos
+#define HCO_ACOS acos
+#define HCO_POW pow
+#define HCO_ATAN2 atan2
+#endif
+
 #define PI 3.1415926535897932384626433  /* constants.f90:5 */
 #define R2P (2.0 * PI)
 #define RSOL 6.9598e10                   /* constants.f90:11 */
@@ -321,7 +343,7 @@ double hco_bbfreq(hco_rng *r, double teff) {  /* random.f90:49-77 */
   z4 = hco_ran(r);
   i = 1;
   while (z0 > inczeta[i]) i = i + 1;
-  return -log(z1 * z2 * z3 * z4) * teff / (double)i;
+  return -HCO_LOG(z1 * z2 * z3 * z4) * teff / (double)i;
 }
 
 /* lucydustfreq.f90:19-157, including its guard chain.  The row index j is
@@ -333,7 +355,7 @@ double hco_lucydustfreq(const hco_dust *d, hco_rng *r, double T0, int id) {
   double lT0, xi, f, P0, P1, P;
   int j, i0, i1, i;
 #define KB(jj, ii) kb[IDX2((jj) - 1, (ii) - 1, nT)]
-  lT0 = log(T0);
+  lT0 = HCO_LOG(T0);
   xi = hco_ran(r);
   j = (int)((double)(nT - 1) * (lT0 - lTint[0]) / (lTint[nT - 1] - lTint[0])) + 1;
   if (j < 1) j = 1;
@@ -387,15 +409,15 @@ double hco_lucydustfreq(const hco_dust *d, hco_rng *r, double T0, int id) {
     /* f keeps its interpolation weight (lucydustfreq.f90:138-139) */
   } else if (xi <= 0.0) {
   } else
-    f = log(xi / P0) / log(P1 / P0);
+    f = HCO_LOG(xi / P0) / HCO_LOG(P1 / P0);
 #undef KB
-  return T0 * nuint[i0 - 1] * pow(nuint[i1 - 1] / nuint[i0 - 1], f);
+  return T0 * nuint[i0 - 1] * HCO_POW(nuint[i1 - 1] / nuint[i0 - 1], f);
 }
 
 double hco_kapp(const hco_dust *d, double T, int id) {  /* dust.f90:114-142 */
   const int nT = d->nT;
   const double *kp = d->kappap + (size_t)nT * id;
-  double lT = log(T), f;
+  double lT = HCO_LOG(T), f;
   int k = (int)((double)(nT - 1) * (lT - d->lTint[0]) / (d->lTint[nT - 1] - d->lTint[0])) + 1;
   if (k < 1 || k > nT - 1) return k < 1 ? kp[0] : kp[nT - 1];
   f = (lT - d->lTint[k - 1]) / (d->lTint[k] - d->lTint[k - 1]);
@@ -413,7 +435,7 @@ double hco_dusttemp(const hco_dust *d, double na, double T0, double ltot, double
   Told = 0.5 * Td;
   while (fabs(Td - Told) > eps * Td && n < 1000) {
     Told = Td;
-    Td = pow((na * ltot) / (4.0 * SIGT * nphot * d->kappav[id] * hco_kapp(d, Td, id)), 0.25) /
+    Td = HCO_POW((na * ltot) / (4.0 * SIGT * nphot * d->kappav[id] * hco_kapp(d, Td, id)), 0.25) /
          11605.0;
     if (mass4 > 0.0)
       Td = Td / mass4;
@@ -508,8 +530,8 @@ void hco_stokes(hco_scat *s, const hco_opac *o, int id, hco_rng *r) {
     ri1 = R2P * xran;
     if (ri1 > PI) {
       ri3 = R2P - ri1;
-      cosi3 = cos(ri3);
-      sini3 = sin(ri3);
+      cosi3 = HCO_COS(ri3);
+      sini3 = HCO_SIN(ri3);
       sin2i3 = 2.0 * sini3 * cosi3;
       cos2i3 = 2.0 * (cosi3 * cosi3) - 1.0;
       a11 = p1;
@@ -559,7 +581,7 @@ void hco_stokes(hco_scat *s, const hco_opac *o, int id, hco_rng *r) {
       }
       cosdph = -cosi2 * cosi3 + sini2 * sini3 * bmu;
       if (fabs(cosdph) > 1.0) cosdph = (cosdph > 1.0) ? 1.0 : -1.0;
-      phi = phip + acos(cosdph);
+      phi = phip + HCO_ACOS(cosdph);
       if (phi > R2P) phi = phi - R2P;
       if (phi < 0.0) phi = phi + R2P;
       sin2i2 = 2.0 * sini2 * cosi2;
@@ -580,8 +602,8 @@ void hco_stokes(hco_scat *s, const hco_opac *o, int id, hco_rng *r) {
       a43 = p4 * cos2i3;
       a44 = p3;
     } else {
-      cosi1 = cos(ri1);
-      sini1 = sin(ri1);
+      cosi1 = HCO_COS(ri1);
+      sini1 = HCO_SIN(ri1);
       sin2i1 = 2.0 * sini1 * cosi1;
       cos2i1 = 2.0 * (cosi1 * cosi1) - 1.0;
       a11 = p1;
@@ -631,7 +653,7 @@ void hco_stokes(hco_scat *s, const hco_opac *o, int id, hco_rng *r) {
       }
       cosdph = -cosi1 * cosi2 + sini1 * sini2 * bmu;
       if (fabs(cosdph) > 1.0) cosdph = (cosdph > 1.0) ? 1.0 : -1.0;
-      phi = phip - acos(cosdph);
+      phi = phip - HCO_ACOS(cosdph);
       if (phi > R2P) phi = phi - R2P;
       if (phi < 0.0) phi = phi + R2P;
       sin2i2 = 2.0 * sini2 * cosi2;
@@ -661,8 +683,8 @@ void hco_stokes(hco_scat *s, const hco_opac *o, int id, hco_rng *r) {
   s->cost = cost;
   s->sint = sint;
   s->phi = phi;
-  s->cosp = cos(phi);
-  s->sinp = sin(phi);
+  s->cosp = HCO_COS(phi);
+  s->sinp = HCO_SIN(phi);
 }
 
 /* stokespeel.f90:1-382: the un-normalised phase-matrix weighting of a peeled
@@ -685,8 +707,8 @@ void hco_stokespeel(double *I, double *Q, double *U, double *V, double *cost, do
   sinbt = sqrt(1.0 - cosb2);
   if (ri1 > PI) {
     ri3 = R2P - ri1;
-    cosi3 = cos(ri3);
-    sini3 = sin(ri3);
+    cosi3 = HCO_COS(ri3);
+    sini3 = HCO_SIN(ri3);
     sin2i3 = 2.0 * sini3 * cosi3;
     cos2i3 = 2.0 * (cosi3 * cosi3) - 1.0;
     a11 = p1;
@@ -740,8 +762,8 @@ void hco_stokespeel(double *I, double *Q, double *U, double *V, double *cost, do
     a43 = p4 * cos2i3;
     a44 = p3;
   } else {
-    cosi1 = cos(ri1);
-    sini1 = sin(ri1);
+    cosi1 = HCO_COS(ri1);
+    sini1 = HCO_SIN(ri1);
     sin2i1 = 2.0 * sini1 * cosi1;
     cos2i1 = 2.0 * (cosi1 * cosi1) - 1.0;
     a11 = p1;
@@ -821,7 +843,7 @@ void hco_findangle3(double cost1, double sint1, double phi1, double cosp1, doubl
     if (*sintnew > 1.e-8) {
       cosx = (cost2 - (cost1 * *costnew)) / (sint1 * *sintnew);
       cosx = clamp1(cosx);
-      x = acos(cosx);
+      x = HCO_ACOS(cosx);
       if (phi2 > phi1) {
         if (phi2 - phi1 > PI) {
           *ri1 = x;
@@ -1055,7 +1077,7 @@ static void peeloff(ctx_t *c, const pkt_t *P0, double sip, int idust2, int peeli
     }
   }
   if (c->tr) c->tr->tau_peel += tsum;
-  factor = exp(-tsum);
+  factor = HCO_EXP(-tsum);
   sip = sip * factor;
   sqp = sqp * factor;
   sup = sup * factor;
@@ -1074,7 +1096,7 @@ static void peeloff(ctx_t *c, const pkt_t *P0, double sip, int idust2, int peeli
   ix = (int)((ximage + s->rmaxi) / (2.0 * s->rmaxi / (double)s->nxhst)) + 1;
   iy = (int)((yimage + s->rmaxi) / (2.0 * s->rmaxi / (double)s->nxhst)) + 1;
   rho2 = ximage * ximage + yimage * yimage;
-  inub = clampi((int)((double)s->nfreq * log(P0->nub / s->numin) / s->lnurat) + 1, 1, s->nfreq);
+  inub = clampi((int)((double)s->nfreq * HCO_LOG(P0->nub / s->numin) / s->lnurat) + 1, 1, s->nfreq);
   if (P0->iflag == 1)
     i_inter = 2;
   else
@@ -1138,8 +1160,8 @@ static void initp_sample(ctx_t *c, pkt_t *P) {
   P->sint = sqrt(1.0 - P->cost * P->cost);
   xran = hco_ran(&c->rng);
   P->phi = R2P * xran;
-  P->cosp = cos(P->phi);
-  P->sinp = sin(P->phi);
+  P->cosp = HCO_COS(P->phi);
+  P->sinp = HCO_SIN(P->phi);
 }
 
 /* initp_transform (initp_common.f90:31-94), including its clamp of costi
@@ -1157,10 +1179,10 @@ static void initp_transform(pkt_t *P, double cosb, double sinb, double lp) {
       cospn = (P->cost - costi * cosb) / (sinti * sinb);
       if (fabs(cospn) > 1.0) cospn = (cospn < 0.0) ? -1.0 : 1.0;
       if (P->phi < PI) {
-        phinew = acos(cospn);
+        phinew = HCO_ACOS(cospn);
         if (phinew > PI) phinew = PI;
       } else {
-        phinew = R2P - acos(cospn);
+        phinew = R2P - HCO_ACOS(cospn);
         if (phinew < PI) phinew = PI;
       }
     } else {
@@ -1175,15 +1197,15 @@ static void initp_transform(pkt_t *P, double cosb, double sinb, double lp) {
   P->phi = P->phi + lp;
   if (P->phi > R2P) P->phi = P->phi - R2P;
   if (P->phi < 0.0) P->phi = P->phi + R2P;
-  P->sinp = sin(P->phi);
-  P->cosp = cos(P->phi);
+  P->sinp = HCO_SIN(P->phi);
+  P->cosp = HCO_COS(P->phi);
 }
 
 static void bin_init(ctx_t *c, double nub, int i_orig_init) {
   const hco_setup *s = c->s;
   int inub;
   if (!c->t->si_init) return;
-  inub = clampi((int)((double)s->nfreq * log(nub / s->numin) / s->lnurat) + 1, 1, s->nfreq);
+  inub = clampi((int)((double)s->nfreq * HCO_LOG(nub / s->numin) / s->lnurat) + 1, 1, s->nfreq);
   c->t->si_init[(size_t)(inub - 1) + (size_t)s->nfreq * (size_t)(i_orig_init - 1)] += 1.0;
   c->t->si_init[(size_t)(inub - 1)] += 1.0;
 }
@@ -1228,7 +1250,7 @@ static void launch_star(ctx_t *c, pkt_t *P, int hotspot) {
       xran = hco_ran(&c->rng);
       lp = R2P * xran;
       if (s->ispot == 1 && ispotph == 1) {
-        double cphi = cos(lp), sphi = sin(lp);
+        double cphi = HCO_COS(lp), sphi = HCO_SIN(lp);
         double dotsp1 = s->scspot1 * sinb * cphi + s->ssspot1 * sinb * sphi + s->cspot1 * cosb;
         double dotsp2 = s->scspot2 * sinb * cphi + s->ssspot2 * sinb * sphi + s->cspot2 * cosb;
         if (s->nspot == 1) {
@@ -1241,11 +1263,11 @@ static void launch_star(ctx_t *c, pkt_t *P, int hotspot) {
     }
     if (s->ispot == 1 && s->fspot < 1.0) {
       double dellon = (lp < PI) ? lp : R2P - lp;
-      double costhp = s->cspot1 * cosb + s->sspot1 * sinb * cos(dellon);
+      double costhp = s->cspot1 * cosb + s->sspot1 * sinb * HCO_COS(dellon);
       if (costhp > s->csprad) ispotph = 1;
       if (s->nspot == 2) {
         dellon = (lp < PI) ? PI - lp : lp - PI;
-        costhp = s->cspot2 * cosb + s->sspot2 * sinb * cos(dellon);
+        costhp = s->cspot2 * cosb + s->sspot2 * sinb * HCO_COS(dellon);
         if (costhp > s->csprad) ispotph = 1;
       }
     }
@@ -1263,8 +1285,8 @@ static void launch_star(ctx_t *c, pkt_t *P, int hotspot) {
     xran = hco_ran(&c->rng);
     lp = R2P * xran;
     P->phi = lp;
-    P->sinp = clamp1(sin(P->phi));
-    P->cosp = clamp1(cos(P->phi));
+    P->sinp = clamp1(HCO_SIN(P->phi));
+    P->cosp = clamp1(HCO_COS(P->phi));
   }
   P->nub = hco_bbfreq(&c->rng, ispotph ? s->tshock : s->tstar);
   if (hotspot) {
@@ -1277,8 +1299,8 @@ static void launch_star(ctx_t *c, pkt_t *P, int hotspot) {
   P->i_orig = 1;
   hco_opacset(c->d, P->nub, &P->op);
   set_kapd_launch(c, P);
-  coslp = cos(lp);
-  sinlp = sin(lp);
+  coslp = HCO_COS(lp);
+  sinlp = HCO_SIN(lp);
   P->zp = cosb * 1.0;
   rp = sinb * 1.0;
   P->xp = rp * coslp;
@@ -1291,7 +1313,7 @@ static void launch_star(ctx_t *c, pkt_t *P, int hotspot) {
   P->iflag = 0;
   P->ir = 1;
   P->on_wall = 1;
-  P->it = hco_locate(g->thetarr, g->ntg, acos(cosb));
+  P->it = hco_locate(g->thetarr, g->ntg, HCO_ACOS(cosb));
   P->ip = hco_locate(g->phiarr, g->npg, lp);
   if (c->ipeel) peel_direct(c, P, cosb, sinb, coslp, sinlp);
 }
@@ -1305,7 +1327,7 @@ static void star_reemit(ctx_t *c, pkt_t *P) {
   c->t->aflux += 1.0;
   cosb = clamp1(P->zp / P->rtot);
   sinb = sqrt(1.0 - cosb * cosb);
-  lp = atan2(P->yp, P->xp);
+  lp = HCO_ATAN2(P->yp, P->xp);
   if (lp < 0.0) lp = lp + R2P;
   P->sip = 1.0;
   P->sqp = P->sup = P->svp = 0.0;
@@ -1321,11 +1343,29 @@ static void star_reemit(ctx_t *c, pkt_t *P) {
   P->uz = P->cost;
   P->ir = 1;
   P->on_wall = 1;
-  if (c->ipeel) peel_direct(c, P, cosb, sinb, cos(lp), sin(lp));
+  if (c->ipeel) peel_direct(c, P, cosb, sinb, HCO_COS(lp), HCO_SIN(lp));
   if (c->tr) c->tr->nstar++;
 }
 
 /* select_dust.f90:1-67 with mask is_any; 0 when the cell has no opacity. */
+int hco_select_dust(const double *kapd, const double *dens, hco_rng *r) {
+  double xran = hco_ran(r), opac = 0.0, frac = 0.0;
+  int id;
+  for (id = 0; id < HCO_NDG; id++) opac = opac + kapd[id] * dens[id];
+  if (opac == 0.0) return 0;
+  for (id = 0; id < HCO_NDG; id++) {
+    frac = frac + kapd[id] * dens[id] / opac;
+    if (xran <= frac + (double)1.e-10f) return id + 1;
+  }
+  return 0;
+}
+
+void hco_dustmat(const hco_opac *o, int id, double cost, double cost2, double *p1, double *p2,
+                 double *p3) {
+  double p4;
+  dustmat(p1, p2, p3, &p4, cost, cost2, o, id);
+}
+
 static int select_dust(ctx_t *c, const pkt_t *P, size_t cell) {
   double xran = hco_ran(&c->rng), opac = 0.0, frac = 0.0;
   int id;
@@ -1358,7 +1398,7 @@ static void bin_exit(ctx_t *c, pkt_t *P, double xpold, double ypold, double zpol
       t->nscat = t->nscat + (double)iscat;
     }
   }
-  inub = clampi((int)((double)s->nfreq * log(P->nub / s->numin) / s->lnurat) + 1, 1, s->nfreq);
+  inub = clampi((int)((double)s->nfreq * HCO_LOG(P->nub / s->numin) / s->lnurat) + 1, 1, s->nfreq);
   if (P->iflag == 1)
     i_inter = 2;
   else
@@ -1423,12 +1463,12 @@ static int propagate(ctx_t *c, pkt_t *P) {
   double xpold = P->xp, ypold = P->yp, zpold = P->zp, tau;
   int iscat = 0, iint = 0, idust2 = 0, st, first = 1;
   int pir = 0, pit = 0, pip = 0;
-  tau = -log(hco_ran(&c->rng));
+  tau = -HCO_LOG(hco_ran(&c->rng));
   st = tauint(c, P, tau);
   for (;;) {
     while (st == ST_STAR) {
       star_reemit(c, P);
-      tau = -log(hco_ran(&c->rng));
+      tau = -HCO_LOG(hco_ran(&c->rng));
       xpold = P->xp;
       ypold = P->yp;
       zpold = P->zp;
@@ -1493,10 +1533,10 @@ static int propagate(ctx_t *c, pkt_t *P) {
         P->sip = 1.0;
         P->sqp = P->sup = P->svp = 0.0;
         P->cost = 1.0 - 2.0 * hco_ran(&c->rng);
-        P->sint = sqrt(1.0 - pow(P->cost, 2.0));
+        P->sint = sqrt(1.0 - HCO_POW(P->cost, 2.0));
         P->phi = R2P * hco_ran(&c->rng);
-        P->cosp = cos(P->phi);
-        P->sinp = sin(P->phi);
+        P->cosp = HCO_COS(P->phi);
+        P->sinp = HCO_SIN(P->phi);
         P->nub = hco_lucydustfreq(c->d, &c->rng, g->tdust[k], idust2 - 1);
         P->uy = P->sint * P->sinp;
         P->ux = P->sint * P->cosp;
@@ -1509,7 +1549,7 @@ static int propagate(ctx_t *c, pkt_t *P) {
         if (c->tr) c->tr->nabs++;
       }
     }
-    tau = -log(hco_ran(&c->rng));
+    tau = -HCO_LOG(hco_ran(&c->rng));
     xpold = P->xp;
     ypold = P->yp;
     zpold = P->zp;
@@ -1565,4 +1605,21 @@ int hco_run_pass(const hco_grid *g, const hco_dust *d, const hco_setup *s, hco_t
     finish_trace(&c, &P, status);
   }
   return 0;
+}
+
+/* Elementary-function probe for the accuracy tests: fn 0 log, 1 exp, 2 sin,
+ * 3 cos, 4 acos, 5 pow(x,y), 6 atan2(x,y). */
+void hco_math(int fn, const double *x, const double *y, double *out, int64_t n) {
+  int64_t i;
+  for (i = 0; i < n; i++) {
+    switch (fn) {
+      case 0: out[i] = HCO_LOG(x[i]); break;
+      case 1: out[i] = HCO_EXP(x[i]); break;
+      case 2: out[i] = HCO_SIN(x[i]); break;
+      case 3: out[i] = HCO_COS(x[i]); break;
+      case 4: out[i] = HCO_ACOS(x[i]); break;
+      case 5: out[i] = HCO_POW(x[i], y[i]); break;
+      default: out[i] = HCO_ATAN2(x[i], y[i]); break;
+    }
+  }
 }

@@ -148,6 +148,10 @@ void hco_findangle3(double cost1, double sint1, double phi1, double cosp1, doubl
                     double cost2, double sint2, double phi2, double cosp2, double sinp2,
                     double *costnew, double *sintnew, double *phinew, double *ri1);
 
+int hco_select_dust(const double *kapd, const double *dens, hco_rng *r);
+void hco_dustmat(const hco_opac *o, int id, double cost, double cost2, double *p1, double *p2,
+                 double *p3);
+
 /* ---- one transport pass over packets [p_begin, p_end) ----
  * ipeel: 0 for the Lucy passes, 1 for the final peeled pass.
  * np: the pass's total packet count (sets the source split and is the

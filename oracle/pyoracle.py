@@ -13,7 +13,8 @@ from typing import Optional
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(_HERE, "libhc_oracle.so")
+LIBS = {"libm": os.path.join(_HERE, "libhc_oracle.so"),
+        "dm": os.path.join(_HERE, "libhc_oracle_dm.so")}
 NDG = 8
 NO = 8
 
@@ -89,19 +90,20 @@ TRACE_DTYPE = np.dtype([
 SCALARS = ("tot", "flux", "sflux", "nscat", "scount", "abflux", "aflux", "sumsub", "sumall",
            "hit", "htot", "n_err_nowall", "n_err_dust", "n_warn")
 
-_lib = None
+_libs = {}
 
 
 def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def load() -> C.CDLL:
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
+def load(variant: str = "libm") -> C.CDLL:
+    """variant "libm": elementary functions of the C library (as the
+    reference's own build); "dm": the deterministic set the GPU uses."""
+    if variant not in _libs:
+        if not os.path.exists(LIBS[variant]):
             build()
-        lib = C.CDLL(LIB)
+        lib = C.CDLL(LIBS[variant])
         lib.hco_run_pass.argtypes = [C.POINTER(Grid), C.POINTER(Dust), C.POINTER(Setup),
                                      C.POINTER(Tallies), C.c_int64, C.c_int64, C.c_int64,
                                      C.c_int, C.c_int32, C.c_int32, C.c_int, C.c_uint64,
@@ -137,8 +139,9 @@ def load() -> C.CDLL:
         lib.hco_findangle3.argtypes = [C.c_double] * 10 + [_dp] * 4
         lib.hco_mulmod.argtypes = [C.c_int32] * 3
         lib.hco_mulmod.restype = C.c_int32
-        _lib = lib
-    return _lib
+        lib.hco_math.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int64]
+        _libs[variant] = lib
+    return _libs[variant]
 
 
 def _p(a, ct=C.c_double):
@@ -148,8 +151,8 @@ def _p(a, ct=C.c_double):
 class Oracle:
     """The oracle bound to a model (hochunk3d_amd.model.Model)."""
 
-    def __init__(self, model):
-        self.lib = load()
+    def __init__(self, model, variant: str = "dm"):
+        self.lib = load(variant)
         m = self.model = model
         f = lambda a: np.asfortranarray(np.asarray(a, np.float64))  # noqa: E731
         self._arrs = {k: f(getattr(m, k)) for k in (
