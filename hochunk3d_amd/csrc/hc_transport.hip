@@ -30,16 +30,12 @@
 
 namespace hc {
 
-enum Scalar {
-  S_TOT = 0, S_FLUX, S_SFLUX, S_NSCAT, S_SCOUNT, S_ABFLUX, S_AFLUX, S_SUMSUB, S_SUMALL,
-  S_HIT, S_HTOT, S_ERR_WALL, S_ERR_DUST, S_WARN, S_COUNT
-};
+constexpr int kGuide = 512;  // bins of the opacity-table guide
 
 struct Params {
   // grid
   int nrg, ntg, npg, nact;
   int act[kMaxSpecies];      // reference species id (1-based) of slot k
-  int slot[kMaxSpecies];     // slot of species id-1, or -1
   const double *r2arr, *tan2arr, *thetarr, *aarr, *barr, *phiarr;
   const double *dens, *densm, *tdust;
   // dust
@@ -48,6 +44,8 @@ struct Params {
   const double *nudust, *kappad, *adust, *gdust, *pldust;
   double kappav[kMaxSpecies];
   const double *lTint, *nuint, *kapBint;
+  const int32_t *guide;      // [ndg][kGuide] bracket start per log-frequency bin
+  double glog0[kMaxSpecies], ginv[kMaxSpecies];
   // sources
   double rstar, rmincgs, tstar, tshock;
   int ips, ispot, nspot, ipeel;
@@ -70,22 +68,44 @@ struct Params {
   hc_trace *trace;
 };
 
+// Launch parameters live in constant memory: every wave reads them with
+// scalar loads and no function has to carry them.
+__constant__ Params c_P;
+
+enum Scalar {
+  S_TOT = 0, S_FLUX, S_SFLUX, S_NSCAT, S_SCOUNT, S_ABFLUX, S_AFLUX, S_SUMSUB, S_SUMALL,
+  S_HIT, S_HTOT, S_ERR_WALL, S_ERR_DUST, S_WARN, S_COUNT
+};
+
 // ---------------------------------------------------------------------------
-// packet state
+// packet state (registers)
 // ---------------------------------------------------------------------------
+template <int NA>
 struct Pkt {
   double x, y, z, ux, uy, uz;
   Dir d;
   Stokes4 s;
-  double nub, rsq, rtot;
+  double nub, rsq;
   int ir, it, ip;
   bool on_wall;
   int iflag, i_orig;
-  double kapd[kMaxSpecies];
+  double kapd[NA], alb[NA];
 };
 
-struct Acc {  // per-lane scalar tallies, flushed once per lane
-  double v[S_COUNT];
+struct Acc {  // per-lane counters, flushed once per lane
+  uint32_t tot, flux, sflux, nscat, scount, abflux, aflux, hit, htot, err_wall, err_dust;
+  double sumsub, sumall;
+};
+
+// Lucy tally of the cell the packet is in: segments accumulate here and go to
+// the device arrays with one atomic per species when the packet leaves the
+// cell (a packet random-walking in an optically thick cell adds thousands of
+// segments to the same cell).
+template <int NA>
+struct LucyAcc {
+  long long cell;
+  double tau[NA];
+  double len;
 };
 
 struct Trace {
@@ -94,7 +114,8 @@ struct Trace {
   unsigned long long hash;
 };
 
-__device__ __forceinline__ size_t cell_of(const Params &P, int ir, int it, int ip) {
+__device__ __forceinline__ size_t cell_of(int ir, int it, int ip) {
+  const Params &P = c_P;
   return (size_t)(ir - 1) + (size_t)P.nrg * ((size_t)(it - 1) + (size_t)P.ntg * (size_t)(ip - 1));
 }
 
@@ -111,18 +132,19 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) {
 }
 
 // Masked opacity of a cell: sum over active species in increasing id.
-__device__ __forceinline__ double cell_opac(const Params &P, const Pkt &k, size_t cell) {
-  const double *dm = P.densm + cell * (size_t)P.nact;
+template <int NA>
+__device__ __forceinline__ double cell_opac(const Pkt<NA> &k, const double *dm) {
   double opac = 0.0;
 #pragma unroll
-  for (int s = 0; s < kMaxSpecies; ++s)
-    if (s < P.nact) opac = opac + k.kapd[s] * dm[s];
+  for (int s = 0; s < NA; ++s)
+    if (s < c_P.nact) opac = opac + k.kapd[s] * dm[s];
   return opac;
 }
 
-__device__ __forceinline__ bool next_wall(const Params &P, double ux, double uy, double uz,
-                                          double x, double y, double z, int ir, int it, int ip,
-                                          bool on_wall, double &dt, int &face) {
+__device__ __forceinline__ bool next_wall(double ux, double uy, double uz, double x, double y,
+                                          double z, int ir, int it, int ip, bool on_wall,
+                                          double &dt, int &face) {
+  const Params &P = c_P;
   Walls w;
   w.reset();
   radface(w, ux, uy, uz, x, y, z, P.r2arr[ir - 1], P.r2arr[ir]);
@@ -132,7 +154,7 @@ __device__ __forceinline__ bool next_wall(const Params &P, double ux, double uy,
   return w.next(on_wall, dt, face);
 }
 
-__device__ __forceinline__ void face_step(const Params &P, int face, int &ir, int &it, int &ip) {
+__device__ __forceinline__ void face_step(int face, int &ir, int &it, int &ip) {
   switch (face) {
     case 1: ir -= 1; break;
     case 2: ir += 1; break;
@@ -141,30 +163,49 @@ __device__ __forceinline__ void face_step(const Params &P, int face, int &ir, in
     case 5: ip -= 1; break;
     default: ip += 1; break;
   }
-  if (ip == 0) ip = P.npg - 1;
-  if (ip == P.npg) ip = 1;
+  if (ip == 0) ip = c_P.npg - 1;
+  if (ip == c_P.npg) ip = 1;
 }
 
 enum { ST_INTERACT = 0, ST_EXIT = 1, ST_STAR = 2, ST_ERROR = 3 };
 
-// Lucy tally of one chord: dtauabs += kapd (1-albedo) rho ds per active
-// species, jmean += ds R*.
-__device__ __forceinline__ void lucy_tally(const Params &P, const Pkt &k, const double *alb,
-                                           size_t cell, double ds) {
+template <int NA>
+__device__ __forceinline__ void lucy_flush(LucyAcc<NA> &la) {
+  const Params &P = c_P;
+  if (la.cell < 0) return;
   if (P.dtauabs) {
-    const double *dm = P.densm + cell * (size_t)P.nact;
-    double *ta = P.dtauabs + cell * (size_t)P.nact;
+    double *ta = P.dtauabs + (size_t)la.cell * (size_t)P.nact;
 #pragma unroll
-    for (int s = 0; s < kMaxSpecies; ++s)
-      if (s < P.nact && dm[s] != 0.0)
-        atomic_add(&ta[s], k.kapd[s] * (1.0 - alb[s]) * dm[s] * ds);
+    for (int s = 0; s < NA; ++s)
+      if (s < P.nact && la.tau[s] != 0.0) atomic_add(&ta[s], la.tau[s]);
   }
-  if (P.jmean) atomic_add(&P.jmean[cell], ds * P.rmincgs);
+  if (P.jmean && la.len != 0.0) atomic_add(&P.jmean[la.cell], la.len * P.rmincgs);
+  la.cell = -1;
+}
+
+// Add one chord of cell `cell` to the per-cell accumulator: dtauabs gains
+// kapd (1-albedo) rho ds per active species, jmean gains ds R*.
+template <int NA>
+__device__ __forceinline__ void lucy_add(LucyAcc<NA> &la, const Pkt<NA> &k, const double *dm,
+                                         size_t cell, double ds) {
+  if (!c_P.dtauabs && !c_P.jmean) return;
+  if (la.cell != (long long)cell) {
+    lucy_flush(la);
+    la.cell = (long long)cell;
+#pragma unroll
+    for (int s = 0; s < NA; ++s) la.tau[s] = 0.0;
+    la.len = 0.0;
+  }
+#pragma unroll
+  for (int s = 0; s < NA; ++s)
+    if (s < c_P.nact && dm[s] != 0.0) la.tau[s] += k.kapd[s] * (1.0 - k.alb[s]) * dm[s] * ds;
+  la.len += ds;
 }
 
 // The cell walk to optical depth tau (tauint reconstruction).
-__device__ int tauint(const Params &P, Pkt &k, const double *alb, double tau, Acc &acc,
-                      Trace &tr, bool tracing) {
+template <int NA, bool TRACE>
+__device__ int tauint(Pkt<NA> &k, double tau, LucyAcc<NA> &la, Acc &acc, Trace &tr) {
+  const Params &P = c_P;
   if (k.on_wall) {
     const bool inner = fabs(k.rsq - P.r2arr[k.ir - 1]) < fabs(k.rsq - P.r2arr[k.ir]);
     const bool outwards = k.ux * k.x + k.uy * k.y + k.uz * k.z > 0.0;
@@ -177,37 +218,36 @@ __device__ int tauint(const Params &P, Pkt &k, const double *alb, double tau, Ac
   }
   double tsum = 0.0;
   for (;;) {
-    const size_t cell = cell_of(P, k.ir, k.it, k.ip);
+    const size_t cell = cell_of(k.ir, k.it, k.ip);
+    const double *dm = P.densm + cell * (size_t)P.nact;
     double dt;
     int face;
-    if (!next_wall(P, k.ux, k.uy, k.uz, k.x, k.y, k.z, k.ir, k.it, k.ip, k.on_wall, dt, face)) {
-      acc.v[S_ERR_WALL] += 1.0;
+    if (!next_wall(k.ux, k.uy, k.uz, k.x, k.y, k.z, k.ir, k.it, k.ip, k.on_wall, dt, face)) {
+      acc.err_wall++;
       return ST_ERROR;
     }
-    const double opac = cell_opac(P, k, cell);
+    const double opac = cell_opac(k, dm);
     const double dtau = opac * dt;
     if (tsum + dtau >= tau) {
       const double ds = (tau - tsum) / opac;
-      lucy_tally(P, k, alb, cell, ds);
+      lucy_add(la, k, dm, cell, ds);
       k.x = k.x + ds * k.ux;
       k.y = k.y + ds * k.uy;
       k.z = k.z + ds * k.uz;
       k.on_wall = false;
       k.rsq = k.x * k.x + k.y * k.y + k.z * k.z;
-      k.rtot = sqrt(k.rsq);
-      if (tracing) tr.path += ds;
+      if constexpr (TRACE) tr.path += ds;
       return ST_INTERACT;
     }
-    lucy_tally(P, k, alb, cell, dt);
+    lucy_add(la, k, dm, cell, dt);
     tsum = tsum + dtau;
-    face_step(P, face, k.ir, k.it, k.ip);
+    face_step(face, k.ir, k.it, k.ip);
     k.x = k.x + dt * k.ux;
     k.y = k.y + dt * k.uy;
     k.z = k.z + dt * k.uz;
     k.on_wall = true;
     k.rsq = k.x * k.x + k.y * k.y + k.z * k.z;
-    k.rtot = sqrt(k.rsq);
-    if (tracing) {
+    if constexpr (TRACE) {
       tr.path += dt;
       tr.ncross++;
       hash_cell(tr, k.ir, k.it, k.ip);
@@ -217,18 +257,66 @@ __device__ int tauint(const Params &P, Pkt &k, const double *alb, double tau, Ac
   }
 }
 
-__device__ __forceinline__ HG hg_of(const Params &P, int id, double nu) {
+// opacset for one species (dust.f90:13-76).  The bracket of nu in the
+// decreasing frequency table is found from a guide table of log-frequency
+// bins and a short forward scan; for a strictly decreasing table that is the
+// same unique bracket the reference's bisection finds.
+__device__ __forceinline__ Opac1 opacset_species(int id, double nu) {
+  const Params &P = c_P;
   const size_t o = (size_t)P.nlammax * id;
-  const Opac1 op = opacset1(P.nudust + o, P.kappad + o, P.adust + o, P.gdust + o, P.pldust + o,
-                            P.nlambda[id], nu);
+  const double *nud = P.nudust + o;
+  const int nl = P.nlambda[id];
+  Opac1 r;
+  int i0, i1;
+  double f;
+  if (nu >= nud[0]) {
+    i0 = i1 = 1;
+    f = 0.0;
+  } else if (nu <= nud[nl - 1]) {
+    i0 = i1 = nl;
+    f = 0.0;
+  } else {
+    int b = (int)((P.glog0[id] - hcm_log(nu)) * P.ginv[id]);
+    b = clampi(b - 1, 0, kGuide - 1);
+    i1 = P.guide[id * kGuide + b];
+    while (!(nu > nud[i1 - 1])) ++i1;
+    i0 = i1 - 1;
+    f = (nu - nud[i1 - 1]) / (nud[i0 - 1] - nud[i1 - 1]);
+  }
+  const double *kd = P.kappad + o, *ad = P.adust + o, *gd = P.gdust + o, *pd = P.pldust + o;
+  if (i0 == i1) {
+    r.g = gd[i0 - 1];
+    r.pl = pd[i0 - 1];
+    r.kappa = kd[i0 - 1];
+    r.albedo = ad[i0 - 1];
+  } else {
+    const double fm1 = 1.0 - f;
+    r.g = fm1 * gd[i1 - 1] + f * gd[i0 - 1];
+    r.pl = fm1 * pd[i1 - 1] + f * pd[i0 - 1];
+    r.kappa = fm1 * kd[i1 - 1] + f * kd[i0 - 1];
+    r.albedo = fm1 * ad[i1 - 1] + f * ad[i0 - 1];
+  }
+  return r;
+}
+
+__device__ __forceinline__ HG hg_of(int id, double nu) {
+  const Opac1 op = opacset_species(id, nu);
   return HG(op.g, op.pl);
 }
 
+struct PeelOut {
+  double tau;
+  int ncross;
+  int err;
+};
+
 // peeloff_3d.f90:5-288 for observer `peelid`.  sip is the peel weight; the
 // remaining Stokes components come from the packet.
-__device__ __attribute__((noinline)) void peeloff(const Params &P, const Pkt &k, double sip,
-                                                  int idust2, int peelid, Acc &acc, Trace &tr,
-                                                  bool tracing) {
+template <int NA>
+__device__ __attribute__((noinline)) PeelOut peeloff(const Pkt<NA> &k, double sip, int idust2,
+                                                     int peelid) {
+  const Params &P = c_P;
+  PeelOut out{0.0, 0, 0};
   Stokes4 s{sip, k.s.Q, k.s.U, k.s.V};
   double x1 = k.x, y1 = k.y, z1 = k.z;
   int ir = k.ir, it = k.it, ip = k.ip;
@@ -240,7 +328,7 @@ __device__ __attribute__((noinline)) void peeloff(const Params &P, const Pkt &k,
     const bool inner = fabs(k.rsq - P.r2arr[ir - 1]) < fabs(k.rsq - P.r2arr[ir]);
     const bool outwards = ux1 * x1 + uy1 * y1 + uz1 * z1 > 0.0;
     if (inner && !outwards) {
-      if (ir == 1) return;
+      if (ir == 1) return out;
       ir = ir - 1;
     }
     if (!inner && outwards) ir = ir + 1;
@@ -249,33 +337,33 @@ __device__ __attribute__((noinline)) void peeloff(const Params &P, const Pkt &k,
   while (ir < P.nrg) {
     double dt;
     int face;
-    if (!next_wall(P, ux1, uy1, uz1, x1, y1, z1, ir, it, ip, on_wall, dt, face)) {
-      acc.v[S_ERR_WALL] += 1.0;
-      return;
+    if (!next_wall(ux1, uy1, uz1, x1, y1, z1, ir, it, ip, on_wall, dt, face)) {
+      out.err = 1;
+      return out;
     }
-    tsum = tsum + cell_opac(P, k, cell_of(P, ir, it, ip)) * dt;
+    tsum = tsum + cell_opac(k, P.densm + cell_of(ir, it, ip) * (size_t)P.nact) * dt;
     if (tsum > kTauMax) {
-      if (tracing) tr.tau_peel += tsum;
-      return;
+      out.tau = tsum;
+      return out;
     }
-    face_step(P, face, ir, it, ip);
+    face_step(face, ir, it, ip);
     x1 = x1 + dt * ux1;
     y1 = y1 + dt * uy1;
     z1 = z1 + dt * uz1;
     on_wall = true;
-    if (tracing) tr.npeelcross++;
+    out.ncross++;
     if (ir == 0) {
-      if (tracing) tr.tau_peel += tsum;
-      return;
+      out.tau = tsum;
+      return out;
     }
   }
-  if (tracing) tr.tau_peel += tsum;
+  out.tau = tsum;
   const double factor = hcm_exp(-tsum);
   s.I = s.I * factor;
   s.Q = s.Q * factor;
   s.U = s.U * factor;
   s.V = s.V * factor;
-  if (k.iflag == 1) peel_weight(k.d, coste, sinte, cospe, sinpe, hg_of(P, idust2, k.nub), s);
+  if (k.iflag == 1) peel_weight(k.d, coste, sinte, cospe, sinpe, hg_of(idust2, k.nub), s);
   const double yimage = z1 * sinte - y1 * coste * sinpe - x1 * coste * cospe;
   const double ximage = y1 * cospe - x1 * sinpe;
   const int ix = (int)((ximage + P.rmaxi) / P.fractxh) + 1;
@@ -307,7 +395,7 @@ __device__ __attribute__((noinline)) void peeloff(const Params &P, const Pkt &k,
       }
     }
   }
-  if (P.imcube && P.img_i && ix <= P.nxhst && iy <= P.nxhst && ix > 0 && iy > 0) {
+  if (P.imcube && ix <= P.nxhst && iy <= P.nxhst && ix > 0 && iy > 0) {
     const size_t nx = (size_t)P.nxhst;
     const size_t q = (size_t)peelid +
                      (size_t)P.npeel * ((size_t)(ix - 1) + nx * ((size_t)(iy - 1) +
@@ -318,14 +406,20 @@ __device__ __attribute__((noinline)) void peeloff(const Params &P, const Pkt &k,
     atomic_add(&P.img_u[q], s.U);
     atomic_add(&P.img_v[q], s.V);
   }
+  return out;
 }
 
-__device__ __forceinline__ double inczeta_at(int i) {
-  // inczeta(i) = sum_{j=i..1} 1/j^4 accumulated from the largest j down, as
-  // zetainit (random.f90:79-97) builds it; evaluated on the fly.
-  double z = 0.0;
-  for (int j = i; j >= 1; --j) z = z + 1.0 / (double)(j * j * j * j);
-  return z;
+template <int NA, bool TRACE>
+__device__ __forceinline__ void peel_all(const Pkt<NA> &k, double sip, int idust2, Acc &acc,
+                                         Trace &tr) {
+  for (int q = 0; q < c_P.npeel; ++q) {
+    const PeelOut o = peeloff(k, sip, idust2, q);
+    acc.err_wall += o.err;
+    if constexpr (TRACE) {
+      tr.tau_peel += o.tau;
+      tr.npeelcross += o.ncross;
+    }
+  }
 }
 
 __constant__ double c_inczeta[146];
@@ -339,11 +433,11 @@ __device__ __forceinline__ double bbfreq(Mrg &rng, double teff) {  // random.f90
 }
 
 // lucydustfreq.f90:19-157 with its guard chain; j clamped to [1,nT-1].
-__device__ double lucydustfreq(const Params &P, Mrg &rng, double T0, int id) {
+__device__ __attribute__((noinline)) double lucydustfreq(double T0, int id, Mrg &rng) {
+  const Params &P = c_P;
   const int nT = P.nT, nnu = P.nnu;
   const double *lT = P.lTint, *nuint = P.nuint;
   const double *kb = P.kapBint + (size_t)nT * nnu * id;
-  auto KB = [&](int jj, int ii) { return kb[(size_t)(jj - 1) + (size_t)nT * (size_t)(ii - 1)]; };
   const double lT0 = hcm_log(T0);
   const double xi = ran(rng);
   int j = (int)((double)(nT - 1) * (lT0 - lT[0]) / (lT[nT - 1] - lT[0])) + 1;
@@ -351,10 +445,15 @@ __device__ double lucydustfreq(const Params &P, Mrg &rng, double T0, int id) {
   double f = (lT0 - lT[j - 1]) / (lT[j] - lT[j - 1]);
   if (f < 0.0) f = 0.0;
   if (f > 1.0) f = 1.0;
+  const double *row = kb + (j - 1);  // KB(j, i) = row[(i-1)*nT], KB(j+1, i) = row[(i-1)*nT + 1]
+  auto KB = [&](int ii) {
+    const double *p = row + (size_t)nT * (size_t)(ii - 1);
+    return (1.0 - f) * p[0] + f * p[1];
+  };
   int i0 = 1, i1 = nnu;
-  double P0 = (1.0 - f) * KB(j, i0) + f * KB(j + 1, i0);
+  double P0 = KB(i0);
   if (xi < P0) return (P0 > 0.0) ? T0 * xi * nuint[0] / P0 : T0 * nuint[0];
-  double P1 = (1.0 - f) * KB(j, i1) + f * KB(j + 1, i1);
+  double P1 = KB(i1);
   if (P1 > 1.0) P1 = 1.0;
   if (xi >= P1) {
     if (P1 < 1.0 && xi < 1.0) return T0 * nuint[nnu - 1] * (1.0 - P1) / (1.0 - xi);
@@ -362,7 +461,7 @@ __device__ double lucydustfreq(const Params &P, Mrg &rng, double T0, int id) {
   }
   while ((i1 - i0) > 1) {
     const int i = (i0 + i1) / 2;
-    double Pm = (1.0 - f) * KB(j, i) + f * KB(j + 1, i);
+    double Pm = KB(i);
     if (Pm > 1.0) Pm = 1.0;
     if (Pm < P0) {
       P0 = Pm;
@@ -395,23 +494,23 @@ __device__ double lucydustfreq(const Params &P, Mrg &rng, double T0, int id) {
 // opacset for all active species: kapd = kappa * R*(cm) * kappa_V and the
 // albedo.  launch_order picks newdisktherm's (kappa*rmincgs)*kappav or
 // propagate's ((kappa*rstar)*rsol)*kappav grouping.
-__device__ __forceinline__ void set_opacity(const Params &P, Pkt &k, double *alb,
-                                            bool launch_order) {
+template <int NA>
+__device__ __forceinline__ void set_opacity(Pkt<NA> &k, bool launch_order) {
+  const Params &P = c_P;
 #pragma unroll
-  for (int s = 0; s < kMaxSpecies; ++s) {
+  for (int s = 0; s < NA; ++s) {
     if (s < P.nact) {
       const int id = P.act[s] - 1;
-      const size_t o = (size_t)P.nlammax * id;
-      const Opac1 op = opacset1(P.nudust + o, P.kappad + o, P.adust + o, P.gdust + o,
-                                P.pldust + o, P.nlambda[id], k.nub);
+      const Opac1 op = opacset_species(id, k.nub);
       k.kapd[s] = launch_order ? op.kappa * P.rmincgs * P.kappav[id]
                                : op.kappa * P.rstar * kRsol * P.kappav[id];
-      alb[s] = op.albedo;
+      k.alb[s] = op.albedo;
     }
   }
 }
 
-__device__ __forceinline__ void initp_sample(Pkt &k, Mrg &rng) {  // initp_common.f90:1-29
+template <int NA>
+__device__ __forceinline__ void initp_sample(Pkt<NA> &k, Mrg &rng) {  // initp_common.f90:1-29
   k.d.cost = sqrt(ran(rng));
   k.d.sint = sqrt(1.0 - k.d.cost * k.d.cost);
   k.d.phi = kR2P * ran(rng);
@@ -419,7 +518,8 @@ __device__ __forceinline__ void initp_sample(Pkt &k, Mrg &rng) {  // initp_commo
   k.d.sinp = hcm_sin(k.d.phi);
 }
 
-__device__ __forceinline__ void initp_transform(Pkt &k, double cosb, double sinb, double lp) {
+template <int NA>
+__device__ __forceinline__ void initp_transform(Pkt<NA> &k, double cosb, double sinb, double lp) {
   if (fabs(cosb) < 0.9999999) {  // initp_common.f90:31-94
     double costi = k.d.cost * cosb - k.d.sint * sinb * k.d.cosp;
     if (fabs(costi) > 1.0) {
@@ -456,31 +556,43 @@ __device__ __forceinline__ void initp_transform(Pkt &k, double cosb, double sinb
 
 __device__ __forceinline__ double clamp1(double x) { return x > 1.0 ? 1.0 : (x < -1.0 ? -1.0 : x); }
 
-__device__ __forceinline__ void bin_init(const Params &P, double nub, int i_orig_init) {
+__device__ __forceinline__ void bin_init(double nub, int i_orig_init) {
+  const Params &P = c_P;
   if (!P.si_init) return;
-  const int inub = clampi((int)((double)P.nfreq * hcm_log(nub / P.numin) / P.lnurat) + 1, 1, P.nfreq);
+  const int inub =
+      clampi((int)((double)P.nfreq * hcm_log(nub / P.numin) / P.lnurat) + 1, 1, P.nfreq);
   atomic_add(&P.si_init[(size_t)(inub - 1) + (size_t)P.nfreq * (size_t)(i_orig_init - 1)], 1.0);
   atomic_add(&P.si_init[(size_t)(inub - 1)], 1.0);
 }
 
-__device__ void peel_direct(const Params &P, const Pkt &k, double cosb, double sinb,
-                            double coslp, double sinlp, Acc &acc, Trace &tr, bool tracing) {
+template <int NA, bool TRACE>
+__device__ void peel_direct(const Pkt<NA> &k, double cosb, double sinb, double coslp,
+                            double sinlp, Acc &acc, Trace &tr) {
+  const Params &P = c_P;
   for (int q = 0; q < P.npeel; ++q) {
+    double w;
     if (!P.ips) {
       const double cosnorm =
           cosb * P.coste[q] + (sinb * P.sinte[q] * (P.cospe[q] * coslp + P.sinpe[q] * sinlp));
-      const double sipnew = 4.0 * k.s.I * cosnorm;
-      if (cosnorm > 0.0) peeloff(P, k, sipnew, 0, q, acc, tr, tracing);
+      if (!(cosnorm > 0.0)) continue;
+      w = 4.0 * k.s.I * cosnorm;
     } else {
-      peeloff(P, k, k.s.I * 2.0, 0, q, acc, tr, tracing);
+      w = k.s.I * 2.0;
+    }
+    const PeelOut o = peeloff(k, w, 0, q);
+    acc.err_wall += o.err;
+    if constexpr (TRACE) {
+      tr.tau_peel += o.tau;
+      tr.npeelcross += o.ncross;
     }
   }
 }
 
 // Star / hot-spot launch (initpspot.f90:3-175 or initp_ps.f90:3-79, then
 // newdisktherm.F90:256-377).
-__device__ void launch_star(const Params &P, Pkt &k, double *alb, Mrg &rng, bool hotspot,
-                            Acc &acc, Trace &tr, bool tracing) {
+template <int NA, bool TRACE>
+__device__ void launch_star(Pkt<NA> &k, Mrg &rng, bool hotspot, Acc &acc, Trace &tr) {
+  const Params &P = c_P;
   double cosb, sinb, lp;
   int ispotph = hotspot ? 1 : 0;
   k.s = Stokes4{1.0, 0.0, 0.0, 0.0};
@@ -526,19 +638,18 @@ __device__ void launch_star(const Params &P, Pkt &k, double *alb, Mrg &rng, bool
   k.nub = bbfreq(rng, ispotph ? P.tshock : P.tstar);
   if (hotspot) {
     if (ran(rng) < 0.5) k.nub = 1.2398 / 0.06 + ran(rng) * (1.2398 / 0.015 - 1.2398 / 0.06);
-    bin_init(P, k.nub, 3);
+    bin_init(k.nub, 3);
   } else {
-    bin_init(P, k.nub, 2);
+    bin_init(k.nub, 2);
   }
   k.i_orig = 1;
-  set_opacity(P, k, alb, true);
+  set_opacity(k, true);
   const double coslp = hcm_cos(lp), sinlp = hcm_sin(lp);
   k.z = cosb * 1.0;
   const double rp = sinb * 1.0;
   k.x = rp * coslp;
   k.y = rp * sinlp;
   k.rsq = k.z * k.z + rp * rp;
-  k.rtot = sqrt(k.rsq);
   k.ux = k.d.sint * k.d.cosp;
   k.uy = k.d.sint * k.d.sinp;
   k.uz = k.d.cost;
@@ -547,14 +658,15 @@ __device__ void launch_star(const Params &P, Pkt &k, double *alb, Mrg &rng, bool
   k.on_wall = true;
   k.it = locate(P.thetarr, P.ntg, hcm_acos(cosb));
   k.ip = locate(P.phiarr, P.npg, lp);
-  if (P.ipeel) peel_direct(P, k, cosb, sinb, coslp, sinlp, acc, tr, tracing);
+  if (P.ipeel) peel_direct<NA, TRACE>(k, cosb, sinb, coslp, sinlp, acc, tr);
 }
 
 // Star re-emission of a packet that reached the stellar surface.
-__device__ void star_reemit(const Params &P, Pkt &k, double *alb, Mrg &rng, Acc &acc,
-                            Trace &tr, bool tracing) {
-  acc.v[S_AFLUX] += 1.0;
-  const double cosb = clamp1(k.z / k.rtot);
+template <int NA, bool TRACE>
+__device__ void star_reemit(Pkt<NA> &k, Mrg &rng, Acc &acc, Trace &tr) {
+  const Params &P = c_P;
+  acc.aflux++;
+  const double cosb = clamp1(k.z / sqrt(k.rsq));
   const double sinb = sqrt(1.0 - cosb * cosb);
   double lp = hcm_atan2(k.y, k.x);
   if (lp < 0.0) lp = lp + kR2P;
@@ -564,28 +676,30 @@ __device__ void star_reemit(const Params &P, Pkt &k, double *alb, Mrg &rng, Acc 
   k.nub = bbfreq(rng, P.tstar);
   k.i_orig = 1;
   k.iflag = 0;
-  set_opacity(P, k, alb, true);
+  set_opacity(k, true);
   k.ux = k.d.sint * k.d.cosp;
   k.uy = k.d.sint * k.d.sinp;
   k.uz = k.d.cost;
   k.ir = 1;
   k.on_wall = true;
-  if (P.ipeel) peel_direct(P, k, cosb, sinb, hcm_cos(lp), hcm_sin(lp), acc, tr, tracing);
-  if (tracing) tr.nstar++;
+  if (P.ipeel) peel_direct<NA, TRACE>(k, cosb, sinb, hcm_cos(lp), hcm_sin(lp), acc, tr);
+  if constexpr (TRACE) tr.nstar++;
 }
 
 // select_dust.f90 over the active species (zero-density slots add nothing).
-__device__ __forceinline__ int select_dust(const Params &P, const Pkt &k, size_t cell, Mrg &rng) {
+template <int NA>
+__device__ __forceinline__ int select_dust(const Pkt<NA> &k, size_t cell, Mrg &rng) {
+  const Params &P = c_P;
   const double xran = ran(rng);
   const double *dn = P.dens + cell * (size_t)P.nact;
   double opac = 0.0, frac = 0.0;
 #pragma unroll
-  for (int s = 0; s < kMaxSpecies; ++s)
+  for (int s = 0; s < NA; ++s)
     if (s < P.nact) opac = opac + k.kapd[s] * dn[s];
   if (opac == 0.0) return -1;
   const double slack = (double)1.e-10f;
 #pragma unroll
-  for (int s = 0; s < kMaxSpecies; ++s) {
+  for (int s = 0; s < NA; ++s) {
     if (s < P.nact) {
       frac = frac + k.kapd[s] * dn[s] / opac;
       if (xran <= frac + slack) return s;
@@ -594,8 +708,10 @@ __device__ __forceinline__ int select_dust(const Params &P, const Pkt &k, size_t
   return -1;
 }
 
-__device__ void bin_exit(const Params &P, Pkt &k, double xpold, double ypold, double zpold,
-                         int iscat, Acc &acc) {
+template <int NA>
+__device__ void bin_exit(Pkt<NA> &k, double xpold, double ypold, double zpold, int iscat,
+                         Acc &acc) {
+  const Params &P = c_P;
   int it = (int)((double)P.nmu * (1.0 - k.d.cost) / 2.0) + 1;
   it = clampi(it, 1, P.nmu);
   if (k.d.phi < 0.0) k.d.phi = k.d.phi + kR2P;
@@ -605,10 +721,10 @@ __device__ void bin_exit(const Params &P, Pkt &k, double xpold, double ypold, do
   const double y = zpold * P.sini[it - 1] - ypold * ut * k.d.sinp - xpold * ut * k.d.cosp;
   const double rho2 = (x * x + y * y);
   if (rho2 < P.aperture2[P.nap - 1] * 1.0001) {
-    acc.v[S_FLUX] += 1.0;
+    acc.flux++;
     if (k.iflag == 1) {
-      acc.v[S_SFLUX] += 1.0;
-      acc.v[S_NSCAT] += (double)iscat;
+      acc.sflux++;
+      acc.nscat += (uint32_t)iscat;
     }
   }
   const int inub =
@@ -641,56 +757,65 @@ __device__ void bin_exit(const Params &P, Pkt &k, double xpold, double ypold, do
       }
     }
   }
-  if (counted) acc.v[S_SUMSUB] += k.s.I;
-  acc.v[S_SUMALL] += k.s.I;
-  if (k.iflag == 1) acc.v[S_SCOUNT] += 1.0;
+  if (counted) acc.sumsub += k.s.I;
+  acc.sumall += k.s.I;
+  if (k.iflag == 1) acc.scount++;
 }
 
 // propagate.f90:1-325 for one packet; returns 0 escaped, 1 killed, 2 error.
-__device__ int propagate(const Params &P, Pkt &k, double *alb, Mrg &rng, Acc &acc, Trace &tr,
-                         bool tracing) {
+template <int NA, bool TRACE>
+__device__ int propagate(Pkt<NA> &k, Mrg &rng, Acc &acc, Trace &tr) {
+  const Params &P = c_P;
+  LucyAcc<NA> la;
+  la.cell = -1;
   double xpold = k.x, ypold = k.y, zpold = k.z;
-  int iscat = 0, iint = 0, pir = 0, pit = 0, pip = 0;
+  int iscat = 0, iint = 0, pir = 0, pit = 0, pip = 0, status = 0;
   bool first = true;
-  int st = tauint(P, k, alb, -hcm_log(ran(rng)), acc, tr, tracing);
+  int st = tauint<NA, TRACE>(k, -hcm_log(ran(rng)), la, acc, tr);
   for (;;) {
     while (st == ST_STAR) {
-      star_reemit(P, k, alb, rng, acc, tr, tracing);
+      star_reemit<NA, TRACE>(k, rng, acc, tr);
       const double tau = -hcm_log(ran(rng));
       xpold = k.x;
       ypold = k.y;
       zpold = k.z;
-      st = tauint(P, k, alb, tau, acc, tr, tracing);
+      st = tauint<NA, TRACE>(k, tau, la, acc, tr);
     }
-    if (st == ST_ERROR) return 2;
+    if (st == ST_ERROR) {
+      status = 2;
+      break;
+    }
     if (st == ST_EXIT) break;
-    const size_t cell = cell_of(P, k.ir, k.it, k.ip);
+    const size_t cell = cell_of(k.ir, k.it, k.ip);
     if (!first) {
       if (k.ir != pir || k.it != pit || k.ip != pip) iint = 0;
       if (iint > P.iintmax) {
         if (P.killed) atomicAdd(&P.killed[cell], 1);
-        acc.v[S_ABFLUX] += 1.0;
-        return 1;
+        acc.abflux++;
+        status = 1;
+        break;
       }
     }
-    const int slot = select_dust(P, k, cell, rng);
+    const int slot = select_dust(k, cell, rng);
     if (slot < 0) {
-      acc.v[S_ERR_DUST] += 1.0;
-      return 2;
+      acc.err_dust++;
+      status = 2;
+      break;
     }
     const int idust2 = P.act[slot] - 1;
-    if (first) acc.v[S_TOT] += 1.0;
+    if (first) acc.tot++;
     first = false;
-    const double albedo = alb[slot];
-    if (ran(rng) <= albedo) {
+    if (ran(rng) <= k.alb[slot]) {
       k.iflag = 1;
       ++iscat;
       ++iint;
-      const HG h = hg_of(P, idust2, k.nub);
-      if (P.ipeel)
-        for (int q = 0; q < P.npeel; ++q) peeloff(P, k, k.s.I, idust2, q, acc, tr, tracing);
-      stokes(k.d, k.s, h, rng, acc.v[S_HTOT], acc.v[S_HIT]);
-      if (tracing) tr.nscat++;
+      const HG h = hg_of(idust2, k.nub);
+      if (P.ipeel) peel_all<NA, TRACE>(k, k.s.I, idust2, acc, tr);
+      double ntr = 0.0, nht = 0.0;
+      stokes(k.d, k.s, h, rng, ntr, nht);
+      acc.htot += (uint32_t)ntr;
+      acc.hit += (uint32_t)nht;
+      if constexpr (TRACE) tr.nscat++;
     } else {
       if (P.dbig) atomicAdd(&P.dbig[cell], 1);
       k.i_orig = (idust2 == 0 || idust2 == 1 || idust2 == 4 || idust2 == 5) ? 2 : 3;
@@ -701,16 +826,15 @@ __device__ int propagate(const Params &P, Pkt &k, double *alb, Mrg &rng, Acc &ac
       k.d.phi = kR2P * ran(rng);
       k.d.cosp = hcm_cos(k.d.phi);
       k.d.sinp = hcm_sin(k.d.phi);
-      k.nub = lucydustfreq(P, rng, P.tdust[cell * (size_t)P.nact + slot], idust2);
+      k.nub = lucydustfreq(P.tdust[cell * (size_t)P.nact + slot], idust2, rng);
       k.uy = k.d.sint * k.d.sinp;
       k.ux = k.d.sint * k.d.cosp;
       k.uz = k.d.cost;
       k.iflag = 0;
-      set_opacity(P, k, alb, false);
-      if (P.ipeel)
-        for (int q = 0; q < P.npeel; ++q) peeloff(P, k, k.s.I, idust2, q, acc, tr, tracing);
+      set_opacity(k, false);
+      if (P.ipeel) peel_all<NA, TRACE>(k, k.s.I, idust2, acc, tr);
       ++iint;
-      if (tracing) tr.nabs++;
+      if constexpr (TRACE) tr.nabs++;
     }
     const double tau = -hcm_log(ran(rng));
     xpold = k.x;
@@ -722,10 +846,11 @@ __device__ int propagate(const Params &P, Pkt &k, double *alb, Mrg &rng, Acc &ac
     pir = k.ir;
     pit = k.it;
     pip = k.ip;
-    st = tauint(P, k, alb, tau, acc, tr, tracing);
+    st = tauint<NA, TRACE>(k, tau, la, acc, tr);
   }
-  bin_exit(P, k, xpold, ypold, zpold, iscat, acc);
-  return 0;
+  lucy_flush(la);
+  if (status == 0) bin_exit(k, xpold, ypold, zpold, iscat, acc);
+  return status;
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -733,23 +858,21 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-__global__ __launch_bounds__(256) void transport_fp64(Params P) {
-  Acc acc;
-#pragma unroll
-  for (int i = 0; i < S_COUNT; ++i) acc.v[i] = 0.0;
-  const bool tracing = P.trace != nullptr;
+template <int NA, bool TRACE>
+__global__ __launch_bounds__(256) void transport_fp64() {
+  const Params &P = c_P;
+  Acc acc{};
   for (;;) {
     const unsigned long long p = atomicAdd(P.next_packet, 1ull);
     if (p >= (unsigned long long)P.np) break;
     Mrg rng = mrg_segment(P.g1, P.g2, P.seg0 + p, P.pow1, P.pow2);
-    Pkt k;
-    double alb[kMaxSpecies];
+    Pkt<NA> k;
     Trace tr{0.0, 0.0, 0, 0, 0, 0, 0, 14695981039346656037ull};
     const long long iphot = (long long)p + 1;
-    launch_star(P, k, alb, rng, iphot < P.npacc + P.npaccstar, acc, tr, tracing);
-    if (tracing) hash_cell(tr, k.ir, k.it, k.ip);
-    const int status = propagate(P, k, alb, rng, acc, tr, tracing);
-    if (tracing) {
+    launch_star<NA, TRACE>(k, rng, iphot < P.npacc + P.npaccstar, acc, tr);
+    if constexpr (TRACE) hash_cell(tr, k.ir, k.it, k.ip);
+    const int status = propagate<NA, TRACE>(k, rng, acc, tr);
+    if constexpr (TRACE) {
       hc_trace &o = P.trace[p];
       o.x = k.x;
       o.y = k.y;
@@ -778,10 +901,15 @@ __global__ __launch_bounds__(256) void transport_fp64(Params P) {
       o.pad = 0;
     }
   }
+  const double v[S_COUNT] = {(double)acc.tot,    (double)acc.flux,     (double)acc.sflux,
+                             (double)acc.nscat,  (double)acc.scount,   (double)acc.abflux,
+                             (double)acc.aflux,  acc.sumsub,           acc.sumall,
+                             (double)acc.hit,    (double)acc.htot,     (double)acc.err_wall,
+                             (double)acc.err_dust, 0.0};
 #pragma unroll
   for (int i = 0; i < S_COUNT; ++i) {
-    const double v = wave_sum(acc.v[i]);
-    if ((threadIdx.x & 63) == 0 && v != 0.0) atomic_add(&P.scal[i], v);
+    const double w = wave_sum(v[i]);
+    if ((threadIdx.x & 63) == 0 && w != 0.0) atomic_add(&P.scal[i], w);
   }
 }
 
@@ -829,7 +957,8 @@ __global__ __launch_bounds__(256) void lucy_temperature(TParams T) {
       int iter = 0;
       while (fabs(Td - Tprev) > 1.e-5 * Td && iter < 1000) {
         Tprev = Td;
-        Td = hcm_pow((na * T.ltot) / (4.0 * kSigT * T.nphot * T.kappav[id] * kapp(T, Td, id)), 0.25) /
+        Td = hcm_pow((na * T.ltot) / (4.0 * kSigT * T.nphot * T.kappav[id] * kapp(T, Td, id)),
+                     0.25) /
              11605.0;
         Td = Td / m4;
         if (Td > 1600.0 / 11605.0) Td = 1600.0 / 11605.0;
@@ -837,8 +966,8 @@ __global__ __launch_bounds__(256) void lucy_temperature(TParams T) {
         ++iter;
       }
       tnew = Td;
+      if (fabs(tnew - told) > 0.01 * told) ++changed;
     }
-    if (fabs(tnew - told) > 0.01 * told) ++changed;
     T.tdust2[i] = tnew;
   }
   for (int off = 32; off > 0; off >>= 1) changed += __shfl_xor(changed, off, 64);
@@ -880,7 +1009,8 @@ struct hc_ctx {
          *kappap = nullptr;
   double *u = nullptr, *sini = nullptr, *aperture2 = nullptr, *coste = nullptr,
          *sinte = nullptr, *cospe = nullptr, *sinpe = nullptr;
-  int32_t *pow1 = nullptr, *pow2 = nullptr;
+  int32_t *pow1 = nullptr, *pow2 = nullptr, *guide = nullptr;
+  double glog0[kMaxSpecies] = {0}, ginv[kMaxSpecies] = {0};
   double *dtauabs = nullptr, *nabs = nullptr, *jmean = nullptr;
   int32_t *killed = nullptr, *dbig = nullptr;
   double *sed = nullptr, *peel = nullptr, *img_i = nullptr, *img_q = nullptr, *img_u = nullptr,
@@ -1088,6 +1218,28 @@ int hc_set_dust(hc_ctx *ctx, const int32_t *nlambda, const double *nudust, const
   if (!rc) rc = upload(ctx, &ctx->kapBint, kapBint, (size_t)d.nT * d.nnu * d.ndg);
   if (!rc) rc = upload(ctx, &ctx->kappap, kappap, (size_t)d.nT * d.ndg);
   if (rc) return rc;
+  // guide table: for log-frequency bin b of species id, the bracket index i1
+  // (1-based, nud[i1-1] < nu) of the bin's largest frequency
+  std::vector<int32_t> guide((size_t)kMaxSpecies * kGuide, 2);
+  for (int id = 0; id < d.ndg; ++id) {
+    const double *nud = nudust + (size_t)d.nlammax * id;
+    const int nl = nlambda[id];
+    for (int i = 1; i < nl; ++i)
+      if (!(nud[i] < nud[i - 1]))
+        return fail(ctx, HC_E_ARG, "hc_set_dust: frequencies must decrease strictly");
+    if (nl < 2) continue;
+    const double L = std::log(nud[0] / nud[nl - 1]);
+    ctx->glog0[id] = std::log(nud[0]);
+    ctx->ginv[id] = (double)kGuide / L;
+    int i1 = 2;
+    for (int b = 0; b < kGuide; ++b) {
+      const double nub = nud[0] * std::exp(-(double)b * L / (double)kGuide) * (1.0 + 1e-9);
+      while (i1 < nl && !(nub > nud[i1 - 1])) ++i1;
+      guide[(size_t)id * kGuide + b] = std::max(2, i1 - 1);
+    }
+  }
+  rc = upload(ctx, &ctx->guide, guide.data(), guide.size());
+  if (rc) return rc;
   ctx->have_dust = true;
   return HC_OK;
 }
@@ -1183,9 +1335,10 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
   P.nact = ctx->nact;
   for (int i = 0; i < kMaxSpecies; ++i) {
     P.act[i] = ctx->act[i];
-    P.slot[i] = ctx->slot[i];
     P.nlambda[i] = ctx->nlambda[i];
     P.kappav[i] = ctx->kappav[i];
+    P.glog0[i] = ctx->glog0[i];
+    P.ginv[i] = ctx->ginv[i];
   }
   P.r2arr = ctx->r2arr;
   P.tan2arr = ctx->tan2arr;
@@ -1207,6 +1360,7 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
   P.lTint = ctx->lTint;
   P.nuint = ctx->nuint;
   P.kapBint = ctx->kapBint;
+  P.guide = ctx->guide;
   const hc_sources &S = ctx->src;
   P.rstar = S.rstar;
   P.rmincgs = S.rstar * kRsol;
@@ -1266,6 +1420,7 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
   P.sed = ctx->sed;
   P.peel = P.ipeel ? ctx->peel : nullptr;
   P.img_i = cube ? ctx->img_i : nullptr;
+  P.imcube = cube ? 1 : 0;
   P.img_q = ctx->img_q;
   P.img_u = ctx->img_u;
   P.img_v = ctx->img_v;
@@ -1274,13 +1429,19 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
   P.next_packet = ctx->counter;
   P.trace = trace ? ctx->trace : nullptr;
 
+  HCK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_P), &P, sizeof P, 0, hipMemcpyHostToDevice, s));
+  void (*kern)() = nullptr;
+  if (ctx->nact <= 4)
+    kern = trace ? transport_fp64<4, true> : transport_fp64<4, false>;
+  else
+    kern = trace ? transport_fp64<8, true> : transport_fp64<8, false>;
   int dev_cus = 0, bpm = 0;
   HCK(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-  HCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpm, transport_fp64, 256, 0));
+  HCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpm, kern, 256, 0));
   long long blocks = (long long)std::max(dev_cus, 1) * std::max(bpm, 1);
   blocks = std::min<long long>(blocks, std::max<long long>((np + 255) / 256, 1));
   HCK(hipEventRecord(ctx->ev0, s));
-  if (np > 0) hipLaunchKernelGGL(transport_fp64, dim3((unsigned)blocks), dim3(256), 0, s, P);
+  if (np > 0) hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), 0, s);
   HCK(hipGetLastError());
   HCK(hipEventRecord(ctx->ev1, s));
   HCK(hipStreamSynchronize(s));
@@ -1410,7 +1571,7 @@ int hc_destroy(hc_ctx *ctx) {
                   ctx->sinpe,  ctx->pow1,    ctx->pow2,    ctx->dtauabs, ctx->nabs,   ctx->jmean,
                   ctx->killed, ctx->dbig,    ctx->sed,     ctx->peel,    ctx->img_i,  ctx->img_q,
                   ctx->img_u,  ctx->img_v,   ctx->si_init, ctx->scal,    ctx->mass4,  ctx->counter,
-                  ctx->trace};
+                  ctx->trace,  ctx->guide};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);

@@ -454,7 +454,8 @@ double hco_dusttemp(const hco_dust *d, double na, double T0, double ltot, double
  * sum_k kapd*(1-albedo)*rho*ds per cell and species, so the cell absorbs
  * (L/N)*dtauabs, and dusttemp's balance with na -> dtauabs gives T.  Cells
  * outside the (nrg-1,ntg-1,npg-1) block and species with zero mass keep
- * 0.1 K, the initarr.f90:53 initial value. */
+ * 0.1 K, the initarr.f90:53 initial value.  Returns the number of dusty
+ * (cell, species) pairs whose T moved by more than 1 %. */
 int64_t hco_tfinal(const hco_grid *g, const hco_dust *d, const double *dtauabs, double ltot,
                    double nphot, double *tdust2) {
   const size_t ncell = (size_t)g->nrg * g->ntg * g->npg;
@@ -467,9 +468,10 @@ int64_t hco_tfinal(const hco_grid *g, const hco_dust *d, const double *dtauabs, 
           size_t c = (size_t)ir + (size_t)g->nrg * ((size_t)it + (size_t)g->ntg * ip);
           size_t k = c + ncell * id;
           double tnew = 0.1 / 11605.0;
-          if (ir < g->nrg - 1 && it < g->ntg - 1 && ip < g->npg - 1 && g->massarr[k] > 0.0)
+          if (ir < g->nrg - 1 && it < g->ntg - 1 && ip < g->npg - 1 && g->massarr[k] > 0.0) {
             tnew = hco_dusttemp(d, dtauabs[k], g->tdust[k], ltot, nphot, g->massarr[k], id);
-          if (fabs(tnew - g->tdust[k]) > 0.01 * g->tdust[k]) nchanged++;
+            if (fabs(tnew - g->tdust[k]) > 0.01 * g->tdust[k]) nchanged++;
+          }
           tdust2[k] = tnew;
         }
   return nchanged;
