@@ -59,7 +59,7 @@ class HcTallies(C.Structure):
         "hit", "htot")] + [
         ("n_err_nowall", C.c_int64), ("n_err_dust", C.c_int64), ("n_warn", C.c_int64),
         ("kernel_ms", C.c_double),
-    ]
+    ] + [(n, C.c_double) for n in ("ncross", "nscat_ev", "nabs_ev", "npeelray", "npeelcross")]
 
 
 TRACE_DTYPE = np.dtype([
@@ -71,7 +71,8 @@ TRACE_DTYPE = np.dtype([
 ], align=True)
 
 SCALARS = ("tot", "flux", "sflux", "nscat", "scount", "abflux", "aflux", "sumsub", "sumall",
-           "hit", "htot", "n_err_nowall", "n_err_dust", "n_warn")
+           "hit", "htot", "n_err_nowall", "n_err_dust", "n_warn", "ncross", "nscat_ev",
+           "nabs_ev", "npeelray", "npeelcross")
 
 
 def library_path() -> str:
@@ -203,6 +204,16 @@ class Transport:
     def set_temperature(self, tdust: np.ndarray):
         self._check(self.lib.hc_set_temperature(self.h, _ptr(_f64(tdust))), "hc_set_temperature")
 
+    def run_scalars(self, np_: int, ipeel: int = 0, seed1: int = 12345678,
+                    seed2: int = 87654321, gen: int = 1, seg0: int = 0):
+        """A pass whose tallies stay on the device (the next Lucy step reads
+        them there); returns the scalar tallies and the kernel time in ms."""
+        t = HcTallies()
+        rc = self.lib.hc_run_iteration(self.h, int(np_), int(ipeel), int(seed1), int(seed2),
+                                       int(gen), int(seg0), C.byref(t), None)
+        self._check(rc, "hc_run_iteration")
+        return {k: getattr(t, k) for k in SCALARS}, t.kernel_ms
+
     def run(self, np_: int, ipeel: int = 0, seed1: int = 12345678, seed2: int = 87654321,
             gen: int = 1, seg0: int = 0, trace: bool = False, images: bool = True) -> PassResult:
         m = self.model
@@ -242,12 +253,19 @@ class Transport:
         return PassResult(dtauabs, nabs, jmean, killed, dbig, sed, peel, img_i, img_q, img_u,
                           img_v, si_init, scal, tr, t.kernel_ms)
 
-    def temperature(self, np_: int):
+    def temperature(self, np_: int, apply: bool = False):
+        """Lucy temperature update from the last pass.  apply=False returns the
+        new temperatures (reference layout); apply=True installs them on the
+        device and returns None."""
         m = self.model
-        t2 = np.zeros((m.nrg, m.ntg, m.npg, 8), order="F")
+        t2 = None if apply else np.zeros((m.nrg, m.ntg, m.npg, 8), order="F")
         n = C.c_int64(0)
-        self._check(self.lib.hc_temperature(self.h, _ptr(_f64(m.massarr)), float(m.ltot),
-                                            float(np_), _ptr(t2), C.byref(n)), "hc_temperature")
+        mass = None
+        if not getattr(self, "_mass_sent", False):
+            mass = _f64(m.massarr)
+            self._mass_sent = True
+        self._check(self.lib.hc_temperature(self.h, _ptr(mass), float(m.ltot), float(np_),
+                                            _ptr(t2), C.byref(n)), "hc_temperature")
         return t2, int(n.value)
 
     def close(self):

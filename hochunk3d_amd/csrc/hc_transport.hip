@@ -74,7 +74,8 @@ __constant__ Params c_P;
 
 enum Scalar {
   S_TOT = 0, S_FLUX, S_SFLUX, S_NSCAT, S_SCOUNT, S_ABFLUX, S_AFLUX, S_SUMSUB, S_SUMALL,
-  S_HIT, S_HTOT, S_ERR_WALL, S_ERR_DUST, S_WARN, S_COUNT
+  S_HIT, S_HTOT, S_ERR_WALL, S_ERR_DUST, S_WARN, S_NCROSS, S_NSCAT_EV, S_NABS_EV, S_NPEELRAY,
+  S_NPEELCROSS, S_COUNT
 };
 
 // ---------------------------------------------------------------------------
@@ -94,6 +95,7 @@ struct Pkt {
 
 struct Acc {  // per-lane counters, flushed once per lane
   uint32_t tot, flux, sflux, nscat, scount, abflux, aflux, hit, htot, err_wall, err_dust;
+  uint32_t ncross, nscat_ev, nabs_ev, npeelray, npeelcross;
   double sumsub, sumall;
 };
 
@@ -106,6 +108,8 @@ struct LucyAcc {
   long long cell;
   double tau[NA];
   double len;
+  double nabs[NA];  // absorptions per species (nabs)
+  int nabs_any;     // absorptions of any species (dbig)
 };
 
 struct Trace {
@@ -180,7 +184,29 @@ __device__ __forceinline__ void lucy_flush(LucyAcc<NA> &la) {
       if (s < P.nact && la.tau[s] != 0.0) atomic_add(&ta[s], la.tau[s]);
   }
   if (P.jmean && la.len != 0.0) atomic_add(&P.jmean[la.cell], la.len * P.rmincgs);
+  if (la.nabs_any) {
+    double *na = P.nabs + (size_t)la.cell * (size_t)P.nact;
+#pragma unroll
+    for (int s = 0; s < NA; ++s)
+      if (s < P.nact && la.nabs[s] != 0.0) atomic_add(&na[s], la.nabs[s]);
+    atomicAdd(&P.dbig[la.cell], la.nabs_any);
+  }
   la.cell = -1;
+}
+
+template <int NA>
+__device__ __forceinline__ void lucy_open(LucyAcc<NA> &la, size_t cell) {
+  if (la.cell != (long long)cell) {
+    lucy_flush(la);
+    la.cell = (long long)cell;
+#pragma unroll
+    for (int s = 0; s < NA; ++s) {
+      la.tau[s] = 0.0;
+      la.nabs[s] = 0.0;
+    }
+    la.len = 0.0;
+    la.nabs_any = 0;
+  }
 }
 
 // Add one chord of cell `cell` to the per-cell accumulator: dtauabs gains
@@ -189,13 +215,7 @@ template <int NA>
 __device__ __forceinline__ void lucy_add(LucyAcc<NA> &la, const Pkt<NA> &k, const double *dm,
                                          size_t cell, double ds) {
   if (!c_P.dtauabs && !c_P.jmean) return;
-  if (la.cell != (long long)cell) {
-    lucy_flush(la);
-    la.cell = (long long)cell;
-#pragma unroll
-    for (int s = 0; s < NA; ++s) la.tau[s] = 0.0;
-    la.len = 0.0;
-  }
+  lucy_open(la, cell);
 #pragma unroll
   for (int s = 0; s < NA; ++s)
     if (s < c_P.nact && dm[s] != 0.0) la.tau[s] += k.kapd[s] * (1.0 - k.alb[s]) * dm[s] * ds;
@@ -247,6 +267,7 @@ __device__ int tauint(Pkt<NA> &k, double tau, LucyAcc<NA> &la, Acc &acc, Trace &
     k.z = k.z + dt * k.uz;
     k.on_wall = true;
     k.rsq = k.x * k.x + k.y * k.y + k.z * k.z;
+    acc.ncross++;
     if constexpr (TRACE) {
       tr.path += dt;
       tr.ncross++;
@@ -415,6 +436,8 @@ __device__ __forceinline__ void peel_all(const Pkt<NA> &k, double sip, int idust
   for (int q = 0; q < c_P.npeel; ++q) {
     const PeelOut o = peeloff(k, sip, idust2, q);
     acc.err_wall += o.err;
+    acc.npeelray++;
+    acc.npeelcross += (uint32_t)o.ncross;
     if constexpr (TRACE) {
       tr.tau_peel += o.tau;
       tr.npeelcross += o.ncross;
@@ -581,6 +604,8 @@ __device__ void peel_direct(const Pkt<NA> &k, double cosb, double sinb, double c
     }
     const PeelOut o = peeloff(k, w, 0, q);
     acc.err_wall += o.err;
+    acc.npeelray++;
+    acc.npeelcross += (uint32_t)o.ncross;
     if constexpr (TRACE) {
       tr.tau_peel += o.tau;
       tr.npeelcross += o.ncross;
@@ -815,11 +840,15 @@ __device__ int propagate(Pkt<NA> &k, Mrg &rng, Acc &acc, Trace &tr) {
       stokes(k.d, k.s, h, rng, ntr, nht);
       acc.htot += (uint32_t)ntr;
       acc.hit += (uint32_t)nht;
+      acc.nscat_ev++;
       if constexpr (TRACE) tr.nscat++;
     } else {
-      if (P.dbig) atomicAdd(&P.dbig[cell], 1);
+      lucy_open(la, cell);
+      la.nabs_any += 1;
+#pragma unroll
+      for (int s2 = 0; s2 < NA; ++s2)
+        if (s2 == slot) la.nabs[s2] += 1.0;
       k.i_orig = (idust2 == 0 || idust2 == 1 || idust2 == 4 || idust2 == 5) ? 2 : 3;
-      if (P.nabs) atomic_add(&P.nabs[cell * (size_t)P.nact + slot], 1.0);
       k.s = Stokes4{1.0, 0.0, 0.0, 0.0};
       k.d.cost = 1.0 - 2.0 * ran(rng);
       k.d.sint = sqrt(1.0 - hcm_pow(k.d.cost, 2.0));
@@ -834,6 +863,7 @@ __device__ int propagate(Pkt<NA> &k, Mrg &rng, Acc &acc, Trace &tr) {
       set_opacity(k, false);
       if (P.ipeel) peel_all<NA, TRACE>(k, k.s.I, idust2, acc, tr);
       ++iint;
+      acc.nabs_ev++;
       if constexpr (TRACE) tr.nabs++;
     }
     const double tau = -hcm_log(ran(rng));
@@ -905,7 +935,9 @@ __global__ __launch_bounds__(256) void transport_fp64() {
                              (double)acc.nscat,  (double)acc.scount,   (double)acc.abflux,
                              (double)acc.aflux,  acc.sumsub,           acc.sumall,
                              (double)acc.hit,    (double)acc.htot,     (double)acc.err_wall,
-                             (double)acc.err_dust, 0.0};
+                             (double)acc.err_dust, 0.0,
+                             (double)acc.ncross, (double)acc.nscat_ev, (double)acc.nabs_ev,
+                             (double)acc.npeelray, (double)acc.npeelcross};
 #pragma unroll
   for (int i = 0; i < S_COUNT; ++i) {
     const double w = wave_sum(v[i]);
@@ -1014,7 +1046,8 @@ struct hc_ctx {
   double *dtauabs = nullptr, *nabs = nullptr, *jmean = nullptr;
   int32_t *killed = nullptr, *dbig = nullptr;
   double *sed = nullptr, *peel = nullptr, *img_i = nullptr, *img_q = nullptr, *img_u = nullptr,
-         *img_v = nullptr, *si_init = nullptr, *scal = nullptr, *mass4 = nullptr;
+         *img_v = nullptr, *si_init = nullptr, *scal = nullptr, *mass4 = nullptr,
+         *tdust2 = nullptr;
   unsigned long long *counter = nullptr;
   hc_trace *trace = nullptr;
   size_t trace_cap = 0;
@@ -1466,6 +1499,11 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
   out->n_err_nowall = (int64_t)scal[S_ERR_WALL];
   out->n_err_dust = (int64_t)scal[S_ERR_DUST];
   out->n_warn = (int64_t)scal[S_WARN];
+  out->ncross = scal[S_NCROSS];
+  out->nscat_ev = scal[S_NSCAT_EV];
+  out->nabs_ev = scal[S_NABS_EV];
+  out->npeelray = scal[S_NPEELRAY];
+  out->npeelcross = scal[S_NPEELCROSS];
   std::vector<double> tmp;
   auto per_species = [&](double *dev, double *host) -> int {
     if (!host) return HC_OK;
@@ -1510,18 +1548,23 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
 
 int hc_temperature(hc_ctx *ctx, const double *massarr, double ltot, double nphot, double *tdust2,
                    int64_t *nchanged) {
-  if (!ctx || !massarr || !tdust2) return HC_E_ARG;
+  if (!ctx) return HC_E_ARG;
   if (!ctx->have_grid || !ctx->have_dust || !ctx->dtauabs)
     return fail(ctx, HC_E_STATE, "hc_temperature needs a completed pass");
+  if (!massarr && !ctx->mass4)
+    return fail(ctx, HC_E_ARG, "hc_temperature: massarr required on the first call");
   const hc_dims &d = ctx->dims;
   HCK(hipSetDevice(ctx->device));
   const size_t nc = ctx->ncellw * (size_t)ctx->nact;
   std::vector<double> tmp;
-  to_device_layout(ctx, massarr, tmp);
-  int rc = upload(ctx, &ctx->mass4, tmp.data(), nc);
+  int rc = 0;
+  if (massarr) {
+    to_device_layout(ctx, massarr, tmp);
+    rc = upload(ctx, &ctx->mass4, tmp.data(), nc);
+    if (rc) return rc;
+  }
+  rc = dalloc(ctx, &ctx->tdust2, nc);
   if (rc) return rc;
-  double *t2 = nullptr;
-  HCK(hipMalloc((void **)&t2, nc * sizeof(double)));
   TParams T;
   std::memset(&T, 0, sizeof T);
   T.nrg = d.nrg;
@@ -1538,7 +1581,7 @@ int hc_temperature(hc_ctx *ctx, const double *massarr, double ltot, double nphot
   T.dtauabs = ctx->dtauabs;
   T.mass4 = ctx->mass4;
   T.tdust = ctx->tdust;
-  T.tdust2 = t2;
+  T.tdust2 = ctx->tdust2;
   T.ltot = ltot;
   T.nphot = nphot;
   T.nchanged = ctx->counter;
@@ -1546,17 +1589,26 @@ int hc_temperature(hc_ctx *ctx, const double *massarr, double ltot, double nphot
   const unsigned blocks = (unsigned)std::min<size_t>((nc + 255) / 256, 4096);
   hipLaunchKernelGGL(lucy_temperature, dim3(std::max(blocks, 1u)), dim3(256), 0, ctx->stream, T);
   HCK(hipGetLastError());
-  HCK(hipStreamSynchronize(ctx->stream));
-  tmp.resize(nc);
-  HCK(hipMemcpy(tmp.data(), t2, nc * sizeof(double), hipMemcpyDeviceToHost));
   unsigned long long nch = 0;
-  HCK(hipMemcpy(&nch, ctx->counter, sizeof nch, hipMemcpyDeviceToHost));
-  HCK(hipFree(t2));
-  for (size_t i = 0; i < ctx->ncellw * (size_t)d.ndg; ++i) tdust2[i] = 0.1 / 11605.0;
-  for (int k = 0; k < ctx->nact; ++k) {
-    double *dst = tdust2 + ctx->ncellw * (size_t)(ctx->act[k] - 1);
-    for (size_t c = 0; c < ctx->ncellw; ++c) dst[c] = tmp[c * (size_t)ctx->nact + k];
+  HCK(hipMemcpyAsync(&nch, ctx->counter, sizeof nch, hipMemcpyDeviceToHost, ctx->stream));
+  if (tdust2) {
+    tmp.resize(nc);
+    HCK(hipMemcpyAsync(tmp.data(), ctx->tdust2, nc * sizeof(double), hipMemcpyDeviceToHost,
+                       ctx->stream));
+    HCK(hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < ctx->ncellw * (size_t)d.ndg; ++i) tdust2[i] = 0.1 / 11605.0;
+    for (int k = 0; k < ctx->nact; ++k) {
+      double *dst = tdust2 + ctx->ncellw * (size_t)(ctx->act[k] - 1);
+      for (size_t c = 0; c < ctx->ncellw; ++c) dst[c] = tmp[c * (size_t)ctx->nact + k];
+    }
+  } else {
+    // apply on the device: tdust <- tdust2 and refresh the sublimation mask
+    HCK(hipMemcpyAsync(ctx->tdust, ctx->tdust2, nc * sizeof(double), hipMemcpyDeviceToDevice,
+                       ctx->stream));
+    const int rc2 = refresh_mask(ctx);
+    if (rc2) return rc2;
   }
+  HCK(hipStreamSynchronize(ctx->stream));
   if (nchanged) *nchanged = (int64_t)nch;
   return HC_OK;
 }
@@ -1571,7 +1623,7 @@ int hc_destroy(hc_ctx *ctx) {
                   ctx->sinpe,  ctx->pow1,    ctx->pow2,    ctx->dtauabs, ctx->nabs,   ctx->jmean,
                   ctx->killed, ctx->dbig,    ctx->sed,     ctx->peel,    ctx->img_i,  ctx->img_q,
                   ctx->img_u,  ctx->img_v,   ctx->si_init, ctx->scal,    ctx->mass4,  ctx->counter,
-                  ctx->trace,  ctx->guide};
+                  ctx->trace,  ctx->guide,   ctx->tdust2};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);

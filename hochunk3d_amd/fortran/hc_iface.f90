@@ -1,0 +1,109 @@
+! hc_iface.f90 -- ISO_C_BINDING interfaces of libhochunk3d_amd.so (include/hc.h)
+! for a Fortran driver.  Module arrays of grid_mod / dust_mod / tts_mod are
+! passed as they are: column-major, the reference's dimensions.
+module hc_iface
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: HC_FP64 = 0, HC_FP32 = 1
+
+  type, bind(C) :: hc_dims
+    integer(c_int32_t) :: nrg, ntg, npg, ndg
+    integer(c_int32_t) :: nfreq, nmu, nph, nap, npeel, nxhst
+    integer(c_int32_t) :: nlammax, nT, nnu
+    integer(c_int32_t) :: precision, imcube
+  end type hc_dims
+
+  type, bind(C) :: hc_sources
+    real(c_double) :: rstar, tstar, tshock
+    integer(c_int32_t) :: limb, ips, ispot, nspot
+    real(c_double) :: fspot, csprad, cspot1, sspot1, scspot1, ssspot1, cspot2, sspot2, &
+                      scspot2, ssspot2
+    real(c_double) :: accfrac, accfrac2, numin, lnurat, rmaxi, ltot
+    integer(c_int32_t) :: iintmax, pad
+    type(c_ptr) :: u, aperture2, coste, sinte, cospe, sinpe
+  end type hc_sources
+
+  type, bind(C) :: hc_tallies
+    type(c_ptr) :: dtauabs, nabs, jmean, killed, dbig, dsg, sed, peel
+    type(c_ptr) :: img_i, img_q, img_u, img_v, si_init
+    real(c_double) :: tot, flux, sflux, nscat, scount, abflux, aflux, sumsub, sumall, hit, htot
+    integer(c_int64_t) :: n_err_nowall, n_err_dust, n_warn
+    real(c_double) :: kernel_ms
+    real(c_double) :: ncross, nscat_ev, nabs_ev, npeelray, npeelcross
+  end type hc_tallies
+
+  interface
+    integer(c_int) function hc_create(dims, device, ctx) bind(C, name='hc_create')
+      import :: c_int, c_ptr, hc_dims
+      type(hc_dims), intent(in) :: dims
+      integer(c_int), value :: device
+      type(c_ptr), intent(out) :: ctx
+    end function hc_create
+
+    integer(c_int) function hc_set_grid(ctx, r2arr, tan2arr, thetarr, aarr, barr, phiarr, &
+                                        densarr, tdust, tsub) bind(C, name='hc_set_grid')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: r2arr(*), tan2arr(*), thetarr(*), aarr(*), barr(*), &
+                                    phiarr(*), densarr(*), tdust(*)
+      real(c_double), value :: tsub
+    end function hc_set_grid
+
+    integer(c_int) function hc_set_temperature(ctx, tdust) bind(C, name='hc_set_temperature')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: tdust(*)
+    end function hc_set_temperature
+
+    integer(c_int) function hc_set_dust(ctx, nlambda, nudust, kappad, adust, gdust, pldust, &
+                                        kappav, lTint, nuint, kapBint, kappap) &
+                                        bind(C, name='hc_set_dust')
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(in) :: nlambda(*)
+      real(c_double), intent(in) :: nudust(*), kappad(*), adust(*), gdust(*), pldust(*), &
+                                    kappav(*), lTint(*), nuint(*), kapBint(*), kappap(*)
+    end function hc_set_dust
+
+    integer(c_int) function hc_set_sources(ctx, src) bind(C, name='hc_set_sources')
+      import :: c_int, c_ptr, hc_sources
+      type(c_ptr), value :: ctx
+      type(hc_sources), intent(in) :: src
+    end function hc_set_sources
+
+    integer(c_int) function hc_run_iteration(ctx, np, ipeel, seed1, seed2, gen, seg0, out, &
+                                             trace) bind(C, name='hc_run_iteration')
+      import :: c_int, c_ptr, c_int64_t, c_int32_t, hc_tallies
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: np, seg0
+      integer(c_int32_t), value :: ipeel, seed1, seed2, gen
+      type(hc_tallies), intent(inout) :: out
+      type(c_ptr), value :: trace
+    end function hc_run_iteration
+
+    integer(c_int) function hc_temperature(ctx, massarr, ltot, nphot, tdust2, nchanged) &
+                                           bind(C, name='hc_temperature')
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: massarr(*)
+      real(c_double), value :: ltot, nphot
+      real(c_double), intent(out) :: tdust2(*)
+      integer(c_int64_t), intent(out) :: nchanged
+    end function hc_temperature
+
+    integer(c_int) function hc_destroy(ctx) bind(C, name='hc_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function hc_destroy
+
+    type(c_ptr) function hc_last_error(ctx) bind(C, name='hc_last_error')
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function hc_last_error
+
+    integer(c_int) function hc_abi_version() bind(C, name='hc_abi_version')
+      import :: c_int
+    end function hc_abi_version
+  end interface
+end module hc_iface

@@ -315,3 +315,14 @@ def config1(**over) -> ModelConfig:
     for k, v in over.items():
         setattr(c, k, v)
     return c
+
+
+def config2(**over) -> ModelConfig:
+    """The full-3-D FP64 correctness model: the disk + envelope + star of
+    config1 with a fractal clumpy envelope (4 levels, 32 clumps per level,
+    D = 2.6, seed 7) on 200 x 199 x 200 cells (~8M)."""
+    c = ModelConfig(nr=200, ntheta=199, nphi=200,
+                    fractal={"n": 32, "levels": 4, "fl": 3.792, "densratio": 0.75, "seed": 7})
+    for k, v in over.items():
+        setattr(c, k, v)
+    return c

@@ -85,6 +85,9 @@ typedef struct {
   double tot, flux, sflux, nscat, scount, abflux, aflux, sumsub, sumall, hit, htot;
   int64_t n_err_nowall, n_err_dust, n_warn;
   double kernel_ms;        /* device time of the transport launch(es) */
+  /* work counters of the pass: wall crossings of transport rays, scattering
+   * and absorption events, peel-off rays and their wall crossings */
+  double ncross, nscat_ev, nabs_ev, npeelray, npeelcross;
 } hc_tallies;
 
 /* Per-packet record of the deterministic-trace mode. */
@@ -128,8 +131,11 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
 
 /* Lucy temperature update on the device from the last pass's dtauabs
  * (reconstructed tfinal, newdisktherm.F90:557 with dusttemp.f90:6-58).
- * massarr (nrg,ntg,npg,ndg) = (rho V)^(1/4); writes tdust2 and the count of
- * (cell,species) whose T moved by more than 1 %. */
+ * massarr (nrg,ntg,npg,ndg) = (rho V)^(1/4); NULL reuses the one given to the
+ * previous call.  With tdust2 non-NULL the new temperatures are written there
+ * (the caller applies them with hc_set_temperature, initarr.f90:78-84); with
+ * tdust2 NULL they are applied on the device directly.  nchanged receives
+ * the number of dusty (cell, species) whose T moved by more than 1 %. */
 int hc_temperature(hc_ctx *ctx, const double *massarr, double ltot, double nphot,
                    double *tdust2, int64_t *nchanged);
 

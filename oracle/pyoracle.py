@@ -241,6 +241,36 @@ class Oracle:
         out["trace"] = tr
         return out
 
+    def run_into(self, out: dict, np_: int, p_begin: int, p_end: int, seg0: int = 0,
+                 seed1: int = 12345678, seed2: int = 87654321, gen: int = 1) -> None:
+        """A Lucy pass over packets [p_begin, p_end) accumulated into the
+        tallies of `out` (from new_tallies); for repeated bounded samples."""
+        t = out["_struct"]
+        rc = self.lib.hco_run_pass(C.byref(self.grid), C.byref(self.dust), C.byref(self.setup),
+                                   C.byref(t), int(np_), int(p_begin), int(p_end), 0,
+                                   int(seed1), int(seed2), int(gen), int(seg0), None)
+        if rc != 0:
+            raise RuntimeError(f"hco_run_pass failed ({rc})")
+
+    def new_tallies(self) -> dict:
+        m = self.model
+        c = m.cfg
+        out = {
+            "dtauabs": np.zeros((m.nrg, m.ntg, m.npg, NDG), order="F"),
+            "nabs": np.zeros((m.nrg, m.ntg, m.npg, NDG), order="F"),
+            "jmean": np.zeros((m.nrg, m.ntg, m.npg), order="F"),
+            "killed": np.zeros((m.nrg, m.ntg, m.npg), np.int32, order="F"),
+            "dbig": np.zeros((m.nrg, m.ntg, m.npg), np.int32, order="F"),
+            "sed": np.zeros((c.nfreq, c.nmu, c.nph, c.nap, NO, 9), order="F"),
+            "si_init": np.zeros((c.nfreq, 5), order="F"),
+        }
+        t = Tallies()
+        t.dtauabs, t.nabs, t.jmean = _p(out["dtauabs"]), _p(out["nabs"]), _p(out["jmean"])
+        t.killed, t.dbig = _p(out["killed"], C.c_int32), _p(out["dbig"], C.c_int32)
+        t.sed, t.si_init = _p(out["sed"]), _p(out["si_init"])
+        out["_struct"] = t
+        return out
+
     def tfinal(self, dtauabs, nphot):
         m = self.model
         t2 = np.zeros((m.nrg, m.ntg, m.npg, NDG), order="F")
