@@ -25,7 +25,8 @@ _ip = C.POINTER(C.c_int32)
 # Every symbol include/hc.h declares.
 EXPORTS = (
     "hc_create", "hc_set_grid", "hc_set_temperature", "hc_set_dust", "hc_set_sources",
-    "hc_run_iteration", "hc_temperature", "hc_destroy", "hc_last_error", "hc_abi_version",
+    "hc_run_iteration", "hc_temperature", "hc_set_option", "hc_destroy", "hc_last_error",
+    "hc_abi_version",
 )
 
 
@@ -76,7 +77,7 @@ SCALARS = ("tot", "flux", "sflux", "nscat", "scount", "abflux", "aflux", "sumsub
 
 
 def library_path() -> str:
-    return os.path.join(_HERE, LIBNAME)
+    return os.environ.get("HOCHUNK3D_AMD_LIB", os.path.join(_HERE, LIBNAME))
 
 
 _lib = None
@@ -100,6 +101,7 @@ def load() -> C.CDLL:
                                      C.c_int32, C.c_uint64, C.POINTER(HcTallies), C.c_void_p]
     lib.hc_temperature.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, _dp,
                                    C.POINTER(C.c_int64)]
+    lib.hc_set_option.argtypes = [C.c_void_p, C.c_char_p, C.c_int64]
     lib.hc_destroy.argtypes = [C.c_void_p]
     lib.hc_last_error.argtypes = [C.c_void_p]
     lib.hc_last_error.restype = C.c_char_p
@@ -195,6 +197,11 @@ class Transport:
             setattr(s, k, v)
         s.u, s.aperture2, s.coste, s.sinte, s.cospe, s.sinpe = [_ptr(a) for a in arrs]
         self._check(self.lib.hc_set_sources(self.h, C.byref(s)), "hc_set_sources")
+
+    def set_option(self, name: str, value: int):
+        """Launch-shape knob (regen_lanes, blocks_per_cu); results do not
+        depend on it."""
+        self._check(self.lib.hc_set_option(self.h, name.encode(), int(value)), "hc_set_option")
 
     def _check(self, rc, what):
         if rc != 0:

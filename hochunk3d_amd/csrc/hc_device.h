@@ -20,6 +20,7 @@ constexpr double kR2P = 2.0 * kPi;
 constexpr double kRsol = 6.9598e10;                   // constants.f90:11
 constexpr double kSigT = 5.670400e-5;                 // constants.f90:21
 constexpr double kHuge = 1.7976931348623157e308;      // huge(1d0)
+constexpr uint32_t kMaxTrials = 1u << 20;             // stokes rejection-loop bound
 constexpr int kMaxSpecies = 8;                        // grid_mod.f90:13
 constexpr int kNo = 8;                                // SED output slots
 constexpr double kTauMax = 25.0;                      // peeloff_3d.f90:40
@@ -241,55 +242,86 @@ __device__ __forceinline__ Opac1 opacset1(const double *__restrict__ nud,
   return o;
 }
 
+// ---------------------------------------------------------------------------
+// Precision of the event arithmetic.  R = double evaluates every expression
+// as the reference does with the deterministic elementary functions of
+// hc_math.h (bit-identical to the CPU oracle); R = float runs the same
+// expressions in FP32 with the hardware's native elementary functions.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double m_log(double x) { return hcm_log(x); }
+__device__ __forceinline__ double m_exp(double x) { return hcm_exp(x); }
+__device__ __forceinline__ double m_sin(double x) { return hcm_sin(x); }
+__device__ __forceinline__ double m_cos(double x) { return hcm_cos(x); }
+__device__ __forceinline__ double m_acos(double x) { return hcm_acos(x); }
+__device__ __forceinline__ double m_pow(double x, double y) { return hcm_pow(x, y); }
+__device__ __forceinline__ double m_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ double m_atan2(double y, double x) { return hcm_atan2(y, x); }
+__device__ __forceinline__ float m_log(float x) { return __logf(x); }
+__device__ __forceinline__ float m_exp(float x) { return __expf(x); }
+__device__ __forceinline__ float m_sin(float x) { return __sinf(x); }
+__device__ __forceinline__ float m_cos(float x) { return __cosf(x); }
+__device__ __forceinline__ float m_acos(float x) { return acosf(x); }
+__device__ __forceinline__ float m_pow(float x, float y) {
+  return y == 2.0f ? x * x : __powf(x, y);
+}
+__device__ __forceinline__ float m_sqrt(float x) { return __fsqrt_rn(x); }
+__device__ __forceinline__ float m_atan2(float y, float x) { return atan2f(y, x); }
+
 // Derived HG terms of a species (dust.f90:68-72).
+template <class R>
 struct HG {
-  double g, g2, g2p1, onemg2, twog, pl;
-  __device__ __forceinline__ HG(double g_, double pl_) : g(g_), pl(pl_) {
+  R g, g2, g2p1, onemg2, twog, pl;
+  __device__ __forceinline__ HG(R g_, R pl_) : g(g_), pl(pl_) {
     g2 = g * g;
-    g2p1 = g2 + 1.0;
-    onemg2 = 1.0 - g2;
-    twog = 2.0 * g;
+    g2p1 = g2 + R(1.0);
+    onemg2 = R(1.0) - g2;
+    twog = R(2.0) * g;
   }
 };
 
 // dustmat.f90:39-66 (p4 = 0).
-__device__ __forceinline__ void dustmat(const HG &h, double cost, double cost2, double &p1,
-                                        double &p2, double &p3) {
-  const double temp = h.g2p1 - h.twog * cost;
-  p1 = 1.5 * h.onemg2 / (2.0 + h.g2) * (1.0 + cost2) / (temp * sqrt(temp));
-  p2 = -h.pl * p1 * (1.0 - cost2) / (1.0 + cost2);
-  p3 = p1 * 2.0 * cost / (1.0 + cost2);
+template <class R>
+__device__ __forceinline__ void dustmat(const HG<R> &h, R cost, R cost2, R &p1, R &p2, R &p3) {
+  const R temp = h.g2p1 - h.twog * cost;
+  p1 = R(1.5) * h.onemg2 / (R(2.0) + h.g2) * (R(1.0) + cost2) / (temp * m_sqrt(temp));
+  p2 = -h.pl * p1 * (R(1.0) - cost2) / (R(1.0) + cost2);
+  p3 = p1 * R(2.0) * cost / (R(1.0) + cost2);
 }
 
-__device__ __forceinline__ double cs_peak(const HG &h) {  // stokes.f90:54-62
-  const double temp = (h.g > 0.0) ? h.g2p1 - h.twog : h.g2p1 + h.twog;
-  return 1.5 * (h.onemg2) / (2.0 + h.g2) * 2.0 / temp / sqrt(temp) * 1.01;
+template <class R>
+__device__ __forceinline__ R cs_peak(const HG<R> &h) {  // stokes.f90:54-62
+  const R temp = (h.g > R(0.0)) ? h.g2p1 - h.twog : h.g2p1 + h.twog;
+  return R(1.5) * (h.onemg2) / (R(2.0) + h.g2) * R(2.0) / temp / m_sqrt(temp) * R(1.01);
 }
 
 // Packet direction and Stokes vector that scattering rotates.
+template <class R>
 struct Dir {
-  double cost, sint, cosp, sinp, phi;
+  R cost, sint, cosp, sinp, phi;
 };
+template <class R>
 struct Stokes4 {
-  double I, Q, U, V;
+  R I, Q, U, V;
 };
 
 // Mueller rows for the scattering frame (stokes.f90:206-240 / 337-369),
 // shared by stokes and stokespeel.  upper selects the ri1 > pi branch.
+template <class R>
 struct Mueller {
-  double a11, a12, a13, a21, a22, a23, a31, a32, a33, a44;
+  R a11, a12, a13, a21, a22, a23, a31, a32, a33, a44;
 };
 
-__device__ __forceinline__ Mueller mueller(bool upper, double p1, double p2, double p3,
-                                           double sini2, double cosi2, double s2a, double c2a) {
-  // s2a/c2a are hcm_sin(2 i1), hcm_cos(2 i1) (lower) or hcm_sin(2 i3), hcm_cos(2 i3) (upper)
-  Mueller m;
-  const double sin2i2 = 2.0 * sini2 * cosi2;
-  const double cos2i2 = 2.0 * (cosi2 * cosi2) - 1.0;
-  const double sin2 = sin2i2 * s2a;
-  const double cos2 = cos2i2 * c2a;
-  const double sin2cos1 = sin2i2 * c2a;
-  const double cos2sin1 = cos2i2 * s2a;
+template <class R>
+__device__ __forceinline__ Mueller<R> mueller(bool upper, R p1, R p2, R p3, R sini2, R cosi2,
+                                              R s2a, R c2a) {
+  // s2a/c2a are sin(2 i1), cos(2 i1) (lower) or sin(2 i3), cos(2 i3) (upper)
+  Mueller<R> m;
+  const R sin2i2 = R(2.0) * sini2 * cosi2;
+  const R cos2i2 = R(2.0) * (cosi2 * cosi2) - R(1.0);
+  const R sin2 = sin2i2 * s2a;
+  const R cos2 = cos2i2 * c2a;
+  const R sin2cos1 = sin2i2 * c2a;
+  const R cos2sin1 = cos2i2 * s2a;
   m.a11 = p1;
   m.a12 = p2 * c2a;
   m.a21 = p2 * cos2i2;
@@ -313,82 +345,86 @@ __device__ __forceinline__ Mueller mueller(bool upper, double p1, double p2, dou
 // stokes.f90:1-392 (idust = 1): rejection sampling of (mu, i1), new direction
 // by spherical trigonometry, Stokes rotation normalised by the accepted
 // probability.  ntrial/nhit count trials and acceptances (htot, hit).
-__device__ __forceinline__ void stokes(Dir &d, Stokes4 &s, const HG &h, Mrg &rng,
-                                       double &ntrial, double &nhit) {
-  const double costp = d.cost, sintp = d.sint, phip = d.phi;
-  double peak = cs_peak(h);
-  double bmu, cosb2, p1, p2, p3, sinbt, ri, c2a, s2a, ci, si, rprob;
+template <class R>
+__device__ __forceinline__ void stokes(Dir<R> &d, Stokes4<R> &s, const HG<R> &h, Mrg &rng,
+                                       uint32_t &ntrial, uint32_t &nhit) {
+  const R kpi = R(kPi), kr2p = R(kR2P);
+  const R costp = d.cost, sintp = d.sint, phip = d.phi;
+  R peak = cs_peak(h);
+  R bmu, cosb2, p1, p2, p3, sinbt, ri, c2a, s2a, ci, si, rprob;
   bool upper;
-  for (;;) {
-    ntrial += 1.0;
-    bmu = 1.0 - 2.0 * ran(rng);
+  for (uint32_t tries = 1;; ++tries) {
+    ntrial += 1;
+    bmu = R(1.0) - R(2.0) * R(ran(rng));
     cosb2 = bmu * bmu;
     dustmat(h, bmu, cosb2, p1, p2, p3);
-    sinbt = sqrt(1.0 - cosb2);
-    const double ri1 = kR2P * ran(rng);
-    upper = ri1 > kPi;
-    ri = upper ? kR2P - ri1 : ri1;
-    ci = hcm_cos(ri);
-    si = hcm_sin(ri);
-    s2a = 2.0 * si * ci;
-    c2a = 2.0 * (ci * ci) - 1.0;
-    const double a12 = p2 * c2a;
-    const double a13 = upper ? p2 * s2a : -p2 * s2a;
+    sinbt = m_sqrt(R(1.0) - cosb2);
+    const R ri1 = kr2p * R(ran(rng));
+    upper = ri1 > kpi;
+    ri = upper ? kr2p - ri1 : ri1;
+    ci = m_cos(ri);
+    si = m_sin(ri);
+    s2a = R(2.0) * si * ci;
+    c2a = R(2.0) * (ci * ci) - R(1.0);
+    const R a12 = p2 * c2a;
+    const R a13 = upper ? p2 * s2a : -p2 * s2a;
     rprob = (p1 * s.I + a12 * s.Q + a13 * s.U) / s.I;
     if (rprob > peak) peak = rprob;
-    if (peak * ran(rng) > rprob) continue;
-    nhit += 1.0;
+    // rejected; a Stokes vector that no angle accepts (non-finite after FP32
+    // rounding) takes the last trial after kMaxTrials
+    if (peak * R(ran(rng)) > rprob && tries < kMaxTrials) continue;
+    nhit += 1;
     break;
   }
-  if (bmu == 1.0) return;  // stokes.f90:102-105
-  double cost, sint, sini2, cosi2;
-  if (bmu == -1.0) {
-    si = 1.0;
-    sini2 = 1.0;
-    ci = 0.0;
-    cosi2 = 0.0;
+  if (bmu == R(1.0)) return;  // stokes.f90:102-105
+  R cost, sint, sini2, cosi2;
+  if (bmu == R(-1.0)) {
+    si = R(1.0);
+    sini2 = R(1.0);
+    ci = R(0.0);
+    cosi2 = R(0.0);
     cost = -costp;
     sint = sintp;
   } else {
     cost = costp * bmu + sintp * sinbt * ci;
-    if (fabs(cost) < 1.0) {
-      sint = fabs(sqrt(1.0 - cost * cost));
+    if (fabs(cost) < R(1.0)) {
+      sint = fabs(m_sqrt(R(1.0) - cost * cost));
       sini2 = si * sintp / sint;
-      const double bott = sint * sinbt;
+      const R bott = sint * sinbt;
       cosi2 = costp / bott - cost * bmu / bott;
-      if (cosi2 >= 1.0) {
-        cosi2 = 1.0;
-        sini2 = 0.0;
+      if (cosi2 >= R(1.0)) {
+        cosi2 = R(1.0);
+        sini2 = R(0.0);
       }
-      if (cosi2 <= -1.0) {
-        cosi2 = -1.0;
-        sini2 = 0.0;
+      if (cosi2 <= R(-1.0)) {
+        cosi2 = R(-1.0);
+        sini2 = R(0.0);
       }
     } else {
-      cosi2 = 0.0;
-      if (cost >= 1.0) {
-        cost = 1.0;
-        cosi2 = -1.0;
+      cosi2 = R(0.0);
+      if (cost >= R(1.0)) {
+        cost = R(1.0);
+        cosi2 = R(-1.0);
       }
-      if (cost <= -1.0) {
-        cost = -1.0;
-        cosi2 = 1.0;
+      if (cost <= R(-1.0)) {
+        cost = R(-1.0);
+        cosi2 = R(1.0);
       }
-      sint = 0.0;
-      sini2 = 0.0;
+      sint = R(0.0);
+      sini2 = R(0.0);
     }
   }
-  double cosdph = upper ? -cosi2 * ci + sini2 * si * bmu : -ci * cosi2 + si * sini2 * bmu;
-  if (fabs(cosdph) > 1.0) cosdph = (cosdph > 1.0) ? 1.0 : -1.0;
-  double phi = upper ? phip + hcm_acos(cosdph) : phip - hcm_acos(cosdph);
-  if (phi > kR2P) phi = phi - kR2P;
-  if (phi < 0.0) phi = phi + kR2P;
-  const Mueller m = mueller(upper, p1, p2, p3, sini2, cosi2, s2a, c2a);
-  const double a = rprob;
-  const double sI = (m.a11 * s.I + m.a12 * s.Q + m.a13 * s.U) / a;
-  const double sQ = (m.a21 * s.I + m.a22 * s.Q + m.a23 * s.U + 0.0 * s.V) / a;
-  const double sU = (m.a31 * s.I + m.a32 * s.Q + m.a33 * s.U + 0.0 * s.V) / a;
-  const double sV = (0.0 * s.Q + 0.0 * s.U + m.a44 * s.V) / a;
+  R cosdph = upper ? -cosi2 * ci + sini2 * si * bmu : -ci * cosi2 + si * sini2 * bmu;
+  if (fabs(cosdph) > R(1.0)) cosdph = (cosdph > R(1.0)) ? R(1.0) : R(-1.0);
+  R phi = upper ? phip + m_acos(cosdph) : phip - m_acos(cosdph);
+  if (phi > kr2p) phi = phi - kr2p;
+  if (phi < R(0.0)) phi = phi + kr2p;
+  const Mueller<R> m = mueller(upper, p1, p2, p3, sini2, cosi2, s2a, c2a);
+  const R a = rprob;
+  const R sI = (m.a11 * s.I + m.a12 * s.Q + m.a13 * s.U) / a;
+  const R sQ = (m.a21 * s.I + m.a22 * s.Q + m.a23 * s.U + R(0.0) * s.V) / a;
+  const R sU = (m.a31 * s.I + m.a32 * s.Q + m.a33 * s.U + R(0.0) * s.V) / a;
+  const R sV = (R(0.0) * s.Q + R(0.0) * s.U + m.a44 * s.V) / a;
   s.I = sI;
   s.Q = sQ;
   s.U = sU;
@@ -396,67 +432,69 @@ __device__ __forceinline__ void stokes(Dir &d, Stokes4 &s, const HG &h, Mrg &rng
   d.cost = cost;
   d.sint = sint;
   d.phi = phi;
-  d.cosp = hcm_cos(phi);
-  d.sinp = hcm_sin(phi);
+  d.cosp = m_cos(phi);
+  d.sinp = m_sin(phi);
 }
 
 // findangle3.f90 (observer azimuth phi2 = 0 as peeloff_3d.f90:168 passes
 // it) followed by stokespeel.f90 for one observer direction (ce, se).
-__device__ __forceinline__ void peel_weight(const Dir &d, double ce, double se, double cpe,
-                                            double spe, const HG &h, Stokes4 &s) {
-  double bmu = d.cost * ce + (d.sint * se * (d.cosp * cpe + d.sinp * spe));
-  if (bmu > 1.0) bmu = 1.0;
-  if (bmu < -1.0) bmu = -1.0;
-  const double sintnew = sqrt(1.0 - bmu * bmu);
-  double ri1 = 0.0;
-  if (d.sint > 1.e-8 && sintnew > 1.e-8) {
-    double cosx = (ce - (d.cost * bmu)) / (d.sint * sintnew);
-    if (cosx > 1.0) cosx = 1.0;
-    if (cosx < -1.0) cosx = -1.0;
-    const double x = hcm_acos(cosx);
-    const double phi2 = 0.0;
+template <class R>
+__device__ __forceinline__ void peel_weight(const Dir<R> &d, R ce, R se, R cpe, R spe,
+                                            const HG<R> &h, Stokes4<R> &s) {
+  const R kpi = R(kPi), kr2p = R(kR2P);
+  R bmu = d.cost * ce + (d.sint * se * (d.cosp * cpe + d.sinp * spe));
+  if (bmu > R(1.0)) bmu = R(1.0);
+  if (bmu < R(-1.0)) bmu = R(-1.0);
+  const R sintnew = m_sqrt(R(1.0) - bmu * bmu);
+  R ri1 = R(0.0);
+  if (d.sint > R(1.e-8) && sintnew > R(1.e-8)) {
+    R cosx = (ce - (d.cost * bmu)) / (d.sint * sintnew);
+    if (cosx > R(1.0)) cosx = R(1.0);
+    if (cosx < R(-1.0)) cosx = R(-1.0);
+    const R x = m_acos(cosx);
+    const R phi2 = R(0.0);
     if (phi2 > d.phi)
-      ri1 = (phi2 - d.phi > kPi) ? x : kR2P - x;
+      ri1 = (phi2 - d.phi > kpi) ? x : kr2p - x;
     else
-      ri1 = (d.phi - phi2 > kPi) ? kR2P - x : x;
+      ri1 = (d.phi - phi2 > kpi) ? kr2p - x : x;
   }
-  const double cosb2 = bmu * bmu;
-  double p1, p2, p3;
+  const R cosb2 = bmu * bmu;
+  R p1, p2, p3;
   dustmat(h, bmu, cosb2, p1, p2, p3);
-  const double sinbt = sqrt(1.0 - cosb2);
-  const bool upper = ri1 > kPi;
-  const double ri = upper ? kR2P - ri1 : ri1;
-  double ci = hcm_cos(ri), si = hcm_sin(ri);
-  const double s2a = 2.0 * si * ci;
-  const double c2a = 2.0 * (ci * ci) - 1.0;
-  if (bmu == 1.0) return;  // stokespeel.f90:116-119
-  double sini2, cosi2;
-  if (bmu == -1.0) {
-    si = 1.0;
-    sini2 = 1.0;
-    ci = 0.0;
-    cosi2 = 0.0;
-  } else if (fabs(ce) < 1.0) {
+  const R sinbt = m_sqrt(R(1.0) - cosb2);
+  const bool upper = ri1 > kpi;
+  const R ri = upper ? kr2p - ri1 : ri1;
+  R ci = m_cos(ri), si = m_sin(ri);
+  const R s2a = R(2.0) * si * ci;
+  const R c2a = R(2.0) * (ci * ci) - R(1.0);
+  if (bmu == R(1.0)) return;  // stokespeel.f90:116-119
+  R sini2, cosi2;
+  if (bmu == R(-1.0)) {
+    si = R(1.0);
+    sini2 = R(1.0);
+    ci = R(0.0);
+    cosi2 = R(0.0);
+  } else if (fabs(ce) < R(1.0)) {
     sini2 = si * d.sint / se;
-    const double bott = se * sinbt;
+    const R bott = se * sinbt;
     cosi2 = d.cost / bott - ce * bmu / bott;
-    if (cosi2 >= 1.0) {
-      cosi2 = 1.0;
-      sini2 = 0.0;
+    if (cosi2 >= R(1.0)) {
+      cosi2 = R(1.0);
+      sini2 = R(0.0);
     }
-    if (cosi2 <= -1.0) {
-      cosi2 = -1.0;
-      sini2 = 0.0;
+    if (cosi2 <= R(-1.0)) {
+      cosi2 = R(-1.0);
+      sini2 = R(0.0);
     }
   } else {
-    cosi2 = (ce >= 1.0) ? -1.0 : 1.0;
-    sini2 = 0.0;
+    cosi2 = (ce >= R(1.0)) ? R(-1.0) : R(1.0);
+    sini2 = R(0.0);
   }
-  const Mueller m = mueller(upper, p1, p2, p3, sini2, cosi2, s2a, c2a);
-  const double sI = (m.a11 * s.I + m.a12 * s.Q + m.a13 * s.U);
-  const double sQ = (m.a21 * s.I + m.a22 * s.Q + m.a23 * s.U + 0.0 * s.V);
-  const double sU = (m.a31 * s.I + m.a32 * s.Q + m.a33 * s.U + 0.0 * s.V);
-  const double sV = (0.0 * s.Q + 0.0 * s.U + m.a44 * s.V);
+  const Mueller<R> m = mueller(upper, p1, p2, p3, sini2, cosi2, s2a, c2a);
+  const R sI = (m.a11 * s.I + m.a12 * s.Q + m.a13 * s.U);
+  const R sQ = (m.a21 * s.I + m.a22 * s.Q + m.a23 * s.U + R(0.0) * s.V);
+  const R sU = (m.a31 * s.I + m.a32 * s.Q + m.a33 * s.U + R(0.0) * s.V);
+  const R sV = (R(0.0) * s.Q + R(0.0) * s.U + m.a44 * s.V);
   s.I = sI;
   s.Q = sQ;
   s.U = sU;

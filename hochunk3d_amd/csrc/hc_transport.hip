@@ -31,6 +31,7 @@
 namespace hc {
 
 constexpr int kGuide = 512;  // bins of the opacity-table guide
+constexpr int kCdfGuide = 128;  // probability bins of the emissivity-CDF guide
 
 struct Params {
   // grid
@@ -44,6 +45,8 @@ struct Params {
   const double *nudust, *kappad, *adust, *gdust, *pldust;
   double kappav[kMaxSpecies];
   const double *lTint, *nuint, *kapBint;
+  // per species id and row j: number of CDF nodes <= b/kCdfGuide, b = 0..kCdfGuide
+  const uint16_t *cdfguide;
   const int32_t *guide;      // [ndg][kGuide] bracket start per log-frequency bin
   double glog0[kMaxSpecies], ginv[kMaxSpecies];
   // sources
@@ -66,6 +69,9 @@ struct Params {
   double *scal;
   unsigned long long *next_packet;
   hc_trace *trace;
+  // launch shape
+  int regen_lanes;  // waiting lanes (1..64) that end a walk phase
+  uint32_t max_work;  // events + crossings after which a packet is dropped
 };
 
 // Launch parameters live in constant memory: every wave reads them with
@@ -81,21 +87,23 @@ enum Scalar {
 // ---------------------------------------------------------------------------
 // packet state (registers)
 // ---------------------------------------------------------------------------
-template <int NA>
+template <class R, int NA>
 struct Pkt {
-  double x, y, z, ux, uy, uz;
-  Dir d;
-  Stokes4 s;
-  double nub, rsq;
+  double x, y, z, ux, uy, uz;  // position and direction of the walk: FP64
+  Dir<R> d;
+  Stokes4<R> s;
+  R nub;
+  double rsq;
   int ir, it, ip;
   bool on_wall;
   int iflag, i_orig;
-  double kapd[NA], alb[NA];
+  double kapd[NA];  // kappa R* kappa_V per active species (walk opacity)
+  R alb[NA];
 };
 
 struct Acc {  // per-lane counters, flushed once per lane
   uint32_t tot, flux, sflux, nscat, scount, abflux, aflux, hit, htot, err_wall, err_dust;
-  uint32_t ncross, nscat_ev, nabs_ev, npeelray, npeelcross;
+  uint32_t ncross, nscat_ev, nabs_ev, npeelray, npeelcross, warn;
   double sumsub, sumall;
 };
 
@@ -103,7 +111,7 @@ struct Acc {  // per-lane counters, flushed once per lane
 // the device arrays with one atomic per species when the packet leaves the
 // cell (a packet random-walking in an optically thick cell adds thousands of
 // segments to the same cell).
-template <int NA>
+template <class R, int NA>
 struct LucyAcc {
   long long cell;
   double tau[NA];
@@ -136,8 +144,8 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) {
 }
 
 // Masked opacity of a cell: sum over active species in increasing id.
-template <int NA>
-__device__ __forceinline__ double cell_opac(const Pkt<NA> &k, const double *dm) {
+template <class R, int NA>
+__device__ __forceinline__ double cell_opac(const Pkt<R, NA> &k, const double *dm) {
   double opac = 0.0;
 #pragma unroll
   for (int s = 0; s < NA; ++s)
@@ -145,6 +153,11 @@ __device__ __forceinline__ double cell_opac(const Pkt<NA> &k, const double *dm) 
   return opac;
 }
 
+// On the polar axis (x = y = 0) every phi wall passes through the point, so
+// phiface's roots are all 0 and the walk would step from phi cell to phi cell
+// without moving; a straight ray leaving the axis keeps the azimuth of its
+// direction, so there the phi walls are skipped and the caller has put the
+// packet in the phi cell of its direction (axis_phi_cell).
 __device__ __forceinline__ bool next_wall(double ux, double uy, double uz, double x, double y,
                                           double z, int ir, int it, int ip, bool on_wall,
                                           double &dt, int &face) {
@@ -153,9 +166,19 @@ __device__ __forceinline__ bool next_wall(double ux, double uy, double uz, doubl
   w.reset();
   radface(w, ux, uy, uz, x, y, z, P.r2arr[ir - 1], P.r2arr[ir]);
   if (P.ntg > 2) thetaface(w, ux, uy, uz, x, y, z, P.tan2arr[it - 1], P.tan2arr[it]);
-  if (P.npg > 2 && P.ntg > 2)
+  if (P.npg > 2 && P.ntg > 2 && !(x == 0.0 && y == 0.0))
     phiface(w, P.aarr[ip - 1], P.barr[ip - 1], P.aarr[ip], P.barr[ip], ux, uy, x, y);
   return w.next(on_wall, dt, face);
+}
+
+// The phi cell a ray leaving the polar axis enters: the one holding the
+// azimuth of its direction (unchanged when the ray runs along the axis).
+__device__ __attribute__((noinline)) int axis_phi_cell(double ux, double uy, int ip) {
+  const Params &P = c_P;
+  if (P.npg <= 2 || (ux == 0.0 && uy == 0.0)) return ip;
+  double p = hcm_atan2(uy, ux);
+  if (p < 0.0) p = p + kR2P;
+  return locate(P.phiarr, P.npg, p);
 }
 
 __device__ __forceinline__ void face_step(int face, int &ir, int &it, int &ip) {
@@ -173,8 +196,8 @@ __device__ __forceinline__ void face_step(int face, int &ir, int &it, int &ip) {
 
 enum { ST_INTERACT = 0, ST_EXIT = 1, ST_STAR = 2, ST_ERROR = 3 };
 
-template <int NA>
-__device__ __forceinline__ void lucy_flush(LucyAcc<NA> &la) {
+template <class R, int NA>
+__device__ __forceinline__ void lucy_flush(LucyAcc<R, NA> &la) {
   const Params &P = c_P;
   if (la.cell < 0) return;
   if (P.dtauabs) {
@@ -194,8 +217,8 @@ __device__ __forceinline__ void lucy_flush(LucyAcc<NA> &la) {
   la.cell = -1;
 }
 
-template <int NA>
-__device__ __forceinline__ void lucy_open(LucyAcc<NA> &la, size_t cell) {
+template <class R, int NA>
+__device__ __forceinline__ void lucy_open(LucyAcc<R, NA> &la, size_t cell) {
   if (la.cell != (long long)cell) {
     lucy_flush(la);
     la.cell = (long long)cell;
@@ -211,20 +234,24 @@ __device__ __forceinline__ void lucy_open(LucyAcc<NA> &la, size_t cell) {
 
 // Add one chord of cell `cell` to the per-cell accumulator: dtauabs gains
 // kapd (1-albedo) rho ds per active species, jmean gains ds R*.
-template <int NA>
-__device__ __forceinline__ void lucy_add(LucyAcc<NA> &la, const Pkt<NA> &k, const double *dm,
+template <class R, int NA>
+__device__ __forceinline__ void lucy_add(LucyAcc<R, NA> &la, const Pkt<R, NA> &k, const double *dm,
                                          size_t cell, double ds) {
   if (!c_P.dtauabs && !c_P.jmean) return;
   lucy_open(la, cell);
 #pragma unroll
   for (int s = 0; s < NA; ++s)
-    if (s < c_P.nact && dm[s] != 0.0) la.tau[s] += k.kapd[s] * (1.0 - k.alb[s]) * dm[s] * ds;
+    if (s < c_P.nact && dm[s] != 0.0) la.tau[s] += k.kapd[s] * (1.0 - (double)k.alb[s]) * dm[s] * ds;
   la.len += ds;
 }
 
-// The cell walk to optical depth tau (tauint reconstruction).
-template <int NA, bool TRACE>
-__device__ int tauint(Pkt<NA> &k, double tau, LucyAcc<NA> &la, Acc &acc, Trace &tr) {
+// The cell walk to optical depth tau (tauint reconstruction), split into
+// its entry (the packet on a radial wall steps into the cell its direction
+// points to) and single crossings, so the event loop can interleave lanes.
+enum { ST_WALK = 4, ST_FETCH = 5, ST_DONE = 6, ST_ABORT = 7 };
+
+template <class R, int NA>
+__device__ __forceinline__ int walk_begin(Pkt<R, NA> &k) {
   const Params &P = c_P;
   if (k.on_wall) {
     const bool inner = fabs(k.rsq - P.r2arr[k.ir - 1]) < fabs(k.rsq - P.r2arr[k.ir]);
@@ -236,46 +263,54 @@ __device__ int tauint(Pkt<NA> &k, double tau, LucyAcc<NA> &la, Acc &acc, Trace &
     if (!inner && outwards) k.ir += 1;
     if (k.ir == P.nrg) return ST_EXIT;
   }
-  double tsum = 0.0;
-  for (;;) {
-    const size_t cell = cell_of(k.ir, k.it, k.ip);
-    const double *dm = P.densm + cell * (size_t)P.nact;
-    double dt;
-    int face;
-    if (!next_wall(k.ux, k.uy, k.uz, k.x, k.y, k.z, k.ir, k.it, k.ip, k.on_wall, dt, face)) {
-      acc.err_wall++;
-      return ST_ERROR;
-    }
-    const double opac = cell_opac(k, dm);
-    const double dtau = opac * dt;
-    if (tsum + dtau >= tau) {
-      const double ds = (tau - tsum) / opac;
-      lucy_add(la, k, dm, cell, ds);
-      k.x = k.x + ds * k.ux;
-      k.y = k.y + ds * k.uy;
-      k.z = k.z + ds * k.uz;
-      k.on_wall = false;
-      k.rsq = k.x * k.x + k.y * k.y + k.z * k.z;
-      if constexpr (TRACE) tr.path += ds;
-      return ST_INTERACT;
-    }
-    lucy_add(la, k, dm, cell, dt);
-    tsum = tsum + dtau;
-    face_step(face, k.ir, k.it, k.ip);
-    k.x = k.x + dt * k.ux;
-    k.y = k.y + dt * k.uy;
-    k.z = k.z + dt * k.uz;
-    k.on_wall = true;
-    k.rsq = k.x * k.x + k.y * k.y + k.z * k.z;
-    acc.ncross++;
-    if constexpr (TRACE) {
-      tr.path += dt;
-      tr.ncross++;
-      hash_cell(tr, k.ir, k.it, k.ip);
-    }
-    if (k.ir == P.nrg) return ST_EXIT;
-    if (k.ir == 0) return ST_STAR;
+  return ST_WALK;
+}
+
+// One step of the walk: to the next wall, or to the interaction point when
+// the optical depth tau is reached inside the current cell.
+template <class R, int NA, bool TRACE>
+__device__ __forceinline__ int walk_step(Pkt<R, NA> &k, double tau, double &tsum,
+                                         LucyAcc<R, NA> &la, Acc &acc, Trace &tr) {
+  const Params &P = c_P;
+  if (k.x == 0.0 && k.y == 0.0) k.ip = axis_phi_cell(k.ux, k.uy, k.ip);
+  const size_t cell = cell_of(k.ir, k.it, k.ip);
+  const double *dm = P.densm + cell * (size_t)P.nact;
+  double dt;
+  int face;
+  if (!next_wall(k.ux, k.uy, k.uz, k.x, k.y, k.z, k.ir, k.it, k.ip, k.on_wall, dt, face)) {
+    acc.err_wall++;
+    return ST_ERROR;
   }
+  const double opac = cell_opac(k, dm);
+  const double dtau = opac * dt;
+  if (tsum + dtau >= tau) {
+    const double ds = (tau - tsum) / opac;
+    lucy_add(la, k, dm, cell, ds);
+    k.x = k.x + ds * k.ux;
+    k.y = k.y + ds * k.uy;
+    k.z = k.z + ds * k.uz;
+    k.on_wall = false;
+    k.rsq = k.x * k.x + k.y * k.y + k.z * k.z;
+    if constexpr (TRACE) tr.path += ds;
+    return ST_INTERACT;
+  }
+  lucy_add(la, k, dm, cell, dt);
+  tsum = tsum + dtau;
+  face_step(face, k.ir, k.it, k.ip);
+  k.x = k.x + dt * k.ux;
+  k.y = k.y + dt * k.uy;
+  k.z = k.z + dt * k.uz;
+  k.on_wall = true;
+  k.rsq = k.x * k.x + k.y * k.y + k.z * k.z;
+  acc.ncross++;
+  if constexpr (TRACE) {
+    tr.path += dt;
+    tr.ncross++;
+    hash_cell(tr, k.ir, k.it, k.ip);
+  }
+  if (k.ir == P.nrg) return ST_EXIT;
+  if (k.ir == 0) return ST_STAR;
+  return ST_WALK;
 }
 
 // opacset for one species (dust.f90:13-76).  The bracket of nu in the
@@ -297,7 +332,8 @@ __device__ __forceinline__ Opac1 opacset_species(int id, double nu) {
     i0 = i1 = nl;
     f = 0.0;
   } else {
-    int b = (int)((P.glog0[id] - hcm_log(nu)) * P.ginv[id]);
+    // the bin only seeds the scan, so a single-precision log suffices
+    int b = (int)((P.glog0[id] - (double)__logf((float)nu)) * P.ginv[id]);
     b = clampi(b - 1, 0, kGuide - 1);
     i1 = P.guide[id * kGuide + b];
     while (!(nu > nud[i1 - 1])) ++i1;
@@ -320,9 +356,10 @@ __device__ __forceinline__ Opac1 opacset_species(int id, double nu) {
   return r;
 }
 
-__device__ __forceinline__ HG hg_of(int id, double nu) {
-  const Opac1 op = opacset_species(id, nu);
-  return HG(op.g, op.pl);
+template <class R>
+__device__ __forceinline__ HG<R> hg_of(int id, R nu) {
+  const Opac1 op = opacset_species(id, (double)nu);
+  return HG<R>(R(op.g), R(op.pl));
 }
 
 struct PeelOut {
@@ -331,17 +368,63 @@ struct PeelOut {
   int err;
 };
 
+// What a peel-off ray needs of the packet at its emission or scattering
+// event.  Records wait in a per-wavefront queue in LDS and are walked
+// together, one (record, observer) pair per lane.
+template <class R, int NA>
+struct PeelRec {
+  double x, y, z, rsq;
+  double kapd[NA];
+  Dir<R> d;
+  Stokes4<R> s;
+  R nub;
+  R cosb, sinb, coslp, sinlp;  // direct stellar emission: surface normal
+  int ir, it, ip;
+  int flags;                   // on_wall | iflag<<1 | i_orig<<2 | idust2<<5 | direct<<9
+  long long pid;
+};
+
+template <class R, int NA>
+__device__ __forceinline__ void make_rec(PeelRec<R, NA> &r, const Pkt<R, NA> &k, int idust2,
+                                         bool direct, long long pid) {
+  r.x = k.x;
+  r.y = k.y;
+  r.z = k.z;
+  r.rsq = k.rsq;
+#pragma unroll
+  for (int s = 0; s < NA; ++s) r.kapd[s] = k.kapd[s];
+  r.d = k.d;
+  r.s = k.s;
+  r.nub = k.nub;
+  r.ir = k.ir;
+  r.it = k.it;
+  r.ip = k.ip;
+  r.flags = (k.on_wall ? 1 : 0) | (k.iflag << 1) | (k.i_orig << 2) | (idust2 << 5) |
+            ((direct ? 1 : 0) << 9);
+  r.pid = pid;
+}
+
+template <class R, int NA>
+__device__ __forceinline__ double rec_opac(const PeelRec<R, NA> &k, const double *dm) {
+  double opac = 0.0;
+#pragma unroll
+  for (int s = 0; s < NA; ++s)
+    if (s < c_P.nact) opac = opac + k.kapd[s] * dm[s];
+  return opac;
+}
+
 // peeloff_3d.f90:5-288 for observer `peelid`.  sip is the peel weight; the
 // remaining Stokes components come from the packet.
-template <int NA>
-__device__ __attribute__((noinline)) PeelOut peeloff(const Pkt<NA> &k, double sip, int idust2,
+template <class R, int NA>
+__device__ __attribute__((noinline)) PeelOut peeloff(const PeelRec<R, NA> &k, double sip,
                                                      int peelid) {
   const Params &P = c_P;
   PeelOut out{0.0, 0, 0};
-  Stokes4 s{sip, k.s.Q, k.s.U, k.s.V};
+  Stokes4<R> s{R(sip), k.s.Q, k.s.U, k.s.V};
   double x1 = k.x, y1 = k.y, z1 = k.z;
   int ir = k.ir, it = k.it, ip = k.ip;
-  bool on_wall = k.on_wall;
+  bool on_wall = (k.flags & 1) != 0;
+  const int iflag = (k.flags >> 1) & 1, i_orig = (k.flags >> 2) & 7, idust2 = (k.flags >> 5) & 15;
   const double cospe = P.cospe[peelid], sinpe = P.sinpe[peelid];
   const double coste = P.coste[peelid], sinte = P.sinte[peelid];
   const double ux1 = sinte * cospe, uy1 = sinte * sinpe, uz1 = coste;
@@ -355,14 +438,19 @@ __device__ __attribute__((noinline)) PeelOut peeloff(const Pkt<NA> &k, double si
     if (!inner && outwards) ir = ir + 1;
   }
   double tsum = 0.0;
+  // a straight ray meets each sphere and each cone at most twice and each
+  // phi half-plane at most once: more crossings than that is a stuck ray
+  const int max_cross = 2 * (P.nrg + P.ntg + P.npg) + 16;
   while (ir < P.nrg) {
     double dt;
     int face;
-    if (!next_wall(ux1, uy1, uz1, x1, y1, z1, ir, it, ip, on_wall, dt, face)) {
+    if (x1 == 0.0 && y1 == 0.0) ip = axis_phi_cell(ux1, uy1, ip);
+    if (out.ncross > max_cross ||
+        !next_wall(ux1, uy1, uz1, x1, y1, z1, ir, it, ip, on_wall, dt, face)) {
       out.err = 1;
       return out;
     }
-    tsum = tsum + cell_opac(k, P.densm + cell_of(ir, it, ip) * (size_t)P.nact) * dt;
+    tsum = tsum + rec_opac(k, P.densm + cell_of(ir, it, ip) * (size_t)P.nact) * dt;
     if (tsum > kTauMax) {
       out.tau = tsum;
       return out;
@@ -379,25 +467,27 @@ __device__ __attribute__((noinline)) PeelOut peeloff(const Pkt<NA> &k, double si
     }
   }
   out.tau = tsum;
-  const double factor = hcm_exp(-tsum);
+  const R factor = m_exp(R(-tsum));
   s.I = s.I * factor;
   s.Q = s.Q * factor;
   s.U = s.U * factor;
   s.V = s.V * factor;
-  if (k.iflag == 1) peel_weight(k.d, coste, sinte, cospe, sinpe, hg_of(idust2, k.nub), s);
+  if (iflag == 1)
+    peel_weight<R>(k.d, R(coste), R(sinte), R(cospe), R(sinpe), hg_of<R>(idust2, k.nub), s);
   const double yimage = z1 * sinte - y1 * coste * sinpe - x1 * coste * cospe;
   const double ximage = y1 * cospe - x1 * sinpe;
   const int ix = (int)((ximage + P.rmaxi) / P.fractxh) + 1;
   const int iy = (int)((yimage + P.rmaxi) / P.fractxh) + 1;
   const double rho2 = ximage * ximage + yimage * yimage;
   const int inub =
-      clampi((int)((double)P.nfreq * hcm_log(k.nub / P.numin) / P.lnurat) + 1, 1, P.nfreq);
-  const int i_inter = (k.iflag == 1) ? 2 : ((k.i_orig == 1) ? 1 : 3);
-  const int indx[3] = {1, 1 + k.i_orig, 5 + i_inter};
+      clampi((int)((double)P.nfreq * (double)m_log(R(k.nub / P.numin)) / P.lnurat) + 1, 1, P.nfreq);
+  const int i_inter = (iflag == 1) ? 2 : ((i_orig == 1) ? 1 : 3);
+  const int indx[3] = {1, 1 + i_orig, 5 + i_inter};
   if (P.peel) {
     const size_t n = (size_t)P.nfreq * P.npeel * P.nap * kNo;
     for (int ia = 0; ia < P.nap; ++ia) {
       if (rho2 < P.aperture2[ia]) {
+#pragma unroll
         for (int j = 0; j < 3; ++j) {
           const size_t q = (size_t)(inub - 1) +
                            (size_t)P.nfreq * ((size_t)peelid +
@@ -422,6 +512,7 @@ __device__ __attribute__((noinline)) PeelOut peeloff(const Pkt<NA> &k, double si
                      (size_t)P.npeel * ((size_t)(ix - 1) + nx * ((size_t)(iy - 1) +
                                                                   nx * (size_t)(inub - 1)));
     const size_t n4 = (size_t)P.npeel * nx * nx * P.nfreq;
+#pragma unroll
     for (int j = 0; j < 3; ++j) atomic_add(&P.img_i[q + n4 * (size_t)(indx[j] - 1)], s.I);
     atomic_add(&P.img_q[q], s.Q);
     atomic_add(&P.img_u[q], s.U);
@@ -430,62 +521,71 @@ __device__ __attribute__((noinline)) PeelOut peeloff(const Pkt<NA> &k, double si
   return out;
 }
 
-template <int NA, bool TRACE>
-__device__ __forceinline__ void peel_all(const Pkt<NA> &k, double sip, int idust2, Acc &acc,
-                                         Trace &tr) {
-  for (int q = 0; q < c_P.npeel; ++q) {
-    const PeelOut o = peeloff(k, sip, idust2, q);
-    acc.err_wall += o.err;
-    acc.npeelray++;
-    acc.npeelcross += (uint32_t)o.ncross;
-    if constexpr (TRACE) {
-      tr.tau_peel += o.tau;
-      tr.npeelcross += o.ncross;
-    }
-  }
-}
-
 __constant__ double c_inczeta[146];
 
-__device__ __forceinline__ double bbfreq(Mrg &rng, double teff) {  // random.f90:49-77
-  const double z0 = c_inczeta[145] * ran(rng);
-  const double z1 = ran(rng), z2 = ran(rng), z3 = ran(rng), z4 = ran(rng);
+template <class R>
+__device__ __forceinline__ R bbfreq(Mrg &rng, R teff) {  // random.f90:49-77
+  const R z0 = R(c_inczeta[145]) * R(ran(rng));
+  const R z1 = R(ran(rng)), z2 = R(ran(rng)), z3 = R(ran(rng)), z4 = R(ran(rng));
   int i = 1;
-  while (z0 > c_inczeta[i]) ++i;
-  return -hcm_log(z1 * z2 * z3 * z4) * teff / (double)i;
+  while (z0 > R(c_inczeta[i])) ++i;
+  return -m_log(z1 * z2 * z3 * z4) * teff / (R)i;
 }
 
 // lucydustfreq.f90:19-157 with its guard chain; j clamped to [1,nT-1].
-__device__ __attribute__((noinline)) double lucydustfreq(double T0, int id, Mrg &rng) {
+template <class R>
+__device__ __forceinline__ R lucydustfreq(double T0d, int id, Mrg &rng) {
   const Params &P = c_P;
   const int nT = P.nT, nnu = P.nnu;
   const double *lT = P.lTint, *nuint = P.nuint;
   const double *kb = P.kapBint + (size_t)nT * nnu * id;
-  const double lT0 = hcm_log(T0);
-  const double xi = ran(rng);
-  int j = (int)((double)(nT - 1) * (lT0 - lT[0]) / (lT[nT - 1] - lT[0])) + 1;
+  const R T0 = R(T0d);
+  const R lT0 = m_log(T0);
+  const R xi = R(ran(rng));
+  int j = (int)((R)(nT - 1) * (lT0 - R(lT[0])) / (R(lT[nT - 1]) - R(lT[0]))) + 1;
   j = clampi(j, 1, nT - 1);
-  double f = (lT0 - lT[j - 1]) / (lT[j] - lT[j - 1]);
-  if (f < 0.0) f = 0.0;
-  if (f > 1.0) f = 1.0;
+  R f = (lT0 - R(lT[j - 1])) / (R(lT[j]) - R(lT[j - 1]));
+  if (f < R(0.0)) f = R(0.0);
+  if (f > R(1.0)) f = R(1.0);
   const double *row = kb + (j - 1);  // KB(j, i) = row[(i-1)*nT], KB(j+1, i) = row[(i-1)*nT + 1]
   auto KB = [&](int ii) {
     const double *p = row + (size_t)nT * (size_t)(ii - 1);
-    return (1.0 - f) * p[0] + f * p[1];
+    return (R(1.0) - f) * R(p[0]) + f * R(p[1]);
   };
   int i0 = 1, i1 = nnu;
-  double P0 = KB(i0);
-  if (xi < P0) return (P0 > 0.0) ? T0 * xi * nuint[0] / P0 : T0 * nuint[0];
-  double P1 = KB(i1);
-  if (P1 > 1.0) P1 = 1.0;
+  R P0 = KB(i0);
+  if (xi < P0) return (P0 > R(0.0)) ? T0 * xi * R(nuint[0]) / P0 : T0 * R(nuint[0]);
+  R P1 = KB(i1);
+  if (P1 > R(1.0)) P1 = R(1.0);
   if (xi >= P1) {
-    if (P1 < 1.0 && xi < 1.0) return T0 * nuint[nnu - 1] * (1.0 - P1) / (1.0 - xi);
-    return T0 * nuint[nnu - 1];
+    if (P1 < R(1.0) && xi < R(1.0)) return T0 * R(nuint[nnu - 1]) * (R(1.0) - P1) / (R(1.0) - xi);
+    return T0 * R(nuint[nnu - 1]);
+  }
+  if (P.cdfguide) {
+    // The interpolated CDF lies between rows j and j+1, so the node counts
+    // of both rows at the probability bin of xi bound the bracket; the
+    // rows are strictly increasing (checked on the host), so the bracket is
+    // the unique one the full bisection finds.
+    const int b = (int)(xi * R(kCdfGuide));
+    const uint16_t *g0 = P.cdfguide + ((size_t)id * nT + (size_t)(j - 1)) * (kCdfGuide + 1);
+    const uint16_t *g1 = g0 + (kCdfGuide + 1);
+    const int lo = max(1, (int)min(g0[b], g1[b]));
+    const int hi = min(nnu, (int)max(g0[b + 1], g1[b + 1]) + 1);
+    if (lo > i0) {
+      i0 = lo;
+      P0 = KB(i0);
+      if (P0 > R(1.0)) P0 = R(1.0);
+    }
+    if (hi < i1) {
+      i1 = hi;
+      P1 = KB(i1);
+      if (P1 > R(1.0)) P1 = R(1.0);
+    }
   }
   while ((i1 - i0) > 1) {
     const int i = (i0 + i1) / 2;
-    double Pm = KB(i);
-    if (Pm > 1.0) Pm = 1.0;
+    R Pm = KB(i);
+    if (Pm > R(1.0)) Pm = R(1.0);
     if (Pm < P0) {
       P0 = Pm;
       i0 = i;
@@ -503,217 +603,223 @@ __device__ __attribute__((noinline)) double lucydustfreq(double T0, int id, Mrg 
     }
   }
   if (xi < P0)
-    f = 0.0;
+    f = R(0.0);
   else if (xi > P1)
-    f = 1.0;
-  else if (P0 <= 0.0)
-    f = 0.5;
-  else if (P1 <= 0.0 || xi <= 0.0) {
-  } else
-    f = hcm_log(xi / P0) / hcm_log(P1 / P0);
-  return T0 * nuint[i0 - 1] * hcm_pow(nuint[i1 - 1] / nuint[i0 - 1], f);
+    f = R(1.0);
+  else if (P0 <= R(0.0))
+    f = R(0.5);
+  else if (P1 <= R(0.0) || xi <= R(0.0)) {
+  } else if (!(P1 > P0))  // nodes that round to one value (FP32): no log(1) = 0 divisor
+    f = R(0.0);
+  else
+    f = m_log(xi / P0) / m_log(P1 / P0);
+  return T0 * R(nuint[i0 - 1]) * m_pow(R(nuint[i1 - 1]) / R(nuint[i0 - 1]), f);
 }
 
 // opacset for all active species: kapd = kappa * R*(cm) * kappa_V and the
 // albedo.  launch_order picks newdisktherm's (kappa*rmincgs)*kappav or
 // propagate's ((kappa*rstar)*rsol)*kappav grouping.
-template <int NA>
-__device__ __forceinline__ void set_opacity(Pkt<NA> &k, bool launch_order) {
+template <class R, int NA>
+__device__ __forceinline__ void set_opacity(Pkt<R, NA> &k, bool launch_order) {
   const Params &P = c_P;
 #pragma unroll
   for (int s = 0; s < NA; ++s) {
     if (s < P.nact) {
       const int id = P.act[s] - 1;
-      const Opac1 op = opacset_species(id, k.nub);
+      const Opac1 op = opacset_species(id, (double)k.nub);
       k.kapd[s] = launch_order ? op.kappa * P.rmincgs * P.kappav[id]
                                : op.kappa * P.rstar * kRsol * P.kappav[id];
-      k.alb[s] = op.albedo;
+      k.alb[s] = R(op.albedo);
     }
   }
 }
 
-template <int NA>
-__device__ __forceinline__ void initp_sample(Pkt<NA> &k, Mrg &rng) {  // initp_common.f90:1-29
-  k.d.cost = sqrt(ran(rng));
-  k.d.sint = sqrt(1.0 - k.d.cost * k.d.cost);
-  k.d.phi = kR2P * ran(rng);
-  k.d.cosp = hcm_cos(k.d.phi);
-  k.d.sinp = hcm_sin(k.d.phi);
+template <class R, int NA>
+__device__ __forceinline__ void initp_sample(Pkt<R, NA> &k, Mrg &rng) {  // initp_common.f90:1-29
+  k.d.cost = m_sqrt(R(ran(rng)));
+  k.d.sint = m_sqrt(R(1.0) - k.d.cost * k.d.cost);
+  k.d.phi = R(kR2P) * R(ran(rng));
+  k.d.cosp = m_cos(k.d.phi);
+  k.d.sinp = m_sin(k.d.phi);
 }
 
-template <int NA>
-__device__ __forceinline__ void initp_transform(Pkt<NA> &k, double cosb, double sinb, double lp) {
-  if (fabs(cosb) < 0.9999999) {  // initp_common.f90:31-94
-    double costi = k.d.cost * cosb - k.d.sint * sinb * k.d.cosp;
-    if (fabs(costi) > 1.0) {
-      if (costi > 1.0) costi = 1.0;
-      if (k.d.cost < -1.0) costi = -1.0;
+template <class R, int NA>
+__device__ __forceinline__ void initp_transform(Pkt<R, NA> &k, R cosb, R sinb, R lp) {
+  const R kpi = R(kPi), kr2p = R(kR2P);
+  if (fabs(cosb) < R(0.9999999)) {  // initp_common.f90:31-94
+    R costi = k.d.cost * cosb - k.d.sint * sinb * k.d.cosp;
+    if (fabs(costi) > R(1.0)) {
+      if (costi > R(1.0)) costi = R(1.0);
+      if (k.d.cost < R(-1.0)) costi = R(-1.0);
     }
-    const double sinti = sqrt(1.0 - costi * costi);
-    double phinew;
-    if (sinti > 0.0000001) {
-      double cospn = (k.d.cost - costi * cosb) / (sinti * sinb);
-      if (fabs(cospn) > 1.0) cospn = (cospn < 0.0) ? -1.0 : 1.0;
-      if (k.d.phi < kPi) {
-        phinew = hcm_acos(cospn);
-        if (phinew > kPi) phinew = kPi;
+    const R sinti = m_sqrt(R(1.0) - costi * costi);
+    R phinew;
+    if (sinti > R(0.0000001)) {
+      R cospn = (k.d.cost - costi * cosb) / (sinti * sinb);
+      if (fabs(cospn) > R(1.0)) cospn = (cospn < R(0.0)) ? R(-1.0) : R(1.0);
+      if (k.d.phi < kpi) {
+        phinew = m_acos(cospn);
+        if (phinew > kpi) phinew = kpi;
       } else {
-        phinew = kR2P - hcm_acos(cospn);
-        if (phinew < kPi) phinew = kPi;
+        phinew = kr2p - m_acos(cospn);
+        if (phinew < kpi) phinew = kpi;
       }
     } else {
-      phinew = 0.0;
+      phinew = R(0.0);
     }
     k.d.phi = phinew;
     k.d.cost = costi;
     k.d.sint = sinti;
-  } else if (cosb < 0.0) {
+  } else if (cosb < R(0.0)) {
     k.d.cost = -k.d.cost;
   }
   k.d.phi = k.d.phi + lp;
-  if (k.d.phi > kR2P) k.d.phi = k.d.phi - kR2P;
-  if (k.d.phi < 0.0) k.d.phi = k.d.phi + kR2P;
-  k.d.sinp = hcm_sin(k.d.phi);
-  k.d.cosp = hcm_cos(k.d.phi);
+  if (k.d.phi > kr2p) k.d.phi = k.d.phi - kr2p;
+  if (k.d.phi < R(0.0)) k.d.phi = k.d.phi + kr2p;
+  k.d.sinp = m_sin(k.d.phi);
+  k.d.cosp = m_cos(k.d.phi);
 }
 
-__device__ __forceinline__ double clamp1(double x) { return x > 1.0 ? 1.0 : (x < -1.0 ? -1.0 : x); }
+template <class R>
+__device__ __forceinline__ R clamp1(R x) {
+  return x > R(1.0) ? R(1.0) : (x < R(-1.0) ? R(-1.0) : x);
+}
 
-__device__ __forceinline__ void bin_init(double nub, int i_orig_init) {
+// Walk direction from the packet's angles.
+template <class R, int NA>
+__device__ __forceinline__ void set_walk_dir(Pkt<R, NA> &k) {
+  if constexpr (sizeof(R) == sizeof(double)) {
+    k.ux = (double)(k.d.sint * k.d.cosp);
+    k.uy = (double)(k.d.sint * k.d.sinp);
+    k.uz = (double)k.d.cost;
+  } else {
+    // FP32 direction cosines are unit only to ~1e-7; the wall roots assume a
+    // unit vector, so the walk direction is renormalised in FP64.
+    const double ux = (double)k.d.sint * (double)k.d.cosp;
+    const double uy = (double)k.d.sint * (double)k.d.sinp;
+    const double uz = (double)k.d.cost;
+    const double inv = 1.0 / sqrt(ux * ux + uy * uy + uz * uz);
+    k.ux = ux * inv;
+    k.uy = uy * inv;
+    k.uz = uz * inv;
+  }
+}
+
+template <class R>
+__device__ __forceinline__ void bin_init(R nub, int i_orig_init) {
   const Params &P = c_P;
   if (!P.si_init) return;
-  const int inub =
-      clampi((int)((double)P.nfreq * hcm_log(nub / P.numin) / P.lnurat) + 1, 1, P.nfreq);
+  const int inub = clampi(
+      (int)((double)P.nfreq * (double)m_log(R(nub / P.numin)) / P.lnurat) + 1, 1, P.nfreq);
   atomic_add(&P.si_init[(size_t)(inub - 1) + (size_t)P.nfreq * (size_t)(i_orig_init - 1)], 1.0);
   atomic_add(&P.si_init[(size_t)(inub - 1)], 1.0);
 }
 
-template <int NA, bool TRACE>
-__device__ void peel_direct(const Pkt<NA> &k, double cosb, double sinb, double coslp,
-                            double sinlp, Acc &acc, Trace &tr) {
-  const Params &P = c_P;
-  for (int q = 0; q < P.npeel; ++q) {
-    double w;
-    if (!P.ips) {
-      const double cosnorm =
-          cosb * P.coste[q] + (sinb * P.sinte[q] * (P.cospe[q] * coslp + P.sinpe[q] * sinlp));
-      if (!(cosnorm > 0.0)) continue;
-      w = 4.0 * k.s.I * cosnorm;
-    } else {
-      w = k.s.I * 2.0;
-    }
-    const PeelOut o = peeloff(k, w, 0, q);
-    acc.err_wall += o.err;
-    acc.npeelray++;
-    acc.npeelcross += (uint32_t)o.ncross;
-    if constexpr (TRACE) {
-      tr.tau_peel += o.tau;
-      tr.npeelcross += o.ncross;
-    }
-  }
-}
-
 // Star / hot-spot launch (initpspot.f90:3-175 or initp_ps.f90:3-79, then
 // newdisktherm.F90:256-377).
-template <int NA, bool TRACE>
-__device__ void launch_star(Pkt<NA> &k, Mrg &rng, bool hotspot, Acc &acc, Trace &tr) {
+template <class R, int NA>
+__device__ __forceinline__ void launch_star(Pkt<R, NA> &k, Mrg &rng, bool hotspot, R *nrm) {
   const Params &P = c_P;
-  double cosb, sinb, lp;
+  const R kpi = R(kPi), kr2p = R(kR2P);
+  R cosb, sinb, lp;
   int ispotph = hotspot ? 1 : 0;
-  k.s = Stokes4{1.0, 0.0, 0.0, 0.0};
+  k.s = Stokes4<R>{R(1.0), R(0.0), R(0.0), R(0.0)};
   if (!P.ips) {
     for (;;) {
       for (;;) {
-        cosb = 1.0 - 2.0 * ran(rng);
-        sinb = sqrt(1.0 - cosb * cosb);
-        if (cosb != 0.0 && fabs(cosb) < 1.0) break;
+        cosb = R(1.0) - R(2.0) * R(ran(rng));
+        sinb = m_sqrt(R(1.0) - cosb * cosb);
+        if (cosb != R(0.0) && fabs(cosb) < R(1.0)) break;
       }
-      lp = kR2P * ran(rng);
+      lp = kr2p * R(ran(rng));
       if (P.ispot == 1 && ispotph == 1) {
-        const double cphi = hcm_cos(lp), sphi = hcm_sin(lp);
-        const double d1 = P.scspot1 * sinb * cphi + P.ssspot1 * sinb * sphi + P.cspot1 * cosb;
-        const double d2 = P.scspot2 * sinb * cphi + P.ssspot2 * sinb * sphi + P.cspot2 * cosb;
-        if (P.nspot == 1 ? (d1 < P.csprad) : (d1 < P.csprad && d2 < P.csprad)) continue;
+        const R cphi = m_cos(lp), sphi = m_sin(lp);
+        const R d1 = R(P.scspot1) * sinb * cphi + R(P.ssspot1) * sinb * sphi + R(P.cspot1) * cosb;
+        const R d2 = R(P.scspot2) * sinb * cphi + R(P.ssspot2) * sinb * sphi + R(P.cspot2) * cosb;
+        const R cs = R(P.csprad);
+        if (P.nspot == 1 ? (d1 < cs) : (d1 < cs && d2 < cs)) continue;
       }
       break;
     }
     if (P.ispot == 1 && P.fspot < 1.0) {
-      double dellon = (lp < kPi) ? lp : kR2P - lp;
-      if (P.cspot1 * cosb + P.sspot1 * sinb * hcm_cos(dellon) > P.csprad) ispotph = 1;
+      R dellon = (lp < kpi) ? lp : kr2p - lp;
+      if (R(P.cspot1) * cosb + R(P.sspot1) * sinb * m_cos(dellon) > R(P.csprad)) ispotph = 1;
       if (P.nspot == 2) {
-        dellon = (lp < kPi) ? kPi - lp : lp - kPi;
-        if (P.cspot2 * cosb + P.sspot2 * sinb * hcm_cos(dellon) > P.csprad) ispotph = 1;
+        dellon = (lp < kpi) ? kpi - lp : lp - kpi;
+        if (R(P.cspot2) * cosb + R(P.sspot2) * sinb * m_cos(dellon) > R(P.csprad)) ispotph = 1;
       }
     }
     initp_sample(k, rng);
     initp_transform(k, cosb, sinb, lp);
   } else {
     for (;;) {
-      cosb = clamp1(1.0 - 2.0 * ran(rng));
-      sinb = clamp1(sqrt(1.0 - cosb * cosb));
-      if (cosb != 0.0) break;
+      cosb = clamp1(R(1.0) - R(2.0) * R(ran(rng)));
+      sinb = clamp1(m_sqrt(R(1.0) - cosb * cosb));
+      if (cosb != R(0.0)) break;
     }
     k.d.cost = cosb;
     k.d.sint = sinb;
-    lp = kR2P * ran(rng);
+    lp = kr2p * R(ran(rng));
     k.d.phi = lp;
-    k.d.sinp = clamp1(hcm_sin(k.d.phi));
-    k.d.cosp = clamp1(hcm_cos(k.d.phi));
+    k.d.sinp = clamp1(m_sin(k.d.phi));
+    k.d.cosp = clamp1(m_cos(k.d.phi));
   }
-  k.nub = bbfreq(rng, ispotph ? P.tshock : P.tstar);
+  k.nub = bbfreq<R>(rng, R(ispotph ? P.tshock : P.tstar));
   if (hotspot) {
-    if (ran(rng) < 0.5) k.nub = 1.2398 / 0.06 + ran(rng) * (1.2398 / 0.015 - 1.2398 / 0.06);
+    if (ran(rng) < 0.5)
+      k.nub = R(1.2398 / 0.06) + R(ran(rng)) * R(1.2398 / 0.015 - 1.2398 / 0.06);
     bin_init(k.nub, 3);
   } else {
     bin_init(k.nub, 2);
   }
   k.i_orig = 1;
   set_opacity(k, true);
-  const double coslp = hcm_cos(lp), sinlp = hcm_sin(lp);
-  k.z = cosb * 1.0;
-  const double rp = sinb * 1.0;
-  k.x = rp * coslp;
-  k.y = rp * sinlp;
+  const R coslp = m_cos(lp), sinlp = m_sin(lp);
+  k.z = (double)cosb * 1.0;
+  const double rp = (double)sinb * 1.0;
+  k.x = rp * (double)coslp;
+  k.y = rp * (double)sinlp;
   k.rsq = k.z * k.z + rp * rp;
-  k.ux = k.d.sint * k.d.cosp;
-  k.uy = k.d.sint * k.d.sinp;
-  k.uz = k.d.cost;
+  set_walk_dir(k);
   k.iflag = 0;
   k.ir = 1;
   k.on_wall = true;
-  k.it = locate(P.thetarr, P.ntg, hcm_acos(cosb));
-  k.ip = locate(P.phiarr, P.npg, lp);
-  if (P.ipeel) peel_direct<NA, TRACE>(k, cosb, sinb, coslp, sinlp, acc, tr);
+  k.it = locate(P.thetarr, P.ntg, (double)m_acos(cosb));
+  k.ip = locate(P.phiarr, P.npg, (double)lp);
+  nrm[0] = cosb;
+  nrm[1] = sinb;
+  nrm[2] = coslp;
+  nrm[3] = sinlp;
 }
 
 // Star re-emission of a packet that reached the stellar surface.
-template <int NA, bool TRACE>
-__device__ void star_reemit(Pkt<NA> &k, Mrg &rng, Acc &acc, Trace &tr) {
+template <class R, int NA>
+__device__ __forceinline__ void star_reemit(Pkt<R, NA> &k, Mrg &rng, Acc &acc, R *nrm) {
   const Params &P = c_P;
   acc.aflux++;
-  const double cosb = clamp1(k.z / sqrt(k.rsq));
-  const double sinb = sqrt(1.0 - cosb * cosb);
-  double lp = hcm_atan2(k.y, k.x);
-  if (lp < 0.0) lp = lp + kR2P;
-  k.s = Stokes4{1.0, 0.0, 0.0, 0.0};
+  const R cosb = clamp1(R(k.z / sqrt(k.rsq)));
+  const R sinb = m_sqrt(R(1.0) - cosb * cosb);
+  R lp = m_atan2(R(k.y), R(k.x));
+  if (lp < R(0.0)) lp = lp + R(kR2P);
+  k.s = Stokes4<R>{R(1.0), R(0.0), R(0.0), R(0.0)};
   initp_sample(k, rng);
   initp_transform(k, cosb, sinb, lp);
-  k.nub = bbfreq(rng, P.tstar);
+  k.nub = bbfreq<R>(rng, R(P.tstar));
   k.i_orig = 1;
   k.iflag = 0;
   set_opacity(k, true);
-  k.ux = k.d.sint * k.d.cosp;
-  k.uy = k.d.sint * k.d.sinp;
-  k.uz = k.d.cost;
+  set_walk_dir(k);
   k.ir = 1;
   k.on_wall = true;
-  if (P.ipeel) peel_direct<NA, TRACE>(k, cosb, sinb, hcm_cos(lp), hcm_sin(lp), acc, tr);
-  if constexpr (TRACE) tr.nstar++;
+  nrm[0] = cosb;
+  nrm[1] = sinb;
+  nrm[2] = m_cos(lp);
+  nrm[3] = m_sin(lp);
 }
 
 // select_dust.f90 over the active species (zero-density slots add nothing).
-template <int NA>
-__device__ __forceinline__ int select_dust(const Pkt<NA> &k, size_t cell, Mrg &rng) {
+template <class R, int NA>
+__device__ __forceinline__ int select_dust(const Pkt<R, NA> &k, size_t cell, Mrg &rng) {
   const Params &P = c_P;
   const double xran = ran(rng);
   const double *dn = P.dens + cell * (size_t)P.nact;
@@ -733,17 +839,18 @@ __device__ __forceinline__ int select_dust(const Pkt<NA> &k, size_t cell, Mrg &r
   return -1;
 }
 
-template <int NA>
-__device__ void bin_exit(Pkt<NA> &k, double xpold, double ypold, double zpold, int iscat,
+template <class R, int NA>
+__device__ __forceinline__ void bin_exit(Pkt<R, NA> &k, double xpold, double ypold, double zpold, int iscat,
                          Acc &acc) {
   const Params &P = c_P;
-  int it = (int)((double)P.nmu * (1.0 - k.d.cost) / 2.0) + 1;
+  int it = (int)((double)P.nmu * (1.0 - (double)k.d.cost) / 2.0) + 1;
   it = clampi(it, 1, P.nmu);
-  if (k.d.phi < 0.0) k.d.phi = k.d.phi + kR2P;
-  const int ip = clampi((int)((double)P.nph * k.d.phi / kR2P) + 1, 1, P.nph);
+  if (k.d.phi < R(0.0)) k.d.phi = k.d.phi + R(kR2P);
+  const int ip = clampi((int)((double)P.nph * (double)k.d.phi / kR2P) + 1, 1, P.nph);
   const double ut = P.u[it - 1];
-  const double x = ypold * k.d.cosp - xpold * k.d.sinp;
-  const double y = zpold * P.sini[it - 1] - ypold * ut * k.d.sinp - xpold * ut * k.d.cosp;
+  const double cp = (double)k.d.cosp, sp = (double)k.d.sinp;
+  const double x = ypold * cp - xpold * sp;
+  const double y = zpold * P.sini[it - 1] - ypold * ut * sp - xpold * ut * cp;
   const double rho2 = (x * x + y * y);
   if (rho2 < P.aperture2[P.nap - 1] * 1.0001) {
     acc.flux++;
@@ -753,7 +860,7 @@ __device__ void bin_exit(Pkt<NA> &k, double xpold, double ypold, double zpold, i
     }
   }
   const int inub =
-      clampi((int)((double)P.nfreq * hcm_log(k.nub / P.numin) / P.lnurat) + 1, 1, P.nfreq);
+      clampi((int)((double)P.nfreq * (double)m_log(R(k.nub / P.numin)) / P.lnurat) + 1, 1, P.nfreq);
   const int i_inter = (k.iflag == 1) ? 2 : ((k.i_orig == 1) ? 1 : 3);
   const int indx[3] = {1, 1 + k.i_orig, 5 + i_inter};
   const bool counted = !(k.iflag == 0 && k.i_orig == 4);
@@ -761,6 +868,7 @@ __device__ void bin_exit(Pkt<NA> &k, double xpold, double ypold, double zpold, i
     const size_t n = (size_t)P.nfreq * P.nmu * P.nph * P.nap * kNo;
     for (int ia = 0; ia < P.nap; ++ia) {
       if (rho2 < P.aperture2[ia]) {
+#pragma unroll
         for (int j = 0; j < 3; ++j) {
           const size_t q =
               (size_t)(inub - 1) +
@@ -769,173 +877,321 @@ __device__ void bin_exit(Pkt<NA> &k, double xpold, double ypold, double zpold, i
                    (size_t)P.nmu * ((size_t)(ip - 1) +
                                     (size_t)P.nph * ((size_t)ia + (size_t)P.nap *
                                                                       (size_t)(indx[j] - 1))));
-          atomic_add(&P.sed[q], k.s.I);
-          atomic_add(&P.sed[q + n], k.s.Q);
-          atomic_add(&P.sed[q + 2 * n], k.s.U);
-          atomic_add(&P.sed[q + 3 * n], k.s.V);
-          atomic_add(&P.sed[q + 4 * n], k.s.I * k.s.I);
-          atomic_add(&P.sed[q + 5 * n], k.s.Q * k.s.Q);
-          atomic_add(&P.sed[q + 6 * n], k.s.U * k.s.U);
-          atomic_add(&P.sed[q + 7 * n], k.s.V * k.s.V);
+          const double sI = k.s.I, sQ = k.s.Q, sU = k.s.U, sV = k.s.V;
+          atomic_add(&P.sed[q], sI);
+          atomic_add(&P.sed[q + n], sQ);
+          atomic_add(&P.sed[q + 2 * n], sU);
+          atomic_add(&P.sed[q + 3 * n], sV);
+          atomic_add(&P.sed[q + 4 * n], sI * sI);
+          atomic_add(&P.sed[q + 5 * n], sQ * sQ);
+          atomic_add(&P.sed[q + 6 * n], sU * sU);
+          atomic_add(&P.sed[q + 7 * n], sV * sV);
           atomic_add(&P.sed[q + 8 * n], 1.0);
         }
       }
     }
   }
-  if (counted) acc.sumsub += k.s.I;
-  acc.sumall += k.s.I;
+  if (counted) acc.sumsub += (double)k.s.I;
+  acc.sumall += (double)k.s.I;
   if (k.iflag == 1) acc.scount++;
 }
 
-// propagate.f90:1-325 for one packet; returns 0 escaped, 1 killed, 2 error.
-template <int NA, bool TRACE>
-__device__ int propagate(Pkt<NA> &k, Mrg &rng, Acc &acc, Trace &tr) {
-  const Params &P = c_P;
-  LucyAcc<NA> la;
-  la.cell = -1;
-  double xpold = k.x, ypold = k.y, zpold = k.z;
-  int iscat = 0, iint = 0, pir = 0, pit = 0, pip = 0, status = 0;
-  bool first = true;
-  int st = tauint<NA, TRACE>(k, -hcm_log(ran(rng)), la, acc, tr);
-  for (;;) {
-    while (st == ST_STAR) {
-      star_reemit<NA, TRACE>(k, rng, acc, tr);
-      const double tau = -hcm_log(ran(rng));
-      xpold = k.x;
-      ypold = k.y;
-      zpold = k.z;
-      st = tauint<NA, TRACE>(k, tau, la, acc, tr);
-    }
-    if (st == ST_ERROR) {
-      status = 2;
-      break;
-    }
-    if (st == ST_EXIT) break;
-    const size_t cell = cell_of(k.ir, k.it, k.ip);
-    if (!first) {
-      if (k.ir != pir || k.it != pit || k.ip != pip) iint = 0;
-      if (iint > P.iintmax) {
-        if (P.killed) atomicAdd(&P.killed[cell], 1);
-        acc.abflux++;
-        status = 1;
-        break;
-      }
-    }
-    const int slot = select_dust(k, cell, rng);
-    if (slot < 0) {
-      acc.err_dust++;
-      status = 2;
-      break;
-    }
-    const int idust2 = P.act[slot] - 1;
-    if (first) acc.tot++;
-    first = false;
-    if (ran(rng) <= k.alb[slot]) {
-      k.iflag = 1;
-      ++iscat;
-      ++iint;
-      const HG h = hg_of(idust2, k.nub);
-      if (P.ipeel) peel_all<NA, TRACE>(k, k.s.I, idust2, acc, tr);
-      double ntr = 0.0, nht = 0.0;
-      stokes(k.d, k.s, h, rng, ntr, nht);
-      acc.htot += (uint32_t)ntr;
-      acc.hit += (uint32_t)nht;
-      acc.nscat_ev++;
-      if constexpr (TRACE) tr.nscat++;
-    } else {
-      lucy_open(la, cell);
-      la.nabs_any += 1;
-#pragma unroll
-      for (int s2 = 0; s2 < NA; ++s2)
-        if (s2 == slot) la.nabs[s2] += 1.0;
-      k.i_orig = (idust2 == 0 || idust2 == 1 || idust2 == 4 || idust2 == 5) ? 2 : 3;
-      k.s = Stokes4{1.0, 0.0, 0.0, 0.0};
-      k.d.cost = 1.0 - 2.0 * ran(rng);
-      k.d.sint = sqrt(1.0 - hcm_pow(k.d.cost, 2.0));
-      k.d.phi = kR2P * ran(rng);
-      k.d.cosp = hcm_cos(k.d.phi);
-      k.d.sinp = hcm_sin(k.d.phi);
-      k.nub = lucydustfreq(P.tdust[cell * (size_t)P.nact + slot], idust2, rng);
-      k.uy = k.d.sint * k.d.sinp;
-      k.ux = k.d.sint * k.d.cosp;
-      k.uz = k.d.cost;
-      k.iflag = 0;
-      set_opacity(k, false);
-      if (P.ipeel) peel_all<NA, TRACE>(k, k.s.I, idust2, acc, tr);
-      ++iint;
-      acc.nabs_ev++;
-      if constexpr (TRACE) tr.nabs++;
-    }
-    const double tau = -hcm_log(ran(rng));
-    xpold = k.x;
-    ypold = k.y;
-    zpold = k.z;
-    k.ux = k.d.sint * k.d.cosp;
-    k.uy = k.d.sint * k.d.sinp;
-    k.uz = k.d.cost;
-    pir = k.ir;
-    pit = k.it;
-    pip = k.ip;
-    st = tauint<NA, TRACE>(k, tau, la, acc, tr);
-  }
-  lucy_flush(la);
-  if (status == 0) bin_exit(k, xpold, ypold, zpold, iscat, acc);
-  return status;
+// Optical depth to the next event, -ln(xi) (propagate.f90:33).  In FP32 a
+// draw within 2^-25 of 1 rounds to 1.0f and would give tau = 0, which the
+// walk cannot place inside an empty cell (0/0); such draws take the
+// smallest FP32 step instead.  The FP64 draw is never 1.
+template <class R>
+__device__ __forceinline__ double draw_tau(Mrg &rng) {
+  const double t = (double)(-m_log(R(ran(rng))));
+  if constexpr (sizeof(R) == sizeof(float)) return t > 0.0 ? t : 5.9604644775390625e-08;
+  return t;
 }
+
+// One lane's packet between events.
+template <class R, int NA>
+struct Lane {
+  Pkt<R, NA> k;
+  Mrg rng;
+  LucyAcc<R, NA> la;
+  double xpold, ypold, zpold;
+  long long pid;
+  int iscat, iint, pir, pit, pip;
+  uint32_t work;  // events + crossings of the packet so far
+  bool first;
+};
 
 __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
 
-template <int NA, bool TRACE>
-__global__ __launch_bounds__(256) void transport_fp64() {
+#ifndef HC_WAVES_PER_EU
+#define HC_WAVES_PER_EU 1  // occupancy floor; below it the compiler would spill
+#endif
+
+// Peel records per wavefront queue.  An iteration adds at most one record
+// per lane, so a queue flushed whenever it holds more than len-64 records
+// never overflows.  Four queues and their counters fit the 64 KiB a
+// workgroup can allocate.
+template <class R, int NA>
+constexpr int peel_queue_len() {
+  constexpr size_t fit = (65536 - 64) / (4 * sizeof(PeelRec<R, NA>));
+  return fit < 96 ? (int)fit : 96;
+}
+static_assert(peel_queue_len<double, 4>() >= 80, "peel queue too short");
+static_assert(peel_queue_len<double, 8>() > 64, "peel queue too short");
+
+template <class R, int NA>
+constexpr size_t peel_lds_bytes() {  // queue counters + four wavefront queues
+  return 64 + 4 * (size_t)peel_queue_len<R, NA>() * sizeof(PeelRec<R, NA>);
+}
+
+// Walk every queued (record, observer) pair of this wavefront: lanes take
+// consecutive rays, so the observers of one record run side by side.
+template <class R, int NA, bool TRACE>
+__device__ __forceinline__ void flush_peels(const PeelRec<R, NA> *q, int n, Acc &acc) {
   const Params &P = c_P;
-  Acc acc{};
-  for (;;) {
-    const unsigned long long p = atomicAdd(P.next_packet, 1ull);
-    if (p >= (unsigned long long)P.np) break;
-    Mrg rng = mrg_segment(P.g1, P.g2, P.seg0 + p, P.pow1, P.pow2);
-    Pkt<NA> k;
-    Trace tr{0.0, 0.0, 0, 0, 0, 0, 0, 14695981039346656037ull};
-    const long long iphot = (long long)p + 1;
-    launch_star<NA, TRACE>(k, rng, iphot < P.npacc + P.npaccstar, acc, tr);
-    if constexpr (TRACE) hash_cell(tr, k.ir, k.it, k.ip);
-    const int status = propagate<NA, TRACE>(k, rng, acc, tr);
+  const int lane = threadIdx.x & 63;
+  const int total = n * P.npeel;
+  for (int r = lane; r < total; r += 64) {
+    const int ri = r / P.npeel, obs = r - ri * P.npeel;
+    const PeelRec<R, NA> &rec = q[ri];
+    double w;
+    if ((rec.flags >> 9) & 1) {  // direct stellar emission (newdisktherm.F90:342-377)
+      if (!P.ips) {
+        const R cosnorm =
+            rec.cosb * R(P.coste[obs]) +
+            (rec.sinb * R(P.sinte[obs]) * (R(P.cospe[obs]) * rec.coslp + R(P.sinpe[obs]) * rec.sinlp));
+        if (!(cosnorm > R(0.0))) continue;
+        w = (double)(R(4.0) * rec.s.I * cosnorm);
+      } else {
+        w = (double)(rec.s.I * R(2.0));
+      }
+    } else {
+      w = (double)rec.s.I;
+    }
+    const PeelOut o = peeloff(rec, w, obs);
+    acc.err_wall += o.err;
+    acc.npeelray++;
+    acc.npeelcross += (uint32_t)o.ncross;
     if constexpr (TRACE) {
-      hc_trace &o = P.trace[p];
-      o.x = k.x;
-      o.y = k.y;
-      o.z = k.z;
-      o.cost = k.d.cost;
-      o.sint = k.d.sint;
-      o.phi = k.d.phi;
-      o.I = k.s.I;
-      o.Q = k.s.Q;
-      o.U = k.s.U;
-      o.V = k.s.V;
-      o.nub = k.nub;
-      o.path = tr.path;
-      o.tau_peel = tr.tau_peel;
-      o.ncross = tr.ncross;
-      o.npeelcross = tr.npeelcross;
-      o.nscat = tr.nscat;
-      o.nabs = tr.nabs;
-      o.nstar = tr.nstar;
-      o.ntrial = 0;
-      o.cellhash = tr.hash;
-      o.status = status;
-      o.ir = k.ir;
-      o.it = k.it;
-      o.ip = k.ip;
-      o.pad = 0;
+      atomicAdd(reinterpret_cast<unsigned long long *>(&P.trace[rec.pid].npeelcross),
+                (unsigned long long)o.ncross);
+      atomic_add(&P.trace[rec.pid].tau_peel, o.tau);
+    }
+  }
+}
+
+// Event loop of the pass (propagate.f90:1-325 around newdisktherm.F90:194-449)
+// with path regeneration.  Each lane holds one packet; a wavefront
+// alternates two phases:
+//  - event phase: every lane whose packet needs an event (launch, star
+//    re-emission, interaction, end of the packet and fetch of the next one)
+//    performs it, and the peel-off records those events queued are walked
+//    once the wavefront's queue fills;
+//  - walk phase: lanes whose packets are between events cross one cell wall
+//    per step; lanes that reach their next event wait, and the phase ends
+//    once regen_lanes lanes wait (or nobody walks).
+// A packet's work therefore never waits for another packet's walk to end,
+// and the event code runs for many lanes at once.
+template <class R, int NA, bool TRACE, bool PEEL>
+__global__ __launch_bounds__(256, HC_WAVES_PER_EU) void transport() {
+  extern __shared__ unsigned char lds[];
+  const Params &P = c_P;
+  const int wave = threadIdx.x >> 6;
+  constexpr int kQ = peel_queue_len<R, NA>();
+  PeelRec<R, NA> *queue = nullptr;
+  int *qcount = nullptr;
+  if constexpr (PEEL) {
+    qcount = reinterpret_cast<int *>(lds);
+    queue = reinterpret_cast<PeelRec<R, NA> *>(lds + 64) + (size_t)wave * kQ;
+    if ((threadIdx.x & 63) == 0) qcount[wave] = 0;
+  }
+  auto push = [&](const Pkt<R, NA> &k, int idust2, bool direct, long long pid, const R *nrm) {
+    if constexpr (PEEL) {
+      const int slot = atomicAdd(&qcount[wave], 1);
+      PeelRec<R, NA> &r = queue[slot];
+      make_rec(r, k, idust2, direct, pid);
+      r.cosb = nrm[0];
+      r.sinb = nrm[1];
+      r.coslp = nrm[2];
+      r.sinlp = nrm[3];
+    }
+  };
+  Acc acc{};
+  Lane<R, NA> L;
+  Trace tr{0.0, 0.0, 0, 0, 0, 0, 0, 14695981039346656037ull};
+  R nrm[4] = {R(0.0), R(0.0), R(0.0), R(0.0)};
+  double tau = 0.0, tsum = 0.0;
+  int st = ST_FETCH;
+  for (;;) {
+    // ---------------- event phase ----------------
+    if (st != ST_WALK && st != ST_DONE) {
+      Pkt<R, NA> &k = L.k;
+      int status = -1;  // >= 0: the packet ends (0 escaped, 1 killed, 2 error)
+      if (st == ST_EXIT) {
+        status = 0;
+      } else if (st == ST_ERROR) {
+        status = 2;
+      } else if (st == ST_ABORT) {  // work budget spent: dropped and counted
+        acc.warn++;
+        status = 2;
+      } else if (st == ST_STAR) {
+        L.work++;
+        star_reemit(k, L.rng, acc, nrm);
+        if constexpr (TRACE) tr.nstar++;
+        push(k, 0, true, L.pid, nrm);
+        tau = draw_tau<R>(L.rng);
+        L.xpold = k.x;
+        L.ypold = k.y;
+        L.zpold = k.z;
+      } else if (st == ST_INTERACT) {
+        L.work++;
+        const size_t cell = cell_of(k.ir, k.it, k.ip);
+        if (!L.first) {
+          if (k.ir != L.pir || k.it != L.pit || k.ip != L.pip) L.iint = 0;
+          if (L.iint > P.iintmax) {
+            if (P.killed) atomicAdd(&P.killed[cell], 1);
+            acc.abflux++;
+            status = 1;
+          }
+        }
+        if (status < 0) {
+          const int slot = select_dust(k, cell, L.rng);
+          if (slot < 0) {
+            acc.err_dust++;
+            status = 2;
+          } else {
+            const int idust2 = P.act[slot] - 1;
+            if (L.first) acc.tot++;
+            L.first = false;
+            R alb_slot = R(0.0);  // k.alb[slot] without a dynamically indexed register array
+#pragma unroll
+            for (int s2 = 0; s2 < NA; ++s2)
+              if (s2 == slot) alb_slot = k.alb[s2];
+            if (ran(L.rng) <= (double)alb_slot) {
+              k.iflag = 1;
+              ++L.iscat;
+              ++L.iint;
+              const HG<R> h = hg_of<R>(idust2, k.nub);
+              push(k, idust2, false, L.pid, nrm);
+              stokes<R>(k.d, k.s, h, L.rng, acc.htot, acc.hit);
+              acc.nscat_ev++;
+              if constexpr (TRACE) tr.nscat++;
+            } else {
+              lucy_open(L.la, cell);
+              L.la.nabs_any += 1;
+#pragma unroll
+              for (int s2 = 0; s2 < NA; ++s2)
+                if (s2 == slot) L.la.nabs[s2] += 1.0;
+              k.i_orig = (idust2 == 0 || idust2 == 1 || idust2 == 4 || idust2 == 5) ? 2 : 3;
+              k.s = Stokes4<R>{R(1.0), R(0.0), R(0.0), R(0.0)};
+              k.d.cost = R(1.0) - R(2.0) * R(ran(L.rng));
+              k.d.sint = m_sqrt(R(1.0) - m_pow(k.d.cost, R(2.0)));
+              k.d.phi = R(kR2P) * R(ran(L.rng));
+              k.d.cosp = m_cos(k.d.phi);
+              k.d.sinp = m_sin(k.d.phi);
+              k.nub = lucydustfreq<R>(P.tdust[cell * (size_t)P.nact + slot], idust2, L.rng);
+              set_walk_dir(k);
+              k.iflag = 0;
+              set_opacity(k, false);
+              push(k, idust2, false, L.pid, nrm);
+              ++L.iint;
+              acc.nabs_ev++;
+              if constexpr (TRACE) tr.nabs++;
+            }
+            tau = draw_tau<R>(L.rng);
+            L.xpold = k.x;
+            L.ypold = k.y;
+            L.zpold = k.z;
+            set_walk_dir(k);
+            L.pir = k.ir;
+            L.pit = k.it;
+            L.pip = k.ip;
+          }
+        }
+      }
+      if (status >= 0) {  // the packet ends
+        lucy_flush(L.la);
+        if (status == 0) bin_exit(k, L.xpold, L.ypold, L.zpold, L.iscat, acc);
+        if constexpr (TRACE) {
+          hc_trace &o = P.trace[L.pid];
+          o.x = k.x;
+          o.y = k.y;
+          o.z = k.z;
+          o.cost = k.d.cost;
+          o.sint = k.d.sint;
+          o.phi = k.d.phi;
+          o.I = k.s.I;
+          o.Q = k.s.Q;
+          o.U = k.s.U;
+          o.V = k.s.V;
+          o.nub = (double)k.nub;
+          o.path = tr.path;
+          o.ncross = tr.ncross;
+          o.nscat = tr.nscat;
+          o.nabs = tr.nabs;
+          o.nstar = tr.nstar;
+          o.ntrial = 0;
+          o.cellhash = tr.hash;
+          o.status = status;
+          o.ir = k.ir;
+          o.it = k.it;
+          o.ip = k.ip;
+          o.pad = 0;
+        }
+        st = ST_FETCH;
+      }
+      if (st == ST_FETCH) {  // take the next packet and launch it
+        const unsigned long long p = atomicAdd(P.next_packet, 1ull);
+        if (p >= (unsigned long long)P.np) {
+          st = ST_DONE;
+        } else {
+          L.pid = (long long)p;
+          L.rng = mrg_segment(P.g1, P.g2, P.seg0 + p, P.pow1, P.pow2);
+          L.la.cell = -1;
+          L.iscat = L.iint = L.pir = L.pit = L.pip = 0;
+          L.work = 0;
+          L.first = true;
+          if constexpr (TRACE) tr = Trace{0.0, 0.0, 0, 0, 0, 0, 0, 14695981039346656037ull};
+          launch_star(k, L.rng, (long long)p + 1 < P.npacc + P.npaccstar, nrm);
+          if constexpr (TRACE) hash_cell(tr, k.ir, k.it, k.ip);
+          push(k, 0, true, L.pid, nrm);
+          L.xpold = k.x;
+          L.ypold = k.y;
+          L.zpold = k.z;
+          tau = draw_tau<R>(L.rng);
+        }
+      }
+      if (st != ST_DONE) {
+        tsum = 0.0;
+        st = walk_begin(k);
+      }
+    }
+    if constexpr (PEEL) {
+      const int n = qcount[wave];
+      const bool last = __ballot(st != ST_DONE) == 0ull;
+      if (n > kQ - 64 || (last && n > 0)) {
+        flush_peels<R, NA, TRACE>(queue, n, acc);
+        if ((threadIdx.x & 63) == 0) qcount[wave] = 0;
+      }
+    }
+    if (__ballot(st != ST_DONE) == 0ull) break;
+    // ---------------- walk phase ----------------
+    for (;;) {
+      if (__ballot(st == ST_WALK) == 0ull) break;
+      if (__popcll(__ballot(st != ST_WALK && st != ST_DONE)) >= P.regen_lanes) break;
+      if (st == ST_WALK) {
+        st = walk_step<R, NA, TRACE>(L.k, tau, tsum, L.la, acc, tr);
+        if (++L.work > P.max_work) st = ST_ABORT;
+      }
     }
   }
   const double v[S_COUNT] = {(double)acc.tot,    (double)acc.flux,     (double)acc.sflux,
                              (double)acc.nscat,  (double)acc.scount,   (double)acc.abflux,
                              (double)acc.aflux,  acc.sumsub,           acc.sumall,
                              (double)acc.hit,    (double)acc.htot,     (double)acc.err_wall,
-                             (double)acc.err_dust, 0.0,
+                             (double)acc.err_dust, (double)acc.warn,
                              (double)acc.ncross, (double)acc.nscat_ev, (double)acc.nabs_ev,
                              (double)acc.npeelray, (double)acc.npeelcross};
 #pragma unroll
@@ -1042,6 +1298,7 @@ struct hc_ctx {
   double *u = nullptr, *sini = nullptr, *aperture2 = nullptr, *coste = nullptr,
          *sinte = nullptr, *cospe = nullptr, *sinpe = nullptr;
   int32_t *pow1 = nullptr, *pow2 = nullptr, *guide = nullptr;
+  uint16_t *cdfguide = nullptr;
   double glog0[kMaxSpecies] = {0}, ginv[kMaxSpecies] = {0};
   double *dtauabs = nullptr, *nabs = nullptr, *jmean = nullptr;
   int32_t *killed = nullptr, *dbig = nullptr;
@@ -1055,6 +1312,10 @@ struct hc_ctx {
   int32_t nlambda[kMaxSpecies] = {0};
   double kappav[kMaxSpecies] = {0};
   hc_sources src;
+  // launch shape
+  int regen_lanes = 16;
+  int blocks_per_cu = 0;
+  uint32_t max_work = 1u << 24;
 };
 
 namespace {
@@ -1130,7 +1391,7 @@ int hc_create(const hc_dims *dims, int device, hc_ctx **out) {
       dims->nap < 1 || dims->npeel < 0 || dims->nxhst < 1 || dims->nlammax < 2 || dims->nT < 2 ||
       dims->nnu < 2)
     return HC_E_ARG;
-  if (dims->precision != HC_FP64) return HC_E_UNSUPPORTED;
+  if (dims->precision != HC_FP64 && dims->precision != HC_FP32) return HC_E_UNSUPPORTED;
   hc_ctx *ctx = new hc_ctx();
   ctx->dims = *dims;
   ctx->device = device;
@@ -1273,6 +1534,38 @@ int hc_set_dust(hc_ctx *ctx, const int32_t *nlambda, const double *nudust, const
   }
   rc = upload(ctx, &ctx->guide, guide.data(), guide.size());
   if (rc) return rc;
+  // emissivity-CDF guide: per species and temperature row, the number of
+  // nodes with kapBint <= b/kCdfGuide.  Used only when every row increases
+  // strictly (then the bracket of an interpolated CDF is unique).
+  {
+    const int nT = d.nT, nnu = d.nnu;
+    std::vector<uint16_t> cg((size_t)d.ndg * nT * (kCdfGuide + 1));
+    bool strict = nnu < 65536;
+    for (int id = 0; id < d.ndg && strict; ++id)
+      for (int j = 0; j < nT && strict; ++j) {
+        const double *kb = kapBint + (size_t)nT * nnu * id + j;
+        for (int i = 1; i < nnu; ++i)
+          if (!(kb[(size_t)nT * i] > kb[(size_t)nT * (i - 1)])) {
+            strict = false;
+            break;
+          }
+        uint16_t *g = cg.data() + ((size_t)id * nT + j) * (kCdfGuide + 1);
+        int cnt = 0;
+        for (int b = 0; b <= kCdfGuide; ++b) {
+          const double v = (double)b / (double)kCdfGuide;
+          while (cnt < nnu && kb[(size_t)nT * cnt] <= v) ++cnt;
+          g[b] = (uint16_t)cnt;
+        }
+      }
+    if (ctx->cdfguide) {
+      (void)hipFree(ctx->cdfguide);
+      ctx->cdfguide = nullptr;
+    }
+    if (strict) {
+      rc = upload(ctx, &ctx->cdfguide, cg.data(), cg.size());
+      if (rc) return rc;
+    }
+  }
   ctx->have_dust = true;
   return HC_OK;
 }
@@ -1352,6 +1645,7 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
   HCK(hipMemsetAsync(ctx->si_init, 0, (size_t)d.nfreq * 5 * sizeof(double), s));
   HCK(hipMemsetAsync(ctx->scal, 0, S_COUNT * sizeof(double), s));
   HCK(hipMemsetAsync(ctx->counter, 0, sizeof(unsigned long long), s));
+  if (trace) HCK(hipMemsetAsync(ctx->trace, 0, (size_t)np * sizeof(hc_trace), s));
   const bool cube = d.imcube && ipeel && ctx->img_i;
   if (cube) {
     HCK(hipMemsetAsync(ctx->img_i, 0, nimg * kNo * sizeof(double), s));
@@ -1394,6 +1688,7 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
   P.nuint = ctx->nuint;
   P.kapBint = ctx->kapBint;
   P.guide = ctx->guide;
+  P.cdfguide = ctx->cdfguide;
   const hc_sources &S = ctx->src;
   P.rstar = S.rstar;
   P.rmincgs = S.rstar * kRsol;
@@ -1445,9 +1740,11 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
   P.seg0 = seg0;
   P.pow1 = ctx->pow1;
   P.pow2 = ctx->pow2;
-  P.dtauabs = out->dtauabs ? ctx->dtauabs : nullptr;
+  // the Lucy tallies always accumulate on the device: the host arrays in
+  // `out` only choose what is downloaded afterwards
+  P.dtauabs = ctx->dtauabs;
   P.nabs = ctx->nabs;
-  P.jmean = out->jmean ? ctx->jmean : nullptr;
+  P.jmean = ctx->jmean;
   P.killed = ctx->killed;
   P.dbig = ctx->dbig;
   P.sed = ctx->sed;
@@ -1461,20 +1758,41 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
   P.scal = ctx->scal;
   P.next_packet = ctx->counter;
   P.trace = trace ? ctx->trace : nullptr;
+  P.regen_lanes = ctx->regen_lanes;
+  P.max_work = ctx->max_work;
 
   HCK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_P), &P, sizeof P, 0, hipMemcpyHostToDevice, s));
   void (*kern)() = nullptr;
-  if (ctx->nact <= 4)
-    kern = trace ? transport_fp64<4, true> : transport_fp64<4, false>;
-  else
-    kern = trace ? transport_fp64<8, true> : transport_fp64<8, false>;
+  size_t shm = 0;
+  const bool fp32 = d.precision == HC_FP32;
+  const bool pk = P.ipeel != 0;
+#define HC_PICK(R_, NA_)                                                                   \
+  kern = trace ? (pk ? transport<R_, NA_, true, true> : transport<R_, NA_, true, false>)  \
+               : (pk ? transport<R_, NA_, false, true> : transport<R_, NA_, false, false>); \
+  shm = pk ? peel_lds_bytes<R_, NA_>() : 0
+  if (ctx->nact <= 4) {
+    if (fp32) {
+      HC_PICK(float, 4);
+    } else {
+      HC_PICK(double, 4);
+    }
+  } else {
+    if (fp32) {
+      HC_PICK(float, 8);
+    } else {
+      HC_PICK(double, 8);
+    }
+  }
+#undef HC_PICK
   int dev_cus = 0, bpm = 0;
   HCK(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-  HCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpm, kern, 256, 0));
-  long long blocks = (long long)std::max(dev_cus, 1) * std::max(bpm, 1);
+  HCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpm, kern, 256, shm));
+  const int per_cu = ctx->blocks_per_cu > 0 ? std::min(ctx->blocks_per_cu, std::max(bpm, 1))
+                                            : std::max(bpm, 1);
+  long long blocks = (long long)std::max(dev_cus, 1) * per_cu;
   blocks = std::min<long long>(blocks, std::max<long long>((np + 255) / 256, 1));
   HCK(hipEventRecord(ctx->ev0, s));
-  if (np > 0) hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), 0, s);
+  if (np > 0) hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), shm, s);
   HCK(hipGetLastError());
   HCK(hipEventRecord(ctx->ev1, s));
   HCK(hipStreamSynchronize(s));
@@ -1613,6 +1931,24 @@ int hc_temperature(hc_ctx *ctx, const double *massarr, double ltot, double nphot
   return HC_OK;
 }
 
+int hc_set_option(hc_ctx *ctx, const char *name, int64_t value) {
+  if (!ctx || !name) return HC_E_ARG;
+  const std::string n(name);
+  if (n == "regen_lanes") {
+    if (value < 1 || value > 64) return fail(ctx, HC_E_ARG, "regen_lanes must be in 1..64");
+    ctx->regen_lanes = (int)value;
+  } else if (n == "max_work") {
+    if (value < 1 || value > 0x7fffffff) return fail(ctx, HC_E_ARG, "max_work must be in 1..2^31-1");
+    ctx->max_work = (uint32_t)value;
+  } else if (n == "blocks_per_cu") {
+    if (value < 0 || value > 64) return fail(ctx, HC_E_ARG, "blocks_per_cu must be in 0..64");
+    ctx->blocks_per_cu = (int)value;
+  } else {
+    return fail(ctx, HC_E_ARG, "hc_set_option: unknown option " + n);
+  }
+  return HC_OK;
+}
+
 int hc_destroy(hc_ctx *ctx) {
   if (!ctx) return HC_OK;
   (void)hipSetDevice(ctx->device);
@@ -1623,7 +1959,7 @@ int hc_destroy(hc_ctx *ctx) {
                   ctx->sinpe,  ctx->pow1,    ctx->pow2,    ctx->dtauabs, ctx->nabs,   ctx->jmean,
                   ctx->killed, ctx->dbig,    ctx->sed,     ctx->peel,    ctx->img_i,  ctx->img_q,
                   ctx->img_u,  ctx->img_v,   ctx->si_init, ctx->scal,    ctx->mass4,  ctx->counter,
-                  ctx->trace,  ctx->guide,   ctx->tdust2};
+                  ctx->trace,  ctx->guide,   ctx->tdust2,  ctx->cdfguide};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);

@@ -92,6 +92,13 @@ module hc_iface
       integer(c_int64_t), intent(out) :: nchanged
     end function hc_temperature
 
+    integer(c_int) function hc_set_option(ctx, name, value) bind(C, name='hc_set_option')
+      import :: c_int, c_ptr, c_char, c_int64_t
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: name(*)   ! NUL-terminated
+      integer(c_int64_t), value :: value
+    end function hc_set_option
+
     integer(c_int) function hc_destroy(ctx) bind(C, name='hc_destroy')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -139,6 +139,21 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
 int hc_temperature(hc_ctx *ctx, const double *massarr, double ltot, double nphot,
                    double *tdust2, int64_t *nchanged);
 
+/* Launch-shape knobs of the transport kernel, set by the autotuner
+ * (hochunk3d_amd/autotune.py); no counterpart in the reference, whose packet
+ * loop is serial.  Names:
+ *   "regen_lanes"   1..64: lanes of a wavefront that must wait for an event
+ *                   before the wavefront stops walking and runs the events
+ *                   (default 16);
+ *   "blocks_per_cu" 0..64: workgroups of 256 lanes per compute unit, 0 =
+ *                   as many as fit (default 0);
+ *   "max_work"      1..2^31-1: events + cell crossings after which a packet
+ *                   is dropped and counted in n_warn (default 2^24, far
+ *                   above any packet of the reference's models; it bounds
+ *                   the kernel's run time against runaway packets).
+ * Results do not depend on the first two. */
+int hc_set_option(hc_ctx *ctx, const char *name, int64_t value);
+
 int hc_destroy(hc_ctx *ctx);
 const char *hc_last_error(const hc_ctx *ctx);
 int hc_abi_version(void);

@@ -923,7 +923,12 @@ static inline double cell_opac(const ctx_t *c, const pkt_t *P, size_t cell) {
 }
 
 /* Walls of the current cell and the next one along (ux,uy,uz)
- * (peeloff_3d.f90:92-103). */
+ * (peeloff_3d.f90:92-103).  On the polar axis (x = y = 0) every phi wall
+ * passes through the point, so phiface's roots are all 0 and the walk would
+ * step from phi cell to phi cell without moving; a straight ray leaving the
+ * axis keeps the azimuth of its direction, so there the phi walls are
+ * skipped and the caller has put the packet in the phi cell of its
+ * direction (axis_phi_cell). */
 static inline int next_wall(const ctx_t *c, double ux, double uy, double uz, double x, double y,
                             double z, int ir, int it, int ip, int on_wall, double *dt, int *face) {
   const hco_grid *g = c->g;
@@ -931,9 +936,19 @@ static inline int next_wall(const ctx_t *c, double ux, double uy, double uz, dou
   hco_reset_t(&w);
   hco_radface(&w, ux, uy, uz, x, y, z, g->r2arr[ir - 1], g->r2arr[ir]);
   if (g->ntg > 2) hco_thetaface(&w, ux, uy, uz, x, y, z, g->tan2arr[it - 1], g->tan2arr[it]);
-  if (g->npg > 2 && g->ntg > 2)
+  if (g->npg > 2 && g->ntg > 2 && !(x == 0.0 && y == 0.0))
     hco_phiface(&w, g->aarr[ip - 1], g->barr[ip - 1], g->aarr[ip], g->barr[ip], ux, uy, x, y);
   return hco_find_next_wall(&w, on_wall, dt, face);
+}
+
+/* The phi cell a ray leaving the polar axis enters: the one holding the
+ * azimuth of its direction (unchanged when the ray runs along the axis). */
+static inline int axis_phi_cell(const hco_grid *g, double ux, double uy, int ip) {
+  double p;
+  if (g->npg <= 2 || (ux == 0.0 && uy == 0.0)) return ip;
+  p = HCO_ATAN2(uy, ux);
+  if (p < 0.0) p = p + R2P;
+  return hco_locate(g->phiarr, g->npg, p);
 }
 
 static inline void face_step(const hco_grid *g, int face, int *ir, int *it, int *ip) {
@@ -947,6 +962,32 @@ static inline void face_step(const hco_grid *g, int face, int *ir, int *it, int 
   }
   if (*ip == 0) *ip = g->npg - 1;
   if (*ip == g->npg) *ip = 1;
+}
+
+/* A straight ray through the empty grid from (x,y,z), starting in cell
+ * (ir,it,ip) (on a radial wall when on_wall), to the outer sphere or the
+ * star: the wall-crossing geometry of tauint/peeloff without opacity, for
+ * tests.  Returns the number of crossings (-1 when no wall is found or the
+ * ray makes more crossings than a straight line can), and the last cell. */
+int hco_walk_ray(const hco_grid *g, double x, double y, double z, double ux, double uy, double uz,
+                 int *ir, int *it, int *ip, int on_wall) {
+  ctx_t c;
+  int n = 0, face;
+  double dt;
+  memset(&c, 0, sizeof c);
+  c.g = g;
+  while (*ir >= 1 && *ir < g->nrg) {
+    if (n > 2 * (g->nrg + g->ntg + g->npg) + 16) return -1;
+    if (x == 0.0 && y == 0.0) *ip = axis_phi_cell(g, ux, uy, *ip);
+    if (!next_wall(&c, ux, uy, uz, x, y, z, *ir, *it, *ip, on_wall, &dt, &face)) return -1;
+    face_step(g, face, ir, it, ip);
+    x = x + dt * ux;
+    y = y + dt * uy;
+    z = z + dt * uz;
+    on_wall = 1;
+    n++;
+  }
+  return n;
 }
 
 enum { ST_INTERACT = 0, ST_EXIT = 1, ST_STAR = 2, ST_ERROR = 3 };
@@ -979,8 +1020,10 @@ static int tauint(ctx_t *c, pkt_t *P, double tau) {
     if (P->ir == g->nrg) return ST_EXIT;
   }
   for (;;) {
-    size_t cell = cell_index(g, P->ir, P->it, P->ip);
+    size_t cell;
     int id;
+    if (P->xp == 0.0 && P->yp == 0.0) P->ip = axis_phi_cell(g, P->ux, P->uy, P->ip);
+    cell = cell_index(g, P->ir, P->it, P->ip);
     if (!next_wall(c, P->ux, P->uy, P->uz, P->xp, P->yp, P->zp, P->ir, P->it, P->ip, P->on_wall,
                    &dt, &face)) {
       t->n_err_nowall++;
@@ -1045,7 +1088,7 @@ static void peeloff(ctx_t *c, const pkt_t *P0, double sip, int idust2, int peeli
   double coste = s->coste[peelid], sinte = s->sinte[peelid];
   double ux1 = sinte * cospe, uy1 = sinte * sinpe, uz1 = coste;
   double tsum = 0.0, dt, opac, factor, ximage, yimage, rho2;
-  int face, ix, iy, inub, ia, i_inter, k, indx[3];
+  int face, ix, iy, inub, ia, i_inter, k, indx[3], nstep = 0;
   if (on_wall) {
     int inner = fabs(rsq1 - g->r2arr[ir - 1]) < fabs(rsq1 - g->r2arr[ir]);
     int outwards = ux1 * xp1 + uy1 * yp1 + uz1 * zp1 > 0.0;
@@ -1056,6 +1099,13 @@ static void peeloff(ctx_t *c, const pkt_t *P0, double sip, int idust2, int peeli
     if (!inner && outwards) ir = ir + 1;
   }
   while (ir < g->nrg) {
+    /* a straight ray meets each sphere and each cone at most twice and each
+     * phi half-plane at most once: more crossings than that is a stuck ray */
+    if (++nstep > 2 * (g->nrg + g->ntg + g->npg) + 17) {
+      t->n_err_nowall++;
+      return;
+    }
+    if (xp1 == 0.0 && yp1 == 0.0) ip = axis_phi_cell(g, ux1, uy1, ip);
     if (!next_wall(c, ux1, uy1, uz1, xp1, yp1, zp1, ir, it, ip, on_wall, &dt, &face)) {
       t->n_err_nowall++;
       return;

@@ -124,6 +124,8 @@ void hco_phiface(hco_walls *w, double a1, double b1, double a2, double b2,
                  double ux, double uy, double x, double y);
 int hco_find_next_wall(const hco_walls *w, int on_wall, double *t, int *face);
 int hco_locate(const double *xx, int n, double x);
+int hco_walk_ray(const hco_grid *g, double x, double y, double z, double ux, double uy, double uz,
+                 int *ir, int *it, int *ip, int on_wall);
 double hco_bbfreq(hco_rng *r, double teff);
 double hco_lucydustfreq(const hco_dust *d, hco_rng *r, double T0, int id);
 double hco_kapp(const hco_dust *d, double T, int id);

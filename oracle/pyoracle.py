@@ -124,6 +124,9 @@ def load(variant: str = "libm") -> C.CDLL:
         lib.hco_find_next_wall.restype = C.c_int
         lib.hco_locate.argtypes = [_dp, C.c_int, C.c_double]
         lib.hco_locate.restype = C.c_int
+        lib.hco_walk_ray.argtypes = ([C.POINTER(Grid)] + [C.c_double] * 6 +
+                                     [C.POINTER(C.c_int)] * 3 + [C.c_int])
+        lib.hco_walk_ray.restype = C.c_int
         lib.hco_bbfreq.argtypes = [C.POINTER(Rng), C.c_double]
         lib.hco_bbfreq.restype = C.c_double
         lib.hco_lucydustfreq.argtypes = [C.POINTER(Dust), C.POINTER(Rng), C.c_double, C.c_int]
@@ -197,6 +200,15 @@ class Oracle:
 
     def set_temperature(self, tdust):
         self._arrs["tdust"][...] = tdust
+
+    def walk_ray(self, x, y, z, ux, uy, uz, ir, it, ip, on_wall=False):
+        """Crossings of a straight ray through the empty grid (hco_walk_ray)
+        and the last cell: (n, ir, it, ip); n = -1 for a lost ray."""
+        cir, cit, cip = C.c_int(ir), C.c_int(it), C.c_int(ip)
+        n = self.lib.hco_walk_ray(C.byref(self.grid), float(x), float(y), float(z), float(ux),
+                                  float(uy), float(uz), C.byref(cir), C.byref(cit),
+                                  C.byref(cip), int(on_wall))
+        return n, cir.value, cit.value, cip.value
 
     def run(self, np_: int, p_begin: int = 0, p_end: Optional[int] = None, ipeel: int = 0,
             seed1: int = 12345678, seed2: int = 87654321, gen: int = 1, seg0: int = 0,

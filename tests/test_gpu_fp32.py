@@ -1,0 +1,65 @@
+"""The FP32 path (FP32 packet directions, Stokes vectors, frequencies and
+event arithmetic with native elementary functions; FP64 positions, wall roots
+and accumulators) against the plain FP64 path on the same packets.
+
+Targets: SED error <= 1 % and Lucy temperature error <= 0.5 % against FP64.
+The two runs draw the same random segments, so their difference is the
+precision effect plus the Monte Carlo noise of histories that diverge."""
+import numpy as np
+import pytest
+
+from hochunk3d_amd import build_model, config1
+from hochunk3d_amd.testing import synthetic_temperature
+
+pytestmark = pytest.mark.gpu
+
+N = 2_000_000
+
+
+@pytest.fixture(scope="module")
+def runs():
+    from hochunk3d_amd._lib import HC_FP32, HC_FP64, Transport
+    m = build_model(config1(nfreq=50))
+    m.tdust[...] = synthetic_temperature(m)
+    out = {}
+    for prec in (HC_FP64, HC_FP32):
+        g = Transport(m, precision=prec)
+        r = g.run(N, ipeel=0)
+        t2, _ = g.temperature(N)
+        g.close()
+        out[prec] = (r, t2)
+    return m, out[HC_FP64], out[HC_FP32]
+
+
+def test_fp32_sed_within_one_percent(runs):
+    m, (r64, _), (r32, _) = runs
+    # SED summed over frequency per inclination bin (slot 0: all photons)
+    s64 = r64.sed[0, :, :, 0, 0, 0].sum(axis=0)
+    s32 = r32.sed[0, :, :, 0, 0, 0].sum(axis=0)
+    err = np.abs(s32 - s64) / s64
+    print("SED per inclination, rel err:", np.round(err, 5))
+    assert err.max() <= 0.01
+    # the spectrum itself, luminosity-weighted
+    f64 = r64.sed[0, :, :, 0, 0, 0].sum(axis=1)
+    f32 = r32.sed[0, :, :, 0, 0, 0].sum(axis=1)
+    werr = np.abs(f32 - f64).sum() / f64.sum()
+    print("luminosity-weighted spectral difference", werr)
+    assert werr <= 0.01
+
+
+def test_fp32_temperature_within_half_percent(runs):
+    m, (r64, t64), (r32, t32) = runs
+    well = r64.nabs >= 2000        # cells with a well-sampled absorbed energy
+    assert well.sum() > 100
+    rel = np.abs(t32[well] - t64[well]) / t64[well]
+    print("T rel err: median", np.median(rel), "p95", np.percentile(rel, 95), "max", rel.max())
+    assert np.median(rel) <= 0.005
+    assert np.percentile(rel, 95) <= 0.01
+
+
+def test_fp32_energy_bookkeeping(runs):
+    m, (r64, _), (r32, _) = runs
+    for k in ("sumall", "flux"):
+        assert abs(r32.scalars[k] - r64.scalars[k]) <= 0.002 * r64.scalars[k], k
+    assert r32.scalars["n_err_nowall"] <= 1e-5 * N
+    assert r32.scalars["n_err_dust"] == 0

@@ -89,3 +89,18 @@ def test_zero_packets(pair):
     g, _ = pair
     r = g.run(0)
     assert r.scalars["sumall"] == 0.0 and r.sed.sum() == 0.0
+
+
+def test_device_resident_pass_feeds_temperature(pair):
+    """A pass whose tallies stay on the device (no host arrays given) must
+    still accumulate dtauabs, so the temperature update that follows it
+    equals the oracle's on the same packets."""
+    g, c = pair
+    n = 20000
+    g.run_scalars(n)
+    t2g, nchg = g.temperature(n)
+    rc = c.run(n, ipeel=0)
+    t2c, nchc = c.tfinal(rc["dtauabs"], n)
+    rel_close(t2g, t2c, rtol=1e-9)
+    assert nchc > 0
+    assert abs(nchg - nchc) <= max(2, nchc // 1000)

@@ -1,0 +1,107 @@
+"""The cell walk's geometry (the oracle's next_wall/face_step, which the GPU
+walk restates) against closed-form counts of where a straight line meets the
+grid's walls: spheres r = r_i, cones/midplane theta = theta_j and phi
+half-planes.  A wrong root, a skipped wall or a wrong face step changes the
+count or the exit cell."""
+import numpy as np
+import pytest
+
+from hochunk3d_amd import build_model, config1
+
+
+@pytest.fixture(scope="module")
+def grid():
+    from oracle.pyoracle import Oracle
+    m = build_model(config1())
+    return m, Oracle(m)
+
+
+def _cell(m, x, y, z):
+    r = np.sqrt(x * x + y * y + z * z)
+    th = np.arccos(z / r)
+    ph = np.arctan2(y, x) % (2 * np.pi)
+    ir = int(np.searchsorted(m.rarr, r, side="right"))
+    it = int(np.searchsorted(m.thetarr, th, side="right"))
+    ip = int(np.searchsorted(m.phiarr, ph, side="right"))
+    return ir, it, ip
+
+
+def _expected_crossings(m, p, u):
+    """Wall crossings of p + t u, t > 0, up to the outer sphere or the star."""
+    p, u = np.asarray(p, float), np.asarray(u, float)
+
+    def sphere(r2):
+        b = p @ u
+        c = p @ p - r2
+        d = b * b - c
+        if d <= 0:
+            return []
+        s = np.sqrt(d)
+        return [t for t in (-b - s, -b + s) if t > 1e-12]
+
+    t_out = max(sphere(m.rarr[-1] ** 2))
+    t_star = [t for t in sphere(m.rarr[0] ** 2)]
+    t_end = min([t_out] + t_star)
+    n = 1  # the final crossing (outer sphere or star)
+    for r in m.rarr[1:-1]:
+        n += sum(1 for t in sphere(r * r) if t < t_end - 1e-9)
+    for th in m.thetarr[1:-1]:
+        if abs(th - np.pi / 2) < 1e-12:
+            roots = [-p[2] / u[2]] if u[2] != 0 else []
+        else:
+            T = np.tan(th) ** 2
+            a = u[0] ** 2 + u[1] ** 2 - T * u[2] ** 2
+            b = 2 * (p[0] * u[0] + p[1] * u[1] - T * p[2] * u[2])
+            c = p[0] ** 2 + p[1] ** 2 - T * p[2] ** 2
+            d = b * b - 4 * a * c
+            roots = [] if d < 0 else [(-b - np.sqrt(d)) / (2 * a), (-b + np.sqrt(d)) / (2 * a)]
+            roots = [t for t in roots if np.sign(p[2] + t * u[2]) == np.sign(np.cos(th))]
+        n += sum(1 for t in roots if 1e-12 < t < t_end - 1e-9)
+    for ph in m.phiarr[:-1]:  # phi = 0 and 2 pi are one half-plane
+        c, s = np.cos(ph), np.sin(ph)
+        den = u[1] * c - u[0] * s
+        if den == 0:
+            continue
+        t = -(p[1] * c - p[0] * s) / den
+        q = p + t * u
+        if 1e-12 < t < t_end - 1e-9 and q[0] * c + q[1] * s > 0:
+            n += 1
+    return n, (t_out <= min(t_star) if t_star else True)
+
+
+def test_random_rays_cross_every_wall_once(grid):
+    m, orc = grid
+    rng = np.random.default_rng(5)
+    rmax = m.rarr[-1]
+    checked = 0
+    while checked < 300:
+        # points spread in log radius so the inner, finely zoned shells are hit
+        r = np.exp(rng.uniform(np.log(1.5), np.log(0.9 * rmax)))
+        v = rng.normal(size=3)
+        p = r * v / np.linalg.norm(v)
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        ir, it, ip = _cell(m, *p)
+        want, escapes = _expected_crossings(m, p, u)
+        n, jr, jt, jp = orc.walk_ray(*p, *u, ir, it, ip, False)
+        assert n == want, (p, u, n, want)
+        assert jr == (m.nrg if escapes else 0)
+        checked += 1
+
+
+def test_ray_leaving_the_polar_axis(grid):
+    """A ray starting on the z axis: every phi wall passes through its start
+    point.  It must leave in the phi cell of its direction, never cross a phi
+    wall, and cross each radial shell beyond it and each cone it passes."""
+    m, orc = grid
+    z0 = 3.0 * m.rarr[1]
+    for phid in (0.3, 2.0, 4.0, 5.9):
+        u = np.array([np.cos(phid), np.sin(phid), 0.0])
+        ir = int(np.searchsorted(m.rarr, z0, side="right"))
+        n, jr, jt, jp = orc.walk_ray(0.0, 0.0, z0, *u, ir, 1, 1, False)
+        rho_end = np.sqrt(m.rarr[-1] ** 2 - z0 ** 2)
+        th_end = np.arctan2(rho_end, z0)
+        n_theta = int(np.sum((m.thetarr[1:-1] > 0) & (m.thetarr[1:-1] < th_end)))
+        assert n == (m.nrg - ir) + n_theta, (phid, n)
+        assert jr == m.nrg and jt == 1 + n_theta
+        assert jp == int(np.searchsorted(m.phiarr, phid, side="right"))
