@@ -1,19 +1,32 @@
 #!/usr/bin/env python3
 """Packets/s of the Lucy transport pass on one MI355X.
 
-A step is one Lucy iteration of the hot path over one batch of packets: the
-transport pass (launch, cell walk, scattering, absorption/re-emission and the
-absorbed-energy tallies) followed by the device temperature update.  Inputs,
-tallies and temperatures stay resident in HBM for the whole timed region.
+A step is one Lucy iteration of the hot path: the transport pass over one
+batch of packets (launch, cell walk, scattering, absorption and re-emission,
+the absorbed-energy tallies) followed by the device temperature update.
+Grid, tables, tallies and temperatures stay resident in HBM for the whole
+timed region; the library's calls are synchronous on their own stream, so
+each step ends with the device idle.
 
-The default workload is the 200 x 199 x 200-cell (~8M cells) fractal disk +
-envelope + star model on the FP64 plain path; warm-up steps are Lucy
-iterations that bring the dust temperatures close to equilibrium before the
-timed steps.  The CPU baseline is the FP64 restatement of the same path
-(oracle/, C library elementary functions), one thread per host core on
-disjoint packet ranges, over a bounded sample of the same workload.
+Workload (default): the 200 x 199 x 200-cell (~8M cells) disk + Ulrich
+envelope + star model with a fractal clumpy envelope, 3 x N_cells = 23.9M
+packets per Lucy iteration as the reference sizes its temperature
+iterations (newtts.F90:168-177).  Warm-up Lucy iterations (untimed, fewer
+packets) bring the dust temperatures towards equilibrium first: the event
+count per packet grows from ~2 at the initial 0.1 K to ~100 near
+equilibrium, so timing the first iterations would measure an almost
+transparent model.
 
-Prints one JSON line.
+`value` is the plain FP64 path.  The FP32 path (FP32 packet state and event
+arithmetic; FP64 positions, wall roots and accumulators) is then timed over
+the same steps from the same temperatures, and the SED of its first step is
+compared with the FP64 SED of the same packets.
+
+The CPU baseline is the FP64 restatement of the same path (oracle/, C library
+elementary functions), one thread per host core on disjoint packet ranges,
+over a bounded sample of the same workload and temperatures.
+
+Prints one JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
 
@@ -24,6 +37,8 @@ import sys
 import threading
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -32,7 +47,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(model, seconds: float, threads: int, seg_base: int):
+def cpu_baseline(model, seconds: float, threads: int, seg_base: int, np_total: int):
     """Oracle packets/s over ~`seconds` of wall time on `threads` threads."""
     from oracle.pyoracle import Oracle
 
@@ -40,12 +55,12 @@ def cpu_baseline(model, seconds: float, threads: int, seg_base: int):
     tallies = [orc.new_tallies() for _ in range(threads)]
     done = [0] * threads
     stop = time.perf_counter() + seconds
-    chunk = 50
+    chunk = 20
 
     def work(i):
-        p = seg_base + i * 10_000_000
+        p = seg_base + i * 100_000_000
         while time.perf_counter() < stop:
-            orc.run_into(tallies[i], 1_000_000, 0, chunk, seg0=p)
+            orc.run_into(tallies[i], np_total, 0, chunk, seg0=p)
             p += chunk
             done[i] += chunk
 
@@ -59,14 +74,54 @@ def cpu_baseline(model, seconds: float, threads: int, seg_base: int):
     return sum(done) / wall, sum(done), wall
 
 
+def sed_errors(s64: np.ndarray, s32: np.ndarray) -> dict:
+    """SED differences of two passes over the same packets: per inclination
+    bin (frequency-integrated, all photons, largest aperture) and the
+    luminosity-weighted spectral difference."""
+    a = s64[0, :, :, :, -1, 0]   # I, (nfreq, nmu, nph), largest aperture, all photons
+    b = s32[0, :, :, :, -1, 0]
+    inc64, inc32 = a.sum(axis=(0, 2)), b.sum(axis=(0, 2))
+    per_inc = np.abs(inc32 - inc64) / np.maximum(inc64, 1e-300)
+    spec64, spec32 = a.sum(axis=(1, 2)), b.sum(axis=(1, 2))
+    return {"per_inclination_max": float(per_inc.max()),
+            "luminosity_weighted": float(np.abs(spec32 - spec64).sum() / spec64.sum())}
+
+
+def timed_steps(tr, steps: int, np_: int, seg: int, label: str):
+    """`steps` Lucy iterations; returns wall s, kernel ms, summed counters and
+    the SED of the first step."""
+    kernel_ms = 0.0
+    tot = {}
+    sed0 = None
+    t_start = time.perf_counter()
+    for k in range(steps):
+        if k == 0:
+            s, ms, sed0 = tr.run_sed(np_, seg0=seg + k * np_)
+        else:
+            s, ms = tr.run_scalars(np_, seg0=seg + k * np_)
+        tr.temperature(np_, apply=True)
+        kernel_ms += ms
+        for key, v in s.items():
+            tot[key] = tot.get(key, 0.0) + v
+        log(f"{label} step {k}: kernel {ms:.0f} ms ({np_ / (ms / 1e3):.3e} packets/s), "
+            f"{(s['nscat_ev'] + s['nabs_ev']) / np_:.1f} events and {s['ncross'] / np_:.0f} "
+            f"crossings per packet, errors wall={s['n_err_nowall']:.0f} "
+            f"dust={s['n_err_dust']:.0f} warn={s['n_warn']:.0f}")
+    wall = time.perf_counter() - t_start
+    return wall, kernel_ms, tot, sed0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=5, help="untimed Lucy iterations")
     ap.add_argument("--config", default="config2", choices=["config1", "config2"])
-    ap.add_argument("--np", type=int, default=2_000_000, help="packets per step")
-    ap.add_argument("--warmup-np", type=int, default=1_000_000)
+    ap.add_argument("--np", type=float, default=0,
+                    help="packets per step (default 3 x cells, as the reference)")
+    ap.add_argument("--warmup-np", type=float, default=0,
+                    help="packets per warm-up iteration (default a quarter of --np)")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 path")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
@@ -81,61 +136,75 @@ def main():
 
     from hochunk3d_amd import build_model, config1, config2
     from hochunk3d_amd import roofline as rf
-    from hochunk3d_amd._lib import Transport
+    from hochunk3d_amd._lib import HC_FP32, HC_FP64, Transport
 
     t0 = time.time()
     cfg = config2() if args.config == "config2" else config1()
     model = build_model(cfg)
+    ncells = int(model.ncells)
+    np_ = int(args.np) if args.np > 0 else 3 * ncells
+    wnp = int(args.warmup_np) if args.warmup_np > 0 else max(np_ // 4, 1)
     log(f"model {args.config}: {model.nrg - 1}x{model.ntg - 1}x{model.npg - 1} cells, "
-        f"built in {time.time() - t0:.1f}s")
-    tr = Transport(model)
+        f"built in {time.time() - t0:.1f}s; {np_} packets per step")
+
+    tr = Transport(model, precision=HC_FP64)
     seg = 0
     for w in range(args.warmup):
-        s, ms = tr.run_scalars(args.warmup_np, seg0=seg)
-        seg += args.warmup_np
-        tr.temperature(args.warmup_np, apply=True)
-        log(f"warmup {w}: {args.warmup_np} packets, kernel {ms:.0f} ms, "
-            f"{args.warmup_np / (ms / 1e3):.3e} packets/s")
+        s, ms = tr.run_scalars(wnp, seg0=seg)
+        seg += wnp
+        _, nch = tr.temperature(wnp, apply=True)
+        log(f"warmup {w}: {wnp} packets, kernel {ms:.0f} ms, {wnp / (ms / 1e3):.3e} packets/s, "
+            f"{(s['nscat_ev'] + s['nabs_ev']) / wnp:.1f} events per packet, "
+            f"T changed in {nch} cell-species")
+    t_start = tr.get_temperature()  # where the timed steps start
 
-    kernel_ms = 0.0
-    tot = {}
-    t_start = time.perf_counter()
-    for k in range(args.steps):
-        s, ms = tr.run_scalars(args.np, seg0=seg)
-        seg += args.np
-        tr.temperature(args.np, apply=True)
-        kernel_ms += ms
-        for key, v in s.items():
-            tot[key] = tot.get(key, 0.0) + v
-        log(f"step {k}: kernel {ms:.0f} ms ({args.np / (ms / 1e3):.3e} packets/s) "
-            f"errors wall={s['n_err_nowall']:.0f} dust={s['n_err_dust']:.0f}")
-    wall = time.perf_counter() - t_start
-    npk = args.np * args.steps
-    value = npk / wall
-    flops = rf.pass_flops(tot, npk)
-    achieved = flops / (kernel_ms / 1e3) / 1e12
+    wall64, kms64, tot64, sed64 = timed_steps(tr, args.steps, np_, seg, "fp64")
+    tr.close()
+    npk = np_ * args.steps
+    value = npk / wall64
+
+    fp32 = None
+    if not args.no_fp32:
+        model.tdust[...] = t_start
+        tr32 = Transport(model, precision=HC_FP32)
+        wall32, kms32, tot32, sed32 = timed_steps(tr32, args.steps, np_, seg, "fp32")
+        tr32.close()
+        err = sed_errors(sed64, sed32)
+        fp32 = {"value": round(npk / wall32, 2), "unit": "packets/s",
+                "speedup_vs_fp64": round(wall64 / wall32, 3),
+                "kernel_ms_per_step": round(kms32 / args.steps, 3),
+                "sed_error_vs_fp64_first_step": {k: round(v, 6) for k, v in err.items()},
+                "note": "same packets (L'Ecuyer-Cote segments) and starting temperatures as "
+                        "the FP64 steps; SED error includes the Monte Carlo noise of histories "
+                        "that diverge"}
+        log(f"fp32: {fp32['value']:.3e} packets/s, x{fp32['speedup_vs_fp64']} vs fp64, "
+            f"SED error {err}")
+
+    flops = rf.pass_flops(tot64, npk)
+    achieved = flops / (kms64 / 1e3) / 1e12
     roof = {
         "bound": "mfma", "achieved": round(achieved, 4), "peak": rf.PEAK_FP64_TFLOPS,
         "unit": "TFLOP/s", "frac": round(achieved / rf.PEAK_FP64_TFLOPS, 5),
         "traffic": None,
-        "note": "VALU FP64 (no MFMA instructions; MI355X's FP64 vector and matrix peaks are "
-                "both 78.6 TFLOP/s); algorithmic FLOPs from the work counters of the pass "
-                "times the per-unit figures of hochunk3d_amd/roofline.py",
-        "kernel_ms_per_step": round(kernel_ms / args.steps, 3),
+        "kernel": "hc::transport<double, 4, false, false>",
+        "note": "FP64 vector ALU work (no MFMA instructions; MI355X's FP64 vector and matrix "
+                "peaks are both 78.6 TFLOP/s); algorithmic FLOPs = work counters of the pass x "
+                "per-unit figures of hochunk3d_amd/roofline.py, over the kernel time measured "
+                "with HIP events on the library's stream",
+        "kernel_ms_per_step": round(kms64 / args.steps, 3),
     }
 
     cpu = None
     if args.cpu_seconds > 0:
-        # temperatures of the GPU run so the CPU sample sees the same workload
-        t2, _ = tr.temperature(args.np)
-        model.tdust[...] = t2
+        model.tdust[...] = t_start
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline: {threads} threads x {args.cpu_seconds:.0f}s")
-        rate, n, w = cpu_baseline(model, args.cpu_seconds, threads, seg_base=seg + 10**9)
+        rate, n, w = cpu_baseline(model, args.cpu_seconds, threads, seg_base=seg + 10**10,
+                                  np_total=np_)
         cpu = {"value": round(rate, 2), "unit": "packets/s", "cores": threads, "kind": "port",
-               "sample": f"{n} packets of the same model and temperatures in {w:.1f}s wall, "
-                         f"FP64 CPU restatement (oracle/, -O2), one thread per core"}
-    tr.close()
+               "sample": f"{n} packets of the same model and starting temperatures in {w:.1f}s "
+                         f"wall, FP64 CPU restatement (oracle/, -O2, C library elementary "
+                         f"functions), one thread per core on disjoint packet segments"}
 
     line = {
         "metric": "photon packets/s on 1 MI355X, FP32-batched vs FP64 plain, at <=1% SED error",
@@ -144,7 +213,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "ms_per_step": round(wall64 / args.steps * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -153,14 +222,16 @@ def main():
         "config": {
             "workload": (f"{args.config}: {model.nrg - 1}x{model.ntg - 1}x{model.npg - 1}-cell "
                          f"disk+envelope+star" + (" with fractal clumps" if cfg.fractal else "")
-                         + f", Lucy pass of {args.np} packets + temperature update per step, "
-                           f"FP64 plain path"),
-            "packets_per_step": args.np,
-            "cells": int(model.ncells),
+                         + f"; per step one Lucy iteration: transport pass of {np_} packets + "
+                           f"temperature update, after {args.warmup} warm-up iterations of "
+                           f"{wnp} packets; FP64 plain path"),
+            "packets_per_step": np_,
+            "cells": ncells,
             "path": "fp64-plain",
         },
         "roofline": roof,
         "cpu_baseline": cpu,
+        "fp32_path": fp32,
     }
     print(json.dumps(line), flush=True)
 

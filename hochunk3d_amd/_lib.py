@@ -24,7 +24,8 @@ _ip = C.POINTER(C.c_int32)
 
 # Every symbol include/hc.h declares.
 EXPORTS = (
-    "hc_create", "hc_set_grid", "hc_set_temperature", "hc_set_dust", "hc_set_sources",
+    "hc_create", "hc_set_grid", "hc_set_temperature", "hc_get_temperature", "hc_set_dust",
+    "hc_set_sources",
     "hc_run_iteration", "hc_temperature", "hc_set_option", "hc_destroy", "hc_last_error",
     "hc_abi_version",
 )
@@ -60,7 +61,8 @@ class HcTallies(C.Structure):
         "hit", "htot")] + [
         ("n_err_nowall", C.c_int64), ("n_err_dust", C.c_int64), ("n_warn", C.c_int64),
         ("kernel_ms", C.c_double),
-    ] + [(n, C.c_double) for n in ("ncross", "nscat_ev", "nabs_ev", "npeelray", "npeelcross")]
+    ] + [(n, C.c_double) for n in ("ncross", "nscat_ev", "nabs_ev", "npeelray", "npeelcross",
+                                   "max_packet_work")]
 
 
 TRACE_DTYPE = np.dtype([
@@ -73,7 +75,7 @@ TRACE_DTYPE = np.dtype([
 
 SCALARS = ("tot", "flux", "sflux", "nscat", "scount", "abflux", "aflux", "sumsub", "sumall",
            "hit", "htot", "n_err_nowall", "n_err_dust", "n_warn", "ncross", "nscat_ev",
-           "nabs_ev", "npeelray", "npeelcross")
+           "nabs_ev", "npeelray", "npeelcross", "max_packet_work")
 
 
 def library_path() -> str:
@@ -95,6 +97,7 @@ def load() -> C.CDLL:
     lib.hc_create.argtypes = [C.POINTER(HcDims), C.c_int, C.POINTER(C.c_void_p)]
     lib.hc_set_grid.argtypes = [C.c_void_p] + [_dp] * 8 + [C.c_double]
     lib.hc_set_temperature.argtypes = [C.c_void_p, _dp]
+    lib.hc_get_temperature.argtypes = [C.c_void_p, _dp]
     lib.hc_set_dust.argtypes = [C.c_void_p, _ip] + [_dp] * 10
     lib.hc_set_sources.argtypes = [C.c_void_p, C.POINTER(HcSources)]
     lib.hc_run_iteration.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
@@ -199,8 +202,7 @@ class Transport:
         self._check(self.lib.hc_set_sources(self.h, C.byref(s)), "hc_set_sources")
 
     def set_option(self, name: str, value: int):
-        """Launch-shape knob (regen_lanes, blocks_per_cu); results do not
-        depend on it."""
+        """Launch-shape knob (regen_lanes, blocks_per_cu, max_work)."""
         self._check(self.lib.hc_set_option(self.h, name.encode(), int(value)), "hc_set_option")
 
     def _check(self, rc, what):
@@ -211,6 +213,12 @@ class Transport:
     def set_temperature(self, tdust: np.ndarray):
         self._check(self.lib.hc_set_temperature(self.h, _ptr(_f64(tdust))), "hc_set_temperature")
 
+    def get_temperature(self) -> np.ndarray:
+        m = self.model
+        t = np.zeros((m.nrg, m.ntg, m.npg, 8), order="F")
+        self._check(self.lib.hc_get_temperature(self.h, _ptr(t)), "hc_get_temperature")
+        return t
+
     def run_scalars(self, np_: int, ipeel: int = 0, seed1: int = 12345678,
                     seed2: int = 87654321, gen: int = 1, seg0: int = 0):
         """A pass whose tallies stay on the device (the next Lucy step reads
@@ -220,6 +228,20 @@ class Transport:
                                        int(gen), int(seg0), C.byref(t), None)
         self._check(rc, "hc_run_iteration")
         return {k: getattr(t, k) for k in SCALARS}, t.kernel_ms
+
+    def run_sed(self, np_: int, seed1: int = 12345678, seed2: int = 87654321, gen: int = 1,
+                seg0: int = 0):
+        """A Lucy pass that downloads only the SED tallies (per-cell tallies
+        stay on the device); returns (scalars, kernel ms, sed[9, nfreq, nmu,
+        nph, nap, 8])."""
+        cfg = self.model.cfg
+        sed = np.zeros((cfg.nfreq, cfg.nmu, cfg.nph, cfg.nap, NO, 9), order="F")
+        t = HcTallies()
+        t.sed = _ptr(sed)
+        rc = self.lib.hc_run_iteration(self.h, int(np_), 0, int(seed1), int(seed2), int(gen),
+                                       int(seg0), C.byref(t), None)
+        self._check(rc, "hc_run_iteration")
+        return {k: getattr(t, k) for k in SCALARS}, t.kernel_ms, np.moveaxis(sed, -1, 0)
 
     def run(self, np_: int, ipeel: int = 0, seed1: int = 12345678, seed2: int = 87654321,
             gen: int = 1, seg0: int = 0, trace: bool = False, images: bool = True) -> PassResult:

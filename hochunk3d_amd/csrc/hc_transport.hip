@@ -103,7 +103,7 @@ struct Pkt {
 
 struct Acc {  // per-lane counters, flushed once per lane
   uint32_t tot, flux, sflux, nscat, scount, abflux, aflux, hit, htot, err_wall, err_dust;
-  uint32_t ncross, nscat_ev, nabs_ev, npeelray, npeelcross, warn;
+  uint32_t ncross, nscat_ev, nabs_ev, npeelray, npeelcross, warn, maxwork;
   double sumsub, sumall;
 };
 
@@ -284,7 +284,8 @@ __device__ __forceinline__ int walk_step(Pkt<R, NA> &k, double tau, double &tsum
   const double opac = cell_opac(k, dm);
   const double dtau = opac * dt;
   if (tsum + dtau >= tau) {
-    const double ds = (tau - tsum) / opac;
+    double ds = (tau - tsum) / opac;
+    if (ds > dt) ds = dt;  // rounding must not carry the packet past the wall
     lucy_add(la, k, dm, cell, ds);
     k.x = k.x + ds * k.ux;
     k.y = k.y + ds * k.uy;
@@ -715,6 +716,20 @@ __device__ __forceinline__ void bin_init(R nub, int i_orig_init) {
   atomic_add(&P.si_init[(size_t)(inub - 1)], 1.0);
 }
 
+// theta and phi cells of the packet's position (FP64, as the oracle's
+// star re-emission: cos(theta) = z / r clamped, phi = atan2(y, x) in [0, 2pi)).
+template <class R, int NA>
+__device__ __forceinline__ void locate_cell_angles(Pkt<R, NA> &k) {
+  const Params &P = c_P;
+  double cb = k.z / sqrt(k.rsq);
+  if (cb > 1.0) cb = 1.0;
+  if (cb < -1.0) cb = -1.0;
+  double lp = hcm_atan2(k.y, k.x);
+  if (lp < 0.0) lp = lp + kR2P;
+  k.it = locate(P.thetarr, P.ntg, hcm_acos(cb));
+  k.ip = locate(P.phiarr, P.npg, lp);
+}
+
 // Star / hot-spot launch (initpspot.f90:3-175 or initp_ps.f90:3-79, then
 // newdisktherm.F90:256-377).
 template <class R, int NA>
@@ -784,8 +799,14 @@ __device__ __forceinline__ void launch_star(Pkt<R, NA> &k, Mrg &rng, bool hotspo
   k.iflag = 0;
   k.ir = 1;
   k.on_wall = true;
-  k.it = locate(P.thetarr, P.ntg, (double)m_acos(cosb));
-  k.ip = locate(P.phiarr, P.npg, (double)lp);
+  if constexpr (sizeof(R) == sizeof(double)) {
+    k.it = locate(P.thetarr, P.ntg, (double)m_acos(cosb));
+    k.ip = locate(P.phiarr, P.npg, (double)lp);
+  } else {
+    // the FP64 position carries the FP32 angles' rounding; its cell is
+    // located from the position itself, so the walk starts consistent
+    locate_cell_angles(k);
+  }
   nrm[0] = cosb;
   nrm[1] = sinb;
   nrm[2] = coslp;
@@ -811,6 +832,11 @@ __device__ __forceinline__ void star_reemit(Pkt<R, NA> &k, Mrg &rng, Acc &acc, R
   set_walk_dir(k);
   k.ir = 1;
   k.on_wall = true;
+  // the impact point can lie a rounding error outside the theta/phi cell the
+  // walk arrived in (where a cone or phi wall meets the stellar surface); the
+  // walk cannot find a wall it is already past, so the cell is located anew
+  // from the point itself
+  locate_cell_angles(k);
   nrm[0] = cosb;
   nrm[1] = sinb;
   nrm[2] = m_cos(lp);
@@ -1112,6 +1138,7 @@ __global__ __launch_bounds__(256, HC_WAVES_PER_EU) void transport() {
         }
       }
       if (status >= 0) {  // the packet ends
+        acc.maxwork = max(acc.maxwork, L.work);
         lucy_flush(L.la);
         if (status == 0) bin_exit(k, L.xpold, L.ypold, L.zpold, L.iscat, acc);
         if constexpr (TRACE) {
@@ -1199,6 +1226,9 @@ __global__ __launch_bounds__(256, HC_WAVES_PER_EU) void transport() {
     const double w = wave_sum(v[i]);
     if ((threadIdx.x & 63) == 0 && w != 0.0) atomic_add(&P.scal[i], w);
   }
+  unsigned mw = acc.maxwork;
+  for (int off = 32; off > 0; off >>= 1) mw = max(mw, (unsigned)__shfl_xor((int)mw, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(P.next_packet + 1, (unsigned long long)mw);
 }
 
 // ---------------------------------------------------------------------------
@@ -1417,7 +1447,7 @@ int hc_create(const hc_dims *dims, int device, hc_ctx **out) {
   int rc = upload(ctx, &ctx->pow1, p1.data(), 64);
   if (!rc) rc = upload(ctx, &ctx->pow2, p2.data(), 64);
   if (!rc) rc = dalloc(ctx, &ctx->scal, (size_t)S_COUNT);
-  if (!rc) rc = dalloc(ctx, &ctx->counter, 1);
+  if (!rc) rc = dalloc(ctx, &ctx->counter, 2);  // next packet, longest packet
   double z[146];
   z[0] = 0.0;
   for (int i = 1; i <= 145; ++i) z[i] = 0.0;
@@ -1483,6 +1513,21 @@ int hc_set_temperature(hc_ctx *ctx, const double *tdust) {
   to_device_layout(ctx, tdust, tmp);
   HCK(hipMemcpy(ctx->tdust, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice));
   return refresh_mask(ctx);
+}
+
+int hc_get_temperature(hc_ctx *ctx, double *tdust) {
+  if (!ctx || !tdust) return HC_E_ARG;
+  if (!ctx->have_grid) return fail(ctx, HC_E_STATE, "hc_get_temperature before hc_set_grid");
+  HCK(hipSetDevice(ctx->device));
+  const size_t nc = ctx->ncellw * (size_t)ctx->nact;
+  std::vector<double> tmp(nc);
+  HCK(hipMemcpy(tmp.data(), ctx->tdust, nc * sizeof(double), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < ctx->ncellw * (size_t)ctx->dims.ndg; ++i) tdust[i] = 0.1 / 11605.0;
+  for (int k = 0; k < ctx->nact; ++k) {
+    double *dst = tdust + ctx->ncellw * (size_t)(ctx->act[k] - 1);
+    for (size_t c = 0; c < ctx->ncellw; ++c) dst[c] = tmp[c * (size_t)ctx->nact + k];
+  }
+  return HC_OK;
 }
 
 int hc_set_dust(hc_ctx *ctx, const int32_t *nlambda, const double *nudust, const double *kappad,
@@ -1644,7 +1689,7 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
   HCK(hipMemsetAsync(ctx->peel, 0, npeelsed * sizeof(double), s));
   HCK(hipMemsetAsync(ctx->si_init, 0, (size_t)d.nfreq * 5 * sizeof(double), s));
   HCK(hipMemsetAsync(ctx->scal, 0, S_COUNT * sizeof(double), s));
-  HCK(hipMemsetAsync(ctx->counter, 0, sizeof(unsigned long long), s));
+  HCK(hipMemsetAsync(ctx->counter, 0, 2 * sizeof(unsigned long long), s));
   if (trace) HCK(hipMemsetAsync(ctx->trace, 0, (size_t)np * sizeof(hc_trace), s));
   const bool cube = d.imcube && ipeel && ctx->img_i;
   if (cube) {
@@ -1822,6 +1867,9 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
   out->nabs_ev = scal[S_NABS_EV];
   out->npeelray = scal[S_NPEELRAY];
   out->npeelcross = scal[S_NPEELCROSS];
+  unsigned long long counters[2] = {0, 0};
+  HCK(hipMemcpy(counters, ctx->counter, sizeof counters, hipMemcpyDeviceToHost));
+  out->max_packet_work = (double)counters[1];
   std::vector<double> tmp;
   auto per_species = [&](double *dev, double *host) -> int {
     if (!host) return HC_OK;

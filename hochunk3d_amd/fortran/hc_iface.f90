@@ -31,6 +31,7 @@ module hc_iface
     integer(c_int64_t) :: n_err_nowall, n_err_dust, n_warn
     real(c_double) :: kernel_ms
     real(c_double) :: ncross, nscat_ev, nabs_ev, npeelray, npeelcross
+    real(c_double) :: max_packet_work
   end type hc_tallies
 
   interface
@@ -55,6 +56,12 @@ module hc_iface
       type(c_ptr), value :: ctx
       real(c_double), intent(in) :: tdust(*)
     end function hc_set_temperature
+
+    integer(c_int) function hc_get_temperature(ctx, tdust) bind(C, name='hc_get_temperature')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: tdust(*)
+    end function hc_get_temperature
 
     integer(c_int) function hc_set_dust(ctx, nlambda, nudust, kappad, adust, gdust, pldust, &
                                         kappav, lTint, nuint, kapBint, kappap) &

@@ -88,6 +88,9 @@ typedef struct {
   /* work counters of the pass: wall crossings of transport rays, scattering
    * and absorption events, peel-off rays and their wall crossings */
   double ncross, nscat_ev, nabs_ev, npeelray, npeelcross;
+  /* the longest packet of the pass: its events + transport-ray crossings
+   * (the pass cannot end before that packet's serial walk does) */
+  double max_packet_work;
 } hc_tallies;
 
 /* Per-packet record of the deterministic-trace mode. */
@@ -111,6 +114,12 @@ int hc_set_grid(hc_ctx *ctx, const double *r2arr, const double *tan2arr, const d
 
 /* New dust temperatures between Lucy iterations (initarr.f90:78-84). */
 int hc_set_temperature(hc_ctx *ctx, const double *tdust);
+
+/* The dust temperatures the device holds (nrg,ntg,npg,ndg) in eV, e.g. after
+ * Lucy iterations applied on the device; species without dust read 0.1 K
+ * (initarr.f90:53).  Replaces reading tts_mod's tdust after the loop
+ * (newtts.F90:312-330). */
+int hc_get_temperature(hc_ctx *ctx, double *tdust);
 
 /* Dust tables of dustinit/setemiss (dust.f90:314-474): nlambda[ndg];
  * nudust..pldust (nlammax,ndg); kappav[ndg]; lTint[nT]; nuint[nnu];

@@ -1033,6 +1033,7 @@ static int tauint(ctx_t *c, pkt_t *P, double tau) {
     dtau = opac * dt;
     if (tsum + dtau >= tau) {
       double ds = (tau - tsum) / opac;
+      if (ds > dt) ds = dt; /* rounding must not carry the packet past the wall */
       if (t->dtauabs)
         for (id = 0; id < HCO_NDG; id++) {
           size_t k = cell + c->ncell * id;
@@ -1374,6 +1375,8 @@ static void launch_star(ctx_t *c, pkt_t *P, int hotspot) {
  * the star (output.F90:378 "flux which gets absorbed (and reemitted) by
  * star").  Reconstruction: re-launch from the impact point with the stellar
  * launch's direction and spectrum sampling, counted in aflux. */
+static inline const hco_grid *g_of(const ctx_t *c) { return c->g; }
+
 static void star_reemit(ctx_t *c, pkt_t *P) {
   double cosb, sinb, lp;
   c->t->aflux += 1.0;
@@ -1395,6 +1398,12 @@ static void star_reemit(ctx_t *c, pkt_t *P) {
   P->uz = P->cost;
   P->ir = 1;
   P->on_wall = 1;
+  /* the impact point can lie a rounding error outside the theta/phi cell the
+   * walk arrived in (where a cone or phi wall meets the stellar surface); the
+   * walk cannot find a wall it is already past, so the cell is located anew
+   * from the point itself */
+  P->it = hco_locate(g_of(c)->thetarr, g_of(c)->ntg, HCO_ACOS(cosb));
+  P->ip = hco_locate(g_of(c)->phiarr, g_of(c)->npg, lp);
   if (c->ipeel) peel_direct(c, P, cosb, sinb, HCO_COS(lp), HCO_SIN(lp));
   if (c->tr) c->tr->nstar++;
 }
