@@ -124,7 +124,21 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 path")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", type=float, default=0,
+                    help="only copy this many bytes device-to-device once (a plain copy kernel "
+                         "of known size that scales the HBM byte counters) and exit")
     args = ap.parse_args()
+
+    if args.copy_calibration > 0:
+        import torch
+        n = int(args.copy_calibration)
+        a = torch.ones(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copied {n} bytes")
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
