@@ -70,7 +70,13 @@ struct Params {
   unsigned long long *next_packet;
   hc_trace *trace;
   // launch shape
-  int regen_lanes;  // waiting lanes (1..64) that end a walk phase
+  int regen_lanes;  // waiting lanes (1..64) of one event kind that end a walk phase
+  int peel_q;       // peel records per wavefront queue
+  int walls_lds;    // bytes of LDS holding the wall arrays
+  int opac_base;    // LDS byte offset of the opacity tables
+  int guide_off;    // byte offset of the opacset guides within them
+  int opac_off[kMaxSpecies];  // table of active species s: offset in doubles
+  int peel_base;    // LDS byte offset of the peel-off queues
   uint32_t max_work;  // events + crossings after which a packet is dropped
 };
 
@@ -143,9 +149,26 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) {
   return v < lo ? lo : (v > hi ? hi : v);
 }
 
+// Dynamic LDS of the transport kernel (layout set on the host, Params):
+//   [0, walls_lds)            wall arrays r2[nrg] t2[ntg] a[npg] b[npg]
+//   [opac_base, +opac_bytes)  per active species s: nudust, kappad, adust,
+//                             gdust, pldust (nlambda doubles each) at
+//                             opac_off[s] doubles, then the int32 opacset
+//                             guides [nact][kGuide] at guide_off bytes
+//   [peel_base, ...)          peel-off queue counters and queues
+extern __shared__ __align__(16) unsigned char hc_lds[];
+
+// The wall arrays of the grid as the walk reads them: staged in LDS by the
+// transport kernel (r^2 of the radial walls, tan^2 of the theta walls, the
+// phi planes' coefficients), so the roots of a crossing wait on no global
+// memory.
+struct WallView {
+  const double *r2, *t2, *a, *b;
+};
+
 // Masked opacity of a cell: sum over active species in increasing id.
 template <class R, int NA>
-__device__ __forceinline__ double cell_opac(const Pkt<R, NA> &k, const double *dm) {
+__device__ __forceinline__ double cell_opac(const Pkt<R, NA> &k, const double (&dm)[NA]) {
   double opac = 0.0;
 #pragma unroll
   for (int s = 0; s < NA; ++s)
@@ -158,16 +181,16 @@ __device__ __forceinline__ double cell_opac(const Pkt<R, NA> &k, const double *d
 // without moving; a straight ray leaving the axis keeps the azimuth of its
 // direction, so there the phi walls are skipped and the caller has put the
 // packet in the phi cell of its direction (axis_phi_cell).
-__device__ __forceinline__ bool next_wall(double ux, double uy, double uz, double x, double y,
-                                          double z, int ir, int it, int ip, bool on_wall,
-                                          double &dt, int &face) {
+__device__ __forceinline__ bool next_wall(const WallView &W, double ux, double uy, double uz,
+                                          double x, double y, double z, int ir, int it, int ip,
+                                          bool on_wall, double &dt, int &face) {
   const Params &P = c_P;
   Walls w;
   w.reset();
-  radface(w, ux, uy, uz, x, y, z, P.r2arr[ir - 1], P.r2arr[ir]);
-  if (P.ntg > 2) thetaface(w, ux, uy, uz, x, y, z, P.tan2arr[it - 1], P.tan2arr[it]);
+  radface(w, ux, uy, uz, x, y, z, W.r2[ir - 1], W.r2[ir]);
+  if (P.ntg > 2) thetaface(w, ux, uy, uz, x, y, z, W.t2[it - 1], W.t2[it]);
   if (P.npg > 2 && P.ntg > 2 && !(x == 0.0 && y == 0.0))
-    phiface(w, P.aarr[ip - 1], P.barr[ip - 1], P.aarr[ip], P.barr[ip], ux, uy, x, y);
+    phiface(w, W.a[ip - 1], W.b[ip - 1], W.a[ip], W.b[ip], ux, uy, x, y);
   return w.next(on_wall, dt, face);
 }
 
@@ -235,8 +258,8 @@ __device__ __forceinline__ void lucy_open(LucyAcc<R, NA> &la, size_t cell) {
 // Add one chord of cell `cell` to the per-cell accumulator: dtauabs gains
 // kapd (1-albedo) rho ds per active species, jmean gains ds R*.
 template <class R, int NA>
-__device__ __forceinline__ void lucy_add(LucyAcc<R, NA> &la, const Pkt<R, NA> &k, const double *dm,
-                                         size_t cell, double ds) {
+__device__ __forceinline__ void lucy_add(LucyAcc<R, NA> &la, const Pkt<R, NA> &k,
+                                         const double (&dm)[NA], size_t cell, double ds) {
   if (!c_P.dtauabs && !c_P.jmean) return;
   lucy_open(la, cell);
 #pragma unroll
@@ -248,13 +271,13 @@ __device__ __forceinline__ void lucy_add(LucyAcc<R, NA> &la, const Pkt<R, NA> &k
 // The cell walk to optical depth tau (tauint reconstruction), split into
 // its entry (the packet on a radial wall steps into the cell its direction
 // points to) and single crossings, so the event loop can interleave lanes.
-enum { ST_WALK = 4, ST_FETCH = 5, ST_DONE = 6, ST_ABORT = 7 };
+enum { ST_WALK = 4, ST_DONE = 6, ST_ABORT = 7, ST_SCAT = 8, ST_ABS = 9, ST_END = 10 };
 
 template <class R, int NA>
-__device__ __forceinline__ int walk_begin(Pkt<R, NA> &k) {
+__device__ __forceinline__ int walk_begin(const WallView &W, Pkt<R, NA> &k) {
   const Params &P = c_P;
   if (k.on_wall) {
-    const bool inner = fabs(k.rsq - P.r2arr[k.ir - 1]) < fabs(k.rsq - P.r2arr[k.ir]);
+    const bool inner = fabs(k.rsq - W.r2[k.ir - 1]) < fabs(k.rsq - W.r2[k.ir]);
     const bool outwards = k.ux * k.x + k.uy * k.y + k.uz * k.z > 0.0;
     if (inner && !outwards) {
       k.ir -= 1;
@@ -268,16 +291,32 @@ __device__ __forceinline__ int walk_begin(Pkt<R, NA> &k) {
 
 // One step of the walk: to the next wall, or to the interaction point when
 // the optical depth tau is reached inside the current cell.
-template <class R, int NA, bool TRACE>
-__device__ __forceinline__ int walk_step(Pkt<R, NA> &k, double tau, double &tsum,
-                                         LucyAcc<R, NA> &la, Acc &acc, Trace &tr) {
+// The masked densities of the packet's cell, into registers.
+template <class R, int NA>
+__device__ __forceinline__ void load_dens(const Pkt<R, NA> &k, double (&dm)[NA]) {
   const Params &P = c_P;
-  if (k.x == 0.0 && k.y == 0.0) k.ip = axis_phi_cell(k.ux, k.uy, k.ip);
+  if (k.ir < 1 || k.ir >= P.nrg) return;  // outside the grid: nothing to read
+  const double *d = P.densm + cell_of(k.ir, k.it, k.ip) * (size_t)P.nact;
+#pragma unroll
+  for (int s = 0; s < NA; ++s) dm[s] = (s < P.nact) ? d[s] : 0.0;
+}
+
+// dm holds the masked densities of the packet's cell on entry and those of
+// the next cell on return: the load for the next crossing is issued as soon
+// as that cell is known, and its latency overlaps the rest of this step.
+template <class R, int NA, bool TRACE>
+__device__ __forceinline__ int walk_step(const WallView &W, Pkt<R, NA> &k, double tau,
+                                         double &tsum, double (&dm)[NA], LucyAcc<R, NA> &la,
+                                         Acc &acc, Trace &tr) {
+  const Params &P = c_P;
+  if (k.x == 0.0 && k.y == 0.0) {
+    k.ip = axis_phi_cell(k.ux, k.uy, k.ip);
+    load_dens(k, dm);
+  }
   const size_t cell = cell_of(k.ir, k.it, k.ip);
-  const double *dm = P.densm + cell * (size_t)P.nact;
   double dt;
   int face;
-  if (!next_wall(k.ux, k.uy, k.uz, k.x, k.y, k.z, k.ir, k.it, k.ip, k.on_wall, dt, face)) {
+  if (!next_wall(W, k.ux, k.uy, k.uz, k.x, k.y, k.z, k.ir, k.it, k.ip, k.on_wall, dt, face)) {
     acc.err_wall++;
     return ST_ERROR;
   }
@@ -298,6 +337,7 @@ __device__ __forceinline__ int walk_step(Pkt<R, NA> &k, double tau, double &tsum
   lucy_add(la, k, dm, cell, dt);
   tsum = tsum + dtau;
   face_step(face, k.ir, k.it, k.ip);
+  load_dens(k, dm);
   k.x = k.x + dt * k.ux;
   k.y = k.y + dt * k.uy;
   k.z = k.z + dt * k.uz;
@@ -314,15 +354,16 @@ __device__ __forceinline__ int walk_step(Pkt<R, NA> &k, double tau, double &tsum
   return ST_WALK;
 }
 
-// opacset for one species (dust.f90:13-76).  The bracket of nu in the
-// decreasing frequency table is found from a guide table of log-frequency
-// bins and a short forward scan; for a strictly decreasing table that is the
-// same unique bracket the reference's bisection finds.
-__device__ __forceinline__ Opac1 opacset_species(int id, double nu) {
+// opacset for one active species (dust.f90:13-76), slot s of the active
+// list, from its tables in LDS.  The bracket of nu in the decreasing
+// frequency table is found from a guide table of log-frequency bins and a
+// short forward scan; for a strictly decreasing table that is the same unique
+// bracket the reference's bisection finds.
+__device__ __forceinline__ Opac1 opacset_species(int s, double nu) {
   const Params &P = c_P;
-  const size_t o = (size_t)P.nlammax * id;
-  const double *nud = P.nudust + o;
+  const int id = P.act[s] - 1;
   const int nl = P.nlambda[id];
+  const double *nud = reinterpret_cast<const double *>(hc_lds + P.opac_base) + P.opac_off[s];
   Opac1 r;
   int i0, i1;
   double f;
@@ -336,12 +377,12 @@ __device__ __forceinline__ Opac1 opacset_species(int id, double nu) {
     // the bin only seeds the scan, so a single-precision log suffices
     int b = (int)((P.glog0[id] - (double)__logf((float)nu)) * P.ginv[id]);
     b = clampi(b - 1, 0, kGuide - 1);
-    i1 = P.guide[id * kGuide + b];
+    i1 = reinterpret_cast<const int32_t *>(hc_lds + P.opac_base + P.guide_off)[s * kGuide + b];
     while (!(nu > nud[i1 - 1])) ++i1;
     i0 = i1 - 1;
     f = (nu - nud[i1 - 1]) / (nud[i0 - 1] - nud[i1 - 1]);
   }
-  const double *kd = P.kappad + o, *ad = P.adust + o, *gd = P.gdust + o, *pd = P.pldust + o;
+  const double *kd = nud + nl, *ad = nud + 2 * nl, *gd = nud + 3 * nl, *pd = nud + 4 * nl;
   if (i0 == i1) {
     r.g = gd[i0 - 1];
     r.pl = pd[i0 - 1];
@@ -357,9 +398,10 @@ __device__ __forceinline__ Opac1 opacset_species(int id, double nu) {
   return r;
 }
 
+// Henyey-Greenstein terms of active species s at frequency nu.
 template <class R>
-__device__ __forceinline__ HG<R> hg_of(int id, R nu) {
-  const Opac1 op = opacset_species(id, (double)nu);
+__device__ __forceinline__ HG<R> hg_of(int s, R nu) {
+  const Opac1 op = opacset_species(s, (double)nu);
   return HG<R>(R(op.g), R(op.pl));
 }
 
@@ -381,12 +423,12 @@ struct PeelRec {
   R nub;
   R cosb, sinb, coslp, sinlp;  // direct stellar emission: surface normal
   int ir, it, ip;
-  int flags;                   // on_wall | iflag<<1 | i_orig<<2 | idust2<<5 | direct<<9
+  int flags;                   // on_wall | iflag<<1 | i_orig<<2 | slot<<5 | direct<<9
   long long pid;
 };
 
 template <class R, int NA>
-__device__ __forceinline__ void make_rec(PeelRec<R, NA> &r, const Pkt<R, NA> &k, int idust2,
+__device__ __forceinline__ void make_rec(PeelRec<R, NA> &r, const Pkt<R, NA> &k, int slot,
                                          bool direct, long long pid) {
   r.x = k.x;
   r.y = k.y;
@@ -400,7 +442,7 @@ __device__ __forceinline__ void make_rec(PeelRec<R, NA> &r, const Pkt<R, NA> &k,
   r.ir = k.ir;
   r.it = k.it;
   r.ip = k.ip;
-  r.flags = (k.on_wall ? 1 : 0) | (k.iflag << 1) | (k.i_orig << 2) | (idust2 << 5) |
+  r.flags = (k.on_wall ? 1 : 0) | (k.iflag << 1) | (k.i_orig << 2) | (slot << 5) |
             ((direct ? 1 : 0) << 9);
   r.pid = pid;
 }
@@ -417,20 +459,20 @@ __device__ __forceinline__ double rec_opac(const PeelRec<R, NA> &k, const double
 // peeloff_3d.f90:5-288 for observer `peelid`.  sip is the peel weight; the
 // remaining Stokes components come from the packet.
 template <class R, int NA>
-__device__ __attribute__((noinline)) PeelOut peeloff(const PeelRec<R, NA> &k, double sip,
-                                                     int peelid) {
+__device__ __attribute__((noinline)) PeelOut peeloff(const WallView &W, const PeelRec<R, NA> &k,
+                                                     double sip, int peelid) {
   const Params &P = c_P;
   PeelOut out{0.0, 0, 0};
   Stokes4<R> s{R(sip), k.s.Q, k.s.U, k.s.V};
   double x1 = k.x, y1 = k.y, z1 = k.z;
   int ir = k.ir, it = k.it, ip = k.ip;
   bool on_wall = (k.flags & 1) != 0;
-  const int iflag = (k.flags >> 1) & 1, i_orig = (k.flags >> 2) & 7, idust2 = (k.flags >> 5) & 15;
+  const int iflag = (k.flags >> 1) & 1, i_orig = (k.flags >> 2) & 7, slot = (k.flags >> 5) & 15;
   const double cospe = P.cospe[peelid], sinpe = P.sinpe[peelid];
   const double coste = P.coste[peelid], sinte = P.sinte[peelid];
   const double ux1 = sinte * cospe, uy1 = sinte * sinpe, uz1 = coste;
   if (on_wall) {
-    const bool inner = fabs(k.rsq - P.r2arr[ir - 1]) < fabs(k.rsq - P.r2arr[ir]);
+    const bool inner = fabs(k.rsq - W.r2[ir - 1]) < fabs(k.rsq - W.r2[ir]);
     const bool outwards = ux1 * x1 + uy1 * y1 + uz1 * z1 > 0.0;
     if (inner && !outwards) {
       if (ir == 1) return out;
@@ -447,7 +489,7 @@ __device__ __attribute__((noinline)) PeelOut peeloff(const PeelRec<R, NA> &k, do
     int face;
     if (x1 == 0.0 && y1 == 0.0) ip = axis_phi_cell(ux1, uy1, ip);
     if (out.ncross > max_cross ||
-        !next_wall(ux1, uy1, uz1, x1, y1, z1, ir, it, ip, on_wall, dt, face)) {
+        !next_wall(W, ux1, uy1, uz1, x1, y1, z1, ir, it, ip, on_wall, dt, face)) {
       out.err = 1;
       return out;
     }
@@ -474,7 +516,7 @@ __device__ __attribute__((noinline)) PeelOut peeloff(const PeelRec<R, NA> &k, do
   s.U = s.U * factor;
   s.V = s.V * factor;
   if (iflag == 1)
-    peel_weight<R>(k.d, R(coste), R(sinte), R(cospe), R(sinpe), hg_of<R>(idust2, k.nub), s);
+    peel_weight<R>(k.d, R(coste), R(sinte), R(cospe), R(sinpe), hg_of<R>(slot, k.nub), s);
   const double yimage = z1 * sinte - y1 * coste * sinpe - x1 * coste * cospe;
   const double ximage = y1 * cospe - x1 * sinpe;
   const int ix = (int)((ximage + P.rmaxi) / P.fractxh) + 1;
@@ -627,7 +669,7 @@ __device__ __forceinline__ void set_opacity(Pkt<R, NA> &k, bool launch_order) {
   for (int s = 0; s < NA; ++s) {
     if (s < P.nact) {
       const int id = P.act[s] - 1;
-      const Opac1 op = opacset_species(id, (double)k.nub);
+      const Opac1 op = opacset_species(s, (double)k.nub);
       k.kapd[s] = launch_order ? op.kappa * P.rmincgs * P.kappav[id]
                                : op.kappa * P.rstar * kRsol * P.kappav[id];
       k.alb[s] = R(op.albedo);
@@ -794,7 +836,16 @@ __device__ __forceinline__ void launch_star(Pkt<R, NA> &k, Mrg &rng, bool hotspo
   const double rp = (double)sinb * 1.0;
   k.x = rp * (double)coslp;
   k.y = rp * (double)sinlp;
-  k.rsq = k.z * k.z + rp * rp;
+  if constexpr (sizeof(R) == sizeof(float)) {
+    // FP32 angles put the point ~1e-7 off the stellar surface: back onto it
+    const double inv = 1.0 / sqrt(k.x * k.x + k.y * k.y + k.z * k.z);
+    k.x *= inv;
+    k.y *= inv;
+    k.z *= inv;
+    k.rsq = k.x * k.x + k.y * k.y + k.z * k.z;
+  } else {
+    k.rsq = k.z * k.z + rp * rp;
+  }
   set_walk_dir(k);
   k.iflag = 0;
   k.ir = 1;
@@ -955,27 +1006,16 @@ __device__ __forceinline__ double wave_sum(double v) {
 #define HC_WAVES_PER_EU 1  // occupancy floor; below it the compiler would spill
 #endif
 
-// Peel records per wavefront queue.  An iteration adds at most one record
-// per lane, so a queue flushed whenever it holds more than len-64 records
-// never overflows.  Four queues and their counters fit the 64 KiB a
-// workgroup can allocate.
-template <class R, int NA>
-constexpr int peel_queue_len() {
-  constexpr size_t fit = (65536 - 64) / (4 * sizeof(PeelRec<R, NA>));
-  return fit < 96 ? (int)fit : 96;
-}
-static_assert(peel_queue_len<double, 4>() >= 80, "peel queue too short");
-static_assert(peel_queue_len<double, 8>() > 64, "peel queue too short");
-
-template <class R, int NA>
-constexpr size_t peel_lds_bytes() {  // queue counters + four wavefront queues
-  return 64 + 4 * (size_t)peel_queue_len<R, NA>() * sizeof(PeelRec<R, NA>);
-}
+// Peel records per wavefront queue (Params::peel_q, sized on the host to the
+// LDS left after the wall arrays).  An iteration adds at most one record per
+// lane, so a queue flushed whenever it holds more than peel_q - 64 records
+// never overflows.
 
 // Walk every queued (record, observer) pair of this wavefront: lanes take
 // consecutive rays, so the observers of one record run side by side.
 template <class R, int NA, bool TRACE>
-__device__ __forceinline__ void flush_peels(const PeelRec<R, NA> *q, int n, Acc &acc) {
+__device__ __forceinline__ void flush_peels(const WallView &W, const PeelRec<R, NA> *q, int n,
+                                            Acc &acc) {
   const Params &P = c_P;
   const int lane = threadIdx.x & 63;
   const int total = n * P.npeel;
@@ -996,7 +1036,7 @@ __device__ __forceinline__ void flush_peels(const PeelRec<R, NA> *q, int n, Acc 
     } else {
       w = (double)rec.s.I;
     }
-    const PeelOut o = peeloff(rec, w, obs);
+    const PeelOut o = peeloff(W, rec, w, obs);
     acc.err_wall += o.err;
     acc.npeelray++;
     acc.npeelcross += (uint32_t)o.ncross;
@@ -1009,35 +1049,67 @@ __device__ __forceinline__ void flush_peels(const PeelRec<R, NA> *q, int n, Acc 
 }
 
 // Event loop of the pass (propagate.f90:1-325 around newdisktherm.F90:194-449)
-// with path regeneration.  Each lane holds one packet; a wavefront
-// alternates two phases:
-//  - event phase: every lane whose packet needs an event (launch, star
-//    re-emission, interaction, end of the packet and fetch of the next one)
-//    performs it, and the peel-off records those events queued are walked
-//    once the wavefront's queue fills;
-//  - walk phase: lanes whose packets are between events cross one cell wall
-//    per step; lanes that reach their next event wait, and the phase ends
-//    once regen_lanes lanes wait (or nobody walks).
-// A packet's work therefore never waits for another packet's walk to end,
-// and the event code runs for many lanes at once.
+// with path regeneration and event sorting.  Each lane holds one packet and
+// is in one state: walking between events, or waiting for one kind of event
+// (scattering, absorption + re-emission, star re-emission, end of the packet
+// and launch of the next).  A wavefront alternates two phases:
+//  - walk phase: walking lanes cross one cell wall per step.  A lane whose
+//    walk reaches an interaction draws, in the reference's order, the dust
+//    species and the scatter/absorb test right away, so it waits for the
+//    event kind it will perform.  The phase ends as soon as ev_min lanes wait
+//    for one kind, or nobody walks;
+//  - event phase: every event kind with at least ev_min waiting lanes runs
+//    (all kinds when nobody walks), so each event branch executes for many
+//    lanes at once; peel-off records queued by the events are walked once the
+//    wavefront's queue fills.
+// A packet's history and its random numbers are those of the serial loop.
 template <class R, int NA, bool TRACE, bool PEEL>
 __global__ __launch_bounds__(256, HC_WAVES_PER_EU) void transport() {
-  extern __shared__ unsigned char lds[];
   const Params &P = c_P;
   const int wave = threadIdx.x >> 6;
-  constexpr int kQ = peel_queue_len<R, NA>();
+  unsigned char *const lds = hc_lds;
+  // wall arrays into LDS: r2[nrg] t2[ntg] a[npg] b[npg]
+  double *wl = reinterpret_cast<double *>(lds);
+  for (int i = threadIdx.x; i < P.nrg; i += blockDim.x) wl[i] = P.r2arr[i];
+  for (int i = threadIdx.x; i < P.ntg; i += blockDim.x) wl[P.nrg + i] = P.tan2arr[i];
+  for (int i = threadIdx.x; i < P.npg; i += blockDim.x) {
+    wl[P.nrg + P.ntg + i] = P.aarr[i];
+    wl[P.nrg + P.ntg + P.npg + i] = P.barr[i];
+  }
+  const WallView W{wl, wl + P.nrg, wl + P.nrg + P.ntg, wl + P.nrg + P.ntg + P.npg};
+  // opacity tables and guides of the active species into LDS
+  {
+    double *ot = reinterpret_cast<double *>(lds + P.opac_base);
+    int32_t *og = reinterpret_cast<int32_t *>(lds + P.opac_base + P.guide_off);
+    for (int s2 = 0; s2 < P.nact; ++s2) {
+      const int id = P.act[s2] - 1, nl = P.nlambda[id];
+      const size_t o = (size_t)P.nlammax * id;
+      double *t = ot + P.opac_off[s2];
+      for (int i = threadIdx.x; i < nl; i += blockDim.x) {
+        t[i] = P.nudust[o + i];
+        t[nl + i] = P.kappad[o + i];
+        t[2 * nl + i] = P.adust[o + i];
+        t[3 * nl + i] = P.gdust[o + i];
+        t[4 * nl + i] = P.pldust[o + i];
+      }
+      for (int i = threadIdx.x; i < kGuide; i += blockDim.x)
+        og[s2 * kGuide + i] = P.guide[(size_t)id * kGuide + i];
+    }
+  }
+  const int kQ = P.peel_q;
   PeelRec<R, NA> *queue = nullptr;
   int *qcount = nullptr;
   if constexpr (PEEL) {
-    qcount = reinterpret_cast<int *>(lds);
-    queue = reinterpret_cast<PeelRec<R, NA> *>(lds + 64) + (size_t)wave * kQ;
+    qcount = reinterpret_cast<int *>(lds + P.peel_base);
+    queue = reinterpret_cast<PeelRec<R, NA> *>(lds + P.peel_base + 64) + (size_t)wave * kQ;
     if ((threadIdx.x & 63) == 0) qcount[wave] = 0;
   }
-  auto push = [&](const Pkt<R, NA> &k, int idust2, bool direct, long long pid, const R *nrm) {
+  __syncthreads();
+  auto push = [&](const Pkt<R, NA> &k, int sp, bool direct, long long pid, const R *nrm) {
     if constexpr (PEEL) {
-      const int slot = atomicAdd(&qcount[wave], 1);
-      PeelRec<R, NA> &r = queue[slot];
-      make_rec(r, k, idust2, direct, pid);
+      const int qi = atomicAdd(&qcount[wave], 1);
+      PeelRec<R, NA> &r = queue[qi];
+      make_rec(r, k, sp, direct, pid);
       r.cosb = nrm[0];
       r.sinb = nrm[1];
       r.coslp = nrm[2];
@@ -1049,127 +1121,160 @@ __global__ __launch_bounds__(256, HC_WAVES_PER_EU) void transport() {
   Trace tr{0.0, 0.0, 0, 0, 0, 0, 0, 14695981039346656037ull};
   R nrm[4] = {R(0.0), R(0.0), R(0.0), R(0.0)};
   double tau = 0.0, tsum = 0.0;
-  int st = ST_FETCH;
+  double dm[NA];  // masked densities of the packet's cell (walk_step prefetches the next)
+#pragma unroll
+  for (int s2 = 0; s2 < NA; ++s2) dm[s2] = 0.0;
+  int st = ST_END, status = -1, slot = 0;  // status < 0: no packet yet
+  const int ev_min = P.regen_lanes;
+
+  // After a walk step: an interaction draws its species and the
+  // scatter/absorb test (select_dust.f90, propagate.f90:60-104); the packet
+  // ends when it escaped, was lost, or interacted iintmax times in one cell.
+  auto after_step = [&](int s) {
+    Pkt<R, NA> &k = L.k;
+    if (s == ST_WALK || s == ST_STAR) return s;
+    if (s == ST_EXIT) {
+      status = 0;
+      return (int)ST_END;
+    }
+    if (s == ST_ERROR || s == ST_ABORT) {
+      if (s == ST_ABORT) acc.warn++;
+      status = 2;
+      return (int)ST_END;
+    }
+    const size_t cell = cell_of(k.ir, k.it, k.ip);
+    if (!L.first) {
+      if (k.ir != L.pir || k.it != L.pit || k.ip != L.pip) L.iint = 0;
+      if (L.iint > P.iintmax) {
+        if (P.killed) atomicAdd(&P.killed[cell], 1);
+        acc.abflux++;
+        status = 1;
+        return (int)ST_END;
+      }
+    }
+    slot = select_dust(k, cell, L.rng);
+    if (slot < 0) {
+      acc.err_dust++;
+      status = 2;
+      return (int)ST_END;
+    }
+    if (L.first) acc.tot++;
+    L.first = false;
+    R alb_slot = R(0.0);  // k.alb[slot] without a dynamically indexed register array
+#pragma unroll
+    for (int s2 = 0; s2 < NA; ++s2)
+      if (s2 == slot) alb_slot = k.alb[s2];
+    return ran(L.rng) <= (double)alb_slot ? (int)ST_SCAT : (int)ST_ABS;
+  };
+  // the walk after an event: new optical depth, entry into the cell
+  auto start_walk = [&]() {
+    Pkt<R, NA> &k = L.k;
+    tau = draw_tau<R>(L.rng);
+    L.xpold = k.x;
+    L.ypold = k.y;
+    L.zpold = k.z;
+    tsum = 0.0;
+    const int s = after_step(walk_begin(W, k));
+    if (s == ST_WALK) load_dens(k, dm);
+    return s;
+  };
+
   for (;;) {
     // ---------------- event phase ----------------
-    if (st != ST_WALK && st != ST_DONE) {
+    {
+      const unsigned long long walkers = __ballot(st == ST_WALK);
+      const int n_s = __popcll(__ballot(st == ST_SCAT));
+      const int n_a = __popcll(__ballot(st == ST_ABS));
+      const int n_t = __popcll(__ballot(st == ST_STAR));
+      const int n_e = __popcll(__ballot(st == ST_END));
+      const bool all = walkers == 0ull;
       Pkt<R, NA> &k = L.k;
-      int status = -1;  // >= 0: the packet ends (0 escaped, 1 killed, 2 error)
-      if (st == ST_EXIT) {
-        status = 0;
-      } else if (st == ST_ERROR) {
-        status = 2;
-      } else if (st == ST_ABORT) {  // work budget spent: dropped and counted
-        acc.warn++;
-        status = 2;
-      } else if (st == ST_STAR) {
+      if (n_s > 0 && (all || n_s >= ev_min) && st == ST_SCAT) {
+        const int idust2 = P.act[slot] - 1;
+        k.iflag = 1;
+        ++L.iscat;
+        ++L.iint;
+        L.work++;
+        const HG<R> h = hg_of<R>(slot, k.nub);
+        push(k, slot, false, L.pid, nrm);
+        stokes<R>(k.d, k.s, h, L.rng, acc.htot, acc.hit);
+        acc.nscat_ev++;
+        if constexpr (TRACE) tr.nscat++;
+        set_walk_dir(k);
+        L.pir = k.ir;
+        L.pit = k.it;
+        L.pip = k.ip;
+        st = start_walk();
+      }
+      if (n_a > 0 && (all || n_a >= ev_min) && st == ST_ABS) {
+        const int idust2 = P.act[slot] - 1;
+        const size_t cell = cell_of(k.ir, k.it, k.ip);
+        L.work++;
+        lucy_open(L.la, cell);
+        L.la.nabs_any += 1;
+#pragma unroll
+        for (int s2 = 0; s2 < NA; ++s2)
+          if (s2 == slot) L.la.nabs[s2] += 1.0;
+        k.i_orig = (idust2 == 0 || idust2 == 1 || idust2 == 4 || idust2 == 5) ? 2 : 3;
+        k.s = Stokes4<R>{R(1.0), R(0.0), R(0.0), R(0.0)};
+        k.d.cost = R(1.0) - R(2.0) * R(ran(L.rng));
+        k.d.sint = m_sqrt(R(1.0) - m_pow(k.d.cost, R(2.0)));
+        k.d.phi = R(kR2P) * R(ran(L.rng));
+        k.d.cosp = m_cos(k.d.phi);
+        k.d.sinp = m_sin(k.d.phi);
+        k.nub = lucydustfreq<R>(P.tdust[cell * (size_t)P.nact + slot], idust2, L.rng);
+        set_walk_dir(k);
+        k.iflag = 0;
+        set_opacity(k, false);
+        push(k, slot, false, L.pid, nrm);
+        ++L.iint;
+        acc.nabs_ev++;
+        if constexpr (TRACE) tr.nabs++;
+        L.pir = k.ir;
+        L.pit = k.it;
+        L.pip = k.ip;
+        st = start_walk();
+      }
+      if (n_t > 0 && (all || n_t >= ev_min) && st == ST_STAR) {
         L.work++;
         star_reemit(k, L.rng, acc, nrm);
         if constexpr (TRACE) tr.nstar++;
         push(k, 0, true, L.pid, nrm);
-        tau = draw_tau<R>(L.rng);
-        L.xpold = k.x;
-        L.ypold = k.y;
-        L.zpold = k.z;
-      } else if (st == ST_INTERACT) {
-        L.work++;
-        const size_t cell = cell_of(k.ir, k.it, k.ip);
-        if (!L.first) {
-          if (k.ir != L.pir || k.it != L.pit || k.ip != L.pip) L.iint = 0;
-          if (L.iint > P.iintmax) {
-            if (P.killed) atomicAdd(&P.killed[cell], 1);
-            acc.abflux++;
-            status = 1;
+        st = start_walk();
+      }
+      if (n_e > 0 && (all || n_e >= ev_min) && st == ST_END) {
+        if (status >= 0) {  // the packet ends (0 escaped, 1 killed, 2 lost)
+          acc.maxwork = max(acc.maxwork, L.work);
+          lucy_flush(L.la);
+          if (status == 0) bin_exit(k, L.xpold, L.ypold, L.zpold, L.iscat, acc);
+          if constexpr (TRACE) {
+            hc_trace &o = P.trace[L.pid];
+            o.x = k.x;
+            o.y = k.y;
+            o.z = k.z;
+            o.cost = k.d.cost;
+            o.sint = k.d.sint;
+            o.phi = k.d.phi;
+            o.I = k.s.I;
+            o.Q = k.s.Q;
+            o.U = k.s.U;
+            o.V = k.s.V;
+            o.nub = (double)k.nub;
+            o.path = tr.path;
+            o.ncross = tr.ncross;
+            o.nscat = tr.nscat;
+            o.nabs = tr.nabs;
+            o.nstar = tr.nstar;
+            o.ntrial = 0;
+            o.cellhash = tr.hash;
+            o.status = status;
+            o.ir = k.ir;
+            o.it = k.it;
+            o.ip = k.ip;
+            o.pad = 0;
           }
         }
-        if (status < 0) {
-          const int slot = select_dust(k, cell, L.rng);
-          if (slot < 0) {
-            acc.err_dust++;
-            status = 2;
-          } else {
-            const int idust2 = P.act[slot] - 1;
-            if (L.first) acc.tot++;
-            L.first = false;
-            R alb_slot = R(0.0);  // k.alb[slot] without a dynamically indexed register array
-#pragma unroll
-            for (int s2 = 0; s2 < NA; ++s2)
-              if (s2 == slot) alb_slot = k.alb[s2];
-            if (ran(L.rng) <= (double)alb_slot) {
-              k.iflag = 1;
-              ++L.iscat;
-              ++L.iint;
-              const HG<R> h = hg_of<R>(idust2, k.nub);
-              push(k, idust2, false, L.pid, nrm);
-              stokes<R>(k.d, k.s, h, L.rng, acc.htot, acc.hit);
-              acc.nscat_ev++;
-              if constexpr (TRACE) tr.nscat++;
-            } else {
-              lucy_open(L.la, cell);
-              L.la.nabs_any += 1;
-#pragma unroll
-              for (int s2 = 0; s2 < NA; ++s2)
-                if (s2 == slot) L.la.nabs[s2] += 1.0;
-              k.i_orig = (idust2 == 0 || idust2 == 1 || idust2 == 4 || idust2 == 5) ? 2 : 3;
-              k.s = Stokes4<R>{R(1.0), R(0.0), R(0.0), R(0.0)};
-              k.d.cost = R(1.0) - R(2.0) * R(ran(L.rng));
-              k.d.sint = m_sqrt(R(1.0) - m_pow(k.d.cost, R(2.0)));
-              k.d.phi = R(kR2P) * R(ran(L.rng));
-              k.d.cosp = m_cos(k.d.phi);
-              k.d.sinp = m_sin(k.d.phi);
-              k.nub = lucydustfreq<R>(P.tdust[cell * (size_t)P.nact + slot], idust2, L.rng);
-              set_walk_dir(k);
-              k.iflag = 0;
-              set_opacity(k, false);
-              push(k, idust2, false, L.pid, nrm);
-              ++L.iint;
-              acc.nabs_ev++;
-              if constexpr (TRACE) tr.nabs++;
-            }
-            tau = draw_tau<R>(L.rng);
-            L.xpold = k.x;
-            L.ypold = k.y;
-            L.zpold = k.z;
-            set_walk_dir(k);
-            L.pir = k.ir;
-            L.pit = k.it;
-            L.pip = k.ip;
-          }
-        }
-      }
-      if (status >= 0) {  // the packet ends
-        acc.maxwork = max(acc.maxwork, L.work);
-        lucy_flush(L.la);
-        if (status == 0) bin_exit(k, L.xpold, L.ypold, L.zpold, L.iscat, acc);
-        if constexpr (TRACE) {
-          hc_trace &o = P.trace[L.pid];
-          o.x = k.x;
-          o.y = k.y;
-          o.z = k.z;
-          o.cost = k.d.cost;
-          o.sint = k.d.sint;
-          o.phi = k.d.phi;
-          o.I = k.s.I;
-          o.Q = k.s.Q;
-          o.U = k.s.U;
-          o.V = k.s.V;
-          o.nub = (double)k.nub;
-          o.path = tr.path;
-          o.ncross = tr.ncross;
-          o.nscat = tr.nscat;
-          o.nabs = tr.nabs;
-          o.nstar = tr.nstar;
-          o.ntrial = 0;
-          o.cellhash = tr.hash;
-          o.status = status;
-          o.ir = k.ir;
-          o.it = k.it;
-          o.ip = k.ip;
-          o.pad = 0;
-        }
-        st = ST_FETCH;
-      }
-      if (st == ST_FETCH) {  // take the next packet and launch it
+        // take the next packet and launch it
         const unsigned long long p = atomicAdd(P.next_packet, 1ull);
         if (p >= (unsigned long long)P.np) {
           st = ST_DONE;
@@ -1180,26 +1285,20 @@ __global__ __launch_bounds__(256, HC_WAVES_PER_EU) void transport() {
           L.iscat = L.iint = L.pir = L.pit = L.pip = 0;
           L.work = 0;
           L.first = true;
+          status = -1;
           if constexpr (TRACE) tr = Trace{0.0, 0.0, 0, 0, 0, 0, 0, 14695981039346656037ull};
           launch_star(k, L.rng, (long long)p + 1 < P.npacc + P.npaccstar, nrm);
           if constexpr (TRACE) hash_cell(tr, k.ir, k.it, k.ip);
           push(k, 0, true, L.pid, nrm);
-          L.xpold = k.x;
-          L.ypold = k.y;
-          L.zpold = k.z;
-          tau = draw_tau<R>(L.rng);
+          st = start_walk();
         }
-      }
-      if (st != ST_DONE) {
-        tsum = 0.0;
-        st = walk_begin(k);
       }
     }
     if constexpr (PEEL) {
       const int n = qcount[wave];
       const bool last = __ballot(st != ST_DONE) == 0ull;
       if (n > kQ - 64 || (last && n > 0)) {
-        flush_peels<R, NA, TRACE>(queue, n, acc);
+        flush_peels<R, NA, TRACE>(W, queue, n, acc);
         if ((threadIdx.x & 63) == 0) qcount[wave] = 0;
       }
     }
@@ -1207,10 +1306,13 @@ __global__ __launch_bounds__(256, HC_WAVES_PER_EU) void transport() {
     // ---------------- walk phase ----------------
     for (;;) {
       if (__ballot(st == ST_WALK) == 0ull) break;
-      if (__popcll(__ballot(st != ST_WALK && st != ST_DONE)) >= P.regen_lanes) break;
+      if (__popcll(__ballot(st == ST_SCAT)) >= ev_min || __popcll(__ballot(st == ST_ABS)) >= ev_min ||
+          __popcll(__ballot(st == ST_END)) >= ev_min || __popcll(__ballot(st == ST_STAR)) >= ev_min)
+        break;
       if (st == ST_WALK) {
-        st = walk_step<R, NA, TRACE>(L.k, tau, tsum, L.la, acc, tr);
-        if (++L.work > P.max_work) st = ST_ABORT;
+        int s = walk_step<R, NA, TRACE>(W, L.k, tau, tsum, dm, L.la, acc, tr);
+        if (++L.work > P.max_work) s = ST_ABORT;
+        st = after_step(s);
       }
     }
   }
@@ -1345,6 +1447,7 @@ struct hc_ctx {
   // launch shape
   int regen_lanes = 16;
   int blocks_per_cu = 0;
+  size_t lds_per_block = 64 * 1024;  // the device's limit (hc_create)
   uint32_t max_work = 1u << 24;
 };
 
@@ -1435,6 +1538,11 @@ int hc_create(const hc_dims *dims, int device, hc_ctx **out) {
     delete ctx;
     return HC_E_HIP;
   }
+  int lds = 0;  // the workgroup LDS limit; 64 KiB when the runtime does not say
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess &&
+      lds >= 64 * 1024)
+    ctx->lds_per_block = (size_t)lds;
+  (void)hipGetLastError();
   // RNG jump tables: (a^(2^30))^(2^k) mod m for k = 0..63
   std::vector<int32_t> p1(64), p2(64);
   int64_t b1 = 1033780774, b2 = 1494757890;  // a1w, a2w (ecuyer_cote_mod.f90:14-15)
@@ -1806,15 +1914,14 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
   P.regen_lanes = ctx->regen_lanes;
   P.max_work = ctx->max_work;
 
-  HCK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_P), &P, sizeof P, 0, hipMemcpyHostToDevice, s));
   void (*kern)() = nullptr;
-  size_t shm = 0;
+  size_t rec_bytes = 0;
   const bool fp32 = d.precision == HC_FP32;
   const bool pk = P.ipeel != 0;
 #define HC_PICK(R_, NA_)                                                                   \
   kern = trace ? (pk ? transport<R_, NA_, true, true> : transport<R_, NA_, true, false>)  \
                : (pk ? transport<R_, NA_, false, true> : transport<R_, NA_, false, false>); \
-  shm = pk ? peel_lds_bytes<R_, NA_>() : 0
+  rec_bytes = sizeof(PeelRec<R_, NA_>)
   if (ctx->nact <= 4) {
     if (fp32) {
       HC_PICK(float, 4);
@@ -1829,6 +1936,42 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
     }
   }
 #undef HC_PICK
+  // dynamic LDS: the wall arrays, then (peel-off passes) four wavefront
+  // queues of peel records; two workgroups share a compute unit
+  const size_t walls = ((size_t)(d.nrg + d.ntg + 2 * d.npg) * sizeof(double) + 63) / 64 * 64;
+  P.walls_lds = (int)walls;
+  size_t ntab = 0;  // doubles of the active species' opacity tables
+  for (int k = 0; k < ctx->nact; ++k) {
+    P.opac_off[k] = (int)ntab;
+    ntab += 5 * (size_t)ctx->nlambda[ctx->act[k] - 1];
+  }
+  P.opac_base = (int)walls;
+  P.guide_off = (int)((ntab * sizeof(double) + 15) / 16 * 16);
+  const size_t opac =
+      ((size_t)P.guide_off + (size_t)ctx->nact * kGuide * sizeof(int32_t) + 63) / 64 * 64;
+  P.peel_base = (int)(walls + opac);
+  size_t shm = walls + opac;
+  P.peel_q = 0;
+  if (pk) {
+    // two workgroups per compute unit when the queues leave room for them
+    const size_t budget = ctx->lds_per_block;
+    const size_t avail = budget > shm + 64 ? budget - shm - 64 : 0;
+    const size_t q = std::min<size_t>(96, avail / (4 * rec_bytes));
+    if (q < 65)
+      return fail(ctx, HC_E_UNSUPPORTED,
+                  "hc_run_iteration: the peel-off queues do not fit the LDS left by this grid's "
+                  "wall arrays and dust tables");
+    P.peel_q = (int)q;
+    shm += 64 + 4 * q * rec_bytes;
+  }
+  if (shm > ctx->lds_per_block)
+    return fail(ctx, HC_E_UNSUPPORTED,
+                "hc_run_iteration: the grid's wall arrays and the dust opacity tables exceed "
+                "the LDS of a workgroup");
+  HCK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_P), &P, sizeof P, 0, hipMemcpyHostToDevice, s));
+  if (shm > 64 * 1024)
+    HCK(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
   int dev_cus = 0, bpm = 0;
   HCK(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
   HCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpm, kern, 256, shm));
