@@ -31,7 +31,7 @@
 namespace hc {
 
 constexpr int kGuide = 512;  // bins of the opacity-table guide
-constexpr int kCdfGuide = 128;  // probability bins of the emissivity-CDF guide
+constexpr int kCdfGuide = 1024;  // probability bins of the emissivity-CDF guide
 
 struct Params {
   // grid
@@ -76,6 +76,7 @@ struct Params {
   int opac_base;    // LDS byte offset of the opacity tables
   int guide_off;    // byte offset of the opacset guides within them
   int opac_off[kMaxSpecies];  // table of active species s: offset in doubles
+  int emis_base;    // LDS byte offset of lTint[nT] and nuint[nnu]
   int peel_base;    // LDS byte offset of the peel-off queues
   uint32_t max_work;  // events + crossings after which a packet is dropped
 };
@@ -166,13 +167,19 @@ struct WallView {
   const double *r2, *t2, *a, *b;
 };
 
+// Per-cell densities are stored once, signed: +rho for a species below its
+// sublimation temperature, -rho above it.  The walk sees the masked density
+// (peeloff_3d.f90:103-110), select_dust the full one (select_dust.f90), from
+// the same load.
+__device__ __forceinline__ double masked(double d) { return d > 0.0 ? d : 0.0; }
+
 // Masked opacity of a cell: sum over active species in increasing id.
 template <class R, int NA>
 __device__ __forceinline__ double cell_opac(const Pkt<R, NA> &k, const double (&dm)[NA]) {
   double opac = 0.0;
 #pragma unroll
   for (int s = 0; s < NA; ++s)
-    if (s < c_P.nact) opac = opac + k.kapd[s] * dm[s];
+    if (s < c_P.nact) opac = opac + k.kapd[s] * masked(dm[s]);
   return opac;
 }
 
@@ -264,7 +271,7 @@ __device__ __forceinline__ void lucy_add(LucyAcc<R, NA> &la, const Pkt<R, NA> &k
   lucy_open(la, cell);
 #pragma unroll
   for (int s = 0; s < NA; ++s)
-    if (s < c_P.nact && dm[s] != 0.0) la.tau[s] += k.kapd[s] * (1.0 - (double)k.alb[s]) * dm[s] * ds;
+    if (s < c_P.nact && dm[s] > 0.0) la.tau[s] += k.kapd[s] * (1.0 - (double)k.alb[s]) * dm[s] * ds;
   la.len += ds;
 }
 
@@ -452,7 +459,7 @@ __device__ __forceinline__ double rec_opac(const PeelRec<R, NA> &k, const double
   double opac = 0.0;
 #pragma unroll
   for (int s = 0; s < NA; ++s)
-    if (s < c_P.nact) opac = opac + k.kapd[s] * dm[s];
+    if (s < c_P.nact) opac = opac + k.kapd[s] * masked(dm[s]);
   return opac;
 }
 
@@ -580,7 +587,9 @@ template <class R>
 __device__ __forceinline__ R lucydustfreq(double T0d, int id, Mrg &rng) {
   const Params &P = c_P;
   const int nT = P.nT, nnu = P.nnu;
-  const double *lT = P.lTint, *nuint = P.nuint;
+  // lTint and nuint from LDS (staged by the transport kernel)
+  const double *lT = reinterpret_cast<const double *>(hc_lds + P.emis_base);
+  const double *nuint = lT + nT;
   const double *kb = P.kapBint + (size_t)nT * nnu * id;
   const R T0 = R(T0d);
   const R lT0 = m_log(T0);
@@ -595,10 +604,24 @@ __device__ __forceinline__ R lucydustfreq(double T0d, int id, Mrg &rng) {
     const double *p = row + (size_t)nT * (size_t)(ii - 1);
     return (R(1.0) - f) * R(p[0]) + f * R(p[1]);
   };
+  // the end nodes and the guide entries do not depend on each other: their
+  // loads are issued together before any of them is needed
+  const double *pe = row + (size_t)nT * (size_t)(nnu - 1);
+  const double a0 = row[0], a1 = row[1], e0 = pe[0], e1 = pe[1];
+  const int bq = clampi((int)(xi * R(kCdfGuide)), 0, kCdfGuide - 1);
+  int g0a = 0, g1a = 0, g0b = 0, g1b = 0;
+  if (P.cdfguide) {
+    const uint16_t *g0 = P.cdfguide + ((size_t)id * nT + (size_t)(j - 1)) * (kCdfGuide + 1);
+    const uint16_t *g1 = g0 + (kCdfGuide + 1);
+    g0a = g0[bq];
+    g1a = g1[bq];
+    g0b = g0[bq + 1];
+    g1b = g1[bq + 1];
+  }
   int i0 = 1, i1 = nnu;
-  R P0 = KB(i0);
+  R P0 = (R(1.0) - f) * R(a0) + f * R(a1);
   if (xi < P0) return (P0 > R(0.0)) ? T0 * xi * R(nuint[0]) / P0 : T0 * R(nuint[0]);
-  R P1 = KB(i1);
+  R P1 = (R(1.0) - f) * R(e0) + f * R(e1);
   if (P1 > R(1.0)) P1 = R(1.0);
   if (xi >= P1) {
     if (P1 < R(1.0) && xi < R(1.0)) return T0 * R(nuint[nnu - 1]) * (R(1.0) - P1) / (R(1.0) - xi);
@@ -609,11 +632,8 @@ __device__ __forceinline__ R lucydustfreq(double T0d, int id, Mrg &rng) {
     // of both rows at the probability bin of xi bound the bracket; the
     // rows are strictly increasing (checked on the host), so the bracket is
     // the unique one the full bisection finds.
-    const int b = (int)(xi * R(kCdfGuide));
-    const uint16_t *g0 = P.cdfguide + ((size_t)id * nT + (size_t)(j - 1)) * (kCdfGuide + 1);
-    const uint16_t *g1 = g0 + (kCdfGuide + 1);
-    const int lo = max(1, (int)min(g0[b], g1[b]));
-    const int hi = min(nnu, (int)max(g0[b + 1], g1[b + 1]) + 1);
+    const int lo = max(1, min(g0a, g1a));
+    const int hi = min(nnu, max(g0b, g1b) + 1);
     if (lo > i0) {
       i0 = lo;
       P0 = KB(i0);
@@ -894,22 +914,22 @@ __device__ __forceinline__ void star_reemit(Pkt<R, NA> &k, Mrg &rng, Acc &acc, R
   nrm[3] = m_sin(lp);
 }
 
-// select_dust.f90 over the active species (zero-density slots add nothing).
+// select_dust.f90 over the active species (zero-density slots add nothing),
+// from the cell's densities already in registers (|signed density|).
 template <class R, int NA>
-__device__ __forceinline__ int select_dust(const Pkt<R, NA> &k, size_t cell, Mrg &rng) {
+__device__ __forceinline__ int select_dust(const Pkt<R, NA> &k, const double (&dm)[NA], Mrg &rng) {
   const Params &P = c_P;
   const double xran = ran(rng);
-  const double *dn = P.dens + cell * (size_t)P.nact;
   double opac = 0.0, frac = 0.0;
 #pragma unroll
   for (int s = 0; s < NA; ++s)
-    if (s < P.nact) opac = opac + k.kapd[s] * dn[s];
+    if (s < P.nact) opac = opac + k.kapd[s] * fabs(dm[s]);
   if (opac == 0.0) return -1;
   const double slack = (double)1.e-10f;
 #pragma unroll
   for (int s = 0; s < NA; ++s) {
     if (s < P.nact) {
-      frac = frac + k.kapd[s] * dn[s] / opac;
+      frac = frac + k.kapd[s] * fabs(dm[s]) / opac;
       if (xran <= frac + slack) return s;
     }
   }
@@ -994,6 +1014,7 @@ struct Lane {
   long long pid;
   int iscat, iint, pir, pit, pip;
   uint32_t work;  // events + crossings of the packet so far
+  double tdust;   // T of the absorbing species in the cell (prefetched for re-emission)
   bool first;
 };
 
@@ -1095,6 +1116,9 @@ __global__ __launch_bounds__(256, HC_WAVES_PER_EU) void transport() {
       for (int i = threadIdx.x; i < kGuide; i += blockDim.x)
         og[s2 * kGuide + i] = P.guide[(size_t)id * kGuide + i];
     }
+    double *em = reinterpret_cast<double *>(lds + P.emis_base);
+    for (int i = threadIdx.x; i < P.nT; i += blockDim.x) em[i] = P.lTint[i];
+    for (int i = threadIdx.x; i < P.nnu; i += blockDim.x) em[P.nT + i] = P.nuint[i];
   }
   const int kQ = P.peel_q;
   PeelRec<R, NA> *queue = nullptr;
@@ -1152,7 +1176,7 @@ __global__ __launch_bounds__(256, HC_WAVES_PER_EU) void transport() {
         return (int)ST_END;
       }
     }
-    slot = select_dust(k, cell, L.rng);
+    slot = select_dust(k, dm, L.rng);
     if (slot < 0) {
       acc.err_dust++;
       status = 2;
@@ -1164,18 +1188,22 @@ __global__ __launch_bounds__(256, HC_WAVES_PER_EU) void transport() {
 #pragma unroll
     for (int s2 = 0; s2 < NA; ++s2)
       if (s2 == slot) alb_slot = k.alb[s2];
-    return ran(L.rng) <= (double)alb_slot ? (int)ST_SCAT : (int)ST_ABS;
+    if (ran(L.rng) <= (double)alb_slot) return (int)ST_SCAT;
+    L.tdust = P.tdust[cell * (size_t)P.nact + slot];  // used when the absorption runs
+    return (int)ST_ABS;
   };
   // the walk after an event: new optical depth, entry into the cell
-  auto start_walk = [&]() {
+  // reload: the event moved the packet to another cell (launch, star)
+  auto start_walk = [&](bool reload) {
     Pkt<R, NA> &k = L.k;
     tau = draw_tau<R>(L.rng);
     L.xpold = k.x;
     L.ypold = k.y;
     L.zpold = k.z;
     tsum = 0.0;
+    const int ir0 = k.ir;
     const int s = after_step(walk_begin(W, k));
-    if (s == ST_WALK) load_dens(k, dm);
+    if (s == ST_WALK && (reload || k.ir != ir0)) load_dens(k, dm);
     return s;
   };
 
@@ -1204,7 +1232,7 @@ __global__ __launch_bounds__(256, HC_WAVES_PER_EU) void transport() {
         L.pir = k.ir;
         L.pit = k.it;
         L.pip = k.ip;
-        st = start_walk();
+        st = start_walk(false);
       }
       if (n_a > 0 && (all || n_a >= ev_min) && st == ST_ABS) {
         const int idust2 = P.act[slot] - 1;
@@ -1222,7 +1250,7 @@ __global__ __launch_bounds__(256, HC_WAVES_PER_EU) void transport() {
         k.d.phi = R(kR2P) * R(ran(L.rng));
         k.d.cosp = m_cos(k.d.phi);
         k.d.sinp = m_sin(k.d.phi);
-        k.nub = lucydustfreq<R>(P.tdust[cell * (size_t)P.nact + slot], idust2, L.rng);
+        k.nub = lucydustfreq<R>(L.tdust, idust2, L.rng);
         set_walk_dir(k);
         k.iflag = 0;
         set_opacity(k, false);
@@ -1233,14 +1261,14 @@ __global__ __launch_bounds__(256, HC_WAVES_PER_EU) void transport() {
         L.pir = k.ir;
         L.pit = k.it;
         L.pip = k.ip;
-        st = start_walk();
+        st = start_walk(false);
       }
       if (n_t > 0 && (all || n_t >= ev_min) && st == ST_STAR) {
         L.work++;
         star_reemit(k, L.rng, acc, nrm);
         if constexpr (TRACE) tr.nstar++;
         push(k, 0, true, L.pid, nrm);
-        st = start_walk();
+        st = start_walk(true);
       }
       if (n_e > 0 && (all || n_e >= ev_min) && st == ST_END) {
         if (status >= 0) {  // the packet ends (0 escaped, 1 killed, 2 lost)
@@ -1290,7 +1318,7 @@ __global__ __launch_bounds__(256, HC_WAVES_PER_EU) void transport() {
           launch_star(k, L.rng, (long long)p + 1 < P.npacc + P.npaccstar, nrm);
           if constexpr (TRACE) hash_cell(tr, k.ir, k.it, k.ip);
           push(k, 0, true, L.pid, nrm);
-          st = start_walk();
+          st = start_walk(true);
         }
       }
     }
@@ -1398,7 +1426,7 @@ __global__ __launch_bounds__(256) void lucy_temperature(TParams T) {
 __global__ void mask_density(const double *dens, const double *tdust, double tsub_ev,
                              double *densm, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (; i < n; i += (size_t)gridDim.x * blockDim.x) densm[i] = tdust[i] < tsub_ev ? dens[i] : 0.0;
+  for (; i < n; i += (size_t)gridDim.x * blockDim.x) densm[i] = tdust[i] < tsub_ev ? dens[i] : -dens[i];
 }
 
 }  // namespace hc
@@ -1949,8 +1977,10 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
   P.guide_off = (int)((ntab * sizeof(double) + 15) / 16 * 16);
   const size_t opac =
       ((size_t)P.guide_off + (size_t)ctx->nact * kGuide * sizeof(int32_t) + 63) / 64 * 64;
-  P.peel_base = (int)(walls + opac);
-  size_t shm = walls + opac;
+  P.emis_base = (int)(walls + opac);
+  const size_t emis = ((size_t)(d.nT + d.nnu) * sizeof(double) + 63) / 64 * 64;
+  P.peel_base = (int)(walls + opac + emis);
+  size_t shm = walls + opac + emis;
   P.peel_q = 0;
   if (pk) {
     // two workgroups per compute unit when the queues leave room for them
@@ -1966,7 +1996,7 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
   }
   if (shm > ctx->lds_per_block)
     return fail(ctx, HC_E_UNSUPPORTED,
-                "hc_run_iteration: the grid's wall arrays and the dust opacity tables exceed "
+                "hc_run_iteration: the grid's wall arrays and the dust tables exceed "
                 "the LDS of a workgroup");
   HCK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_P), &P, sizeof P, 0, hipMemcpyHostToDevice, s));
   if (shm > 64 * 1024)
