@@ -47,6 +47,9 @@ class HcSources(C.Structure):
         ("iintmax", C.c_int32), ("pad", C.c_int32),
         ("u", _dp), ("aperture2", _dp), ("coste", _dp), ("sinte", _dp), ("cospe", _dp),
         ("sinpe", _dp),
+        ("rddustmin", C.c_double), ("rmaxdmax", C.c_double), ("rmax", C.c_double),
+        ("massc", C.c_double), ("mdot", C.c_double),
+        ("fmass", C.c_double * 8), ("z1", C.c_double * 8), ("bflare", C.c_double * 8),
     ]
 
 
@@ -196,6 +199,11 @@ class Transport:
         s.ltot = m.ltot
         s.iintmax = 10000
         s.nspot = 1
+        s.accfrac = m.accfrac
+        s.rddustmin, s.rmaxdmax, s.rmax = m.rddustmin, m.rmaxdmax, m.rarr[-1]
+        s.massc, s.mdot = m.cfg.massc, m.mdot
+        for i in range(8):
+            s.fmass[i], s.z1[i], s.bflare[i] = m.fmass8[i], m.z1_8[i], m.bflare8[i]
         for k, v in over.items():
             setattr(s, k, v)
         s.u, s.aperture2, s.coste, s.sinte, s.cospe, s.sinpe = [_ptr(a) for a in arrs]

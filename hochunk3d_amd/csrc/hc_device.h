@@ -19,6 +19,8 @@ constexpr double kPi = 3.1415926535897932384626433;  // constants.f90:5
 constexpr double kR2P = 2.0 * kPi;
 constexpr double kRsol = 6.9598e10;                   // constants.f90:11
 constexpr double kSigT = 5.670400e-5;                 // constants.f90:21
+constexpr double kGn = 6.67428e-8;                    // constants.f90:19
+constexpr double kMsol = 1.989e33;                    // constants.f90:13
 constexpr double kHuge = 1.7976931348623157e308;      // huge(1d0)
 constexpr uint32_t kMaxTrials = 1u << 20;             // stokes rejection-loop bound
 constexpr int kMaxSpecies = 8;                        // grid_mod.f90:13

@@ -78,6 +78,11 @@ struct Params {
   int opac_off[kMaxSpecies];  // table of active species s: offset in doubles
   int emis_base;    // LDS byte offset of lTint[nT] and nuint[nnu]
   int peel_base;    // LDS byte offset of the peel-off queues
+  // disk-accretion launch (initpacc.f90, emit_gasacc.f90)
+  const double *costarr;  // cos of the theta walls (gridset.f90:580-599)
+  double rddustmin, rmaxdmax, rmax, massc, mdot;
+  double fmass[kMaxSpecies], z1[kMaxSpecies], bflare[kMaxSpecies];
+  int slot_of[kMaxSpecies];  // active slot of reference species id, -1 if none
   uint32_t max_work;  // events + crossings after which a packet is dropped
 };
 
@@ -792,6 +797,134 @@ __device__ __forceinline__ void locate_cell_angles(Pkt<R, NA> &k) {
   k.ip = locate(P.phiarr, P.npg, lp);
 }
 
+// Species masks of grid_mod.f90:15-20 (0-based ids).
+__constant__ const int c_is_disk[8] = {1, 1, 0, 0, 1, 1, 0, 0};
+__constant__ const int c_is_thermal_disk[8] = {1, 1, 0, 0, 0, 0, 0, 0};
+
+// initpacc.f90:27-52: the emission point of a disk-accretion packet.  r from
+// p(1/r) ~ 1 - sqrt(1/r) by rejection within (rddustmin, rmaxdmax), a
+// uniform azimuth (rantheta, random.f90:28-47) and a Gaussian height of the
+// mass-weighted scale height (gasdev, random.f90:140-159); redrawn outside
+// rmax.  The reference's gasdev keeps the second deviate of a pair for its
+// next call, across packets; a packet here owns its generator segment, so
+// every call draws a fresh pair and returns the deviate returned first.
+__device__ __forceinline__ void acc_position(Mrg &rng, double &x, double &y, double &z) {
+  const Params &P = c_P;
+  for (;;) {
+    double rp = 0.0;
+    while (rp >= P.rmaxdmax || rp <= P.rddustmin) {
+      double winv = ran(rng);
+      while (ran(rng) > 1.0 - sqrt(winv)) winv = ran(rng);
+      rp = 1.0 / winv;
+    }
+    double r2 = 2.0, xx = 0.0, yy = 0.0;
+    while (r2 > 1.0) {
+      xx = ran(rng);
+      xx = xx + xx - 1.0;
+      yy = ran(rng);
+      yy = yy + yy - 1.0;
+      r2 = xx * xx + yy * yy;
+    }
+    const double rr = sqrt(r2);
+    x = rp * (xx / rr);
+    y = rp * (yy / rr);
+    double zsum = 0.0;
+#pragma unroll
+    for (int id = 0; id < 8; ++id)
+      if (c_is_disk[id]) zsum = zsum + P.fmass[id] * P.z1[id] * hcm_pow(rp, P.bflare[id]);
+    double v1, v2, rsq;
+    for (;;) {
+      v1 = 2.0 * ran(rng) - 1.0;
+      v2 = 2.0 * ran(rng) - 1.0;
+      rsq = v1 * v1 + v2 * v2;
+      if (rsq < 1.0 && rsq > 0.0) break;
+    }
+    z = zsum * (v2 * sqrt(-2.0 * hcm_log(rsq) / rsq));
+    if (z * z + rp * rp < P.rmax * P.rmax) return;
+  }
+}
+
+// Disk-accretion launch: initpacc.f90:3-122 without the diffusion layer,
+// then emit_common.f90:25-78 (Lucy re-emission at the cell's temperature) or
+// emit_gasacc.f90:1-44 (Pringle 1981 gas emission where the cell has no
+// thermal disk dust), then newdisktherm.F90:219-250.  The reference picks
+// the emitting species with the kapd the previous packet left (select_dust
+// reads the module global); packets here are independent, so the thermal
+// disk species are weighted by kappa_V(id) rho(id), which is that choice
+// whenever the species share one dust table.  Returns the emitting slot
+// (0 for gas emission).
+template <class R, int NA>
+__device__ __forceinline__ int launch_acc(const WallView &W, Pkt<R, NA> &k, Mrg &rng) {
+  const Params &P = c_P;
+  double x, y, z;
+  acc_position(rng, x, y, z);
+  k.x = x;
+  k.y = y;
+  k.z = z;
+  k.s = Stokes4<R>{R(1.0), R(0.0), R(0.0), R(0.0)};
+  k.rsq = x * x + y * y + z * z;
+  const double rtot = sqrt(k.rsq);
+  k.ir = locate(W.r2, P.nrg, k.rsq);
+  k.it = locate(P.costarr, P.ntg, z / rtot);
+  double lp = hcm_atan2(y, x);
+  if (lp < 0.0) lp = lp + kR2P;
+  k.ip = locate(P.phiarr, P.npg, lp);
+  const size_t cell = cell_of(k.ir, k.it, k.ip);
+  const double *dn = P.densm + cell * (size_t)P.nact;  // signed: |.| is the density
+  double rho[8];
+  double dsum = 0.0;
+#pragma unroll
+  for (int id = 0; id < 8; ++id) {
+    const int sl = P.slot_of[id];
+    rho[id] = (c_is_thermal_disk[id] && sl >= 0) ? fabs(dn[sl]) : 0.0;
+    if (c_is_thermal_disk[id]) dsum = dsum + rho[id];
+  }
+  int idust2 = -1;
+  if (dsum > 0.0) {  // select_dust(is_thermal_disk), select_dust.f90:17-42
+    const double xran = ran(rng);
+    double opac = 0.0, frac = 0.0;
+#pragma unroll
+    for (int id = 0; id < 8; ++id)
+      if (c_is_thermal_disk[id]) opac = opac + P.kappav[id] * rho[id];
+#pragma unroll
+    for (int id = 0; id < 8; ++id) {
+      if (c_is_thermal_disk[id] && idust2 < 0) {
+        frac = frac + P.kappav[id] * rho[id] / opac;
+        if (xran <= frac + (double)1.e-10f) idust2 = id;
+      }
+    }
+  }
+  int slot = 0;
+  if (idust2 >= 0) {  // initpacc.f90:95-115, then the Lucy re-emission
+    slot = P.slot_of[idust2];
+    atomicAdd(&P.dbig[cell], 1);
+    atomic_add(&P.nabs[cell * (size_t)P.nact + slot], 1.0);
+    k.d.cost = R(1.0) - R(2.0) * R(ran(rng));
+    k.d.sint = m_sqrt(R(1.0) - m_pow(k.d.cost, R(2.0)));
+    k.d.phi = R(kR2P) * R(ran(rng));
+    k.d.cosp = m_cos(k.d.phi);
+    k.d.sinp = m_sin(k.d.phi);
+    k.nub = lucydustfreq<R>(P.tdust[cell * (size_t)P.nact + slot], idust2, rng);
+  } else {  // emit_gasacc.f90:22-42
+    const double t0 = 3.0 * kGn * (kMsol * P.massc) * P.mdot /
+                      (8.0 * kPi * kSigT * hcm_pow(rtot * P.rstar * kRsol, 3.0));
+    const double tgas = hcm_pow(t0 * (1.0 - sqrt(1.0 / rtot)), 0.25) / 11605.0;
+    k.nub = bbfreq<R>(rng, R(tgas));
+    k.d.cost = R(1.0) - R(2.0) * R(ran(rng));
+    k.d.sint = m_sqrt(R(1.0) - m_pow(k.d.cost, R(2.0)));
+    k.d.phi = R(kR2P) * R(ran(rng));
+    k.d.cosp = m_cos(k.d.phi);
+    k.d.sinp = m_sin(k.d.phi);
+  }
+  set_walk_dir(k);
+  bin_init(k.nub, 4);  // newdisktherm.F90:219-250
+  set_opacity(k, true);
+  k.iflag = 0;
+  k.i_orig = 2;
+  k.on_wall = false;
+  return slot;
+}
+
 // Star / hot-spot launch (initpspot.f90:3-175 or initp_ps.f90:3-79, then
 // newdisktherm.F90:256-377).
 template <class R, int NA>
@@ -1315,9 +1448,16 @@ __global__ __launch_bounds__(256, HC_WAVES_PER_EU) void transport() {
           L.first = true;
           status = -1;
           if constexpr (TRACE) tr = Trace{0.0, 0.0, 0, 0, 0, 0, 0, 14695981039346656037ull};
-          launch_star(k, L.rng, (long long)p + 1 < P.npacc + P.npaccstar, nrm);
-          if constexpr (TRACE) hash_cell(tr, k.ir, k.it, k.ip);
-          push(k, 0, true, L.pid, nrm);
+          const long long iphot = (long long)p + 1;
+          if (iphot < P.npacc) {  // newdisktherm.F90:209
+            const int sl = launch_acc(W, k, L.rng);
+            if constexpr (TRACE) hash_cell(tr, k.ir, k.it, k.ip);
+            push(k, sl, false, L.pid, nrm);
+          } else {
+            launch_star(k, L.rng, iphot < P.npacc + P.npaccstar, nrm);
+            if constexpr (TRACE) hash_cell(tr, k.ir, k.it, k.ip);
+            push(k, 0, true, L.pid, nrm);
+          }
           st = start_walk(true);
         }
       }
@@ -1450,7 +1590,7 @@ struct hc_ctx {
   double tsub = 1600.0;
   // device buffers
   double *r2arr = nullptr, *tan2arr = nullptr, *thetarr = nullptr, *aarr = nullptr,
-         *barr = nullptr, *phiarr = nullptr;
+         *barr = nullptr, *phiarr = nullptr, *costarr = nullptr;
   double *dens = nullptr, *densm = nullptr, *tdust = nullptr;
   double *nudust = nullptr, *kappad = nullptr, *adust = nullptr, *gdust = nullptr,
          *pldust = nullptr, *lTint = nullptr, *nuint = nullptr, *kapBint = nullptr,
@@ -1628,6 +1768,15 @@ int hc_set_grid(hc_ctx *ctx, const double *r2arr, const double *tan2arr, const d
   if (!rc) rc = upload(ctx, &ctx->aarr, aarr, (size_t)d.npg);
   if (!rc) rc = upload(ctx, &ctx->barr, barr, (size_t)d.npg);
   if (!rc) rc = upload(ctx, &ctx->phiarr, phiarr, (size_t)d.npg);
+  {
+    // costarr as gridset.f90:580-599 builds it: exact at the poles (tan2 = 0)
+    // and at a midplane wall (tan2 = -1), cos(theta) elsewhere
+    std::vector<double> ct(d.ntg);
+    for (int i = 0; i < d.ntg; ++i)
+      ct[i] = tan2arr[i] == 0.0 ? (i == 0 ? 1.0 : -1.0)
+                                : (tan2arr[i] == -1.0 ? 0.0 : std::cos(thetarr[i]));
+    if (!rc) rc = upload(ctx, &ctx->costarr, ct.data(), (size_t)d.ntg);
+  }
   if (rc) return rc;
   std::vector<double> tmp;
   to_device_layout(ctx, densarr, tmp);
@@ -1758,8 +1907,14 @@ int hc_set_sources(hc_ctx *ctx, const hc_sources *src) {
                                                      !src->sinpe)))
     return fail(ctx, HC_E_ARG, "hc_set_sources: null array");
   if (src->limb != 0) return fail(ctx, HC_E_UNSUPPORTED, "limb-darkened launch not supported");
-  if (src->accfrac != 0.0)
-    return fail(ctx, HC_E_UNSUPPORTED, "disk-accretion launch not supported");
+  if (src->accfrac < 0.0 || src->accfrac >= 1.0 || src->accfrac2 < 0.0 || src->accfrac2 >= 1.0)
+    return fail(ctx, HC_E_ARG, "hc_set_sources: accfrac and accfrac2 must lie in [0, 1)");
+  if (src->accfrac > 0.0 &&
+      !(src->rddustmin >= 1.0 && src->rmaxdmax > src->rddustmin && src->rmax > 1.0 &&
+        src->massc > 0.0 && src->mdot >= 0.0))
+    return fail(ctx, HC_E_ARG,
+                "hc_set_sources: disk accretion needs 1 <= rddustmin < rmaxdmax, rmax > 1, "
+                "massc > 0, mdot >= 0");
   HCK(hipSetDevice(ctx->device));
   std::vector<double> sini(d.nmu);
   for (int i = 0; i < d.nmu; ++i) sini[i] = std::sqrt(1.0 - src->u[i] * src->u[i]);
@@ -1941,6 +2096,18 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
   P.trace = trace ? ctx->trace : nullptr;
   P.regen_lanes = ctx->regen_lanes;
   P.max_work = ctx->max_work;
+  P.costarr = ctx->costarr;
+  P.rddustmin = S.rddustmin;
+  P.rmaxdmax = S.rmaxdmax;
+  P.rmax = S.rmax;
+  P.massc = S.massc;
+  P.mdot = S.mdot;
+  for (int i = 0; i < kMaxSpecies; ++i) {
+    P.fmass[i] = i < 8 ? S.fmass[i] : 0.0;
+    P.z1[i] = i < 8 ? S.z1[i] : 0.0;
+    P.bflare[i] = i < 8 ? S.bflare[i] : 0.0;
+    P.slot_of[i] = ctx->slot[i];
+  }
 
   void (*kern)() = nullptr;
   size_t rec_bytes = 0;
@@ -2174,6 +2341,7 @@ int hc_destroy(hc_ctx *ctx) {
   if (!ctx) return HC_OK;
   (void)hipSetDevice(ctx->device);
   void *bufs[] = {ctx->r2arr,  ctx->tan2arr, ctx->thetarr, ctx->aarr,    ctx->barr,   ctx->phiarr,
+                  ctx->costarr,
                   ctx->dens,   ctx->densm,   ctx->tdust,   ctx->nudust,  ctx->kappad, ctx->adust,
                   ctx->gdust,  ctx->pldust,  ctx->lTint,   ctx->nuint,   ctx->kapBint, ctx->kappap,
                   ctx->u,      ctx->sini,    ctx->aperture2, ctx->coste, ctx->sinte,  ctx->cospe,

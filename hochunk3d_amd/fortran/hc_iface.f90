@@ -22,6 +22,9 @@ module hc_iface
     real(c_double) :: accfrac, accfrac2, numin, lnurat, rmaxi, ltot
     integer(c_int32_t) :: iintmax, pad
     type(c_ptr) :: u, aperture2, coste, sinte, cospe, sinpe
+    real(c_double) :: rddustmin, rmaxdmax, rmax
+    real(c_double) :: massc, mdot
+    real(c_double) :: fmass(8), z1(8), bflare(8)
   end type hc_sources
 
   type, bind(C) :: hc_tallies

@@ -82,6 +82,7 @@ class ModelConfig:
     # sources
     accfrac2: float = 0.0          # hot-spot luminosity fraction
     tshock: float = 8000.0
+    mdot_disk: float = 0.0         # disk accretion rate, Msun/yr (0: no accretion photons)
     fractal: Optional[dict] = None
     dust_table: Optional[np.ndarray] = field(default=None, repr=False)
 
@@ -119,6 +120,14 @@ class Model:
     sinpe: np.ndarray
     rmaxi: float
     autors: float
+    # disk-accretion launch (initpacc.f90, gridset.f90:72-143)
+    accfrac: float = 0.0
+    mdot: float = 0.0              # g/s
+    rddustmin: float = 1.0
+    rmaxdmax: float = 1.0
+    fmass8: tuple = (0.0,) * 8
+    z1_8: tuple = (0.0,) * 8
+    bflare8: tuple = (0.0,) * 8
 
     @property
     def ncells(self) -> int:
@@ -228,10 +237,12 @@ def build_model(cfg: ModelConfig) -> Model:
     # flared disks (densdiskg.f90 with CZMIN = R100: z1 = h100 / 100AU^b)
     rp = R * sint
     zp = R * cost
+    fm8, z18, b8 = [0.0] * NDG, [0.0] * NDG, [0.0] * NDG
     for id_, (fm, zr) in enumerate(zip(cfg.fmass, cfg.zscale_rel)):
         if fm <= 0.0:
             continue
         z1 = zr * cfg.h100_au * autors / (100.0 * autors) ** cfg.b_exp
+        fm8[id_], z18[id_], b8[id_] = float(fm), float(z1), float(cfg.b_exp)
         zmr = z1 * rp ** cfg.b_exp
         gexp = (zp / zmr) ** 2
         shape = rp ** (-cfg.a_exp) * np.where(gexp < 120.0, np.exp(-0.5 * gexp), 0.0)
@@ -279,9 +290,14 @@ def build_model(cfg: ModelConfig) -> Model:
     table = cfg.dust_table if cfg.dust_table is not None else synthetic_dust_table()
     dust = dustinit(table)
 
-    # luminosity and binning (gridset.f90 lstar, setup.f90:36-40, initarr.f90:24-28)
+    # luminosity and binning (gridset.f90 lstar, setup.f90:36-40, initarr.f90:24-28).
+    # Disk accretion (gridset.f90:123,143): lacc = G M mdot / (2 Rmin,dust);
+    # the hot spot keeps the fraction accfrac2 of the total.
     lstar = 4.0 * PI * SIGT * cfg.tstar ** 4 * rmincgs ** 2
-    ltot = lstar
+    mdot = cfg.mdot_disk * MSOL / 3.15576e7
+    lacc = GN * (MSOL * cfg.massc) * mdot / 2.0 / (rmind * rmincgs)
+    ltot = (lstar + lacc) / (1.0 - cfg.accfrac2)
+    accfrac = lacc / ltot
     numin = 1.2398 / 5000.0
     numax = 1.2398 / 0.01
     lnurat = np.log(numax / numin)
@@ -304,7 +320,8 @@ def build_model(cfg: ModelConfig) -> Model:
     return Model(cfg, nrg, ntg, npg, rarr, r2arr, thetarr, tan2arr, phiarr, aarr, barr, densarr,
                  vcell, massarr, tdust, dust, cfg.rstar, tstar_ev, cfg.tshock / 11605.0, ltot,
                  numin, float(lnurat), u, aperture2, np.cos(thete), np.sin(thete), np.cos(phie),
-                 np.sin(phie), cfg.rmaxi_au * autors, autors)
+                 np.sin(phie), cfg.rmaxi_au * autors, autors, accfrac, mdot, rmind, rmaxd,
+                 tuple(fm8), tuple(z18), tuple(b8))
 
 
 def config1(**over) -> ModelConfig:

@@ -66,6 +66,14 @@ typedef struct {
   const double *u;         /* [nmu] cos(theta) bin centres (initarr.f90:24-28) */
   const double *aperture2; /* [nap] squared aperture radii, stellar radii^2 */
   const double *coste, *sinte, *cospe, *sinpe; /* [npeel] observer directions */
+  /* disk-accretion launch (initpacc.f90, emit_gasacc.f90), used when
+   * accfrac > 0: emitting radii rddustmin < r < rmaxdmax, the grid's outer
+   * radius rmax (stellar radii); per species fmass, z1 (scale height at
+   * r = 1) and b (flaring exponent) of the flared disks; the stellar mass
+   * massc (Msun) and the disk accretion rate mdot (g/s) of the gas emission */
+  double rddustmin, rmaxdmax, rmax;
+  double massc, mdot;
+  double fmass[8], z1[8], bflare[8];
 } hc_sources;
 
 /* Caller-owned host outputs written by hc_run_iteration.  Any pointer may be

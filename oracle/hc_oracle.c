@@ -43,6 +43,8 @@
 #define R2P (2.0 * PI)
 #define RSOL 6.9598e10                   /* constants.f90:11 */
 #define SIGT 5.670400e-5                 /* constants.f90:21 */
+#define GN 6.67428e-8                    /* constants.f90:19 */
+#define MSOL 1.989e33                    /* constants.f90:13 */
 #define HUGE_T DBL_MAX
 
 /* ------------------------------------------------------------------------ */
@@ -1280,6 +1282,175 @@ static void peel_direct(ctx_t *c, pkt_t *P, double cosb, double sinb, double cos
   }
 }
 
+/* Species masks of grid_mod.f90:15-20 (0-based ids). */
+static const int IS_DISK[HCO_NDG] = {1, 1, 0, 0, 1, 1, 0, 0};
+static const int IS_THERMAL_DISK[HCO_NDG] = {1, 1, 0, 0, 0, 0, 0, 0};
+
+/* rantheta, random.f90:28-47: a uniform azimuth from a point in the unit
+ * disk (sin and cos of it). */
+static void rantheta(hco_rng *r, double *st, double *ct) {
+  double x, y, x2, y2, r2 = 2.0, rr;
+  while (r2 > 1.0) {
+    x = hco_ran(r);
+    x = x + x - 1.0;
+    y = hco_ran(r);
+    y = y + y - 1.0;
+    x2 = x * x;
+    y2 = y * y;
+    r2 = x2 + y2;
+  }
+  rr = sqrt(r2);
+  *st = y / rr;
+  *ct = x / rr;
+}
+
+/* gasdev, random.f90:140-159 (polar Box-Muller).  The reference keeps the
+ * second deviate of a pair for its next call, across packets; each packet
+ * here draws from its own generator segment, so every call draws a fresh
+ * pair and returns the deviate the reference returns first. */
+static double gasdev(hco_rng *r) {
+  double v1, v2, rsq, fac;
+  for (;;) {
+    v1 = 2.0 * hco_ran(r) - 1.0;
+    v2 = 2.0 * hco_ran(r) - 1.0;
+    rsq = v1 * v1 + v2 * v2;
+    if (rsq < 1.0 && rsq > 0.0) break;
+  }
+  fac = sqrt(-2.0 * HCO_LOG(rsq) / rsq);
+  return v2 * fac;
+}
+
+void hco_acc_position(const hco_setup *s, hco_rng *r, double *xyz) {
+  double rp, winv, sp, cp, zsum, zp;
+  int id;
+  for (;;) {
+    rp = 0.0;
+    while (rp >= s->rmaxdmax || rp <= s->rddustmin) {
+      winv = hco_ran(r);
+      while (hco_ran(r) > 1.0 - sqrt(winv)) winv = hco_ran(r);
+      rp = 1.0 / winv;
+    }
+    rantheta(r, &sp, &cp);
+    xyz[0] = rp * cp;
+    xyz[1] = rp * sp;
+    zsum = 0.0;
+    for (id = 0; id < HCO_NDG; id++)
+      if (IS_DISK[id]) zsum = zsum + s->fmass[id] * s->z1[id] * HCO_POW(rp, s->bflare[id]);
+    zp = zsum * gasdev(r);
+    xyz[2] = zp;
+    if (zp * zp + rp * rp < s->rmax * s->rmax) break;
+  }
+}
+
+/* costarr of gridset.f90:580-599: exact at the poles and at a midplane
+ * wall, the C library's cos elsewhere (as the library computes it). */
+static double cost_wall(const hco_grid *g, int it) {
+  if (g->tan2arr[it] == 0.0) return it == 0 ? 1.0 : -1.0;
+  if (g->tan2arr[it] == -1.0) return 0.0;
+  return cos(g->thetarr[it]);
+}
+
+static int locate_cost(const hco_grid *g, double cosb) {  /* nr.f:131-153 on costarr */
+  int jl = 0, ju = g->ntg + 1, jm;
+  while (ju - jl > 1) {
+    jm = (ju + jl) / 2;
+    if ((cost_wall(g, g->ntg - 1) > cost_wall(g, 0)) == (cosb > cost_wall(g, jm - 1)))
+      jl = jm;
+    else
+      ju = jm;
+  }
+  if (jl == 0 && cosb == cost_wall(g, 0)) jl = 1;
+  return jl;
+}
+
+static void set_kapd_launch(ctx_t *c, pkt_t *P);
+static void bin_init(ctx_t *c, double nub, int i_orig_init);
+static void peel_all(ctx_t *c, const pkt_t *P, double sip, int idust2);
+
+/* Disk-accretion launch: initpacc.f90:3-122 without the diffusion layer,
+ * emit_common.f90:25-78 (Lucy branch) or emit_gasacc.f90:1-44, then
+ * newdisktherm.F90:219-250.  The reference picks the emitting species with
+ * the kapd the previous packet left (select_dust reads the module global);
+ * a packet-independent restatement weights the thermal disk species by
+ * kappa_V(id) * rho(id), which is that choice whenever the species share one
+ * dust table (kapd(id) = kappa(nu) R* kappa_V(id) for any nu). */
+static void launch_acc(ctx_t *c, pkt_t *P) {
+  const hco_setup *s = c->s;
+  const hco_grid *g = c->g;
+  const hco_dust *d = c->d;
+  hco_tallies *t = c->t;
+  double xyz[3], cosb, lp, dsum = 0.0, opac = 0.0, frac = 0.0, xran, tgas, t0;
+  int ir, it, ip, id, idust2 = -1;
+  size_t cell;
+  hco_acc_position(s, &c->rng, xyz);
+  P->xp = xyz[0];
+  P->yp = xyz[1];
+  P->zp = xyz[2];
+  P->sip = 1.0;
+  P->sqp = P->sup = P->svp = 0.0;
+  P->rsq = P->xp * P->xp + P->yp * P->yp + P->zp * P->zp;
+  P->rtot = sqrt(P->rsq);
+  ir = hco_locate(g->r2arr, g->nrg, P->rsq);
+  cosb = P->zp / P->rtot;
+  it = locate_cost(g, cosb);
+  lp = HCO_ATAN2(P->yp, P->xp);
+  if (lp < 0.0) lp = lp + R2P;
+  ip = hco_locate(g->phiarr, g->npg, lp);
+  P->ir = ir;
+  P->it = it;
+  P->ip = ip;
+  cell = cell_index(g, ir, it, ip);
+  for (id = 0; id < HCO_NDG; id++)
+    if (IS_THERMAL_DISK[id]) dsum = dsum + g->densarr[cell + c->ncell * id];
+  if (dsum > 0.0) {  /* select_dust(is_thermal_disk), select_dust.f90:17-42 */
+    xran = hco_ran(&c->rng);
+    for (id = 0; id < HCO_NDG; id++)
+      if (IS_THERMAL_DISK[id]) opac = opac + d->kappav[id] * g->densarr[cell + c->ncell * id];
+    for (id = 0; id < HCO_NDG; id++) {
+      if (IS_THERMAL_DISK[id]) {
+        frac = frac + d->kappav[id] * g->densarr[cell + c->ncell * id] / opac;
+        if (xran <= frac + (double)1.e-10f) {
+          idust2 = id;
+          break;
+        }
+      }
+    }
+  }
+  if (idust2 >= 0) {  /* initpacc.f90:95-115 (the thermal disk species are not VSG/PAH) */
+    if (t->dbig) t->dbig[cell] += 1;
+    if (t->nabs) t->nabs[cell + c->ncell * idust2] += 1.0;
+    /* emit_common.f90:57-70 -> reemit.f90:15-33 at the cell's temperature */
+    P->cost = 1.0 - 2.0 * hco_ran(&c->rng);
+    P->sint = sqrt(1.0 - HCO_POW(P->cost, 2.0));
+    P->phi = R2P * hco_ran(&c->rng);
+    P->cosp = HCO_COS(P->phi);
+    P->sinp = HCO_SIN(P->phi);
+    P->nub = hco_lucydustfreq(d, &c->rng, g->tdust[cell + c->ncell * idust2], idust2);
+  } else {
+    /* emit_gasacc.f90:22-42: Pringle (1981) surface temperature */
+    t0 = 3.0 * GN * (MSOL * s->massc) * s->mdot /
+         (8.0 * PI * SIGT * HCO_POW(P->rtot * s->rstar * RSOL, 3.0));
+    tgas = HCO_POW(t0 * (1.0 - sqrt(1.0 / P->rtot)), 0.25) / 11605.0;
+    P->nub = hco_bbfreq(&c->rng, tgas);
+    P->cost = 1.0 - 2.0 * hco_ran(&c->rng);
+    P->sint = sqrt(1.0 - HCO_POW(P->cost, 2.0));
+    P->phi = R2P * hco_ran(&c->rng);
+    P->cosp = HCO_COS(P->phi);
+    P->sinp = HCO_SIN(P->phi);
+  }
+  P->uy = P->sint * P->sinp;
+  P->ux = P->sint * P->cosp;
+  P->uz = P->cost;
+  /* newdisktherm.F90:219-250 */
+  hco_opacset(d, P->nub, &P->op);
+  bin_init(c, P->nub, 4);
+  set_kapd_launch(c, P);
+  P->iflag = 0;
+  P->i_orig = 2;
+  P->on_wall = 0;
+  if (c->ipeel) peel_all(c, P, P->sip, idust2 < 0 ? 0 : idust2);
+}
+
 /* Star / hot-spot launch: initpspot.f90:3-175 (or initp_ps.f90:3-79 for the
  * point-source mode), then newdisktherm.F90:256-337.  hotspot selects the
  * hot-spot branch of the photon loop (newdisktherm.F90:256-273): the packet
@@ -1648,7 +1819,6 @@ int hco_run_pass(const hco_grid *g, const hco_dust *d, const hco_setup *s, hco_t
   npacc = (int64_t)((double)np * s->accfrac);
   npaccstar = (int64_t)(s->accfrac2 * (double)np);
   if (ns + npacc != np) ns = ns + np - (ns + npacc);
-  if (npacc > 1) return -1; /* disk-accretion launch is not restated */
   for (p = p_begin; p < p_end; p++) {
     pkt_t P;
     int64_t iphot = p + 1;
@@ -1660,7 +1830,10 @@ int hco_run_pass(const hco_grid *g, const hco_dust *d, const hco_setup *s, hco_t
       memset(c.tr, 0, sizeof *c.tr);
       c.tr->cellhash = 14695981039346656037ull;
     }
-    launch_star(&c, &P, iphot < npacc + npaccstar ? 1 : 0);
+    if (iphot < npacc)  /* newdisktherm.F90:209 */
+      launch_acc(&c, &P);
+    else
+      launch_star(&c, &P, iphot < npacc + npaccstar ? 1 : 0);
     hash_cell(c.tr, P.ir, P.it, P.ip);
     status = propagate(&c, &P);
     finish_trace(&c, &P, status);

@@ -71,6 +71,10 @@ typedef struct {
   int imcube;
   int iintmax;                  /* 10000 */
   double ltot;                  /* erg/s */
+  /* disk-accretion launch (initpacc.f90, emit_gasacc.f90) */
+  double rddustmin, rmaxdmax, rmax;  /* stellar radii */
+  double massc, mdot;                /* Msun, g/s */
+  double fmass[HCO_NDG], z1[HCO_NDG], bflare[HCO_NDG];
 } hco_setup;
 
 typedef struct {
@@ -124,6 +128,8 @@ void hco_phiface(hco_walls *w, double a1, double b1, double a2, double b2,
                  double ux, double uy, double x, double y);
 int hco_find_next_wall(const hco_walls *w, int on_wall, double *t, int *face);
 int hco_locate(const double *xx, int n, double x);
+/* initpacc.f90:27-52: the emission point of a disk-accretion packet. */
+void hco_acc_position(const hco_setup *s, hco_rng *r, double *xyz);
 int hco_walk_ray(const hco_grid *g, double x, double y, double z, double ux, double uy, double uz,
                  int *ir, int *it, int *ip, int on_wall);
 double hco_bbfreq(hco_rng *r, double teff);

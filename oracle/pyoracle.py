@@ -48,7 +48,10 @@ class Setup(C.Structure):
         ("npeel", C.c_int), ("nxhst", C.c_int), ("numin", C.c_double), ("lnurat", C.c_double),
         ("u", _dp), ("aperture2", _dp), ("coste", _dp), ("sinte", _dp), ("cospe", _dp),
         ("sinpe", _dp), ("rmaxi", C.c_double), ("imcube", C.c_int), ("iintmax", C.c_int),
-        ("ltot", C.c_double)]
+        ("ltot", C.c_double),
+        ("rddustmin", C.c_double), ("rmaxdmax", C.c_double), ("rmax", C.c_double),
+        ("massc", C.c_double), ("mdot", C.c_double),
+        ("fmass", C.c_double * NDG), ("z1", C.c_double * NDG), ("bflare", C.c_double * NDG)]
 
 
 class Tallies(C.Structure):
@@ -127,6 +130,7 @@ def load(variant: str = "libm") -> C.CDLL:
         lib.hco_walk_ray.argtypes = ([C.POINTER(Grid)] + [C.c_double] * 6 +
                                      [C.POINTER(C.c_int)] * 3 + [C.c_int])
         lib.hco_walk_ray.restype = C.c_int
+        lib.hco_acc_position.argtypes = [C.POINTER(Setup), C.POINTER(Rng), _dp]
         lib.hco_bbfreq.argtypes = [C.POINTER(Rng), C.c_double]
         lib.hco_bbfreq.restype = C.c_double
         lib.hco_lucydustfreq.argtypes = [C.POINTER(Dust), C.POINTER(Rng), C.c_double, C.c_int]
@@ -186,6 +190,11 @@ class Oracle:
         s.iplanckst = 1
         s.nspot = 1
         s.accfrac2 = c.accfrac2
+        s.accfrac = m.accfrac
+        s.rddustmin, s.rmaxdmax, s.rmax = m.rddustmin, m.rmaxdmax, float(m.rarr[-1])
+        s.massc, s.mdot = c.massc, m.mdot
+        for i in range(NDG):
+            s.fmass[i], s.z1[i], s.bflare[i] = m.fmass8[i], m.z1_8[i], m.bflare8[i]
         s.nfreq, s.nmu, s.nph, s.nap, s.npeel, s.nxhst = (c.nfreq, c.nmu, c.nph, c.nap, c.npeel,
                                                           c.nximg)
         s.numin, s.lnurat = m.numin, m.lnurat

@@ -104,3 +104,38 @@ def test_device_resident_pass_feeds_temperature(pair):
     rel_close(t2g, t2c, rtol=1e-9)
     assert nchc > 0
     assert abs(nchg - nchc) <= max(2, nchc // 1000)
+
+
+@pytest.fixture(scope="module")
+def acc_pair():
+    """The same model with disk accretion: about a fifth of the packets are
+    accretion photons (initpacc.f90), emitted inside the disk."""
+    from hochunk3d_amd._lib import Transport
+    from oracle.pyoracle import Oracle
+    m = build_model(config1(nximg=41, nfreq=100, mdot_disk=3e-6))
+    m.tdust[...] = synthetic_temperature(m)
+    assert m.accfrac > 0.1
+    g = Transport(m, device=0)
+    yield g, Oracle(m)
+    g.close()
+
+
+def test_accretion_lucy_pass_matches_oracle(acc_pair):
+    g, c = acc_pair
+    n = 8000
+    rg = g.run(n, ipeel=0, trace=True)
+    rc = c.run(n, ipeel=0, trace=True)
+    compare_traces(rg.trace, rc["trace"])
+    _check_tallies(rg, rc)
+    # accretion photons bin into the disk-origin slot of the initial spectrum
+    assert rg.si_init[:, 3].sum() > 0
+
+
+def test_accretion_peeled_pass_matches_oracle(acc_pair):
+    g, c = acc_pair
+    n = 1000
+    rg = g.run(n, ipeel=1, trace=True)
+    rc = c.run(n, ipeel=1, trace=True)
+    compare_traces(rg.trace, rc["trace"])
+    _check_tallies(rg, rc)
+    rel_close(rg.peel, rc["peel"], rtol=1e-9, atol=1e-12)
