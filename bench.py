@@ -122,6 +122,8 @@ def main():
     ap.add_argument("--warmup-np", type=float, default=0,
                     help="packets per warm-up iteration (default a quarter of --np)")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 path")
+    ap.add_argument("--peel-np", type=float, default=100_000,
+                    help="packets of the final peeled pass timed after the Lucy steps (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", type=float, default=0,
@@ -173,9 +175,26 @@ def main():
     t_start = tr.get_temperature()  # where the timed steps start
 
     wall64, kms64, tot64, sed64 = timed_steps(tr, args.steps, np_, seg, "fp64")
-    tr.close()
     npk = np_ * args.steps
     value = npk / wall64
+
+    # the final pass of the run (newtts.F90:347-349): peel-off towards every
+    # observer from every emission and scattering, images on
+    peel = None
+    if args.peel_np > 0:
+        pnp = int(args.peel_np)
+        t0 = time.perf_counter()
+        s, ms = tr.run_scalars(pnp, ipeel=1, seg0=seg + npk)
+        wallp = time.perf_counter() - t0
+        peel = {"packets": pnp, "value": round(pnp / wallp, 2), "unit": "packets/s",
+                "kernel_ms": round(ms, 3),
+                "peel_rays_per_packet": round(s["npeelray"] / pnp, 1),
+                "peel_crossings_per_packet": round(s["npeelcross"] / pnp, 1),
+                "observers": cfg.npeel, "image_pixels": f"{cfg.nximg}x{cfg.nximg}"}
+        log(f"peeled pass: {pnp} packets in {wallp * 1e3:.0f} ms ({pnp / wallp:.3e} packets/s), "
+            f"{s['npeelray'] / pnp:.0f} peel rays and {s['npeelcross'] / pnp:.0f} peel-ray "
+            f"crossings per packet")
+    tr.close()
 
     fp32 = None
     if not args.no_fp32:
@@ -246,6 +265,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "fp32_path": fp32,
+        "peeled_pass_fp64": peel,
     }
     print(json.dumps(line), flush=True)
 
