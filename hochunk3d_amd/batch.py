@@ -1,0 +1,125 @@
+"""Batched parameter-grid runs: many models' Lucy iterations in flight at once.
+
+A pass cannot end before its longest packet, a random walk of 1e5-1e6 events
+in an optically thick disk, and while that packet finishes most of the GPU is
+idle.  Independent models (a parameter-grid fit) fill that time: each model
+gets its own context, and with it its own HIP stream, and a pool of host
+threads keeps `streams` models' passes queued, so the blocks of one model's
+pass run beside another model's tail.  Every model's results are those of
+running it alone (its packets draw from their own generator segments); only
+the order of the floating-point tally sums may differ.
+
+The model parameters follow the parameter-grid configuration of the build's
+benchmark set: radial cells uniform in 50-250, theta cells 60-200, phi cells
+1-128, disk mass log-uniform 1e-4-0.1 Msun, infall rate log-uniform
+1e-7-1e-4 Msun/yr, cavity opening 5-40 deg, stellar temperature 3000-10000 K,
+packets per model log-uniform 1e6-2e7.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import time
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from .model import ModelConfig, build_model
+
+
+@dataclass
+class GridModel:
+    cfg: ModelConfig
+    packets: int
+
+
+def parameter_grid(n: int, seed: int = 4, packet_scale: float = 1.0,
+                   max_cells: Optional[int] = None) -> List[GridModel]:
+    """n model configurations drawn from the parameter-grid ranges (fixed
+    seed).  packet_scale multiplies the drawn packet counts (a bounded
+    sample of the grid); max_cells redraws models with more cells."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < n:
+        nr = int(rng.integers(50, 251))
+        nth = int(rng.integers(60, 201))
+        nph = int(rng.integers(1, 129))
+        if max_cells is not None and nr * nth * nph > max_cells:
+            continue
+        cfg = ModelConfig(
+            nr=nr, ntheta=nth, nphi=nph,
+            massd=float(10.0 ** rng.uniform(-4.0, -1.0)),
+            rate=float(10.0 ** rng.uniform(-7.0, -4.0)),
+            cavity_deg=float(rng.uniform(5.0, 40.0)),
+            tstar=float(rng.uniform(3000.0, 10000.0)),
+            npeel=1, nximg=1, imcube=False, nfreq=50, nmu=10)
+        npk = int(10.0 ** rng.uniform(6.0, np.log10(2.0e7)) * packet_scale)
+        out.append(GridModel(cfg, max(npk, 1)))
+    return out
+
+
+@dataclass
+class BatchResult:
+    packets: int
+    wall_s: float          # the Lucy iterations of the whole batch
+    setup_s: float         # building the models and uploading them
+    kernel_ms: float       # summed over models (passes overlap)
+    per_model: list = field(default_factory=list)
+
+    @property
+    def rate(self) -> float:
+        return self.packets / self.wall_s
+
+
+def build_models(models: List[GridModel], workers: int = 4) -> list:
+    """The grids, densities and dust of every model (host arrays)."""
+    with cf.ThreadPoolExecutor(max_workers=max(1, workers)) as pool:
+        return list(pool.map(lambda gm: build_model(gm.cfg), models))
+
+
+def _iterate(tr, packets: int, iterations: int):
+    kms, seg, scal = 0.0, 0, []
+    for _ in range(iterations):
+        s, ms = tr.run_scalars(packets, seg0=seg)
+        seg += packets
+        tr.temperature(packets, apply=True)
+        kms += ms
+        scal.append(s)
+    return kms, scal
+
+
+def run_batch(models: List[GridModel], iterations: int = 3, streams: int = 4,
+              precision: int = 0, options: Optional[dict] = None,
+              built: Optional[list] = None) -> BatchResult:
+    """Lucy iterations of every model from its initial temperatures.
+
+    Every model is built (unless `built` holds build_models' result) and
+    uploaded first, and all of them stay resident in HBM; then a pool of
+    `streams` host threads runs the models' iterations, largest packet count
+    first, each model on its own context and stream.  `wall_s` covers the
+    iterations only."""
+    from ._lib import Transport
+    t0 = time.perf_counter()
+    if built is None:
+        built = build_models(models)
+    trs = [Transport(m, precision=precision) for m in built]
+    for tr in trs:
+        for k, v in (options or {}).items():
+            tr.set_option(k, v)
+    setup = time.perf_counter() - t0
+    order = sorted(range(len(models)), key=lambda i: -models[i].packets)
+    try:
+        t1 = time.perf_counter()
+        with cf.ThreadPoolExecutor(max_workers=max(1, streams)) as pool:
+            futs = {i: pool.submit(_iterate, trs[i], models[i].packets, iterations)
+                    for i in order}
+            res = {i: f.result() for i, f in futs.items()}
+        wall = time.perf_counter() - t1
+        per = [{"packets": models[i].packets * iterations, "kernel_ms": res[i][0],
+                "scalars": res[i][1], "tdust": trs[i].get_temperature()}
+               for i in range(len(models))]
+    finally:
+        for tr in trs:
+            tr.close()
+    return BatchResult(sum(p["packets"] for p in per), wall, setup,
+                       sum(p["kernel_ms"] for p in per), per)

@@ -84,21 +84,18 @@ struct Walls {
     tp2 = kHuge;
     in1 = ip1 = ip2 = 0;
   }
+  // closest_wall.f90:24-45 as selects (a NaN root changes nothing)
   __device__ __forceinline__ void insert(double t, int i) {
-    if (t < 0.0) {
-      if (t > tn1) {
-        tn1 = t;
-        in1 = i;
-      }
-    } else if (t < tp1) {
-      tp2 = tp1;
-      ip2 = ip1;
-      tp1 = t;
-      ip1 = i;
-    } else if (t < tp2) {
-      tp2 = t;
-      ip2 = i;
-    }
+    const bool neg = t < 0.0;
+    const bool c0 = neg && t > tn1;
+    const bool c1 = !neg && t < tp1;
+    const bool c2 = !neg && !c1 && t < tp2;
+    tn1 = c0 ? t : tn1;
+    in1 = c0 ? i : in1;
+    tp2 = c1 ? tp1 : (c2 ? t : tp2);
+    ip2 = c1 ? ip1 : (c2 ? i : ip2);
+    tp1 = c1 ? t : tp1;
+    ip1 = c1 ? i : ip1;
   }
   // find_next_wall: on a wall, skip the root that is that wall.
   __device__ __forceinline__ bool next(bool on_wall, double &t, int &face) const {
@@ -113,23 +110,22 @@ struct Walls {
   }
 };
 
+// radface.f90:44-62.  A ray that misses a sphere (descr <= 0) inserts NaN
+// roots, which change nothing.
 __device__ __forceinline__ void radface(Walls &w, double ux, double uy, double uz, double x,
                                         double y, double z, double r2a, double r2b) {
+  const double nan = __builtin_nan("");
   const double bb = 2.0 * (x * ux + y * uy + uz * z);
   double cc = x * x + y * y + z * z - r2a;
   double descr = bb * bb - 4.0 * cc;
-  if (descr > 0.0) {
-    descr = sqrt(descr);
-    w.insert((-bb + descr) * 0.5, 1);
-    w.insert((-bb - descr) * 0.5, 1);
-  }
+  descr = descr > 0.0 ? sqrt(descr) : nan;
+  w.insert((-bb + descr) * 0.5, 1);
+  w.insert((-bb - descr) * 0.5, 1);
   cc = x * x + y * y + z * z - r2b;
   descr = bb * bb - 4.0 * cc;
-  if (descr > 0.0) {
-    descr = sqrt(descr);
-    w.insert((-bb + descr) * 0.5, 2);
-    w.insert((-bb - descr) * 0.5, 2);
-  }
+  descr = descr > 0.0 ? sqrt(descr) : nan;
+  w.insert((-bb + descr) * 0.5, 2);
+  w.insert((-bb - descr) * 0.5, 2);
 }
 
 // One theta wall: cone x^2+y^2 = t2 z^2 (t2 > 0), the midplane (t2 = -1) or
@@ -179,12 +175,16 @@ __device__ __forceinline__ void thetaface(Walls &w, double ux, double uy, double
   }
 }
 
+// phiface.f90:43-57; a ray parallel to a plane inserts a NaN root.
 __device__ __forceinline__ void phiface(Walls &w, double a1, double b1, double a2, double b2,
                                         double ux, double uy, double x, double y) {
+  const double nan = __builtin_nan("");
   const double d1 = (a1 * ux + b1 * uy);
-  if (d1 != 0.0) w.insert(-(a1 * x + b1 * y) / d1, 5);
+  const double t1 = -(a1 * x + b1 * y) / d1;
+  w.insert(d1 != 0.0 ? t1 : nan, 5);
   const double d2 = (a2 * ux + b2 * uy);
-  if (d2 != 0.0) w.insert(-(a2 * x + b2 * y) / d2, 6);
+  const double t2 = -(a2 * x + b2 * y) / d2;
+  w.insert(d2 != 0.0 ? t2 : nan, 6);
 }
 
 // nr.f:131-153 bisection.

@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -71,6 +72,7 @@ struct Params {
   hc_trace *trace;
   // launch shape
   int regen_lanes;  // waiting lanes (1..64) of one event kind that end a walk phase
+  int regen_run_lanes;  // then every kind with this many waiters runs (<= regen_lanes)
   int peel_q;       // peel records per wavefront queue
   int walls_lds;    // bytes of LDS holding the wall arrays
   int opac_base;    // LDS byte offset of the opacity tables
@@ -86,9 +88,22 @@ struct Params {
   uint32_t max_work;  // events + crossings after which a packet is dropped
 };
 
-// Launch parameters live in constant memory: every wave reads them with
-// scalar loads and no function has to carry them.
-__constant__ Params c_P;
+// Launch parameters are the transport kernel's argument: every wave reads
+// them from the kernel-argument segment with scalar loads, no inlined helper
+// has to carry them, and passes of several contexts can run at once on their
+// own streams.  Only code inlined into the kernel may call kernel_params():
+// a function the compiler emits as a call does not receive the segment's
+// address (the builtin yields a null pointer there), so every function marked
+// noinline takes what it needs as arguments.
+static_assert(sizeof(Params) <= 4096, "launch parameters exceed the kernel-argument limit");
+__device__ __forceinline__ const Params &kernel_params() {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return *reinterpret_cast<const Params *>(__builtin_amdgcn_kernarg_segment_ptr());
+#else
+  return *static_cast<const Params *>(nullptr);  // the host pass never runs device code
+#endif
+}
+#define c_P (::hc::kernel_params())
 
 enum Scalar {
   S_TOT = 0, S_FLUX, S_SFLUX, S_NSCAT, S_SCOUNT, S_ABFLUX, S_AFLUX, S_SUMSUB, S_SUMALL,
@@ -208,12 +223,19 @@ __device__ __forceinline__ bool next_wall(const WallView &W, double ux, double u
 
 // The phi cell a ray leaving the polar axis enters: the one holding the
 // azimuth of its direction (unchanged when the ray runs along the axis).
-__device__ __attribute__((noinline)) int axis_phi_cell(double ux, double uy, int ip) {
-  const Params &P = c_P;
-  if (P.npg <= 2 || (ux == 0.0 && uy == 0.0)) return ip;
+// Not inlined (a rare path), so it takes the phi walls as arguments: the
+// launch parameters are read from the kernel-argument segment, whose address
+// only the kernel's own code has.
+__device__ __attribute__((noinline)) int axis_phi_cell_call(const double *phiarr, int npg,
+                                                            double ux, double uy, int ip) {
+  if (npg <= 2 || (ux == 0.0 && uy == 0.0)) return ip;
   double p = hcm_atan2(uy, ux);
   if (p < 0.0) p = p + kR2P;
-  return locate(P.phiarr, P.npg, p);
+  return locate(phiarr, npg, p);
+}
+
+__device__ __forceinline__ int axis_phi_cell(double ux, double uy, int ip) {
+  return axis_phi_cell_call(c_P.phiarr, c_P.npg, ux, uy, ip);
 }
 
 __device__ __forceinline__ void face_step(int face, int &ir, int &it, int &ip) {
@@ -471,8 +493,8 @@ __device__ __forceinline__ double rec_opac(const PeelRec<R, NA> &k, const double
 // peeloff_3d.f90:5-288 for observer `peelid`.  sip is the peel weight; the
 // remaining Stokes components come from the packet.
 template <class R, int NA>
-__device__ __attribute__((noinline)) PeelOut peeloff(const WallView &W, const PeelRec<R, NA> &k,
-                                                     double sip, int peelid) {
+__device__ __forceinline__ PeelOut peeloff(const WallView &W, const PeelRec<R, NA> &k, double sip,
+                                           int peelid) {
   const Params &P = c_P;
   PeelOut out{0.0, 0, 0};
   Stokes4<R> s{R(sip), k.s.Q, k.s.U, k.s.V};
@@ -1217,8 +1239,15 @@ __device__ __forceinline__ void flush_peels(const WallView &W, const PeelRec<R, 
 //    lanes at once; peel-off records queued by the events are walked once the
 //    wavefront's queue fills.
 // A packet's history and its random numbers are those of the serial loop.
+#ifndef HC_BLOCK_LUCY
+#define HC_BLOCK_LUCY 256  // lanes per workgroup of the passes without peel-off
+#endif
+constexpr int kBlockPeel = 256;  // four wavefront peel queues per workgroup
+constexpr int kBlockLucy = HC_BLOCK_LUCY;
+
 template <class R, int NA, bool TRACE, bool PEEL>
-__global__ __launch_bounds__(256, HC_WAVES_PER_EU) void transport() {
+__global__ __launch_bounds__(PEEL ? kBlockPeel : kBlockLucy, HC_WAVES_PER_EU) void transport(
+    Params /* read through kernel_params() */) {
   const Params &P = c_P;
   const int wave = threadIdx.x >> 6;
   unsigned char *const lds = hc_lds;
@@ -1283,6 +1312,7 @@ __global__ __launch_bounds__(256, HC_WAVES_PER_EU) void transport() {
   for (int s2 = 0; s2 < NA; ++s2) dm[s2] = 0.0;
   int st = ST_END, status = -1, slot = 0;  // status < 0: no packet yet
   const int ev_min = P.regen_lanes;
+  const int ev_run = min(P.regen_run_lanes, ev_min);
 
   // After a walk step: an interaction draws its species and the
   // scatter/absorb test (select_dust.f90, propagate.f90:60-104); the packet
@@ -1348,10 +1378,12 @@ __global__ __launch_bounds__(256, HC_WAVES_PER_EU) void transport() {
       const int n_a = __popcll(__ballot(st == ST_ABS));
       const int n_t = __popcll(__ballot(st == ST_STAR));
       const int n_e = __popcll(__ballot(st == ST_END));
+      // once one kind has ev_min waiters, every kind with ev_run waiters runs
       const bool all = walkers == 0ull;
+      const bool trig = n_s >= ev_min || n_a >= ev_min || n_t >= ev_min || n_e >= ev_min;
+      const int lim = all ? 1 : (trig ? ev_run : ev_min);
       Pkt<R, NA> &k = L.k;
-      if (n_s > 0 && (all || n_s >= ev_min) && st == ST_SCAT) {
-        const int idust2 = P.act[slot] - 1;
+      if (n_s > 0 && n_s >= lim && st == ST_SCAT) {
         k.iflag = 1;
         ++L.iscat;
         ++L.iint;
@@ -1367,7 +1399,7 @@ __global__ __launch_bounds__(256, HC_WAVES_PER_EU) void transport() {
         L.pip = k.ip;
         st = start_walk(false);
       }
-      if (n_a > 0 && (all || n_a >= ev_min) && st == ST_ABS) {
+      if (n_a > 0 && n_a >= lim && st == ST_ABS) {
         const int idust2 = P.act[slot] - 1;
         const size_t cell = cell_of(k.ir, k.it, k.ip);
         L.work++;
@@ -1396,14 +1428,14 @@ __global__ __launch_bounds__(256, HC_WAVES_PER_EU) void transport() {
         L.pip = k.ip;
         st = start_walk(false);
       }
-      if (n_t > 0 && (all || n_t >= ev_min) && st == ST_STAR) {
+      if (n_t > 0 && n_t >= lim && st == ST_STAR) {
         L.work++;
         star_reemit(k, L.rng, acc, nrm);
         if constexpr (TRACE) tr.nstar++;
         push(k, 0, true, L.pid, nrm);
         st = start_walk(true);
       }
-      if (n_e > 0 && (all || n_e >= ev_min) && st == ST_END) {
+      if (n_e > 0 && n_e >= lim && st == ST_END) {
         if (status >= 0) {  // the packet ends (0 escaped, 1 killed, 2 lost)
           acc.maxwork = max(acc.maxwork, L.work);
           lucy_flush(L.la);
@@ -1614,6 +1646,7 @@ struct hc_ctx {
   hc_sources src;
   // launch shape
   int regen_lanes = 16;
+  int regen_run_lanes = 16;
   int blocks_per_cu = 0;
   size_t lds_per_block = 64 * 1024;  // the device's limit (hc_create)
   uint32_t max_work = 1u << 24;
@@ -1633,6 +1666,18 @@ int fail(hc_ctx *c, int code, const std::string &msg) {
       return fail(ctx, HC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// Copies between host and device go through the context's own (non-blocking)
+// stream: the null stream of a plain hipMemcpy would order them after the
+// work of every other context on the device, and contexts that run side by
+// side would wait for each other's passes.
+#define HC_COPY_(dst, src, bytes, kind)                                               \
+  do {                                                                               \
+    HCK(hipMemcpyAsync((dst), (src), (bytes), (kind), ctx->stream));                 \
+    HCK(hipStreamSynchronize(ctx->stream));                                          \
+  } while (0)
+#define HC_D2H(dst, src, bytes) HC_COPY_(dst, src, bytes, hipMemcpyDeviceToHost)
+#define HC_H2D(dst, src, bytes) HC_COPY_(dst, src, bytes, hipMemcpyHostToDevice)
+
 template <class T>
 int dalloc(hc_ctx *ctx, T **p, size_t n) {
   if (*p) return HC_OK;
@@ -1644,7 +1689,7 @@ template <class T>
 int upload(hc_ctx *ctx, T **p, const T *h, size_t n) {
   int rc = dalloc(ctx, p, n);
   if (rc) return rc;
-  HCK(hipMemcpy(*p, h, n * sizeof(T), hipMemcpyHostToDevice));
+  HC_H2D(*p, h, n * sizeof(T));
   return HC_OK;
 }
 
@@ -1699,7 +1744,7 @@ int hc_create(const hc_dims *dims, int device, hc_ctx **out) {
   ctx->ncellw = (size_t)dims->nrg * dims->ntg * dims->npg;
   for (int i = 0; i < kMaxSpecies; ++i) ctx->slot[i] = -1;
   hipError_t e = hipSetDevice(device);
-  if (e == hipSuccess) e = hipStreamCreate(&ctx->stream);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
   if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
   if (e != hipSuccess) {
@@ -1731,7 +1776,19 @@ int hc_create(const hc_dims *dims, int device, hc_ctx **out) {
     const double t = 1.0 / (double)(j * j * j * j);
     for (int i = 145; i >= j; --i) z[i] = z[i] + t;
   }
-  if (!rc && hipMemcpyToSymbol(HIP_SYMBOL(c_inczeta), z, sizeof z) != hipSuccess) rc = HC_E_HIP;
+  // the table is the same for every context: written once per device, so a
+  // context made while another one's pass runs does not write what it reads
+  if (!rc) {
+    static std::mutex zeta_mu;
+    static std::vector<int> zeta_done;
+    std::lock_guard<std::mutex> lock(zeta_mu);
+    if (std::find(zeta_done.begin(), zeta_done.end(), device) == zeta_done.end()) {
+      if (hipMemcpyToSymbol(HIP_SYMBOL(c_inczeta), z, sizeof z) != hipSuccess)
+        rc = HC_E_HIP;
+      else
+        zeta_done.push_back(device);
+    }
+  }
   if (rc) {
     hc_destroy(ctx);
     return rc;
@@ -1796,7 +1853,7 @@ int hc_set_temperature(hc_ctx *ctx, const double *tdust) {
   HCK(hipSetDevice(ctx->device));
   std::vector<double> tmp;
   to_device_layout(ctx, tdust, tmp);
-  HCK(hipMemcpy(ctx->tdust, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice));
+  HC_H2D(ctx->tdust, tmp.data(), tmp.size() * sizeof(double));
   return refresh_mask(ctx);
 }
 
@@ -1806,7 +1863,7 @@ int hc_get_temperature(hc_ctx *ctx, double *tdust) {
   HCK(hipSetDevice(ctx->device));
   const size_t nc = ctx->ncellw * (size_t)ctx->nact;
   std::vector<double> tmp(nc);
-  HCK(hipMemcpy(tmp.data(), ctx->tdust, nc * sizeof(double), hipMemcpyDeviceToHost));
+  HC_D2H(tmp.data(), ctx->tdust, nc * sizeof(double));
   for (size_t i = 0; i < ctx->ncellw * (size_t)ctx->dims.ndg; ++i) tdust[i] = 0.1 / 11605.0;
   for (int k = 0; k < ctx->nact; ++k) {
     double *dst = tdust + ctx->ncellw * (size_t)(ctx->act[k] - 1);
@@ -2095,6 +2152,7 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
   P.next_packet = ctx->counter;
   P.trace = trace ? ctx->trace : nullptr;
   P.regen_lanes = ctx->regen_lanes;
+  P.regen_run_lanes = ctx->regen_run_lanes;
   P.max_work = ctx->max_work;
   P.costarr = ctx->costarr;
   P.rddustmin = S.rddustmin;
@@ -2109,7 +2167,7 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
     P.slot_of[i] = ctx->slot[i];
   }
 
-  void (*kern)() = nullptr;
+  void (*kern)(Params) = nullptr;
   size_t rec_bytes = 0;
   const bool fp32 = d.precision == HC_FP32;
   const bool pk = P.ipeel != 0;
@@ -2165,19 +2223,19 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
     return fail(ctx, HC_E_UNSUPPORTED,
                 "hc_run_iteration: the grid's wall arrays and the dust tables exceed "
                 "the LDS of a workgroup");
-  HCK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_P), &P, sizeof P, 0, hipMemcpyHostToDevice, s));
   if (shm > 64 * 1024)
     HCK(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
   int dev_cus = 0, bpm = 0;
   HCK(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-  HCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpm, kern, 256, shm));
+  const int block = pk ? kBlockPeel : kBlockLucy;
+  HCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpm, kern, block, shm));
   const int per_cu = ctx->blocks_per_cu > 0 ? std::min(ctx->blocks_per_cu, std::max(bpm, 1))
                                             : std::max(bpm, 1);
   long long blocks = (long long)std::max(dev_cus, 1) * per_cu;
-  blocks = std::min<long long>(blocks, std::max<long long>((np + 255) / 256, 1));
+  blocks = std::min<long long>(blocks, std::max<long long>((np + block - 1) / block, 1));
   HCK(hipEventRecord(ctx->ev0, s));
-  if (np > 0) hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), shm, s);
+  if (np > 0) hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(block), shm, s, P);
   HCK(hipGetLastError());
   HCK(hipEventRecord(ctx->ev1, s));
   HCK(hipStreamSynchronize(s));
@@ -2187,7 +2245,7 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
 
   // download
   double scal[S_COUNT];
-  HCK(hipMemcpy(scal, ctx->scal, sizeof scal, hipMemcpyDeviceToHost));
+  HC_D2H(scal, ctx->scal, sizeof scal);
   out->tot = scal[S_TOT];
   out->flux = scal[S_FLUX];
   out->sflux = scal[S_SFLUX];
@@ -2208,13 +2266,13 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
   out->npeelray = scal[S_NPEELRAY];
   out->npeelcross = scal[S_NPEELCROSS];
   unsigned long long counters[2] = {0, 0};
-  HCK(hipMemcpy(counters, ctx->counter, sizeof counters, hipMemcpyDeviceToHost));
+  HC_D2H(counters, ctx->counter, sizeof counters);
   out->max_packet_work = (double)counters[1];
   std::vector<double> tmp;
   auto per_species = [&](double *dev, double *host) -> int {
     if (!host) return HC_OK;
     tmp.resize(nc);
-    HCK(hipMemcpy(tmp.data(), dev, nc * sizeof(double), hipMemcpyDeviceToHost));
+    HC_D2H(tmp.data(), dev, nc * sizeof(double));
     std::memset(host, 0, ctx->ncellw * (size_t)d.ndg * sizeof(double));
     for (int k = 0; k < ctx->nact; ++k) {
       double *dst = host + ctx->ncellw * (size_t)(ctx->act[k] - 1);
@@ -2225,30 +2283,29 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
   if ((rc = per_species(ctx->dtauabs, out->dtauabs))) return rc;
   if ((rc = per_species(ctx->nabs, out->nabs))) return rc;
   if (out->jmean)
-    HCK(hipMemcpy(out->jmean, ctx->jmean, ctx->ncellw * sizeof(double), hipMemcpyDeviceToHost));
+    HC_D2H(out->jmean, ctx->jmean, ctx->ncellw * sizeof(double));
   if (out->killed)
-    HCK(hipMemcpy(out->killed, ctx->killed, ctx->ncellw * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HC_D2H(out->killed, ctx->killed, ctx->ncellw * sizeof(int32_t));
   if (out->dbig)
-    HCK(hipMemcpy(out->dbig, ctx->dbig, ctx->ncellw * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HC_D2H(out->dbig, ctx->dbig, ctx->ncellw * sizeof(int32_t));
   if (out->dsg) std::memset(out->dsg, 0, ctx->ncellw * sizeof(int32_t));
-  if (out->sed) HCK(hipMemcpy(out->sed, ctx->sed, nsed * sizeof(double), hipMemcpyDeviceToHost));
+  if (out->sed) HC_D2H(out->sed, ctx->sed, nsed * sizeof(double));
   if (out->peel && d.npeel > 0)
-    HCK(hipMemcpy(out->peel, ctx->peel, npeelsed * sizeof(double), hipMemcpyDeviceToHost));
+    HC_D2H(out->peel, ctx->peel, npeelsed * sizeof(double));
   if (out->si_init)
-    HCK(hipMemcpy(out->si_init, ctx->si_init, (size_t)d.nfreq * 5 * sizeof(double),
-                  hipMemcpyDeviceToHost));
+    HC_D2H(out->si_init, ctx->si_init, (size_t)d.nfreq * 5 * sizeof(double));
   if (cube) {
     if (out->img_i)
-      HCK(hipMemcpy(out->img_i, ctx->img_i, nimg * kNo * sizeof(double), hipMemcpyDeviceToHost));
+      HC_D2H(out->img_i, ctx->img_i, nimg * kNo * sizeof(double));
     if (out->img_q)
-      HCK(hipMemcpy(out->img_q, ctx->img_q, nimg * sizeof(double), hipMemcpyDeviceToHost));
+      HC_D2H(out->img_q, ctx->img_q, nimg * sizeof(double));
     if (out->img_u)
-      HCK(hipMemcpy(out->img_u, ctx->img_u, nimg * sizeof(double), hipMemcpyDeviceToHost));
+      HC_D2H(out->img_u, ctx->img_u, nimg * sizeof(double));
     if (out->img_v)
-      HCK(hipMemcpy(out->img_v, ctx->img_v, nimg * sizeof(double), hipMemcpyDeviceToHost));
+      HC_D2H(out->img_v, ctx->img_v, nimg * sizeof(double));
   }
   if (trace && np > 0)
-    HCK(hipMemcpy(trace, ctx->trace, (size_t)np * sizeof(hc_trace), hipMemcpyDeviceToHost));
+    HC_D2H(trace, ctx->trace, (size_t)np * sizeof(hc_trace));
   return HC_OK;
 }
 
@@ -2325,6 +2382,9 @@ int hc_set_option(hc_ctx *ctx, const char *name, int64_t value) {
   if (n == "regen_lanes") {
     if (value < 1 || value > 64) return fail(ctx, HC_E_ARG, "regen_lanes must be in 1..64");
     ctx->regen_lanes = (int)value;
+  } else if (n == "regen_run_lanes") {
+    if (value < 1 || value > 64) return fail(ctx, HC_E_ARG, "regen_run_lanes must be in 1..64");
+    ctx->regen_run_lanes = (int)value;
   } else if (n == "max_work") {
     if (value < 1 || value > 0x7fffffff) return fail(ctx, HC_E_ARG, "max_work must be in 1..2^31-1");
     ctx->max_work = (uint32_t)value;
