@@ -77,27 +77,31 @@ def build_models(models: List[GridModel], workers: int = 4) -> list:
         return list(pool.map(lambda gm: build_model(gm.cfg), models))
 
 
-def _iterate(tr, packets: int, iterations: int):
-    kms, seg, scal = 0.0, 0, []
-    for _ in range(iterations):
-        s, ms = tr.run_scalars(packets, seg0=seg)
+def _iterate(tr, packets: int, iterations: int, sed_last: bool):
+    kms, seg, scal, sed = 0.0, 0, [], None
+    for i in range(iterations):
+        if sed_last and i == iterations - 1:
+            s, ms, sed = tr.run_sed(packets, seg0=seg)
+        else:
+            s, ms = tr.run_scalars(packets, seg0=seg)
         seg += packets
         tr.temperature(packets, apply=True)
         kms += ms
         scal.append(s)
-    return kms, scal
+    return kms, scal, sed
 
 
 def run_batch(models: List[GridModel], iterations: int = 3, streams: int = 4,
               precision: int = 0, options: Optional[dict] = None,
-              built: Optional[list] = None) -> BatchResult:
+              built: Optional[list] = None, sed_last: bool = False) -> BatchResult:
     """Lucy iterations of every model from its initial temperatures.
 
     Every model is built (unless `built` holds build_models' result) and
     uploaded first, and all of them stay resident in HBM; then a pool of
     `streams` host threads runs the models' iterations, largest packet count
     first, each model on its own context and stream.  `wall_s` covers the
-    iterations only."""
+    iterations only.  sed_last downloads the SED of every model's last
+    iteration (per_model[i]["sed"])."""
     from ._lib import Transport
     t0 = time.perf_counter()
     if built is None:
@@ -111,12 +115,12 @@ def run_batch(models: List[GridModel], iterations: int = 3, streams: int = 4,
     try:
         t1 = time.perf_counter()
         with cf.ThreadPoolExecutor(max_workers=max(1, streams)) as pool:
-            futs = {i: pool.submit(_iterate, trs[i], models[i].packets, iterations)
+            futs = {i: pool.submit(_iterate, trs[i], models[i].packets, iterations, sed_last)
                     for i in order}
             res = {i: f.result() for i, f in futs.items()}
         wall = time.perf_counter() - t1
         per = [{"packets": models[i].packets * iterations, "kernel_ms": res[i][0],
-                "scalars": res[i][1], "tdust": trs[i].get_temperature()}
+                "scalars": res[i][1], "sed": res[i][2], "tdust": trs[i].get_temperature()}
                for i in range(len(models))]
     finally:
         for tr in trs:

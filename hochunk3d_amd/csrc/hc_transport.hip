@@ -171,7 +171,8 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) {
 }
 
 // Dynamic LDS of the transport kernel (layout set on the host, Params):
-//   [0, walls_lds)            wall arrays r2[nrg] t2[ntg] a[npg] b[npg]
+//   [0, walls_lds)            wall arrays r2[nrg] t2[ntg] a[npg] b[npg],
+//                             then rr[nrg] tc[ntg] ts[ntg] (wall_clearance)
 //   [opac_base, +opac_bytes)  per active species s: nudust, kappad, adust,
 //                             gdust, pldust (nlambda doubles each) at
 //                             opac_off[s] doubles, then the int32 opacset
@@ -185,6 +186,7 @@ extern __shared__ __align__(16) unsigned char hc_lds[];
 // memory.
 struct WallView {
   const double *r2, *t2, *a, *b;
+  const double *rr, *tc, *ts;  // radii; |cos| and sin of the theta walls
 };
 
 // Per-cell densities are stored once, signed: +rho for a species below its
@@ -323,6 +325,32 @@ __device__ __forceinline__ int walk_begin(const WallView &W, Pkt<R, NA> &k) {
   return ST_WALK;
 }
 
+// A lower bound of the distance from the packet to every wall surface of its
+// cell: |r - R| to the two spheres, the meridional distance
+// |rho |cos t| - |z| sin t| to the two double cones (no point of either nappe
+// is closer), and |a x + b y| to the two full phi planes.  Every root the
+// walk can take lies at least that far along the unit direction, so an
+// interaction point closer than the bound is reached before any wall.  The
+// margin, 1e-6 r, is far above the rounding of the roots, also for rays
+// almost tangent to a wall; a packet off its cell's range gets a negative
+// bound.
+template <class R, int NA>
+__device__ __forceinline__ double wall_clearance(const WallView &W, const Pkt<R, NA> &k) {
+  const Params &P = c_P;
+  const double r = sqrt(k.rsq);
+  double d = fmin(r - W.rr[k.ir - 1], W.rr[k.ir] - r);
+  if (P.ntg > 2) {
+    const double rho = sqrt(k.x * k.x + k.y * k.y), az = fabs(k.z);
+    d = fmin(d, fabs(rho * W.tc[k.it - 1] - az * W.ts[k.it - 1]));
+    d = fmin(d, fabs(rho * W.tc[k.it] - az * W.ts[k.it]));
+  }
+  if (P.npg > 2) {
+    d = fmin(d, fabs(W.a[k.ip - 1] * k.x + W.b[k.ip - 1] * k.y));
+    d = fmin(d, fabs(W.a[k.ip] * k.x + W.b[k.ip] * k.y));
+  }
+  return d - 1e-6 * r;
+}
+
 // One step of the walk: to the next wall, or to the interaction point when
 // the optical depth tau is reached inside the current cell.
 // The masked densities of the packet's cell, into registers.
@@ -348,17 +376,7 @@ __device__ __forceinline__ int walk_step(const WallView &W, Pkt<R, NA> &k, doubl
     load_dens(k, dm);
   }
   const size_t cell = cell_of(k.ir, k.it, k.ip);
-  double dt;
-  int face;
-  if (!next_wall(W, k.ux, k.uy, k.uz, k.x, k.y, k.z, k.ir, k.it, k.ip, k.on_wall, dt, face)) {
-    acc.err_wall++;
-    return ST_ERROR;
-  }
-  const double opac = cell_opac(k, dm);
-  const double dtau = opac * dt;
-  if (tsum + dtau >= tau) {
-    double ds = (tau - tsum) / opac;
-    if (ds > dt) ds = dt;  // rounding must not carry the packet past the wall
+  auto interact = [&](double ds) {
     lucy_add(la, k, dm, cell, ds);
     k.x = k.x + ds * k.ux;
     k.y = k.y + ds * k.uy;
@@ -367,6 +385,26 @@ __device__ __forceinline__ int walk_step(const WallView &W, Pkt<R, NA> &k, doubl
     k.rsq = k.x * k.x + k.y * k.y + k.z * k.z;
     if constexpr (TRACE) tr.path += ds;
     return ST_INTERACT;
+  };
+  const double opac = cell_opac(k, dm);
+  // After an interaction (the packet is off the walls) the next one is often
+  // closer than any wall of the cell: then the walls' roots are not needed,
+  // and the point is the one the full step below reaches.
+  if (!k.on_wall) {
+    const double ds = (tau - tsum) / opac;  // no shortcut for opac = 0 (inf or NaN)
+    if (ds < wall_clearance(W, k)) return interact(ds);
+  }
+  double dt;
+  int face;
+  if (!next_wall(W, k.ux, k.uy, k.uz, k.x, k.y, k.z, k.ir, k.it, k.ip, k.on_wall, dt, face)) {
+    acc.err_wall++;
+    return ST_ERROR;
+  }
+  const double dtau = opac * dt;
+  if (tsum + dtau >= tau) {
+    double ds = (tau - tsum) / opac;
+    if (ds > dt) ds = dt;  // rounding must not carry the packet past the wall
+    return interact(ds);
   }
   lucy_add(la, k, dm, cell, dt);
   tsum = tsum + dtau;
@@ -1259,7 +1297,17 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : kBlockLucy, HC_WAVES_PER_EU) vo
     wl[P.nrg + P.ntg + i] = P.aarr[i];
     wl[P.nrg + P.ntg + P.npg + i] = P.barr[i];
   }
-  const WallView W{wl, wl + P.nrg, wl + P.nrg + P.ntg, wl + P.nrg + P.ntg + P.npg};
+  // radii and the theta walls' |cos t|, sin t (the midplane: 0, 1)
+  double *wr = wl + P.nrg + P.ntg + 2 * P.npg;
+  for (int i = threadIdx.x; i < P.nrg; i += blockDim.x) wr[i] = sqrt(P.r2arr[i]);
+  for (int i = threadIdx.x; i < P.ntg; i += blockDim.x) {
+    const double t2 = P.tan2arr[i];
+    const double c = t2 < 0.0 ? 0.0 : 1.0 / sqrt(1.0 + t2);
+    wr[P.nrg + i] = c;
+    wr[P.nrg + P.ntg + i] = t2 < 0.0 ? 1.0 : sqrt(t2) * c;
+  }
+  const WallView W{wl,         wl + P.nrg,          wl + P.nrg + P.ntg, wl + P.nrg + P.ntg + P.npg,
+                   wr,         wr + P.nrg,          wr + P.nrg + P.ntg};
   // opacity tables and guides of the active species into LDS
   {
     double *ot = reinterpret_cast<double *>(lds + P.opac_base);
@@ -2191,7 +2239,8 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
 #undef HC_PICK
   // dynamic LDS: the wall arrays, then (peel-off passes) four wavefront
   // queues of peel records; two workgroups share a compute unit
-  const size_t walls = ((size_t)(d.nrg + d.ntg + 2 * d.npg) * sizeof(double) + 63) / 64 * 64;
+  const size_t walls =
+      ((size_t)(2 * d.nrg + 3 * d.ntg + 2 * d.npg) * sizeof(double) + 63) / 64 * 64;
   P.walls_lds = (int)walls;
   size_t ntab = 0;  // doubles of the active species' opacity tables
   for (int k = 0; k < ctx->nact; ++k) {

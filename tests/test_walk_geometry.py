@@ -105,3 +105,72 @@ def test_ray_leaving_the_polar_axis(grid):
         assert n == (m.nrg - ir) + n_theta, (phid, n)
         assert jr == m.nrg and jt == 1 + n_theta
         assert jp == int(np.searchsorted(m.phiarr, phid, side="right"))
+
+
+def _clearance(m, p, ir, it, ip):
+    """The walk's interior test bound (wall_clearance in the transport kernel):
+    |r - R| to the cell's spheres, |rho |cos t| - |z| sin t| to its double
+    cones, |a x + b y| to its full phi planes, less 1e-6 r."""
+    x, y, z = p
+    r = np.sqrt(x * x + y * y + z * z)
+    d = min(r - m.rarr[ir - 1], m.rarr[ir] - r)
+    rho = np.hypot(x, y)
+    for j in (it - 1, it):
+        t2 = m.tan2arr[j]
+        c, s = (0.0, 1.0) if t2 < 0 else (1 / np.sqrt(1 + t2), np.sqrt(t2 / (1 + t2)))
+        d = min(d, abs(rho * c - abs(z) * s))
+    for j in (ip - 1, ip):
+        d = min(d, abs(m.aarr[j] * x + m.barr[j] * y))
+    return d - 1e-6 * r
+
+
+def test_interior_clearance_is_a_lower_bound_of_the_first_wall(grid):
+    """Brute force over directions: the first wall root along any direction
+    is at least the bound, and the bound is close to the nearest one."""
+    m, _ = grid
+    rng = np.random.default_rng(5)
+    u = rng.normal(size=(4000, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    tight = 0
+    for _ in range(60):
+        ir, it, ip = (int(rng.integers(2, m.nrg - 1)), int(rng.integers(1, m.ntg)),
+                      int(rng.integers(1, m.npg)))
+        r = m.rarr[ir - 1] + rng.uniform(0.05, 0.95) * (m.rarr[ir] - m.rarr[ir - 1])
+        th = m.thetarr[it - 1] + rng.uniform(0.05, 0.95) * (m.thetarr[it] - m.thetarr[it - 1])
+        ph = m.phiarr[ip - 1] + rng.uniform(0.05, 0.95) * (m.phiarr[ip] - m.phiarr[ip - 1])
+        p = np.array([r * np.sin(th) * np.cos(ph), r * np.sin(th) * np.sin(ph), r * np.cos(th)])
+        # first root along each direction over the cell's wall surfaces
+        b = u @ p
+        first = np.full(len(u), np.inf)
+        for R in (m.rarr[ir - 1], m.rarr[ir]):
+            disc = b * b - (p @ p - R * R)
+            ok = disc > 0
+            for sgn in (-1, 1):
+                t = np.where(ok, -b + sgn * np.sqrt(np.where(ok, disc, 0)), np.inf)
+                first = np.where((t > 0) & (t < first), t, first)
+        for j in (it - 1, it):
+            t2 = m.tan2arr[j]
+            if t2 == 0.0:
+                continue
+            if t2 < 0:
+                t = np.where(u[:, 2] != 0, -p[2] / np.where(u[:, 2] != 0, u[:, 2], 1), np.inf)
+                first = np.where((t > 0) & (t < first), t, first)
+                continue
+            a = u[:, 0] ** 2 + u[:, 1] ** 2 - t2 * u[:, 2] ** 2
+            bb = 2 * (p[0] * u[:, 0] + p[1] * u[:, 1] - t2 * p[2] * u[:, 2])
+            c = p[0] ** 2 + p[1] ** 2 - t2 * p[2] ** 2
+            disc = bb * bb - 4 * a * c
+            ok = (disc > 0) & (a != 0)
+            for sgn in (-1, 1):
+                t = np.where(ok, (-bb + sgn * np.sqrt(np.where(ok, disc, 0))) /
+                             np.where(ok, 2 * a, 1), np.inf)
+                first = np.where((t > 0) & (t < first), t, first)
+        for j in (ip - 1, ip):
+            den = m.aarr[j] * u[:, 0] + m.barr[j] * u[:, 1]
+            t = np.where(den != 0, -(m.aarr[j] * p[0] + m.barr[j] * p[1]) /
+                         np.where(den != 0, den, 1), np.inf)
+            first = np.where((t > 0) & (t < first), t, first)
+        c = _clearance(m, p, ir, it, ip)
+        assert c <= first.min() * (1 + 1e-12), (ir, it, ip, c, first.min())
+        tight += c > 0.9 * first.min()
+    assert tight >= 50
