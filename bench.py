@@ -126,7 +126,7 @@ def batched_grid(args) -> dict:
         f"{sum(m.packets for m in models)} packets per iteration, built in "
         f"{time.perf_counter() - t0:.1f}s")
     b32 = run_batch(models, args.batch_iters, streams=args.batch_streams, precision=HC_FP32,
-                    built=built, sed_last=True)
+                    built=built, sed_last=True, graph=True)
     log(f"batch fp32, {args.batch_streams} streams: {b32.rate:.3e} packets/s "
         f"({b32.wall_s:.2f}s)")
     b64 = run_batch(models, args.batch_iters, streams=1, precision=HC_FP64, built=built,
@@ -142,7 +142,9 @@ def batched_grid(args) -> dict:
                      f"{args.batch_iters} iterations from the initial temperatures; grids "
                      f"resident in HBM before the timed region"),
         "fp32_batched": {"value": round(b32.rate, 2), "unit": "packets/s",
-                         "wall_s": round(b32.wall_s, 3), "streams": args.batch_streams},
+                         "wall_s": round(b32.wall_s, 3), "streams": args.batch_streams,
+                         "lucy_iterations": "replayed from one captured HIP graph per model "
+                                            "(the SED-downloading last one called directly)"},
         "fp64_plain": {"value": round(b64.rate, 2), "unit": "packets/s",
                        "wall_s": round(b64.wall_s, 3), "streams": 1},
         "speedup": round(b64.wall_s / b32.wall_s, 3),
@@ -169,6 +171,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 path")
     ap.add_argument("--peel-np", type=float, default=100_000,
                     help="packets of the final peeled pass timed after the Lucy steps (0: skip)")
+    ap.add_argument("--autotune-np", type=float, default=2_000_000,
+                    help="packets per autotuning probe pass (0: default launch shape)")
     ap.add_argument("--batch-models", type=int, default=16,
                     help="parameter-grid models of the batched section (0: skip)")
     ap.add_argument("--batch-scale", type=float, default=0.05,
@@ -224,6 +228,14 @@ def main():
             f"{(s['nscat_ev'] + s['nabs_ev']) / wnp:.1f} events per packet, "
             f"T changed in {nch} cell-species")
     t_start = tr.get_temperature()  # where the timed steps start
+    tuned = {}
+    if args.autotune_np > 0:
+        from hochunk3d_amd.autotune import tune_transport
+        r = tune_transport(tr, int(args.autotune_np))
+        tuned["fp64"] = {"options": r.options, "gain_vs_default": round(r.gain, 3),
+                         "probes": len(r.trials)}
+        log(f"autotune fp64: {r.options}, x{r.gain:.3f} vs default on {len(r.trials)} probes "
+            f"of {int(args.autotune_np)} packets")
 
     wall64, kms64, tot64, sed64 = timed_steps(tr, args.steps, np_, seg, "fp64")
     npk = np_ * args.steps
@@ -234,6 +246,11 @@ def main():
     peel = None
     if args.peel_np > 0:
         pnp = int(args.peel_np)
+        if args.autotune_np > 0:
+            r = tune_transport(tr, max(pnp // 5, 1000), ipeel=1)
+            tuned["fp64_peeled"] = {"options": r.options, "gain_vs_default": round(r.gain, 3),
+                                    "probes": len(r.trials)}
+            log(f"autotune peeled pass: {r.options}, x{r.gain:.3f} vs default")
         t0 = time.perf_counter()
         s, ms = tr.run_scalars(pnp, ipeel=1, seg0=seg + npk)
         wallp = time.perf_counter() - t0
@@ -251,6 +268,11 @@ def main():
     if not args.no_fp32:
         model.tdust[...] = t_start
         tr32 = Transport(model, precision=HC_FP32)
+        if args.autotune_np > 0:
+            r = tune_transport(tr32, int(args.autotune_np))
+            tuned["fp32"] = {"options": r.options, "gain_vs_default": round(r.gain, 3),
+                             "probes": len(r.trials)}
+            log(f"autotune fp32: {r.options}, x{r.gain:.3f} vs default")
         wall32, kms32, tot32, sed32 = timed_steps(tr32, args.steps, np_, seg, "fp32")
         tr32.close()
         err = sed_errors(sed64, sed32)
@@ -322,6 +344,7 @@ def main():
         "fp32_path": fp32,
         "peeled_pass_fp64": peel,
         "batched_grid": batch,
+        "autotune": tuned or None,
     }
     print(json.dumps(line), flush=True)
 

@@ -26,7 +26,8 @@ _ip = C.POINTER(C.c_int32)
 EXPORTS = (
     "hc_create", "hc_set_grid", "hc_set_temperature", "hc_get_temperature", "hc_set_dust",
     "hc_set_sources",
-    "hc_run_iteration", "hc_temperature", "hc_set_option", "hc_destroy", "hc_last_error",
+    "hc_run_iteration", "hc_temperature", "hc_run_lucy", "hc_set_option", "hc_destroy",
+    "hc_last_error",
     "hc_abi_version",
 )
 
@@ -107,6 +108,9 @@ def load() -> C.CDLL:
                                      C.c_int32, C.c_uint64, C.POINTER(HcTallies), C.c_void_p]
     lib.hc_temperature.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, _dp,
                                    C.POINTER(C.c_int64)]
+    lib.hc_run_lucy.argtypes = [C.c_void_p, C.c_int32, C.c_int64, C.c_int32, C.c_int32,
+                                C.c_int32, C.c_uint64, _dp, C.c_double, C.POINTER(HcTallies),
+                                C.POINTER(C.c_int64)]
     lib.hc_set_option.argtypes = [C.c_void_p, C.c_char_p, C.c_int64]
     lib.hc_destroy.argtypes = [C.c_void_p]
     lib.hc_last_error.argtypes = [C.c_void_p]
@@ -210,7 +214,8 @@ class Transport:
         self._check(self.lib.hc_set_sources(self.h, C.byref(s)), "hc_set_sources")
 
     def set_option(self, name: str, value: int):
-        """Launch-shape knob (regen_lanes, blocks_per_cu, max_work)."""
+        """Launch-shape knob: regen_lanes, regen_run_lanes, peel_queue,
+        blocks_per_cu or max_work (include/hc.h documents each)."""
         self._check(self.lib.hc_set_option(self.h, name.encode(), int(value)), "hc_set_option")
 
     def _check(self, rc, what):
@@ -304,6 +309,24 @@ class Transport:
         self._check(self.lib.hc_temperature(self.h, _ptr(mass), float(m.ltot), float(np_),
                                             _ptr(t2), C.byref(n)), "hc_temperature")
         return t2, int(n.value)
+
+    def run_lucy(self, niter: int, np_: int, seed1: int = 12345678, seed2: int = 87654321,
+                 gen: int = 1, seg0: int = 0):
+        """niter Lucy iterations (pass + device temperature update) replayed
+        from one captured HIP graph; iteration i uses segments
+        seg0 + i*np_ ...  Returns (scalars of the last pass, device ms of all
+        iterations, changed cell-species of the last update)."""
+        m = self.model
+        mass = None
+        if not getattr(self, "_mass_sent", False):
+            mass = _f64(m.massarr)
+            self._mass_sent = True
+        t = HcTallies()
+        n = C.c_int64(0)
+        self._check(self.lib.hc_run_lucy(self.h, int(niter), int(np_), int(seed1), int(seed2),
+                                         int(gen), int(seg0), _ptr(mass), float(m.ltot),
+                                         C.byref(t), C.byref(n)), "hc_run_lucy")
+        return {k: getattr(t, k) for k in SCALARS}, t.kernel_ms, int(n.value)
 
     def close(self):
         if getattr(self, "h", None):

@@ -77,8 +77,23 @@ def build_models(models: List[GridModel], workers: int = 4) -> list:
         return list(pool.map(lambda gm: build_model(gm.cfg), models))
 
 
-def _iterate(tr, packets: int, iterations: int, sed_last: bool):
+def _iterate(tr, packets: int, iterations: int, sed_last: bool, graph: bool):
     kms, seg, scal, sed = 0.0, 0, [], None
+    if graph:
+        # all iterations but an SED-downloading last one from one captured
+        # graph; only the last one's scalars come back
+        ng = iterations - 1 if sed_last else iterations
+        if ng > 0:
+            s, ms, _ = tr.run_lucy(ng, packets, seg0=0)
+            kms += ms
+            seg += ng * packets
+            scal.append(s)
+        if sed_last:
+            s, ms, sed = tr.run_sed(packets, seg0=seg)
+            tr.temperature(packets, apply=True)
+            kms += ms
+            scal.append(s)
+        return kms, scal, sed
     for i in range(iterations):
         if sed_last and i == iterations - 1:
             s, ms, sed = tr.run_sed(packets, seg0=seg)
@@ -93,7 +108,8 @@ def _iterate(tr, packets: int, iterations: int, sed_last: bool):
 
 def run_batch(models: List[GridModel], iterations: int = 3, streams: int = 4,
               precision: int = 0, options: Optional[dict] = None,
-              built: Optional[list] = None, sed_last: bool = False) -> BatchResult:
+              built: Optional[list] = None, sed_last: bool = False,
+              graph: bool = False) -> BatchResult:
     """Lucy iterations of every model from its initial temperatures.
 
     Every model is built (unless `built` holds build_models' result) and
@@ -101,7 +117,9 @@ def run_batch(models: List[GridModel], iterations: int = 3, streams: int = 4,
     `streams` host threads runs the models' iterations, largest packet count
     first, each model on its own context and stream.  `wall_s` covers the
     iterations only.  sed_last downloads the SED of every model's last
-    iteration (per_model[i]["sed"])."""
+    iteration (per_model[i]["sed"]).  graph runs the iterations through
+    hc_run_lucy (one captured HIP graph per model, replayed); the scalars
+    then come from the last replay only."""
     from ._lib import Transport
     t0 = time.perf_counter()
     if built is None:
@@ -115,7 +133,8 @@ def run_batch(models: List[GridModel], iterations: int = 3, streams: int = 4,
     try:
         t1 = time.perf_counter()
         with cf.ThreadPoolExecutor(max_workers=max(1, streams)) as pool:
-            futs = {i: pool.submit(_iterate, trs[i], models[i].packets, iterations, sed_last)
+            futs = {i: pool.submit(_iterate, trs[i], models[i].packets, iterations, sed_last,
+                                   graph)
                     for i in order}
             res = {i: f.result() for i, f in futs.items()}
         wall = time.perf_counter() - t1

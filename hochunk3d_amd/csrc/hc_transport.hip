@@ -62,6 +62,7 @@ struct Params {
   // rng
   int32_t g1, g2;
   unsigned long long seg0;
+  const unsigned long long *seg_dev;  // segment base on the device (graph replays), or null
   const int32_t *pow1, *pow2;
   // tallies
   double *dtauabs, *nabs, *jmean;
@@ -1287,6 +1288,8 @@ template <class R, int NA, bool TRACE, bool PEEL>
 __global__ __launch_bounds__(PEEL ? kBlockPeel : kBlockLucy, HC_WAVES_PER_EU) void transport(
     Params /* read through kernel_params() */) {
   const Params &P = c_P;
+  // packet p draws from generator segment seg_base + p
+  const unsigned long long seg_base = P.seg_dev ? *P.seg_dev : P.seg0;
   const int wave = threadIdx.x >> 6;
   unsigned char *const lds = hc_lds;
   // wall arrays into LDS: r2[nrg] t2[ntg] a[npg] b[npg]
@@ -1521,7 +1524,7 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : kBlockLucy, HC_WAVES_PER_EU) vo
           st = ST_DONE;
         } else {
           L.pid = (long long)p;
-          L.rng = mrg_segment(P.g1, P.g2, P.seg0 + p, P.pow1, P.pow2);
+          L.rng = mrg_segment(P.g1, P.g2, seg_base + p, P.pow1, P.pow2);
           L.la.cell = -1;
           L.iscat = L.iint = L.pir = L.pit = L.pip = 0;
           L.work = 0;
@@ -1686,6 +1689,7 @@ struct hc_ctx {
          *img_v = nullptr, *si_init = nullptr, *scal = nullptr, *mass4 = nullptr,
          *tdust2 = nullptr;
   unsigned long long *counter = nullptr;
+  unsigned long long *segdev = nullptr;  // segment base of graph-replayed passes
   hc_trace *trace = nullptr;
   size_t trace_cap = 0;
   // host copies of scalars
@@ -1696,6 +1700,7 @@ struct hc_ctx {
   int regen_lanes = 16;
   int regen_run_lanes = 16;
   int blocks_per_cu = 0;
+  int peel_queue = 0;  // records per wavefront peel queue, 0 = 96 or what fits
   size_t lds_per_block = 64 * 1024;  // the device's limit (hc_create)
   uint32_t max_work = 1u << 24;
 };
@@ -1760,11 +1765,17 @@ void to_device_layout(const hc_ctx *ctx, const double *ref, std::vector<double> 
   }
 }
 
-int refresh_mask(hc_ctx *ctx) {
+int queue_mask(hc_ctx *ctx) {
   const size_t n = ctx->ncellw * (size_t)ctx->nact;
   hipLaunchKernelGGL(mask_density, dim3(1024), dim3(256), 0, ctx->stream, ctx->dens, ctx->tdust,
                      ctx->tsub / 11605.0, ctx->densm, n);
   HCK(hipGetLastError());
+  return HC_OK;
+}
+
+int refresh_mask(hc_ctx *ctx) {
+  const int rc = queue_mask(ctx);
+  if (rc) return rc;
   HCK(hipStreamSynchronize(ctx->stream));
   return HC_OK;
 }
@@ -2038,14 +2049,47 @@ int hc_set_sources(hc_ctx *ctx, const hc_sources *src) {
   return HC_OK;
 }
 
-int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int32_t seed2,
-                     int32_t gen, uint64_t seg0, hc_tallies *out, hc_trace *trace) {
-  if (!ctx || !out) return HC_E_ARG;
+}  // extern "C"
+
+namespace {
+
+// One transport pass, ready to launch: kernel, parameters, launch shape and
+// the sizes of its tallies.
+struct PassLaunch {
+  void (*kern)(Params) = nullptr;
+  Params P;
+  size_t shm = 0;
+  unsigned blocks = 0;
+  int block = 0;
+  bool cube = false;
+  size_t nc = 0, nsed = 0, npeelsed = 0, nimg = 0;
+};
+
+__global__ void zero_words(uint32_t *p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+// Zero `bytes` (a multiple of 4) of device memory on the context's stream.
+int zero_bytes(hc_ctx *ctx, void *p, size_t bytes) {
+  const size_t n = bytes / 4;
+  if (!p || n == 0) return HC_OK;
+  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 2048);
+  hipLaunchKernelGGL(zero_words, dim3(blocks), dim3(256), 0, ctx->stream,
+                     static_cast<uint32_t *>(p), n);
+  HCK(hipGetLastError());
+  return HC_OK;
+}
+
+// Tallies allocated, launch parameters and LDS layout of one pass; queues no
+// device work.
+int prepare_pass(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int32_t seed2,
+                 int32_t gen, uint64_t seg0, bool trace, PassLaunch &L) {
   if (!ctx->have_grid || !ctx->have_dust || !ctx->have_src)
-    return fail(ctx, HC_E_STATE, "hc_run_iteration: grid, dust and sources must be set");
+    return fail(ctx, HC_E_STATE, "transport pass: grid, dust and sources must be set");
   if (np < 0 || gen < 1 || gen > 32 || seed1 <= 0 || seed1 >= 2147483563 || seed2 <= 0 ||
       seed2 >= 2147483399)
-    return fail(ctx, HC_E_ARG, "hc_run_iteration: bad np or seeds");
+    return fail(ctx, HC_E_ARG, "transport pass: bad np or seeds");
   const hc_dims &d = ctx->dims;
   HCK(hipSetDevice(ctx->device));
   const size_t nc = ctx->ncellw * (size_t)ctx->nact;
@@ -2075,27 +2119,8 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
     if (rc) return rc;
     ctx->trace_cap = (size_t)np;
   }
-  hipStream_t s = ctx->stream;
-  HCK(hipMemsetAsync(ctx->dtauabs, 0, nc * sizeof(double), s));
-  HCK(hipMemsetAsync(ctx->nabs, 0, nc * sizeof(double), s));
-  HCK(hipMemsetAsync(ctx->jmean, 0, ctx->ncellw * sizeof(double), s));
-  HCK(hipMemsetAsync(ctx->killed, 0, ctx->ncellw * sizeof(int32_t), s));
-  HCK(hipMemsetAsync(ctx->dbig, 0, ctx->ncellw * sizeof(int32_t), s));
-  HCK(hipMemsetAsync(ctx->sed, 0, nsed * sizeof(double), s));
-  HCK(hipMemsetAsync(ctx->peel, 0, npeelsed * sizeof(double), s));
-  HCK(hipMemsetAsync(ctx->si_init, 0, (size_t)d.nfreq * 5 * sizeof(double), s));
-  HCK(hipMemsetAsync(ctx->scal, 0, S_COUNT * sizeof(double), s));
-  HCK(hipMemsetAsync(ctx->counter, 0, 2 * sizeof(unsigned long long), s));
-  if (trace) HCK(hipMemsetAsync(ctx->trace, 0, (size_t)np * sizeof(hc_trace), s));
   const bool cube = d.imcube && ipeel && ctx->img_i;
-  if (cube) {
-    HCK(hipMemsetAsync(ctx->img_i, 0, nimg * kNo * sizeof(double), s));
-    HCK(hipMemsetAsync(ctx->img_q, 0, nimg * sizeof(double), s));
-    HCK(hipMemsetAsync(ctx->img_u, 0, nimg * sizeof(double), s));
-    HCK(hipMemsetAsync(ctx->img_v, 0, nimg * sizeof(double), s));
-  }
-
-  Params P;
+  Params &P = L.P;
   std::memset(&P, 0, sizeof P);
   P.nrg = d.nrg;
   P.ntg = d.ntg;
@@ -2215,7 +2240,7 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
     P.slot_of[i] = ctx->slot[i];
   }
 
-  void (*kern)(Params) = nullptr;
+  void (*&kern)(Params) = L.kern;
   size_t rec_bytes = 0;
   const bool fp32 = d.precision == HC_FP32;
   const bool pk = P.ipeel != 0;
@@ -2260,17 +2285,18 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
     // two workgroups per compute unit when the queues leave room for them
     const size_t budget = ctx->lds_per_block;
     const size_t avail = budget > shm + 64 ? budget - shm - 64 : 0;
-    const size_t q = std::min<size_t>(96, avail / (4 * rec_bytes));
+    const size_t want = ctx->peel_queue > 0 ? (size_t)ctx->peel_queue : 96;
+    const size_t q = std::min<size_t>(want, avail / (4 * rec_bytes));
     if (q < 65)
       return fail(ctx, HC_E_UNSUPPORTED,
-                  "hc_run_iteration: the peel-off queues do not fit the LDS left by this grid's "
+                  "transport pass: the peel-off queues do not fit the LDS left by this grid's "
                   "wall arrays and dust tables");
     P.peel_q = (int)q;
     shm += 64 + 4 * q * rec_bytes;
   }
   if (shm > ctx->lds_per_block)
     return fail(ctx, HC_E_UNSUPPORTED,
-                "hc_run_iteration: the grid's wall arrays and the dust tables exceed "
+                "transport pass: the grid's wall arrays and the dust tables exceed "
                 "the LDS of a workgroup");
   if (shm > 64 * 1024)
     HCK(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
@@ -2283,16 +2309,48 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
                                             : std::max(bpm, 1);
   long long blocks = (long long)std::max(dev_cus, 1) * per_cu;
   blocks = std::min<long long>(blocks, std::max<long long>((np + block - 1) / block, 1));
-  HCK(hipEventRecord(ctx->ev0, s));
-  if (np > 0) hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(block), shm, s, P);
-  HCK(hipGetLastError());
-  HCK(hipEventRecord(ctx->ev1, s));
-  HCK(hipStreamSynchronize(s));
-  float ms = 0.f;
-  HCK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-  out->kernel_ms = ms;
+  L.shm = shm;
+  L.blocks = (unsigned)blocks;
+  L.block = block;
+  L.cube = cube;
+  L.nc = nc;
+  L.nsed = nsed;
+  L.npeelsed = npeelsed;
+  L.nimg = nimg;
+  return HC_OK;
+}
 
-  // download
+// The pass's tallies and counters to zero, with kernels (a graph captured
+// from the stream then holds no memset nodes).
+int zero_tallies(hc_ctx *ctx, const PassLaunch &L, int64_t np, bool trace) {
+  const hc_dims &d = ctx->dims;
+  const size_t nc = L.nc;
+  int rc = HC_OK;
+  auto z = [&](void *p, size_t bytes) {
+    if (!rc) rc = zero_bytes(ctx, p, bytes);
+  };
+  z(ctx->dtauabs, nc * sizeof(double));
+  z(ctx->nabs, nc * sizeof(double));
+  z(ctx->jmean, ctx->ncellw * sizeof(double));
+  z(ctx->killed, ctx->ncellw * sizeof(int32_t));
+  z(ctx->dbig, ctx->ncellw * sizeof(int32_t));
+  z(ctx->sed, L.nsed * sizeof(double));
+  z(ctx->peel, L.npeelsed * sizeof(double));
+  z(ctx->si_init, (size_t)d.nfreq * 5 * sizeof(double));
+  z(ctx->scal, S_COUNT * sizeof(double));
+  z(ctx->counter, 2 * sizeof(unsigned long long));
+  if (trace) z(ctx->trace, (size_t)np * sizeof(hc_trace));
+  if (L.cube) {
+    z(ctx->img_i, L.nimg * kNo * sizeof(double));
+    z(ctx->img_q, L.nimg * sizeof(double));
+    z(ctx->img_u, L.nimg * sizeof(double));
+    z(ctx->img_v, L.nimg * sizeof(double));
+  }
+  return rc;
+}
+
+// The scalar tallies of the last pass into out.
+int download_scalars(hc_ctx *ctx, hc_tallies *out) {
   double scal[S_COUNT];
   HC_D2H(scal, ctx->scal, sizeof scal);
   out->tot = scal[S_TOT];
@@ -2314,6 +2372,106 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
   out->nabs_ev = scal[S_NABS_EV];
   out->npeelray = scal[S_NPEELRAY];
   out->npeelcross = scal[S_NPEELCROSS];
+  return HC_OK;
+}
+
+__global__ void copy_words(const double *src, double *dst, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i < n; i += (size_t)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
+__global__ void advance_segment(unsigned long long *seg, unsigned long long np) {
+  if (threadIdx.x == 0) *seg += np;
+}
+
+// Device temperature update (tfinal): mass4 uploaded when given, tdust2
+// allocated, kernel parameters.
+int prepare_temperature(hc_ctx *ctx, const double *massarr, double ltot, double nphot,
+                        TParams &T) {
+  if (!ctx->have_grid || !ctx->have_dust || !ctx->dtauabs)
+    return fail(ctx, HC_E_STATE, "temperature update needs a completed pass");
+  if (!massarr && !ctx->mass4)
+    return fail(ctx, HC_E_ARG, "temperature update: massarr required on the first call");
+  const hc_dims &d = ctx->dims;
+  HCK(hipSetDevice(ctx->device));
+  const size_t nc = ctx->ncellw * (size_t)ctx->nact;
+  int rc = 0;
+  if (massarr) {
+    std::vector<double> tmp;
+    to_device_layout(ctx, massarr, tmp);
+    rc = upload(ctx, &ctx->mass4, tmp.data(), nc);
+    if (rc) return rc;
+  }
+  rc = dalloc(ctx, &ctx->tdust2, nc);
+  if (rc) return rc;
+  std::memset(&T, 0, sizeof T);
+  T.nrg = d.nrg;
+  T.ntg = d.ntg;
+  T.npg = d.npg;
+  T.nact = ctx->nact;
+  T.nT = d.nT;
+  for (int i = 0; i < kMaxSpecies; ++i) {
+    T.act[i] = ctx->act[i];
+    T.kappav[i] = ctx->kappav[i];
+  }
+  T.lTint = ctx->lTint;
+  T.kappap = ctx->kappap;
+  T.dtauabs = ctx->dtauabs;
+  T.mass4 = ctx->mass4;
+  T.tdust = ctx->tdust;
+  T.tdust2 = ctx->tdust2;
+  T.ltot = ltot;
+  T.nphot = nphot;
+  T.nchanged = ctx->counter;
+  return HC_OK;
+}
+
+// Queue the update on the stream (no synchronisation, so it can be captured):
+// the changed-cell counter zeroed, the new temperatures into tdust2 and, with
+// apply, copied into tdust and the sublimation mask refreshed.
+int queue_temperature(hc_ctx *ctx, const TParams &T, bool apply) {
+  const size_t nc = ctx->ncellw * (size_t)ctx->nact;
+  int rc = zero_bytes(ctx, ctx->counter, sizeof(unsigned long long));
+  if (rc) return rc;
+  const unsigned blocks = (unsigned)std::min<size_t>((nc + 255) / 256, 4096);
+  hipLaunchKernelGGL(lucy_temperature, dim3(std::max(blocks, 1u)), dim3(256), 0, ctx->stream, T);
+  HCK(hipGetLastError());
+  if (apply) {
+    hipLaunchKernelGGL(copy_words, dim3(std::max(std::min(blocks, 2048u), 1u)), dim3(256), 0,
+                       ctx->stream, (const double *)ctx->tdust2, ctx->tdust, nc);
+    HCK(hipGetLastError());
+    rc = queue_mask(ctx);
+  }
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int32_t seed2,
+                     int32_t gen, uint64_t seg0, hc_tallies *out, hc_trace *trace) {
+  if (!ctx || !out) return HC_E_ARG;
+  PassLaunch L;
+  int rc = prepare_pass(ctx, np, ipeel, seed1, seed2, gen, seg0, trace != nullptr, L);
+  if (!rc) rc = zero_tallies(ctx, L, np, trace != nullptr);
+  if (rc) return rc;
+  const hc_dims &d = ctx->dims;
+  const size_t nc = L.nc, nsed = L.nsed, npeelsed = L.npeelsed, nimg = L.nimg;
+  const bool cube = L.cube;
+  hipStream_t s = ctx->stream;
+  HCK(hipEventRecord(ctx->ev0, s));
+  if (np > 0) hipLaunchKernelGGL(L.kern, dim3(L.blocks), dim3(L.block), L.shm, s, L.P);
+  HCK(hipGetLastError());
+  HCK(hipEventRecord(ctx->ev1, s));
+  HCK(hipStreamSynchronize(s));
+  float ms = 0.f;
+  HCK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  out->kernel_ms = ms;
+
+  // download
+  rc = download_scalars(ctx, out);
+  if (rc) return rc;
   unsigned long long counters[2] = {0, 0};
   HC_D2H(counters, ctx->counter, sizeof counters);
   out->max_packet_work = (double)counters[1];
@@ -2361,67 +2519,83 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
 int hc_temperature(hc_ctx *ctx, const double *massarr, double ltot, double nphot, double *tdust2,
                    int64_t *nchanged) {
   if (!ctx) return HC_E_ARG;
-  if (!ctx->have_grid || !ctx->have_dust || !ctx->dtauabs)
-    return fail(ctx, HC_E_STATE, "hc_temperature needs a completed pass");
-  if (!massarr && !ctx->mass4)
-    return fail(ctx, HC_E_ARG, "hc_temperature: massarr required on the first call");
-  const hc_dims &d = ctx->dims;
-  HCK(hipSetDevice(ctx->device));
-  const size_t nc = ctx->ncellw * (size_t)ctx->nact;
-  std::vector<double> tmp;
-  int rc = 0;
-  if (massarr) {
-    to_device_layout(ctx, massarr, tmp);
-    rc = upload(ctx, &ctx->mass4, tmp.data(), nc);
-    if (rc) return rc;
-  }
-  rc = dalloc(ctx, &ctx->tdust2, nc);
-  if (rc) return rc;
   TParams T;
-  std::memset(&T, 0, sizeof T);
-  T.nrg = d.nrg;
-  T.ntg = d.ntg;
-  T.npg = d.npg;
-  T.nact = ctx->nact;
-  T.nT = d.nT;
-  for (int i = 0; i < kMaxSpecies; ++i) {
-    T.act[i] = ctx->act[i];
-    T.kappav[i] = ctx->kappav[i];
-  }
-  T.lTint = ctx->lTint;
-  T.kappap = ctx->kappap;
-  T.dtauabs = ctx->dtauabs;
-  T.mass4 = ctx->mass4;
-  T.tdust = ctx->tdust;
-  T.tdust2 = ctx->tdust2;
-  T.ltot = ltot;
-  T.nphot = nphot;
-  T.nchanged = ctx->counter;
-  HCK(hipMemsetAsync(ctx->counter, 0, sizeof(unsigned long long), ctx->stream));
-  const unsigned blocks = (unsigned)std::min<size_t>((nc + 255) / 256, 4096);
-  hipLaunchKernelGGL(lucy_temperature, dim3(std::max(blocks, 1u)), dim3(256), 0, ctx->stream, T);
-  HCK(hipGetLastError());
+  int rc = prepare_temperature(ctx, massarr, ltot, nphot, T);
+  if (!rc) rc = queue_temperature(ctx, T, tdust2 == nullptr);
+  if (rc) return rc;
+  const hc_dims &d = ctx->dims;
+  const size_t nc = ctx->ncellw * (size_t)ctx->nact;
   unsigned long long nch = 0;
-  HCK(hipMemcpyAsync(&nch, ctx->counter, sizeof nch, hipMemcpyDeviceToHost, ctx->stream));
+  HC_D2H(&nch, ctx->counter, sizeof nch);
   if (tdust2) {
-    tmp.resize(nc);
-    HCK(hipMemcpyAsync(tmp.data(), ctx->tdust2, nc * sizeof(double), hipMemcpyDeviceToHost,
-                       ctx->stream));
-    HCK(hipStreamSynchronize(ctx->stream));
+    std::vector<double> tmp(nc);
+    HC_D2H(tmp.data(), ctx->tdust2, nc * sizeof(double));
     for (size_t i = 0; i < ctx->ncellw * (size_t)d.ndg; ++i) tdust2[i] = 0.1 / 11605.0;
     for (int k = 0; k < ctx->nact; ++k) {
       double *dst = tdust2 + ctx->ncellw * (size_t)(ctx->act[k] - 1);
       for (size_t c = 0; c < ctx->ncellw; ++c) dst[c] = tmp[c * (size_t)ctx->nact + k];
     }
-  } else {
-    // apply on the device: tdust <- tdust2 and refresh the sublimation mask
-    HCK(hipMemcpyAsync(ctx->tdust, ctx->tdust2, nc * sizeof(double), hipMemcpyDeviceToDevice,
-                       ctx->stream));
-    const int rc2 = refresh_mask(ctx);
-    if (rc2) return rc2;
   }
-  HCK(hipStreamSynchronize(ctx->stream));
   if (nchanged) *nchanged = (int64_t)nch;
+  return HC_OK;
+}
+
+int hc_run_lucy(hc_ctx *ctx, int32_t niter, int64_t np, int32_t seed1, int32_t seed2,
+                int32_t gen, uint64_t seg0, const double *massarr, double ltot, hc_tallies *out,
+                int64_t *nchanged) {
+  if (!ctx || !out) return HC_E_ARG;
+  if (niter < 1) return fail(ctx, HC_E_ARG, "hc_run_lucy: niter must be >= 1");
+  PassLaunch L;
+  int rc = prepare_pass(ctx, np, 0, seed1, seed2, gen, seg0, false, L);
+  TParams T;
+  if (!rc) rc = prepare_temperature(ctx, massarr, ltot, (double)np, T);
+  if (!rc) rc = dalloc(ctx, &ctx->segdev, 1);
+  if (rc) return rc;
+  L.P.seg_dev = ctx->segdev;
+  hipStream_t s = ctx->stream;
+  const unsigned long long seg_h = seg0;
+  HC_H2D(ctx->segdev, &seg_h, sizeof seg_h);
+  // one Lucy iteration as a graph: zero the tallies, the pass, the
+  // temperature update applied on the device, the next segment base
+  HCK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int crc = zero_tallies(ctx, L, np, false);
+  if (!crc && np > 0) {
+    hipLaunchKernelGGL(L.kern, dim3(L.blocks), dim3(L.block), L.shm, s, L.P);
+    if (hipGetLastError() != hipSuccess) crc = fail(ctx, HC_E_HIP, "hc_run_lucy: pass launch");
+  }
+  if (!crc) crc = queue_temperature(ctx, T, true);
+  if (!crc) {
+    hipLaunchKernelGGL(advance_segment, dim3(1), dim3(64), 0, s, ctx->segdev,
+                       (unsigned long long)np);
+    if (hipGetLastError() != hipSuccess) crc = fail(ctx, HC_E_HIP, "hc_run_lucy: launch");
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t ce = hipStreamEndCapture(s, &graph);
+  if (crc || ce != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    return crc ? crc : fail(ctx, HC_E_HIP, std::string("hc_run_lucy: capture: ") +
+                                               hipGetErrorString(ce));
+  }
+  hipGraphExec_t exec = nullptr;
+  hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e == hipSuccess) e = hipEventRecord(ctx->ev0, s);
+  for (int i = 0; i < niter && e == hipSuccess; ++i) e = hipGraphLaunch(exec, s);
+  if (e == hipSuccess) e = hipEventRecord(ctx->ev1, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess)
+    return fail(ctx, HC_E_HIP, std::string("hc_run_lucy: replay: ") + hipGetErrorString(e));
+  float ms = 0.f;
+  HCK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  out->kernel_ms = ms;
+  rc = download_scalars(ctx, out);
+  if (rc) return rc;
+  unsigned long long counters[2] = {0, 0};
+  HC_D2H(counters, ctx->counter, sizeof counters);
+  if (nchanged) *nchanged = (int64_t)counters[0];
+  out->max_packet_work = (double)counters[1];
   return HC_OK;
 }
 
@@ -2437,6 +2611,10 @@ int hc_set_option(hc_ctx *ctx, const char *name, int64_t value) {
   } else if (n == "max_work") {
     if (value < 1 || value > 0x7fffffff) return fail(ctx, HC_E_ARG, "max_work must be in 1..2^31-1");
     ctx->max_work = (uint32_t)value;
+  } else if (n == "peel_queue") {
+    if (value != 0 && (value < 65 || value > 1024))
+      return fail(ctx, HC_E_ARG, "peel_queue must be 0 or in 65..1024");
+    ctx->peel_queue = (int)value;
   } else if (n == "blocks_per_cu") {
     if (value < 0 || value > 64) return fail(ctx, HC_E_ARG, "blocks_per_cu must be in 0..64");
     ctx->blocks_per_cu = (int)value;
@@ -2457,7 +2635,7 @@ int hc_destroy(hc_ctx *ctx) {
                   ctx->sinpe,  ctx->pow1,    ctx->pow2,    ctx->dtauabs, ctx->nabs,   ctx->jmean,
                   ctx->killed, ctx->dbig,    ctx->sed,     ctx->peel,    ctx->img_i,  ctx->img_q,
                   ctx->img_u,  ctx->img_v,   ctx->si_init, ctx->scal,    ctx->mass4,  ctx->counter,
-                  ctx->trace,  ctx->guide,   ctx->tdust2,  ctx->cdfguide};
+                  ctx->trace,  ctx->guide,   ctx->tdust2,  ctx->cdfguide, ctx->segdev};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);

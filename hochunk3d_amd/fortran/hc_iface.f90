@@ -102,6 +102,20 @@ module hc_iface
       integer(c_int64_t), intent(out) :: nchanged
     end function hc_temperature
 
+    ! niter Lucy iterations replayed from one captured HIP graph (the body of
+    ! newtts.F90:304-330 without the convergence test)
+    integer(c_int) function hc_run_lucy(ctx, niter, np, seed1, seed2, gen, seg0, massarr, &
+                                        ltot, out, nchanged) bind(C, name='hc_run_lucy')
+      import :: c_int, c_ptr, c_double, c_int64_t, c_int32_t, hc_tallies
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: niter, seed1, seed2, gen
+      integer(c_int64_t), value :: np, seg0
+      real(c_double), intent(in) :: massarr(*)
+      real(c_double), value :: ltot
+      type(hc_tallies), intent(inout) :: out
+      integer(c_int64_t), intent(out) :: nchanged
+    end function hc_run_lucy
+
     integer(c_int) function hc_set_option(ctx, name, value) bind(C, name='hc_set_option')
       import :: c_int, c_ptr, c_char, c_int64_t
       type(c_ptr), value :: ctx

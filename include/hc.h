@@ -156,19 +156,42 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
 int hc_temperature(hc_ctx *ctx, const double *massarr, double ltot, double nphot,
                    double *tdust2, int64_t *nchanged);
 
+/* niter Lucy iterations without peel-off, each one hc_run_iteration pass of
+ * np packets followed by hc_temperature applied on the device: the body of
+ * the driver's iteration loop (newtts.F90:304-330) without its convergence
+ * test, which the caller keeps.  Iteration i (0-based) draws from segments
+ * seg0 + i*np + p, so the result is that of the same calls made one by one
+ * (up to the order of the floating-point tally sums).  One iteration is
+ * captured from the context's stream as a HIP graph and replayed niter
+ * times with no host round trip.  out receives the scalar tallies and
+ * max_packet_work of the last pass and, in kernel_ms, the device time of
+ * all iterations; its array pointers are not used.  nchanged (nullable)
+ * receives the last update's changed (cell, species) count.  massarr as in
+ * hc_temperature. */
+int hc_run_lucy(hc_ctx *ctx, int32_t niter, int64_t np, int32_t seed1, int32_t seed2,
+                int32_t gen, uint64_t seg0, const double *massarr, double ltot,
+                hc_tallies *out, int64_t *nchanged);
+
 /* Launch-shape knobs of the transport kernel, set by the autotuner
  * (hochunk3d_amd/autotune.py); no counterpart in the reference, whose packet
  * loop is serial.  Names:
- *   "regen_lanes"   1..64: lanes of a wavefront that must wait for an event
- *                   before the wavefront stops walking and runs the events
- *                   (default 16);
+ *   "regen_lanes"   1..64: lanes of a wavefront that must wait for one event
+ *                   kind before the wavefront stops walking and runs the
+ *                   events (default 16);
+ *   "regen_run_lanes" 1..64: once the walk has stopped, every event kind
+ *                   with at least this many waiting lanes runs (capped at
+ *                   regen_lanes; default 16);
+ *   "peel_queue"    0 or 65..1024: peel-off records per wavefront queue in
+ *                   LDS (0 = 96, or fewer when the LDS left by the grid's
+ *                   walls and dust tables holds fewer);
  *   "blocks_per_cu" 0..64: workgroups of 256 lanes per compute unit, 0 =
  *                   as many as fit (default 0);
  *   "max_work"      1..2^31-1: events + cell crossings after which a packet
  *                   is dropped and counted in n_warn (default 2^24, far
  *                   above any packet of the reference's models; it bounds
  *                   the kernel's run time against runaway packets).
- * Results do not depend on the first two. */
+ * Apart from max_work, which drops the packets that exceed it, results do
+ * not depend on them beyond the order of the floating-point tally sums. */
 int hc_set_option(hc_ctx *ctx, const char *name, int64_t value);
 
 int hc_destroy(hc_ctx *ctx);
