@@ -295,12 +295,16 @@ def main():
     roof = {
         "bound": "mfma", "achieved": round(achieved, 4), "peak": rf.PEAK_FP64_TFLOPS,
         "unit": "TFLOP/s", "frac": round(achieved / rf.PEAK_FP64_TFLOPS, 5),
-        "traffic": None,
+        "traffic": (rf.CONFIG2_STEP_HBM_READ + rf.CONFIG2_STEP_HBM_WRITE
+                    if args.config == "config2" and np_ == rf.CONFIG2_STEP_PACKETS else None),
         "kernel": "hc::transport<double, 4, false, false>",
         "note": "FP64 vector ALU work (no MFMA instructions; MI355X's FP64 vector and matrix "
                 "peaks are both 78.6 TFLOP/s); algorithmic FLOPs = work counters of the pass x "
                 "per-unit figures of hochunk3d_amd/roofline.py, over the kernel time measured "
-                "with HIP events on the library's stream",
+                "with HIP events on the library's stream; traffic = HBM bytes of one step "
+                "measured with rocprofv3 FETCH_SIZE/WRITE_SIZE (scaled by a 256 MiB copy's "
+                "readings) on this workload, 6.1x the algorithmic density bytes "
+                f"({rf.B_CROSS_FP64} B per crossing)",
         "kernel_ms_per_step": round(kms64 / args.steps, 3),
     }
 
