@@ -74,7 +74,6 @@ struct Params {
   // launch shape
   int regen_lanes;  // waiting lanes (1..64) of one event kind that end a walk phase
   int regen_run_lanes;  // then every kind with this many waiters runs (<= regen_lanes)
-  int mixed_roots;      // FP32 path: wall roots with FP32 square roots and divisions
   int peel_q;       // peel records per wavefront queue
   int walls_lds;    // bytes of LDS holding the wall arrays
   int opac_base;    // LDS byte offset of the opacity tables
@@ -223,108 +222,6 @@ __device__ __forceinline__ bool next_wall(const WallView &W, double ux, double u
   if (P.npg > 2 && P.ntg > 2 && !(x == 0.0 && y == 0.0))
     phiface(w, W.a[ip - 1], W.b[ip - 1], W.a[ip], W.b[ip], ux, uy, x, y);
   return w.next(on_wall, dt, face);
-}
-
-// next_wall for the FP32 path: the quadratics' coefficients, discriminants
-// and the phi planes' numerators and denominators in FP64 (where walls
-// cancel: a point near a wall, a ray grazing one), only the square roots and
-// divisions in FP32 with the native instructions (~1 ulp).  Both roots of a
-// quadratic come from the cancellation-free pair q = -(b + sign(b) sqrt(D))/2,
-// t = q/a and t = c/q, so every root has a relative error of a few 1e-7 and
-// a crossing lands within that fraction of its step from the wall.  The
-// face, polar-axis and nappe rules are next_wall's.
-__device__ __forceinline__ float rcp_f32(float x) { return __builtin_amdgcn_rcpf(x); }
-
-__device__ __forceinline__ void quad_roots_mixed(Walls &w, double a, double b, double c,
-                                                 double z, double uz, int face, bool split) {
-  const double disc = b * b - 4.0 * a * c;
-  if (!(disc > 0.0)) return;
-  const double sq = (double)__builtin_amdgcn_sqrtf((float)disc);
-  const double q = -0.5 * (b + (b >= 0.0 ? sq : -sq));
-  const float qf = (float)q;
-  const double t1 = (double)(qf * rcp_f32((float)a));
-  const double t2 = (double)((float)c * rcp_f32(qf));
-  if (split) {
-    w.insert(t1, (z + t1 * uz) > 0.0 ? 3 : 4);
-    w.insert(t2, (z + t2 * uz) > 0.0 ? 3 : 4);
-  } else {
-    w.insert(t1, face);
-    w.insert(t2, face);
-  }
-}
-
-__device__ __forceinline__ void cone_roots_mixed(Walls &w, double ux, double uy, double uz,
-                                                 double x, double y, double z, double t2, int face,
-                                                 bool split) {
-  if (t2 == 0.0) return;
-  if (t2 == -1.0) {
-    if (uz != 0.0) w.insert((double)((float)(-z) * rcp_f32((float)uz)), face);
-    return;
-  }
-  const double a = ux * ux + uy * uy - t2 * uz * uz;
-  const double b = 2.0 * (x * ux + y * uy - t2 * z * uz);
-  const double c = x * x + y * y - t2 * z * z;
-  if (a == 0.0) {
-    if (b != 0.0) {
-      const double t = (double)((float)(-c) * rcp_f32((float)b));
-      w.insert(t, split ? ((z + t * uz) > 0.0 ? 3 : 4) : face);
-    }
-    return;
-  }
-  quad_roots_mixed(w, a, b, c, z, uz, face, split);
-}
-
-__device__ __forceinline__ bool next_wall_mixed(const WallView &W, double ux, double uy,
-                                                double uz, double x, double y, double z, int ir,
-                                                int it, int ip, bool on_wall, double &dt,
-                                                int &face) {
-  const Params &P = c_P;
-  Walls w;
-  w.reset();
-  const double b = 2.0 * (x * ux + y * uy + uz * z);
-  const double rsq = x * x + y * y + z * z;
-  quad_roots_mixed(w, 1.0, b, rsq - W.r2[ir - 1], 0.0, 0.0, 1, false);
-  quad_roots_mixed(w, 1.0, b, rsq - W.r2[ir], 0.0, 0.0, 2, false);
-  if (P.ntg > 2) {
-    const double t2a = W.t2[it - 1], t2b = W.t2[it];
-    if (t2a == t2b && t2a > 0.0) {
-      cone_roots_mixed(w, ux, uy, uz, x, y, z, t2a, 3, true);
-    } else {
-      cone_roots_mixed(w, ux, uy, uz, x, y, z, t2a, 3, false);
-      cone_roots_mixed(w, ux, uy, uz, x, y, z, t2b, 4, false);
-    }
-  }
-  if (P.npg > 2 && P.ntg > 2 && !(x == 0.0 && y == 0.0)) {
-    const double nan = __builtin_nan("");
-    const double d1 = W.a[ip - 1] * ux + W.b[ip - 1] * uy;
-    const double n1 = -(W.a[ip - 1] * x + W.b[ip - 1] * y);
-    w.insert(d1 != 0.0 ? (double)((float)n1 * rcp_f32((float)d1)) : nan, 5);
-    const double d2 = W.a[ip] * ux + W.b[ip] * uy;
-    const double n2 = -(W.a[ip] * x + W.b[ip] * y);
-    w.insert(d2 != 0.0 ? (double)((float)n2 * rcp_f32((float)d2)) : nan, 6);
-  }
-  // Where two roots the choice compares lie within 1e-5 of each other (a ray
-  // through a cell corner), the FP32 roots could order them differently from
-  // FP64 and step into a cell the position is not in: the caller then takes
-  // the FP64 roots.  Elsewhere their 1e-7 error cannot change the choice.
-  const double tol = 1e-5;
-  if (w.tp2 - w.tp1 <= tol * w.tp2) return false;
-  if (on_wall && fabs(fabs(w.tn1) - fabs(w.tp1)) <= tol * fmax(fabs(w.tn1), fabs(w.tp1)))
-    return false;
-  return w.next(on_wall, dt, face);
-}
-
-// The wall roots of the path's precision: on the FP32 path the mixed roots,
-// and the FP64 roots where those leave the choice open or find no wall.
-template <class R>
-__device__ __forceinline__ bool next_wall_r(const WallView &W, double ux, double uy, double uz,
-                                            double x, double y, double z, int ir, int it, int ip,
-                                            bool on_wall, double &dt, int &face) {
-  if constexpr (sizeof(R) == sizeof(float)) {
-    if (c_P.mixed_roots && next_wall_mixed(W, ux, uy, uz, x, y, z, ir, it, ip, on_wall, dt, face))
-      return true;
-  }
-  return next_wall(W, ux, uy, uz, x, y, z, ir, it, ip, on_wall, dt, face);
 }
 
 // The phi cell a ray leaving the polar axis enters: the one holding the
@@ -500,8 +397,7 @@ __device__ __forceinline__ int walk_step(const WallView &W, Pkt<R, NA> &k, doubl
   }
   double dt;
   int face;
-  if (!next_wall_r<R>(W, k.ux, k.uy, k.uz, k.x, k.y, k.z, k.ir, k.it, k.ip, k.on_wall, dt,
-                      face)) {
+  if (!next_wall(W, k.ux, k.uy, k.uz, k.x, k.y, k.z, k.ir, k.it, k.ip, k.on_wall, dt, face)) {
     acc.err_wall++;
     return ST_ERROR;
   }
@@ -509,9 +405,10 @@ __device__ __forceinline__ int walk_step(const WallView &W, Pkt<R, NA> &k, doubl
   if (tsum + dtau >= tau) {
     double ds = (tau - tsum) / opac;
     if constexpr (sizeof(R) == sizeof(float)) {
-      // the FP32 path's roots are good to a few 1e-7 of the step: an
-      // interaction at the wall stays 1e-6 of the step inside the cell, so
-      // that a walk from it cannot start beyond a wall of its cell
+      // an interaction that rounding puts at the wall stays 1e-6 of the step
+      // inside the cell, so the walk from it starts, in any new direction,
+      // from a point its cell's walls bound (the FP64 path keeps the
+      // reference's clamp to the wall itself)
       const double dmax = dt * (1.0 - 1e-6);
       if (ds > dmax) ds = dmax;
     } else {
@@ -674,7 +571,7 @@ __device__ __forceinline__ PeelOut peeloff(const WallView &W, const PeelRec<R, N
     int face;
     if (x1 == 0.0 && y1 == 0.0) ip = axis_phi_cell(ux1, uy1, ip);
     if (out.ncross > max_cross ||
-        !next_wall_r<R>(W, ux1, uy1, uz1, x1, y1, z1, ir, it, ip, on_wall, dt, face)) {
+        !next_wall(W, ux1, uy1, uz1, x1, y1, z1, ir, it, ip, on_wall, dt, face)) {
       out.err = 1;
       return out;
     }
@@ -1813,7 +1710,6 @@ struct hc_ctx {
   int regen_run_lanes = 16;
   int blocks_per_cu = 0;
   int peel_queue = 0;  // records per wavefront peel queue, 0 = 96 or what fits
-  int fp32_roots = 0;  // FP32 path: 1 mixed-precision wall roots, 0 FP64 roots
   size_t lds_per_block = 64 * 1024;  // the device's limit (hc_create)
   uint32_t max_work = 1u << 24;
 };
@@ -2339,7 +2235,6 @@ int prepare_pass(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int32_t 
   P.trace = trace ? ctx->trace : nullptr;
   P.regen_lanes = ctx->regen_lanes;
   P.regen_run_lanes = ctx->regen_run_lanes;
-  P.mixed_roots = ctx->fp32_roots;
   P.max_work = ctx->max_work;
   P.costarr = ctx->costarr;
   P.rddustmin = S.rddustmin;
@@ -2725,9 +2620,6 @@ int hc_set_option(hc_ctx *ctx, const char *name, int64_t value) {
   } else if (n == "max_work") {
     if (value < 1 || value > 0x7fffffff) return fail(ctx, HC_E_ARG, "max_work must be in 1..2^31-1");
     ctx->max_work = (uint32_t)value;
-  } else if (n == "fp32_roots") {
-    if (value != 0 && value != 1) return fail(ctx, HC_E_ARG, "fp32_roots must be 0 or 1");
-    ctx->fp32_roots = (int)value;
   } else if (n == "peel_queue") {
     if (value != 0 && (value < 65 || value > 1024))
       return fail(ctx, HC_E_ARG, "peel_queue must be 0 or in 65..1024");

@@ -181,14 +181,6 @@ int hc_run_lucy(hc_ctx *ctx, int32_t niter, int64_t np, int32_t seed1, int32_t s
  *   "regen_run_lanes" 1..64: once the walk has stopped, every event kind
  *                   with at least this many waiting lanes runs (capped at
  *                   regen_lanes; default 16);
- *   "fp32_roots"    0 or 1: on the FP32 path, 1 takes the square roots and
- *                   divisions of the wall roots in FP32 (coefficients and
- *                   discriminants stay FP64; near-corner choices and walks
- *                   that find no wall fall back to the FP64 roots).  Default
- *                   0 (FP64 roots): with 1 the 64x64x32-cell model's pass is
- *                   1.6x faster, but about 1.6e-5 of the packets lose the
- *                   radial cell of their position and end with no wall found
- *                   (counted in n_err_nowall);
  *   "peel_queue"    0 or 65..1024: peel-off records per wavefront queue in
  *                   LDS (0 = 96, or fewer when the LDS left by the grid's
  *                   walls and dust tables holds fewer);
