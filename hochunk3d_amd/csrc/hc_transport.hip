@@ -1808,6 +1808,10 @@ int hc_create(const hc_dims *dims, int device, hc_ctx **out) {
   if (dims->precision != HC_FP64 && dims->precision != HC_FP32) return HC_E_UNSUPPORTED;
   hc_ctx *ctx = new hc_ctx();
   ctx->dims = *dims;
+  // FP32 packets very rarely lose their walk to rounding and circle (one in
+  // ~2e6 on the 64x64x32-cell model); 2^21 events + crossings, still ~7x the
+  // longest packet of the benchmark models, bounds the time such a lane holds
+  if (dims->precision == HC_FP32) ctx->max_work = 1u << 21;
   ctx->device = device;
   ctx->ncellw = (size_t)dims->nrg * dims->ntg * dims->npg;
   for (int i = 0; i < kMaxSpecies; ++i) ctx->slot[i] = -1;

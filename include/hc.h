@@ -187,9 +187,10 @@ int hc_run_lucy(hc_ctx *ctx, int32_t niter, int64_t np, int32_t seed1, int32_t s
  *   "blocks_per_cu" 0..64: workgroups of 256 lanes per compute unit, 0 =
  *                   as many as fit (default 0);
  *   "max_work"      1..2^31-1: events + cell crossings after which a packet
- *                   is dropped and counted in n_warn (default 2^24, far
- *                   above any packet of the reference's models; it bounds
- *                   the kernel's run time against runaway packets).
+ *                   is dropped and counted in n_warn (default 2^24 on the
+ *                   FP64 path and 2^21 on the FP32 path, both far above any
+ *                   packet of the reference's models, ~3e5; it bounds the
+ *                   kernel's run time against runaway packets).
  * Apart from max_work, which drops the packets that exceed it, results do
  * not depend on them beyond the order of the floating-point tally sums. */
 int hc_set_option(hc_ctx *ctx, const char *name, int64_t value);
