@@ -139,3 +139,28 @@ def test_accretion_peeled_pass_matches_oracle(acc_pair):
     compare_traces(rg.trace, rc["trace"])
     _check_tallies(rg, rc)
     rel_close(rg.peel, rc["peel"], rtol=1e-9, atol=1e-12)
+
+
+def test_even_theta_walls_fractal_model_matches_oracle():
+    """The 200x199x200-cell benchmark grid has an even theta-wall count: its
+    midplane cell lies between the two nappes of one cone.  The same layout
+    (and the fractal clumps) on a small grid, traced packet by packet."""
+    from hochunk3d_amd import config2
+    from hochunk3d_amd._lib import Transport
+    from oracle.pyoracle import Oracle
+    m = build_model(config2(nr=40, ntheta=41, nphi=24, nximg=21, nfreq=50))
+    assert m.ntg % 2 == 0 and m.tan2arr[m.ntg // 2] == m.tan2arr[m.ntg // 2 - 1]
+    m.tdust[...] = synthetic_temperature(m)
+    g = Transport(m)
+    try:
+        rg = g.run(6000, ipeel=0, trace=True)
+        rp = g.run(400, ipeel=1, trace=True, seg0=10_000)
+    finally:
+        g.close()
+    c = Oracle(m)
+    rc = c.run(6000, ipeel=0, trace=True)
+    compare_traces(rg.trace, rc["trace"])
+    _check_tallies(rg, rc)
+    cp = c.run(400, ipeel=1, trace=True, seg0=10_000)
+    compare_traces(rp.trace, cp["trace"])
+    rel_close(rp.peel, cp["peel"], rtol=1e-9, atol=1e-12)
