@@ -1,5 +1,6 @@
 """The parameter-grid generator: ranges, determinism and the cell cap."""
 import numpy as np
+import pytest
 
 from hochunk3d_amd.batch import parameter_grid
 
@@ -33,3 +34,18 @@ def test_cell_cap_and_packet_scale():
     g = parameter_grid(20, packet_scale=1e-3, max_cells=100_000)
     assert all(m.cfg.nr * m.cfg.ntheta * m.cfg.nphi <= 100_000 for m in g)
     assert all(1_000 <= m.packets <= 20_000 for m in g)
+
+
+def test_temperature_comparison_flags_real_differences():
+    from hochunk3d_amd.testing import assert_temperatures_close
+    rng = np.random.default_rng(1)
+    t = rng.uniform(0.01, 0.1, size=(20, 20, 10, 4))
+    assert_temperatures_close(t * (1 + 1e-6), t, True)
+    few = t.copy()
+    few.flat[:5] *= 1.003                       # a few cells of diverged histories
+    assert_temperatures_close(few, t, False)
+    with pytest.raises(AssertionError):
+        assert_temperatures_close(few, t, True)
+    noisy = t * (1 + 0.01 * rng.standard_normal(t.shape))   # other packets: MC noise
+    with pytest.raises(AssertionError):
+        assert_temperatures_close(noisy, t, False)

@@ -11,7 +11,8 @@ import pytest
 
 from hochunk3d_amd.batch import parameter_grid, run_batch
 from hochunk3d_amd.model import build_model
-from hochunk3d_amd.testing import compare_traces, synthetic_temperature
+from hochunk3d_amd.testing import (assert_temperatures_close, compare_traces,
+                                   synthetic_temperature)
 
 pytestmark = pytest.mark.gpu
 
@@ -41,11 +42,17 @@ def test_concurrent_batch_matches_sequential():
     par = run_batch(models, iterations=2, streams=4)
     assert seq.packets == par.packets == 2 * sum(gm.packets for gm in models)
     for a, b in zip(seq.per_model, par.per_model):
-        for sa, sb in zip(a["scalars"], b["scalars"]):
+        for i, (sa, sb) in enumerate(zip(a["scalars"], b["scalars"])):
+            # the first pass starts from the same temperatures: the same
+            # histories; later passes from temperatures equal up to the sum
+            # order of the tallies, where a few packets in a million may differ
             for k in ("ncross", "nscat_ev", "nabs_ev", "n_err_nowall", "n_err_dust"):
-                assert sa[k] == sb[k], k
-            np.testing.assert_allclose(sb["flux"], sa["flux"], rtol=1e-12)
-        np.testing.assert_allclose(b["tdust"], a["tdust"], rtol=1e-9)
+                if i == 0:
+                    assert sa[k] == sb[k], k
+                else:
+                    assert abs(sa[k] - sb[k]) <= 1e-5 * max(sa[k], 1.0), k
+            np.testing.assert_allclose(sb["flux"], sa["flux"], rtol=1e-12 if i == 0 else 1e-5)
+        assert_temperatures_close(b["tdust"], a["tdust"], False)
     # the same batch with every model's iterations replayed from a graph
     grf = run_batch(models, iterations=2, streams=4, graph=True)
     for a, b in zip(seq.per_model, grf.per_model):
@@ -53,4 +60,4 @@ def test_concurrent_batch_matches_sequential():
         # bits (tally sum order): a few packets in a million may differ
         for k in ("ncross", "nscat_ev", "nabs_ev"):
             assert abs(a["scalars"][-1][k] - b["scalars"][-1][k]) <= 1e-5 * a["scalars"][-1][k], k
-        np.testing.assert_allclose(b["tdust"], a["tdust"], rtol=1e-6)
+        assert_temperatures_close(b["tdust"], a["tdust"], False)
