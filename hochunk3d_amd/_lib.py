@@ -27,7 +27,7 @@ EXPORTS = (
     "hc_create", "hc_set_grid", "hc_set_temperature", "hc_get_temperature", "hc_set_dust",
     "hc_set_sources",
     "hc_run_iteration", "hc_temperature", "hc_run_lucy", "hc_set_option", "hc_destroy",
-    "hc_last_error",
+    "hc_last_error", "hc_build_density",
     "hc_abi_version",
 )
 
@@ -67,6 +67,18 @@ class HcTallies(C.Structure):
         ("kernel_ms", C.c_double),
     ] + [(n, C.c_double) for n in ("ncross", "nscat_ev", "nabs_ev", "npeelray", "npeelcross",
                                    "max_packet_work")]
+
+
+class HcDensityModel(C.Structure):
+    _fields_ = [
+        ("nrg", C.c_int32), ("ntg", C.c_int32), ("npg", C.c_int32), ("ndisk", C.c_int32),
+        ("rarr", _dp), ("thetarr", _dp), ("phiarr", _dp), ("fractal", _dp),
+        ("rmincgs", C.c_double), ("disk_id", C.c_int32 * 8),
+        ("disk_fmass", C.c_double * 8), ("disk_z1", C.c_double * 8), ("disk_b", C.c_double * 8),
+    ] + [(n, C.c_double) for n in ("a_exp", "rmind", "rmaxd", "massd", "rc", "rhoe0", "rmine",
+                                   "rmax", "rhoamb")] + [
+        ("env_id", C.c_int32), ("cav_id", C.c_int32),
+    ] + [(n, C.c_double) for n in ("c1e", "cavity_exp", "rho_cavity", "r1000")]
 
 
 TRACE_DTYPE = np.dtype([
@@ -112,6 +124,7 @@ def load() -> C.CDLL:
                                 C.c_int32, C.c_uint64, _dp, C.c_double, C.POINTER(HcTallies),
                                 C.POINTER(C.c_int64)]
     lib.hc_set_option.argtypes = [C.c_void_p, C.c_char_p, C.c_int64]
+    lib.hc_build_density.argtypes = [C.POINTER(HcDensityModel), C.c_int, _dp, _dp, _dp]
     lib.hc_destroy.argtypes = [C.c_void_p]
     lib.hc_last_error.argtypes = [C.c_void_p]
     lib.hc_last_error.restype = C.c_char_p
@@ -338,3 +351,30 @@ class Transport:
             self.close()
         except Exception:
             pass
+
+
+def build_density(fields: dict, nrg: int, ntg: int, npg: int, device: int = 0):
+    """hc_build_density: (densarr, massarr, vcell) in the reference layout.
+    `fields` holds the hc_density_model members (arrays as NumPy arrays)."""
+    lib = load()
+    keep = {}
+    m = HcDensityModel()
+    for k, v in fields.items():
+        if k in ("rarr", "thetarr", "phiarr", "fractal"):
+            if v is None:
+                continue
+            keep[k] = np.asfortranarray(v, dtype=np.float64)
+            setattr(m, k, keep[k].ctypes.data_as(_dp))
+        elif k in ("disk_id", "disk_fmass", "disk_z1", "disk_b"):
+            arr = getattr(m, k)
+            for i, x in enumerate(v):
+                arr[i] = x
+        else:
+            setattr(m, k, v)
+    dens = np.zeros((nrg, ntg, npg, 8), order="F")
+    mass = np.zeros((nrg, ntg, npg, 8), order="F")
+    vol = np.zeros((nrg, ntg, npg), order="F")
+    rc = lib.hc_build_density(C.byref(m), int(device), _ptr(dens), _ptr(mass), _ptr(vol))
+    if rc != 0:
+        raise HcError(f"hc_build_density failed ({rc})")
+    return dens, mass, vol

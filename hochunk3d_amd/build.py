@@ -24,7 +24,7 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
-SOURCES = ["hc_transport.hip"]
+SOURCES = ["hc_transport.hip", "hc_grid.hip"]
 
 
 def build(verbose: bool = False) -> str:

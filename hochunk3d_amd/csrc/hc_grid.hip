@@ -1,0 +1,230 @@
+// Cell densities, volumes and (rho V)^(1/4) of the benchmark models on the
+// GPU: the density part of gridset (gridset.f90:1002-1083) for two flared
+// disks (densdiskg.f90), an Ulrich infalling envelope (densenv.f90, the
+// streamline cubic solved by bisection) and a polynomial bipolar cavity
+// (envset.f90:25-33), with an optional clump multiplier of the envelope
+// (fractal_sph.f90 via gridset.f90:796-870, computed by the caller).
+//
+// One thread per cell evaluates the analytic densities at the cell centre
+// (gridset.f90:494-499, 609-620: log-mean radius, root-mean-square theta) and
+// the cell volume; the disk species are normalised to their mass with one
+// FP64 sum per species (block partial sums, then one atomic per block).  The
+// outputs use the reference's layout, (nrg,ntg,npg[,ndg]) column-major with
+// the last wall index of each dimension zero.
+
+#include "../../include/hc.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace hc_grid {
+
+constexpr double kMsol = 1.989e33;
+
+struct Dev {
+  int nrg, ntg, npg, ndisk;
+  const double *rarr, *thetarr, *phiarr, *mult;
+  double rmincgs;
+  int disk_id[8];
+  double disk_z1[8], disk_b[8], disk_norm[8];
+  double a_exp, rmind, rmaxd;
+  double rc, rhoe0, rmine, rmax, rhoamb;
+  int env_id, cav_id;
+  double c1e, cavity_exp, rho_cavity, r1000;
+  double *dens, *vol;  // (nrg,ntg,npg,8) and (nrg,ntg,npg)
+  double *mass_sum;    // [8] disk species mass before normalisation
+};
+
+// mu0 in [0,1] with mu0^3 + (rx - 1) mu0 - rx mu = 0: 60 bisection steps, as
+// the host construction does, so both land on the same root.
+__device__ double zerod(double xmu, double rx) {
+  double lo = 0.0, hi = 1.0;
+  for (int i = 0; i < 60; ++i) {
+    const double mid = 0.5 * (lo + hi);
+    const double f = mid * mid * mid + (rx - 1.0) * mid - rx * xmu;
+    if (f > 0.0)
+      hi = mid;
+    else
+      lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+__global__ __launch_bounds__(256) void densities(Dev D) {
+  const size_t nci = (size_t)(D.nrg - 1), ncj = (size_t)(D.ntg - 1), nck = (size_t)(D.npg - 1);
+  const size_t nc = nci * ncj * nck;
+  const size_t wall = (size_t)D.nrg * D.ntg * D.npg;
+  double part[8];
+  for (int d = 0; d < 8; ++d) part[d] = 0.0;
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
+       c += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(c % nci);
+    const int j = (int)((c / nci) % ncj);
+    const int k = (int)(c / (nci * ncj));
+    const size_t w = (size_t)i + (size_t)D.nrg * ((size_t)j + (size_t)D.ntg * (size_t)k);
+    const double r0 = D.rarr[i], r1 = D.rarr[i + 1];
+    const double t0 = D.thetarr[j], t1 = D.thetarr[j + 1];
+    const double R = pow(10.0, 0.5 * (log10(r0) + log10(r1)));
+    const double th = sqrt(0.5 * (t0 * t0 + t1 * t1));
+    const double vol = -(r1 * r1 * r1 - r0 * r0 * r0) * (cos(t1) - cos(t0)) *
+                       (D.phiarr[k + 1] - D.phiarr[k]) / 3.0 * D.rmincgs * D.rmincgs * D.rmincgs;
+    D.vol[w] = vol;
+    const double cost = cos(th), sint = sin(th);
+    const double rp = R * sint, zp = R * cost;
+    for (int d = 0; d < D.ndisk; ++d) {
+      const double zmr = D.disk_z1[d] * pow(rp, D.disk_b[d]);
+      const double q = zp / zmr;
+      const double gexp = q * q;
+      double shape = pow(rp, -D.a_exp) * (gexp < 120.0 ? exp(-0.5 * gexp) : 0.0);
+      if (!(rp >= D.rmind && rp <= D.rmaxd)) shape = 0.0;
+      D.dens[w + wall * (size_t)D.disk_id[d]] = shape;
+      part[d] += shape * vol;
+    }
+    // Ulrich envelope
+    const double rx = R / D.rc;
+    const double xmu = fabs(cost);
+    const double mu0 = fmax(zerod(xmu, rx), 1e-12);
+    const double factor = 1.0 / ((xmu / mu0) + 2.0 * (mu0 * mu0) / rx);
+    double env = D.rhoe0 * pow(rx, -1.5) * factor / sqrt(1.0 + xmu / mu0);
+    env = R > D.rmine ? fmax(env, D.rhoamb) : 0.0;
+    if (!(R < D.rmax)) env = 0.0;
+    // polynomial cavity
+    const double zup = D.c1e * pow(R * sint, D.cavity_exp);
+    const bool inhole = R * xmu > zup;
+    const double rr = R / D.r1000;
+    double cav = D.rho_cavity / (rr * rr);
+    if (!(R > D.rmine && R < D.rmax)) cav = 0.0;
+    double e = inhole ? 0.0 : env;
+    if (D.mult) e *= D.mult[c];
+    D.dens[w + wall * (size_t)D.env_id] = e;
+    D.dens[w + wall * (size_t)D.cav_id] = inhole ? cav : 0.0;
+  }
+  // per-species partial sums: wavefront reduction, one atomic per wavefront
+  for (int d = 0; d < D.ndisk; ++d) {
+    double v = part[d];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(&D.mass_sum[d], v);
+  }
+}
+
+__global__ __launch_bounds__(256) void normalise(Dev D, double *massarr) {
+  const size_t nci = (size_t)(D.nrg - 1), ncj = (size_t)(D.ntg - 1), nck = (size_t)(D.npg - 1);
+  const size_t nc = nci * ncj * nck;
+  const size_t wall = (size_t)D.nrg * D.ntg * D.npg;
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
+       c += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(c % nci);
+    const int j = (int)((c / nci) % ncj);
+    const int k = (int)(c / (nci * ncj));
+    const size_t w = (size_t)i + (size_t)D.nrg * ((size_t)j + (size_t)D.ntg * (size_t)k);
+    for (int d = 0; d < D.ndisk; ++d) {
+      const double m = D.mass_sum[d];
+      double &x = D.dens[w + wall * (size_t)D.disk_id[d]];
+      x = m > 0.0 ? x * (D.disk_norm[d] / m) : 0.0;
+    }
+    const double vol = D.vol[w];
+    for (int s = 0; s < 8; ++s) massarr[w + wall * (size_t)s] = pow(D.dens[w + wall * (size_t)s] * vol, 0.25);
+  }
+}
+
+}  // namespace hc_grid
+
+extern "C" int hc_build_density(const hc_density_model *m, int device, double *densarr,
+                                double *massarr, double *vcell) {
+  using namespace hc_grid;
+  if (!m || !densarr || !massarr || !vcell) return HC_E_ARG;
+  if (m->nrg < 2 || m->ntg < 2 || m->npg < 2 || m->ndisk < 0 || m->ndisk > 8 ||
+      m->env_id < 0 || m->env_id > 7 || m->cav_id < 0 || m->cav_id > 7 || !m->rarr ||
+      !m->thetarr || !m->phiarr)
+    return HC_E_ARG;
+  for (int d = 0; d < m->ndisk; ++d)
+    if (m->disk_id[d] < 0 || m->disk_id[d] > 7) return HC_E_ARG;
+  const size_t wall = (size_t)m->nrg * m->ntg * m->npg;
+  const size_t nc = (size_t)(m->nrg - 1) * (m->ntg - 1) * (m->npg - 1);
+  if (hipSetDevice(device) != hipSuccess) return HC_E_HIP;
+  hipStream_t s = nullptr;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return HC_E_HIP;
+  double *d_r = nullptr, *d_t = nullptr, *d_p = nullptr, *d_mult = nullptr, *d_dens = nullptr,
+         *d_vol = nullptr, *d_mass = nullptr, *d_sum = nullptr;
+  int rc = HC_OK;
+  auto ok = [&](hipError_t e) {
+    if (e != hipSuccess && rc == HC_OK) rc = HC_E_HIP;
+    return rc == HC_OK;
+  };
+  ok(hipMalloc((void **)&d_r, m->nrg * sizeof(double))) &&
+      ok(hipMalloc((void **)&d_t, m->ntg * sizeof(double))) &&
+      ok(hipMalloc((void **)&d_p, m->npg * sizeof(double))) &&
+      ok(hipMalloc((void **)&d_dens, wall * 8 * sizeof(double))) &&
+      ok(hipMalloc((void **)&d_mass, wall * 8 * sizeof(double))) &&
+      ok(hipMalloc((void **)&d_vol, wall * sizeof(double))) &&
+      ok(hipMalloc((void **)&d_sum, 8 * sizeof(double))) &&
+      (!m->fractal || ok(hipMalloc((void **)&d_mult, nc * sizeof(double)))) &&
+      ok(hipMemcpyAsync(d_r, m->rarr, m->nrg * sizeof(double), hipMemcpyHostToDevice, s)) &&
+      ok(hipMemcpyAsync(d_t, m->thetarr, m->ntg * sizeof(double), hipMemcpyHostToDevice, s)) &&
+      ok(hipMemcpyAsync(d_p, m->phiarr, m->npg * sizeof(double), hipMemcpyHostToDevice, s)) &&
+      (!m->fractal ||
+       ok(hipMemcpyAsync(d_mult, m->fractal, nc * sizeof(double), hipMemcpyHostToDevice, s))) &&
+      ok(hipMemsetAsync(d_dens, 0, wall * 8 * sizeof(double), s)) &&
+      ok(hipMemsetAsync(d_mass, 0, wall * 8 * sizeof(double), s)) &&
+      ok(hipMemsetAsync(d_vol, 0, wall * sizeof(double), s)) &&
+      ok(hipMemsetAsync(d_sum, 0, 8 * sizeof(double), s));
+  if (rc == HC_OK) {
+    Dev D;
+    std::memset(&D, 0, sizeof D);
+    D.nrg = m->nrg;
+    D.ntg = m->ntg;
+    D.npg = m->npg;
+    D.ndisk = m->ndisk;
+    D.rarr = d_r;
+    D.thetarr = d_t;
+    D.phiarr = d_p;
+    D.mult = d_mult;
+    D.rmincgs = m->rmincgs;
+    for (int d = 0; d < m->ndisk; ++d) {
+      D.disk_id[d] = m->disk_id[d];
+      D.disk_z1[d] = m->disk_z1[d];
+      D.disk_b[d] = m->disk_b[d];
+      D.disk_norm[d] = m->disk_fmass[d] * m->massd * kMsol;
+    }
+    D.a_exp = m->a_exp;
+    D.rmind = m->rmind;
+    D.rmaxd = m->rmaxd;
+    D.rc = m->rc;
+    D.rhoe0 = m->rhoe0;
+    D.rmine = m->rmine;
+    D.rmax = m->rmax;
+    D.rhoamb = m->rhoamb;
+    D.env_id = m->env_id;
+    D.cav_id = m->cav_id;
+    D.c1e = m->c1e;
+    D.cavity_exp = m->cavity_exp;
+    D.rho_cavity = m->rho_cavity;
+    D.r1000 = m->r1000;
+    D.dens = d_dens;
+    D.vol = d_vol;
+    D.mass_sum = d_sum;
+    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>((nc + 255) / 256, 8192));
+    hipLaunchKernelGGL(densities, dim3(blocks), dim3(256), 0, s, D);
+    ok(hipGetLastError());
+    if (rc == HC_OK) {
+      hipLaunchKernelGGL(normalise, dim3(blocks), dim3(256), 0, s, D, d_mass);
+      ok(hipGetLastError());
+    }
+    ok(rc == HC_OK ? hipMemcpyAsync(densarr, d_dens, wall * 8 * sizeof(double),
+                                    hipMemcpyDeviceToHost, s)
+                   : hipSuccess) &&
+        ok(hipMemcpyAsync(massarr, d_mass, wall * 8 * sizeof(double), hipMemcpyDeviceToHost,
+                          s)) &&
+        ok(hipMemcpyAsync(vcell, d_vol, wall * sizeof(double), hipMemcpyDeviceToHost, s)) &&
+        ok(hipStreamSynchronize(s));
+  }
+  (void)hipStreamSynchronize(s);
+  for (double *b : {d_r, d_t, d_p, d_mult, d_dens, d_vol, d_mass, d_sum})
+    if (b) (void)hipFree(b);
+  (void)hipStreamDestroy(s);
+  return rc;
+}

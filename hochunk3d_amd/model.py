@@ -184,7 +184,124 @@ def theta_walls(ntg: int) -> np.ndarray:
     return th
 
 
-def build_model(cfg: ModelConfig) -> Model:
+def _disk_params(cfg: ModelConfig, autors: float):
+    """Per disk species: (id, mass fraction, z1 = h(r=1), flaring b)
+    (densdiskg.f90 with CZMIN = R100: z1 = h100 / 100AU^b)."""
+    out = []
+    for id_, (fm, zr) in enumerate(zip(cfg.fmass, cfg.zscale_rel)):
+        if fm <= 0.0:
+            continue
+        z1 = zr * cfg.h100_au * autors / (100.0 * autors) ** cfg.b_exp
+        out.append((id_, float(fm), float(z1), float(cfg.b_exp)))
+    return out
+
+
+def _envelope_params(cfg: ModelConfig, rmincgs: float, autors: float, rmax: float):
+    """Ulrich envelope density scale and polynomial cavity (envset.f90)."""
+    rc = cfg.rc_au * autors
+    rdcgs = rc * rmincgs
+    const = 1.0 / 3.15576e7 * np.sqrt(MSOL) / (4.0 * PI) / np.sqrt(GN)
+    rhoe0 = const * cfg.rate / np.sqrt(cfg.massc) / rdcgs ** 1.5
+    r1max = rmax * np.tan(np.deg2rad(cfg.cavity_deg))
+    c1e = rmax / r1max ** cfg.cavity_exp
+    return rc, rhoe0, c1e
+
+
+def _densities_host(cfg, rarr, thetarr, phiarr, rmincgs, autors, rmind, rmine, rmax, rmaxd):
+    """Densities, cell volumes and (rho V)^(1/4) in NumPy."""
+    nrg, ntg, npg = len(rarr), len(thetarr), len(phiarr)
+    # cell centres (gridset.f90:494-499, 609-620, 653-655)
+    rave = 10.0 ** (0.5 * (np.log10(rarr[:-1]) + np.log10(rarr[1:])))
+    thave = np.sqrt(0.5 * (thetarr[:-1] ** 2 + thetarr[1:] ** 2))
+    phave = 0.5 * (phiarr[:-1] + phiarr[1:])
+    R, TH, PH = np.meshgrid(rave, thave, phave, indexing="ij")
+    dr3 = (rarr[1:] ** 3 - rarr[:-1] ** 3)[:, None, None]
+    dcost = (np.cos(thetarr[1:]) - np.cos(thetarr[:-1]))[None, :, None]
+    dph = np.diff(phiarr)[None, None, :]
+    vol = -dr3 * dcost * dph / 3.0 * rmincgs ** 3
+    cost = np.cos(TH)
+    sint = np.sin(TH)
+
+    nc = (nrg - 1, ntg - 1, npg - 1)
+    dens = np.zeros(nc + (NDG,))
+
+    # flared disks (densdiskg.f90)
+    rp = R * sint
+    zp = R * cost
+    fm8, z18, b8 = [0.0] * NDG, [0.0] * NDG, [0.0] * NDG
+    for id_, fm, z1, b in _disk_params(cfg, autors):
+        fm8[id_], z18[id_], b8[id_] = fm, z1, b
+        zmr = z1 * rp ** cfg.b_exp
+        gexp = (zp / zmr) ** 2
+        shape = rp ** (-cfg.a_exp) * np.where(gexp < 120.0, np.exp(-0.5 * gexp), 0.0)
+        shape = np.where((rp >= rmind) & (rp <= rmaxd), shape, 0.0)
+        m = np.sum(shape * vol)
+        if m > 0:
+            dens[..., id_] = shape * (fm * cfg.massd * MSOL / m)
+
+    # Ulrich envelope (envset.f90, densenv.f90) with a polynomial cavity
+    rc, rhoe0, c1e = _envelope_params(cfg, rmincgs, autors, rmax)
+    cosa = np.abs(cost)
+    rx = R / rc
+    xmu = cosa
+    mu0 = np.maximum(_zerod(xmu, rx), 1e-12)
+    factor = 1.0 / ((xmu / mu0) + 2.0 * (mu0 ** 2) / rx)
+    env = rhoe0 * rx ** (-1.5) * factor / np.sqrt(1.0 + xmu / mu0)
+    env = np.where(R > rmine, np.maximum(env, cfg.rhoamb), 0.0)
+    env = np.where(R < rmax, env, 0.0)
+    # polynomial cavity z > c1e rp^ex1 (envset.f90:25-33, z01 = 0)
+    zup = c1e * (R * sint) ** cfg.cavity_exp
+    inhole = (R * cosa) > zup
+    cav = cfg.rho_cavity * (R / (1000.0 * autors)) ** (-2.0)
+    cav = np.where((R > rmine) & (R < rmax), cav, 0.0)
+    dens[..., 2] = np.where(inhole, 0.0, env)
+    dens[..., 3] = np.where(inhole, cav, 0.0)
+
+    if cfg.fractal:
+        from .fractal import fractal_multiplier
+        mult = fractal_multiplier(rarr, thetarr, phiarr, **cfg.fractal)
+        dens[..., 2] *= mult
+
+    densarr = np.zeros((nrg, ntg, npg, NDG), order="F")
+    densarr[:-1, :-1, :-1, :] = dens
+    vcell = np.zeros((nrg, ntg, npg), order="F")
+    vcell[:-1, :-1, :-1] = vol
+    massarr = np.zeros_like(densarr)
+    massarr[:-1, :-1, :-1, :] = (dens * vol[..., None]) ** 0.25
+    return densarr, vcell, massarr, fm8, z18, b8
+
+
+def _densities_device(cfg, rarr, thetarr, phiarr, rmincgs, autors, rmind, rmine, rmax, rmaxd,
+                      device):
+    """The same quantities from hc_build_density on GPU `device` (the clump
+    multiplier, a point count, stays on the host)."""
+    from ._lib import build_density
+    nrg, ntg, npg = len(rarr), len(thetarr), len(phiarr)
+    disks = _disk_params(cfg, autors)
+    rc, rhoe0, c1e = _envelope_params(cfg, rmincgs, autors, rmax)
+    mult = None
+    if cfg.fractal:
+        from .fractal import fractal_multiplier
+        mult = fractal_multiplier(rarr, thetarr, phiarr, **cfg.fractal)
+    fields = dict(
+        nrg=nrg, ntg=ntg, npg=npg, ndisk=len(disks), rarr=rarr, thetarr=thetarr, phiarr=phiarr,
+        fractal=mult, rmincgs=rmincgs, disk_id=[d[0] for d in disks],
+        disk_fmass=[d[1] for d in disks], disk_z1=[d[2] for d in disks],
+        disk_b=[d[3] for d in disks], a_exp=cfg.a_exp, rmind=rmind, rmaxd=rmaxd,
+        massd=cfg.massd, rc=rc, rhoe0=rhoe0, rmine=rmine, rmax=rmax, rhoamb=cfg.rhoamb,
+        env_id=2, cav_id=3, c1e=c1e, cavity_exp=cfg.cavity_exp, rho_cavity=cfg.rho_cavity,
+        r1000=1000.0 * autors)
+    densarr, massarr, vcell = build_density(fields, nrg, ntg, npg, device)
+    fm8, z18, b8 = [0.0] * NDG, [0.0] * NDG, [0.0] * NDG
+    for id_, fm, z1, b in disks:
+        fm8[id_], z18[id_], b8[id_] = fm, z1, b
+    return densarr, vcell, massarr, fm8, z18, b8
+
+
+def build_model(cfg: ModelConfig, device: Optional[int] = None) -> Model:
+    """The model's walls, densities, dust and sources.  With `device` the
+    densities, cell volumes and (rho V)^(1/4) come from hc_build_density on
+    that GPU; otherwise they are computed on the host with NumPy."""
     nrg, ntg, npg = cfg.nr + 1, cfg.ntheta + 1, cfg.nphi + 1
     rmincgs = cfg.rstar * RSOL
     autors = AU2CM / RSOL / cfg.rstar
@@ -219,72 +336,12 @@ def build_model(cfg: ModelConfig) -> Model:
     aarr = np.sin(phiarr)
     barr = -np.cos(phiarr)
 
-    # cell centres (gridset.f90:494-499, 609-620, 653-655)
-    rave = 10.0 ** (0.5 * (np.log10(rarr[:-1]) + np.log10(rarr[1:])))
-    thave = np.sqrt(0.5 * (thetarr[:-1] ** 2 + thetarr[1:] ** 2))
-    phave = 0.5 * (phiarr[:-1] + phiarr[1:])
-    R, TH, PH = np.meshgrid(rave, thave, phave, indexing="ij")
-    dr3 = (rarr[1:] ** 3 - rarr[:-1] ** 3)[:, None, None]
-    dcost = (np.cos(thetarr[1:]) - np.cos(thetarr[:-1]))[None, :, None]
-    dph = np.diff(phiarr)[None, None, :]
-    vol = -dr3 * dcost * dph / 3.0 * rmincgs ** 3
-    cost = np.cos(TH)
-    sint = np.sin(TH)
-
-    nc = (nrg - 1, ntg - 1, npg - 1)
-    dens = np.zeros(nc + (NDG,))
-
-    # flared disks (densdiskg.f90 with CZMIN = R100: z1 = h100 / 100AU^b)
-    rp = R * sint
-    zp = R * cost
-    fm8, z18, b8 = [0.0] * NDG, [0.0] * NDG, [0.0] * NDG
-    for id_, (fm, zr) in enumerate(zip(cfg.fmass, cfg.zscale_rel)):
-        if fm <= 0.0:
-            continue
-        z1 = zr * cfg.h100_au * autors / (100.0 * autors) ** cfg.b_exp
-        fm8[id_], z18[id_], b8[id_] = float(fm), float(z1), float(cfg.b_exp)
-        zmr = z1 * rp ** cfg.b_exp
-        gexp = (zp / zmr) ** 2
-        shape = rp ** (-cfg.a_exp) * np.where(gexp < 120.0, np.exp(-0.5 * gexp), 0.0)
-        shape = np.where((rp >= rmind) & (rp <= rmaxd), shape, 0.0)
-        m = np.sum(shape * vol)
-        if m > 0:
-            dens[..., id_] = shape * (fm * cfg.massd * MSOL / m)
-
-    # Ulrich envelope (envset.f90, densenv.f90) with a polynomial cavity
-    rc = cfg.rc_au * autors
-    rdcgs = rc * rmincgs
-    const = 1.0 / 3.15576e7 * np.sqrt(MSOL) / (4.0 * PI) / np.sqrt(GN)
-    rhoe0 = const * cfg.rate / np.sqrt(cfg.massc) / rdcgs ** 1.5
-    cosa = np.abs(cost)
-    rx = R / rc
-    xmu = cosa
-    mu0 = np.maximum(_zerod(xmu, rx), 1e-12)
-    factor = 1.0 / ((xmu / mu0) + 2.0 * (mu0 ** 2) / rx)
-    env = rhoe0 * rx ** (-1.5) * factor / np.sqrt(1.0 + xmu / mu0)
-    env = np.where(R > rmine, np.maximum(env, cfg.rhoamb), 0.0)
-    env = np.where(R < rmax, env, 0.0)
-    # polynomial cavity z > c1e rp^ex1 (envset.f90:25-33, z01 = 0)
-    r1max = rmax * np.tan(np.deg2rad(cfg.cavity_deg))
-    c1e = rmax / r1max ** cfg.cavity_exp
-    zup = c1e * (R * sint) ** cfg.cavity_exp
-    inhole = (R * cosa) > zup
-    cav = cfg.rho_cavity * (R / (1000.0 * autors)) ** (-2.0)
-    cav = np.where((R > rmine) & (R < rmax), cav, 0.0)
-    dens[..., 2] = np.where(inhole, 0.0, env)
-    dens[..., 3] = np.where(inhole, cav, 0.0)
-
-    if cfg.fractal:
-        from .fractal import fractal_multiplier
-        mult = fractal_multiplier(rarr, thetarr, phiarr, **cfg.fractal)
-        dens[..., 2] *= mult
-
-    densarr = np.zeros((nrg, ntg, npg, NDG), order="F")
-    densarr[:-1, :-1, :-1, :] = dens
-    vcell = np.zeros((nrg, ntg, npg), order="F")
-    vcell[:-1, :-1, :-1] = vol
-    massarr = np.zeros_like(densarr)
-    massarr[:-1, :-1, :-1, :] = (dens * vol[..., None]) ** 0.25
+    if device is None:
+        densarr, vcell, massarr, fm8, z18, b8 = _densities_host(
+            cfg, rarr, thetarr, phiarr, rmincgs, autors, rmind, rmine, rmax, rmaxd)
+    else:
+        densarr, vcell, massarr, fm8, z18, b8 = _densities_device(
+            cfg, rarr, thetarr, phiarr, rmincgs, autors, rmind, rmine, rmax, rmaxd, device)
     tdust = np.full((nrg, ntg, npg, NDG), 0.1 / 11605.0, order="F")
 
     table = cfg.dust_table if cfg.dust_table is not None else synthetic_dust_table()

@@ -195,6 +195,32 @@ int hc_run_lucy(hc_ctx *ctx, int32_t niter, int64_t np, int32_t seed1, int32_t s
  * not depend on them beyond the order of the floating-point tally sums. */
 int hc_set_option(hc_ctx *ctx, const char *name, int64_t value);
 
+/* Analytic densities of the benchmark models, for grid construction on the
+ * GPU: the density part of gridset (gridset.f90:1002-1083) with flared disks
+ * (densdiskg.f90), an Ulrich envelope (densenv.f90) and a polynomial bipolar
+ * cavity (envset.f90:25-33), evaluated at the cell centres of the walls
+ * (gridset.f90:494-499, 609-620).  Lengths in stellar radii. */
+typedef struct {
+  int32_t nrg, ntg, npg;
+  int32_t ndisk;                      /* flared disks, <= 8 */
+  const double *rarr, *thetarr, *phiarr; /* walls [nrg], [ntg], [npg] */
+  const double *fractal;              /* (nrg-1,ntg-1,npg-1) envelope multiplier or NULL */
+  double rmincgs;                     /* stellar radius, cm */
+  int32_t disk_id[8];                 /* 0-based species of each disk */
+  double disk_fmass[8], disk_z1[8], disk_b[8]; /* mass fraction, h(r=1), flaring */
+  double a_exp, rmind, rmaxd, massd;  /* density exponent, radial range, disk mass (Msun) */
+  double rc, rhoe0, rmine, rmax, rhoamb; /* centrifugal radius, density scale (g/cm^3) */
+  int32_t env_id, cav_id;             /* 0-based species of envelope and cavity */
+  double c1e, cavity_exp, rho_cavity, r1000; /* z > c1e rp^ex; rho at r1000 (1000 AU) */
+} hc_density_model;
+
+/* Fills densarr and massarr = (rho V)^(1/4), both (nrg,ntg,npg,8), and vcell
+ * (nrg,ntg,npg) in cm^3, column-major with the last wall index zero; the
+ * disks carry their masses.  No counterpart in the reference's loop: it
+ * replaces the host-side evaluation of those formulas in grid setup. */
+int hc_build_density(const hc_density_model *model, int device, double *densarr,
+                     double *massarr, double *vcell);
+
 int hc_destroy(hc_ctx *ctx);
 const char *hc_last_error(const hc_ctx *ctx);
 int hc_abi_version(void);

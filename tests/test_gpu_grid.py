@@ -1,0 +1,52 @@
+"""Grid densities on the GPU (hc_build_density) against the host construction
+of the same formulas: flared disks with their mass normalisation, the Ulrich
+envelope (the streamline cubic), the polynomial cavity, the clump multiplier,
+cell volumes and (rho V)^(1/4).  The two differ only in the last bits of the
+elementary functions and of the disk-mass sums."""
+import time
+
+import numpy as np
+import pytest
+
+from hochunk3d_amd import build_model, config1, config2
+from hochunk3d_amd.batch import parameter_grid
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "64x64x32 disk+envelope": config1(),
+    "40x41x24 fractal, even theta walls": config2(nr=40, ntheta=41, nphi=24),
+    "grid model 196x176x1": parameter_grid(12, max_cells=200_000)[10].cfg,
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_device_densities_match_host(name):
+    cfg = CASES[name]
+    h = build_model(cfg)
+    d = build_model(cfg, device=0)
+    for k in ("densarr", "massarr", "vcell"):
+        a, b = getattr(d, k), getattr(h, k)
+        assert a.shape == b.shape
+        np.testing.assert_array_equal(a == 0.0, b == 0.0, err_msg=k)
+        np.testing.assert_allclose(a, b, rtol=1e-11, atol=0.0, err_msg=k)
+    assert d.fmass8 == h.fmass8 and d.z1_8 == h.z1_8 and d.bflare8 == h.bflare8
+    # the disks carry their mass
+    for id_, fm in enumerate(h.fmass8):
+        if fm > 0:
+            m = (d.densarr[..., id_] * d.vcell).sum()
+            assert abs(m / (fm * cfg.massd * 1.989e33) - 1.0) < 1e-12
+
+
+def test_device_build_time_of_the_8m_cell_model():
+    cfg = config2()
+    build_model(config1(), device=0)          # first use of the device
+    t0 = time.perf_counter()
+    d = build_model(cfg, device=0)
+    td = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    h = build_model(cfg)
+    th = time.perf_counter() - t0
+    print(f"200x199x200 model: host {th:.2f} s, with device densities {td:.2f} s")
+    np.testing.assert_allclose(d.densarr, h.densarr, rtol=1e-11, atol=0.0)
+    assert td < th
