@@ -27,7 +27,7 @@ EXPORTS = (
     "hc_create", "hc_set_grid", "hc_set_temperature", "hc_get_temperature", "hc_set_dust",
     "hc_set_sources",
     "hc_run_iteration", "hc_temperature", "hc_run_lucy", "hc_set_option", "hc_destroy",
-    "hc_last_error", "hc_build_density",
+    "hc_last_error", "hc_build_density", "hc_build_emissivity",
     "hc_abi_version",
 )
 
@@ -125,6 +125,8 @@ def load() -> C.CDLL:
                                 C.POINTER(C.c_int64)]
     lib.hc_set_option.argtypes = [C.c_void_p, C.c_char_p, C.c_int64]
     lib.hc_build_density.argtypes = [C.POINTER(HcDensityModel), C.c_int, _dp, _dp, _dp]
+    lib.hc_build_emissivity.argtypes = [C.c_int, C.c_int32, _dp, _dp, _dp, C.c_int32, _dp,
+                                        C.c_int32, _dp, C.c_int32, _dp, _dp, _dp, _dp]
     lib.hc_destroy.argtypes = [C.c_void_p]
     lib.hc_last_error.argtypes = [C.c_void_p]
     lib.hc_last_error.restype = C.c_char_p
@@ -378,3 +380,18 @@ def build_density(fields: dict, nrg: int, ntg: int, npg: int, device: int = 0):
     if rc != 0:
         raise HcError(f"hc_build_density failed ({rc})")
     return dens, mass, vol
+
+
+def build_emissivity(nud, kd, ad, lTint, nuint, xq, wq, device: int = 0):
+    """hc_build_emissivity: (kapBint (nT, nnu) F-order, kappap (nT,))."""
+    lib = load()
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (nud, kd, ad, lTint, nuint, xq, wq)]
+    nT, nnu = len(a[3]), len(a[4])
+    kapB = np.zeros((nT, nnu), order="F")
+    kp = np.zeros(nT)
+    rc = lib.hc_build_emissivity(int(device), len(a[0]), _ptr(a[0]), _ptr(a[1]), _ptr(a[2]),
+                                 nT, _ptr(a[3]), nnu, _ptr(a[4]), len(a[5]), _ptr(a[5]),
+                                 _ptr(a[6]), _ptr(kapB), _ptr(kp))
+    if rc != 0:
+        raise HcError(f"hc_build_emissivity failed ({rc})")
+    return kapB, kp

@@ -8,9 +8,13 @@
 // One thread per cell evaluates the analytic densities at the cell centre
 // (gridset.f90:494-499, 609-620: log-mean radius, root-mean-square theta) and
 // the cell volume; the disk species are normalised to their mass with one
-// FP64 sum per species (block partial sums, then one atomic per block).  The
-// outputs use the reference's layout, (nrg,ntg,npg[,ndg]) column-major with
-// the last wall index of each dimension zero.
+// FP64 sum per species (wavefront partial sums, one atomic per wavefront).
+// The outputs use the reference's layout, (nrg,ntg,npg[,ndg]) column-major
+// with the last wall index of each dimension zero.
+//
+// Also here: the dust emissivity tables of setemiss, one thread per
+// (temperature, frequency bin), then one per temperature row for the
+// cumulative sum.
 
 #include "../../include/hc.h"
 
@@ -131,7 +135,149 @@ __global__ __launch_bounds__(256) void normalise(Dev D, double *massarr) {
   }
 }
 
+// setemiss (dust.f90:394-474) on a (temperature, frequency bin) grid: the
+// emissivity kappa_abs B_nu / B integrated over each bin of nuint (units of
+// kT) with the caller's quadrature rule, kappa_abs linear in nu between the
+// bin ends (opacset's linear interpolation of the dust table at each end).
+struct Emis {
+  int nl, nT, nnu, nq;
+  const double *nud, *kd, *ad;  // [nl], nud decreasing
+  const double *lTint, *nuint, *xq, *wq;
+  double *bins;                 // (nT, nnu) bin integrals, then the table
+  double *kappap;               // [nT]
+};
+
+// np.interp-style linear interpolation in the dust table at nu (eV),
+// clamped to the end values; returns (1 - albedo) kappa.
+__device__ double kap_abs(const Emis &E, double nu) {
+  // the table decreases in nu: index 0 has the highest frequency
+  if (nu >= E.nud[0]) return (1.0 - E.ad[0]) * E.kd[0];
+  if (nu <= E.nud[E.nl - 1]) return (1.0 - E.ad[E.nl - 1]) * E.kd[E.nl - 1];
+  int lo = 0, hi = E.nl - 1;  // nud[lo] > nu >= nud[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (E.nud[mid] > nu)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  // increasing abscissa x0 = nud[hi] <= nu < x1 = nud[lo]
+  const double x0 = E.nud[hi], x1 = E.nud[lo];
+  const double sk = (E.kd[lo] - E.kd[hi]) / (x1 - x0);
+  const double sa = (E.ad[lo] - E.ad[hi]) / (x1 - x0);
+  const double k = sk * (nu - x0) + E.kd[hi];
+  const double a = sa * (nu - x0) + E.ad[hi];
+  return (1.0 - a) * k;
+}
+
+__device__ double planck_norm(double nu, double T) {  // mathsubs.f90:1-24
+  const double x = nu / T;
+  if (x == 0.0) return 0.0;
+  return 0.1539897 * (x * x * x) / expm1(x);
+}
+
+__global__ __launch_bounds__(256) void emis_bins(Emis E) {
+  const size_t n = (size_t)E.nT * E.nnu;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(t % (size_t)E.nT);
+    const int j = (int)(t / (size_t)E.nT);
+    const double T = exp(E.lTint[i]);
+    const double nu1 = E.nuint[j] * T;
+    const double nu0 = j > 0 ? E.nuint[j - 1] * T : 0.0;
+    const double dnu = nu1 - nu0;
+    const double kap1 = kap_abs(E, nu1), kap0 = kap_abs(E, nu0);
+    double acc = 0.0;
+    for (int q = 0; q < E.nq; ++q) {
+      const double f = E.xq[q];
+      acc += E.wq[q] * ((1.0 - f) * kap0 + f * kap1) * planck_norm(nu0 + f * dnu, T);
+    }
+    E.bins[t] = (dnu / T) * acc;
+  }
+}
+
+// One thread per temperature row: the cumulative sum in frequency, its total
+// (the Planck mean, kappap) and the normalised table with a last node of 1.
+__global__ __launch_bounds__(256) void emis_cumulate(Emis E) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= E.nT) return;
+  double cum = 0.0;
+  for (int j = 0; j < E.nnu; ++j) {
+    cum += E.bins[i + (size_t)E.nT * j];
+    E.bins[i + (size_t)E.nT * j] = cum;
+  }
+  E.kappap[i] = cum;
+  for (int j = 0; j < E.nnu - 1; ++j) E.bins[i + (size_t)E.nT * j] /= cum;
+  E.bins[i + (size_t)E.nT * (E.nnu - 1)] = 1.0;
+}
+
 }  // namespace hc_grid
+
+extern "C" int hc_build_emissivity(int device, int32_t nl, const double *nud, const double *kd,
+                                   const double *ad, int32_t nT, const double *lTint,
+                                   int32_t nnu, const double *nuint, int32_t nq,
+                                   const double *xq, const double *wq, double *kapBint,
+                                   double *kappap) {
+  using namespace hc_grid;
+  if (nl < 2 || nT < 1 || nnu < 2 || nq < 1 || nq > 64 || !nud || !kd || !ad || !lTint ||
+      !nuint || !xq || !wq || !kapBint || !kappap)
+    return HC_E_ARG;
+  if (hipSetDevice(device) != hipSuccess) return HC_E_HIP;
+  hipStream_t s = nullptr;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return HC_E_HIP;
+  const size_t nb = (size_t)nT * nnu;
+  double *d_tab = nullptr, *d_ax = nullptr, *d_bins = nullptr, *d_kp = nullptr;
+  int rc = HC_OK;
+  auto ok = [&](hipError_t e) {
+    if (e != hipSuccess && rc == HC_OK) rc = HC_E_HIP;
+    return rc == HC_OK;
+  };
+  // table: nud, kd, ad; axes: lTint, nuint, xq, wq
+  const size_t nax = (size_t)nT + nnu + 2 * (size_t)nq;
+  ok(hipMalloc((void **)&d_tab, 3 * (size_t)nl * sizeof(double))) &&
+      ok(hipMalloc((void **)&d_ax, nax * sizeof(double))) &&
+      ok(hipMalloc((void **)&d_bins, nb * sizeof(double))) &&
+      ok(hipMalloc((void **)&d_kp, (size_t)nT * sizeof(double))) &&
+      ok(hipMemcpyAsync(d_tab, nud, nl * sizeof(double), hipMemcpyHostToDevice, s)) &&
+      ok(hipMemcpyAsync(d_tab + nl, kd, nl * sizeof(double), hipMemcpyHostToDevice, s)) &&
+      ok(hipMemcpyAsync(d_tab + 2 * nl, ad, nl * sizeof(double), hipMemcpyHostToDevice, s)) &&
+      ok(hipMemcpyAsync(d_ax, lTint, nT * sizeof(double), hipMemcpyHostToDevice, s)) &&
+      ok(hipMemcpyAsync(d_ax + nT, nuint, nnu * sizeof(double), hipMemcpyHostToDevice, s)) &&
+      ok(hipMemcpyAsync(d_ax + nT + nnu, xq, nq * sizeof(double), hipMemcpyHostToDevice, s)) &&
+      ok(hipMemcpyAsync(d_ax + nT + nnu + nq, wq, nq * sizeof(double), hipMemcpyHostToDevice, s));
+  if (rc == HC_OK) {
+    Emis E;
+    E.nl = nl;
+    E.nT = nT;
+    E.nnu = nnu;
+    E.nq = nq;
+    E.nud = d_tab;
+    E.kd = d_tab + nl;
+    E.ad = d_tab + 2 * nl;
+    E.lTint = d_ax;
+    E.nuint = d_ax + nT;
+    E.xq = d_ax + nT + nnu;
+    E.wq = d_ax + nT + nnu + nq;
+    E.bins = d_bins;
+    E.kappap = d_kp;
+    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>((nb + 255) / 256, 8192));
+    hipLaunchKernelGGL(emis_bins, dim3(blocks), dim3(256), 0, s, E);
+    ok(hipGetLastError());
+    if (rc == HC_OK) {
+      hipLaunchKernelGGL(emis_cumulate, dim3((nT + 255) / 256), dim3(256), 0, s, E);
+      ok(hipGetLastError());
+    }
+    ok(rc == HC_OK ? hipMemcpyAsync(kapBint, d_bins, nb * sizeof(double), hipMemcpyDeviceToHost, s)
+                   : hipSuccess) &&
+        ok(hipMemcpyAsync(kappap, d_kp, nT * sizeof(double), hipMemcpyDeviceToHost, s)) &&
+        ok(hipStreamSynchronize(s));
+  }
+  (void)hipStreamSynchronize(s);
+  for (double *b : {d_tab, d_ax, d_bins, d_kp})
+    if (b) (void)hipFree(b);
+  (void)hipStreamDestroy(s);
+  return rc;
+}
 
 extern "C" int hc_build_density(const hc_density_model *m, int device, double *densarr,
                                 double *massarr, double *vcell) {

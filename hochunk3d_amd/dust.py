@@ -111,11 +111,15 @@ def _planck_norm(nu: np.ndarray, T: np.ndarray) -> np.ndarray:
     return np.where(x == 0.0, 0.0, b)
 
 
-def setemiss_species(nud, kd, ad, lTint, nuint, nquad: int = 8):
-    """kapBint (nT, nnu) normalised and kappap (nT,) for one species."""
+def setemiss_species(nud, kd, ad, lTint, nuint, nquad: int = 8, device=None):
+    """kapBint (nT, nnu) normalised and kappap (nT,) for one species; with
+    `device`, computed by hc_build_emissivity on that GPU."""
     xg, wg = np.polynomial.legendre.leggauss(nquad)
     fq = 0.5 * (xg + 1.0)
     wq = 0.5 * wg
+    if device is not None:
+        from ._lib import build_emissivity
+        return build_emissivity(nud, kd, ad, lTint, nuint, fq, wq, device)
     T = np.exp(lTint)[:, None]                      # (nT,1)
     nu1 = nuint[None, :] * T                        # (nT,nnu)
     nu0 = np.concatenate([np.zeros((len(lTint), 1)), nu1[:, :-1]], axis=1)
@@ -136,9 +140,10 @@ def setemiss_species(nud, kd, ad, lTint, nuint, nquad: int = 8):
     return kapB, kappap
 
 
-def dustinit(tables) -> DustTables:
+def dustinit(tables, device=None) -> DustTables:
     """Build dust_mod's arrays from ndg tables (a list of (n,7) arrays, or one
-    table used for every species)."""
+    table used for every species); `device` computes the emissivity tables
+    on that GPU."""
     if isinstance(tables, np.ndarray):
         tables = [tables] * NDG
     if len(tables) != NDG:
@@ -163,7 +168,7 @@ def dustinit(tables) -> DustTables:
         kappav[id_] = kv
         key = id(tab)
         if key not in cache:
-            cache[key] = setemiss_species(nud, kd, ad, lTint, nuint)
+            cache[key] = setemiss_species(nud, kd, ad, lTint, nuint, device=device)
         kapBint[:, :, id_], kappap[:, id_] = cache[key]
     return DustTables(nlambda, arrs["nud"], arrs["kd"], arrs["ad"], arrs["gd"], arrs["pd"],
                       kappav, lTint, nuint, kapBint, kappap)

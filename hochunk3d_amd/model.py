@@ -300,8 +300,9 @@ def _densities_device(cfg, rarr, thetarr, phiarr, rmincgs, autors, rmind, rmine,
 
 def build_model(cfg: ModelConfig, device: Optional[int] = None) -> Model:
     """The model's walls, densities, dust and sources.  With `device` the
-    densities, cell volumes and (rho V)^(1/4) come from hc_build_density on
-    that GPU; otherwise they are computed on the host with NumPy."""
+    densities, cell volumes and (rho V)^(1/4) come from hc_build_density and
+    the emissivity tables from hc_build_emissivity on that GPU; otherwise
+    they are computed on the host with NumPy."""
     nrg, ntg, npg = cfg.nr + 1, cfg.ntheta + 1, cfg.nphi + 1
     rmincgs = cfg.rstar * RSOL
     autors = AU2CM / RSOL / cfg.rstar
@@ -345,7 +346,7 @@ def build_model(cfg: ModelConfig, device: Optional[int] = None) -> Model:
     tdust = np.full((nrg, ntg, npg, NDG), 0.1 / 11605.0, order="F")
 
     table = cfg.dust_table if cfg.dust_table is not None else synthetic_dust_table()
-    dust = dustinit(table)
+    dust = dustinit(table, device=device)
 
     # luminosity and binning (gridset.f90 lstar, setup.f90:36-40, initarr.f90:24-28).
     # Disk accretion (gridset.f90:123,143): lacc = G M mdot / (2 Rmin,dust);

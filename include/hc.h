@@ -221,6 +221,18 @@ typedef struct {
 int hc_build_density(const hc_density_model *model, int device, double *densarr,
                      double *massarr, double *vcell);
 
+/* setemiss for one species (dust.f90:394-474) on the GPU: kapBint (nT,nnu),
+ * the cumulative emissivity kappa_abs B_nu over the frequency bins of nuint
+ * (units of kT) normalised to 1 at the last node, and kappap[nT], its total.
+ * Each bin is integrated with the caller's rule (nq nodes xq in [0,1], weights
+ * wq summing to 1), kappa_abs = (1 - albedo) kappa interpolated linearly in nu
+ * between the bin's ends from the dust table nud (decreasing), kd, ad [nl].
+ * The reference integrates adaptively to 1e-5 instead. */
+int hc_build_emissivity(int device, int32_t nl, const double *nud, const double *kd,
+                        const double *ad, int32_t nT, const double *lTint, int32_t nnu,
+                        const double *nuint, int32_t nq, const double *xq, const double *wq,
+                        double *kapBint, double *kappap);
+
 int hc_destroy(hc_ctx *ctx);
 const char *hc_last_error(const hc_ctx *ctx);
 int hc_abi_version(void);

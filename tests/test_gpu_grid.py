@@ -50,3 +50,24 @@ def test_device_build_time_of_the_8m_cell_model():
     print(f"200x199x200 model: host {th:.2f} s, with device densities {td:.2f} s")
     np.testing.assert_allclose(d.densarr, h.densarr, rtol=1e-11, atol=0.0)
     assert td < th
+
+
+@pytest.mark.parametrize("kappa_v,albedo_max", [(200.0, 0.6), (35.0, 0.9)])
+def test_device_emissivity_tables_match_host(kappa_v, albedo_max):
+    """setemiss on the GPU (hc_build_emissivity) against the host quadrature:
+    the same 8-point rule per bin, the same cumulative sums."""
+    from hochunk3d_amd.dust import dustinit, synthetic_dust_table
+    tab = synthetic_dust_table(kappa_v=kappa_v, albedo_max=albedo_max)
+    t0 = time.perf_counter()
+    h = dustinit(tab)
+    th = time.perf_counter() - t0
+    dustinit(tab, device=0)
+    t0 = time.perf_counter()
+    d = dustinit(tab, device=0)
+    td = time.perf_counter() - t0
+    print(f"emissivity tables: host {th:.3f} s, device {td:.3f} s")
+    np.testing.assert_allclose(d.kappap, h.kappap, rtol=1e-11, atol=0.0)
+    np.testing.assert_allclose(d.kapBint, h.kapBint, rtol=1e-10, atol=1e-300)
+    # a normalised cumulative table: non-decreasing, 1 at the last node
+    kb = d.kapBint[:, :, 0]
+    assert np.all(np.diff(kb, axis=1) >= 0.0) and np.all(kb[:, -1] == 1.0)
