@@ -87,6 +87,17 @@ def sed_errors(s64: np.ndarray, s32: np.ndarray) -> dict:
             "luminosity_weighted": float(np.abs(spec32 - spec64).sum() / spec64.sum())}
 
 
+def temperature_errors(model, t64: np.ndarray, t32: np.ndarray) -> dict:
+    """Dust-temperature differences of the two paths after the same Lucy
+    steps from the same temperatures, over the (cell, species) with dust:
+    mass-weighted mean, median and 99th percentile of |T32/T64 - 1|."""
+    dusty = model.densarr > 0.0
+    rel = np.abs(t32[dusty] / t64[dusty] - 1.0)
+    w = (model.densarr * model.vcell[..., None])[dusty]
+    return {"mass_weighted_mean": float(np.sum(w * rel) / np.sum(w)),
+            "median": float(np.median(rel)), "p99": float(np.percentile(rel, 99))}
+
+
 def timed_steps(tr, steps: int, np_: int, seg: int, label: str):
     """`steps` Lucy iterations; returns wall s, kernel ms, summed counters and
     the SED of the first step."""
@@ -238,6 +249,7 @@ def main():
             f"of {int(args.autotune_np)} packets")
 
     wall64, kms64, tot64, sed64 = timed_steps(tr, args.steps, np_, seg, "fp64")
+    t64_end = tr.get_temperature()
     npk = np_ * args.steps
     value = npk / wall64
 
@@ -274,17 +286,20 @@ def main():
                              "probes": len(r.trials)}
             log(f"autotune fp32: {r.options}, x{r.gain:.3f} vs default")
         wall32, kms32, tot32, sed32 = timed_steps(tr32, args.steps, np_, seg, "fp32")
+        t32_end = tr32.get_temperature()
         tr32.close()
         err = sed_errors(sed64, sed32)
+        terr = temperature_errors(model, t64_end, t32_end)
         fp32 = {"value": round(npk / wall32, 2), "unit": "packets/s",
                 "speedup_vs_fp64": round(wall64 / wall32, 3),
                 "kernel_ms_per_step": round(kms32 / args.steps, 3),
                 "sed_error_vs_fp64_first_step": {k: round(v, 6) for k, v in err.items()},
+                "temperature_error_vs_fp64_after_steps": {k: round(v, 6) for k, v in terr.items()},
                 "note": "same packets (L'Ecuyer-Cote segments) and starting temperatures as "
                         "the FP64 steps; SED error includes the Monte Carlo noise of histories "
                         "that diverge"}
         log(f"fp32: {fp32['value']:.3e} packets/s, x{fp32['speedup_vs_fp64']} vs fp64, "
-            f"SED error {err}")
+            f"SED error {err}, temperature error {terr}")
 
     batch = None
     if args.batch_models > 0:
