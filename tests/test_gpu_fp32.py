@@ -63,3 +63,31 @@ def test_fp32_energy_bookkeeping(runs):
         assert abs(r32.scalars[k] - r64.scalars[k]) <= 0.002 * r64.scalars[k], k
     assert r32.scalars["n_err_nowall"] <= 1e-5 * N
     assert r32.scalars["n_err_dust"] == 0
+
+
+def test_fp32_peeled_images_and_sed_within_one_percent():
+    """The final peeled pass on the FP32 path against FP64 on the same
+    packets: per observer, the image flux (Stokes I summed over pixels and
+    frequencies), the polarised fraction of the image sums, and the peeled
+    SED summed over frequency."""
+    from hochunk3d_amd._lib import HC_FP32, HC_FP64, Transport
+    m = build_model(config1(nfreq=50, nximg=41))
+    m.tdust[...] = synthetic_temperature(m)
+    out = {}
+    for prec in (HC_FP64, HC_FP32):
+        g = Transport(m, precision=prec)
+        out[prec] = g.run(200_000, ipeel=1)
+        g.close()
+    a, b = out[HC_FP64], out[HC_FP32]
+    ia = a.img_i[..., 0].sum(axis=(1, 2, 3))     # (npeel,) all photons
+    ib = b.img_i[..., 0].sum(axis=(1, 2, 3))
+    err = np.abs(ib - ia) / ia
+    print("image flux per observer, rel err:", np.round(err, 5))
+    assert err.max() <= 0.01
+    qa = np.hypot(a.img_q.sum(axis=(1, 2, 3)), a.img_u.sum(axis=(1, 2, 3))) / ia
+    qb = np.hypot(b.img_q.sum(axis=(1, 2, 3)), b.img_u.sum(axis=(1, 2, 3))) / ib
+    print("polarised fraction per observer:", np.round(qa, 5), np.round(qb, 5))
+    assert np.abs(qb - qa).max() <= 0.002
+    pa = a.peel[0, :, :, 0, 0].sum(axis=0)        # peeled SED, I, per observer
+    pb = b.peel[0, :, :, 0, 0].sum(axis=0)
+    assert (np.abs(pb - pa) / pa).max() <= 0.01
