@@ -494,8 +494,8 @@ struct PeelOut {
 };
 
 // What a peel-off ray needs of the packet at its emission or scattering
-// event.  Records wait in a per-wavefront queue in LDS and are walked
-// together, one (record, observer) pair per lane.
+// event.  Records wait in a per-wavefront queue in LDS; the wavefront's lanes
+// walk their (record, observer) rays (flush_peels).
 template <class R, int NA>
 struct PeelRec {
   double x, y, z, rsq;
@@ -538,69 +538,122 @@ __device__ __forceinline__ double rec_opac(const PeelRec<R, NA> &k, const double
   return opac;
 }
 
-// peeloff_3d.f90:5-288 for observer `peelid`.  sip is the peel weight; the
-// remaining Stokes components come from the packet.
+// peeloff_3d.f90:5-288, one (record, observer) ray at a time, split so that
+// a wavefront can walk many rays with lane refill: peel_begin sets the ray up
+// (peeloff_3d.f90:5-72), peel_step crosses one wall (:74-154), peel_tally
+// adds the ray to the peeled SED and the image cube (:155-288).  The
+// arithmetic of a ray is that of the serial routine, in the same order.
+enum { kPeelIdle = 0, kPeelWalk = 1, kPeelTally = 2 };
+
+struct PeelWalk {
+  double x, y, z, ux, uy, uz, tsum, w, tau;
+  int ir, it, ip, ncross, err, r;
+  bool on_wall;
+};
+
+// Ray r = (record r / npeel, observer r % npeel) of the queue.  Returns
+// kPeelIdle for a direct stellar ray the observer cannot see (not a ray) or
+// for a ray that ends at once (counted by the caller when S.r >= 0).
 template <class R, int NA>
-__device__ __forceinline__ PeelOut peeloff(const WallView &W, const PeelRec<R, NA> &k, double sip,
-                                           int peelid) {
+__device__ __forceinline__ int peel_begin(const WallView &W, const PeelRec<R, NA> *q, int r,
+                                          PeelWalk &S) {
   const Params &P = c_P;
-  PeelOut out{0.0, 0, 0};
-  Stokes4<R> s{R(sip), k.s.Q, k.s.U, k.s.V};
-  double x1 = k.x, y1 = k.y, z1 = k.z;
-  int ir = k.ir, it = k.it, ip = k.ip;
-  bool on_wall = (k.flags & 1) != 0;
-  const int iflag = (k.flags >> 1) & 1, i_orig = (k.flags >> 2) & 7, slot = (k.flags >> 5) & 15;
-  const double cospe = P.cospe[peelid], sinpe = P.sinpe[peelid];
-  const double coste = P.coste[peelid], sinte = P.sinte[peelid];
-  const double ux1 = sinte * cospe, uy1 = sinte * sinpe, uz1 = coste;
-  if (on_wall) {
-    const bool inner = fabs(k.rsq - W.r2[ir - 1]) < fabs(k.rsq - W.r2[ir]);
-    const bool outwards = ux1 * x1 + uy1 * y1 + uz1 * z1 > 0.0;
-    if (inner && !outwards) {
-      if (ir == 1) return out;
-      ir = ir - 1;
+  const int ri = r / P.npeel, obs = r - ri * P.npeel;
+  const PeelRec<R, NA> &k = q[ri];
+  S.r = -1;
+  if ((k.flags >> 9) & 1) {  // direct stellar emission (newdisktherm.F90:342-377)
+    if (!P.ips) {
+      const R cosnorm =
+          k.cosb * R(P.coste[obs]) +
+          (k.sinb * R(P.sinte[obs]) * (R(P.cospe[obs]) * k.coslp + R(P.sinpe[obs]) * k.sinlp));
+      if (!(cosnorm > R(0.0))) return kPeelIdle;
+      S.w = (double)(R(4.0) * k.s.I * cosnorm);
+    } else {
+      S.w = (double)(k.s.I * R(2.0));
     }
-    if (!inner && outwards) ir = ir + 1;
+  } else {
+    S.w = (double)k.s.I;
   }
-  double tsum = 0.0;
+  S.r = r;
+  S.tau = 0.0;
+  S.ncross = 0;
+  S.err = 0;
+  S.x = k.x;
+  S.y = k.y;
+  S.z = k.z;
+  S.ir = k.ir;
+  S.it = k.it;
+  S.ip = k.ip;
+  S.on_wall = (k.flags & 1) != 0;
+  const double cospe = P.cospe[obs], sinpe = P.sinpe[obs];
+  const double coste = P.coste[obs], sinte = P.sinte[obs];
+  S.ux = sinte * cospe;
+  S.uy = sinte * sinpe;
+  S.uz = coste;
+  if (S.on_wall) {
+    const bool inner = fabs(k.rsq - W.r2[S.ir - 1]) < fabs(k.rsq - W.r2[S.ir]);
+    const bool outwards = S.ux * S.x + S.uy * S.y + S.uz * S.z > 0.0;
+    if (inner && !outwards) {
+      if (S.ir == 1) return kPeelIdle;
+      S.ir = S.ir - 1;
+    }
+    if (!inner && outwards) S.ir = S.ir + 1;
+  }
+  S.tsum = 0.0;
+  return S.ir < P.nrg ? kPeelWalk : kPeelTally;
+}
+
+template <class R, int NA>
+__device__ __forceinline__ int peel_step(const WallView &W, const PeelRec<R, NA> &k, PeelWalk &S) {
+  const Params &P = c_P;
   // a straight ray meets each sphere and each cone at most twice and each
   // phi half-plane at most once: more crossings than that is a stuck ray
   const int max_cross = 2 * (P.nrg + P.ntg + P.npg) + 16;
-  while (ir < P.nrg) {
-    double dt;
-    int face;
-    if (x1 == 0.0 && y1 == 0.0) ip = axis_phi_cell(ux1, uy1, ip);
-    if (out.ncross > max_cross ||
-        !next_wall(W, ux1, uy1, uz1, x1, y1, z1, ir, it, ip, on_wall, dt, face)) {
-      out.err = 1;
-      return out;
-    }
-    tsum = tsum + rec_opac(k, P.densm + cell_of(ir, it, ip) * (size_t)P.nact) * dt;
-    if (tsum > kTauMax) {
-      out.tau = tsum;
-      return out;
-    }
-    face_step(face, ir, it, ip);
-    x1 = x1 + dt * ux1;
-    y1 = y1 + dt * uy1;
-    z1 = z1 + dt * uz1;
-    on_wall = true;
-    out.ncross++;
-    if (ir == 0) {
-      out.tau = tsum;
-      return out;
-    }
+  double dt;
+  int face;
+  if (S.x == 0.0 && S.y == 0.0) S.ip = axis_phi_cell(S.ux, S.uy, S.ip);
+  if (S.ncross > max_cross ||
+      !next_wall(W, S.ux, S.uy, S.uz, S.x, S.y, S.z, S.ir, S.it, S.ip, S.on_wall, dt, face)) {
+    S.err = 1;
+    return kPeelIdle;
   }
-  out.tau = tsum;
-  const R factor = m_exp(R(-tsum));
+  S.tsum = S.tsum + rec_opac(k, P.densm + cell_of(S.ir, S.it, S.ip) * (size_t)P.nact) * dt;
+  if (S.tsum > kTauMax) {
+    S.tau = S.tsum;
+    return kPeelIdle;
+  }
+  face_step(face, S.ir, S.it, S.ip);
+  S.x = S.x + dt * S.ux;
+  S.y = S.y + dt * S.uy;
+  S.z = S.z + dt * S.uz;
+  S.on_wall = true;
+  S.ncross++;
+  if (S.ir == 0) {
+    S.tau = S.tsum;
+    return kPeelIdle;
+  }
+  if (S.ir < P.nrg) return kPeelWalk;
+  S.tau = S.tsum;
+  return kPeelTally;
+}
+
+template <class R, int NA>
+__device__ __forceinline__ void peel_tally(const PeelRec<R, NA> &k, const PeelWalk &S) {
+  const Params &P = c_P;
+  const int peelid = S.r % P.npeel;
+  Stokes4<R> s{R(S.w), k.s.Q, k.s.U, k.s.V};
+  const int iflag = (k.flags >> 1) & 1, i_orig = (k.flags >> 2) & 7, slot = (k.flags >> 5) & 15;
+  const double cospe = P.cospe[peelid], sinpe = P.sinpe[peelid];
+  const double coste = P.coste[peelid], sinte = P.sinte[peelid];
+  const R factor = m_exp(R(-S.tsum));
   s.I = s.I * factor;
   s.Q = s.Q * factor;
   s.U = s.U * factor;
   s.V = s.V * factor;
   if (iflag == 1)
     peel_weight<R>(k.d, R(coste), R(sinte), R(cospe), R(sinpe), hg_of<R>(slot, k.nub), s);
-  const double yimage = z1 * sinte - y1 * coste * sinpe - x1 * coste * cospe;
-  const double ximage = y1 * cospe - x1 * sinpe;
+  const double yimage = S.z * sinte - S.y * coste * sinpe - S.x * coste * cospe;
+  const double ximage = S.y * cospe - S.x * sinpe;
   const int ix = (int)((ximage + P.rmaxi) / P.fractxh) + 1;
   const int iy = (int)((yimage + P.rmaxi) / P.fractxh) + 1;
   const double rho2 = ximage * ximage + yimage * yimage;
@@ -643,7 +696,6 @@ __device__ __forceinline__ PeelOut peeloff(const WallView &W, const PeelRec<R, N
     atomic_add(&P.img_u[q], s.U);
     atomic_add(&P.img_v[q], s.V);
   }
-  return out;
 }
 
 __constant__ double c_inczeta[146];
@@ -1237,37 +1289,64 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // Walk every queued (record, observer) pair of this wavefront: lanes take
 // consecutive rays, so the observers of one record run side by side.
+// The wavefront walks the rays of its peel queue with lane refill: a lane
+// whose ray ends takes the next (record, observer) pair, so the wavefront
+// steps while rays remain instead of waiting, batch by batch, for the
+// longest ray of 64.  Rays that reach the grid's edge wait for their tally
+// until kPeelTallyMin lanes do (or nothing walks), so the tally code also
+// runs for many lanes at once.
+constexpr int kPeelTallyMin = 16;
+
 template <class R, int NA, bool TRACE>
 __device__ __forceinline__ void flush_peels(const WallView &W, const PeelRec<R, NA> *q, int n,
                                             Acc &acc) {
   const Params &P = c_P;
   const int lane = threadIdx.x & 63;
+  const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   const int total = n * P.npeel;
-  for (int r = lane; r < total; r += 64) {
-    const int ri = r / P.npeel, obs = r - ri * P.npeel;
-    const PeelRec<R, NA> &rec = q[ri];
-    double w;
-    if ((rec.flags >> 9) & 1) {  // direct stellar emission (newdisktherm.F90:342-377)
-      if (!P.ips) {
-        const R cosnorm =
-            rec.cosb * R(P.coste[obs]) +
-            (rec.sinb * R(P.sinte[obs]) * (R(P.cospe[obs]) * rec.coslp + R(P.sinpe[obs]) * rec.sinlp));
-        if (!(cosnorm > R(0.0))) continue;
-        w = (double)(R(4.0) * rec.s.I * cosnorm);
-      } else {
-        w = (double)(rec.s.I * R(2.0));
-      }
-    } else {
-      w = (double)rec.s.I;
-    }
-    const PeelOut o = peeloff(W, rec, w, obs);
-    acc.err_wall += o.err;
+  int base = 0;  // next pair to hand out (the same on every lane)
+  PeelWalk S;
+  S.r = -1;
+  int st = kPeelIdle;
+  auto ray_done = [&]() {
     acc.npeelray++;
-    acc.npeelcross += (uint32_t)o.ncross;
+    acc.npeelcross += (uint32_t)S.ncross;
+    acc.err_wall += (uint32_t)S.err;
     if constexpr (TRACE) {
-      atomicAdd(reinterpret_cast<unsigned long long *>(&P.trace[rec.pid].npeelcross),
-                (unsigned long long)o.ncross);
-      atomic_add(&P.trace[rec.pid].tau_peel, o.tau);
+      const long long pid = q[S.r / P.npeel].pid;
+      atomicAdd(reinterpret_cast<unsigned long long *>(&P.trace[pid].npeelcross),
+                (unsigned long long)S.ncross);
+      atomic_add(&P.trace[pid].tau_peel, S.err ? 0.0 : S.tau);
+    }
+    S.r = -1;
+  };
+  for (;;) {
+    const unsigned long long idle = __ballot(st == kPeelIdle);
+    if (st == kPeelIdle) {
+      const int r = base + __popcll(idle & below);
+      if (r < total) {
+        st = peel_begin(W, q, r, S);
+        if (st == kPeelIdle && S.r >= 0) ray_done();  // ended before its first crossing
+      }
+    }
+    base += __popcll(idle);
+    const unsigned long long walking = __ballot(st == kPeelWalk);
+    const unsigned long long tally = __ballot(st == kPeelTally);
+    if (walking == 0ull && tally == 0ull) {
+      if (base >= total) break;
+      continue;
+    }
+    if (tally != 0ull && (__popcll(tally) >= kPeelTallyMin || walking == 0ull)) {
+      if (st == kPeelTally) {
+        peel_tally(q[S.r / P.npeel], S);
+        ray_done();
+        st = kPeelIdle;
+      }
+      continue;
+    }
+    if (st == kPeelWalk) {
+      st = peel_step(W, q[S.r / P.npeel], S);
+      if (st == kPeelIdle) ray_done();
     }
   }
 }
