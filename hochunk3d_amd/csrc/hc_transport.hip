@@ -101,7 +101,7 @@ __device__ __forceinline__ const Params &kernel_params() {
 #if defined(__HIP_DEVICE_COMPILE__)
   return *reinterpret_cast<const Params *>(__builtin_amdgcn_kernarg_segment_ptr());
 #else
-  return *static_cast<const Params *>(nullptr);  // the host pass never runs device code
+  __builtin_unreachable();  // the host pass never runs device code
 #endif
 }
 #define c_P (::hc::kernel_params())
