@@ -180,7 +180,7 @@ def main():
     ap.add_argument("--warmup-np", type=float, default=0,
                     help="packets per warm-up iteration (default a quarter of --np)")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 path")
-    ap.add_argument("--peel-np", type=float, default=100_000,
+    ap.add_argument("--peel-np", type=float, default=500_000,
                     help="packets of the final peeled pass timed after the Lucy steps (0: skip)")
     ap.add_argument("--autotune-np", type=float, default=2_000_000,
                     help="packets per autotuning probe pass (0: default launch shape)")
