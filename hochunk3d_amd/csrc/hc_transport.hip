@@ -547,7 +547,7 @@ enum { kPeelIdle = 0, kPeelWalk = 1, kPeelTally = 2 };
 
 struct PeelWalk {
   double x, y, z, ux, uy, uz, tsum, w, tau;
-  int ir, it, ip, ncross, err, r;
+  int ir, it, ip, ncross, err, r, rec, obs;  // r = rec * npeel + obs
   bool on_wall;
 };
 
@@ -575,6 +575,8 @@ __device__ __forceinline__ int peel_begin(const WallView &W, const PeelRec<R, NA
     S.w = (double)k.s.I;
   }
   S.r = r;
+  S.rec = ri;
+  S.obs = obs;
   S.tau = 0.0;
   S.ncross = 0;
   S.err = 0;
@@ -640,7 +642,7 @@ __device__ __forceinline__ int peel_step(const WallView &W, const PeelRec<R, NA>
 template <class R, int NA>
 __device__ __forceinline__ void peel_tally(const PeelRec<R, NA> &k, const PeelWalk &S) {
   const Params &P = c_P;
-  const int peelid = S.r % P.npeel;
+  const int peelid = S.obs;
   Stokes4<R> s{R(S.w), k.s.Q, k.s.U, k.s.V};
   const int iflag = (k.flags >> 1) & 1, i_orig = (k.flags >> 2) & 7, slot = (k.flags >> 5) & 15;
   const double cospe = P.cospe[peelid], sinpe = P.sinpe[peelid];
@@ -1313,7 +1315,7 @@ __device__ __forceinline__ void flush_peels(const WallView &W, const PeelRec<R, 
     acc.npeelcross += (uint32_t)S.ncross;
     acc.err_wall += (uint32_t)S.err;
     if constexpr (TRACE) {
-      const long long pid = q[S.r / P.npeel].pid;
+      const long long pid = q[S.rec].pid;
       atomicAdd(reinterpret_cast<unsigned long long *>(&P.trace[pid].npeelcross),
                 (unsigned long long)S.ncross);
       atomic_add(&P.trace[pid].tau_peel, S.err ? 0.0 : S.tau);
@@ -1338,14 +1340,14 @@ __device__ __forceinline__ void flush_peels(const WallView &W, const PeelRec<R, 
     }
     if (tally != 0ull && (__popcll(tally) >= kPeelTallyMin || walking == 0ull)) {
       if (st == kPeelTally) {
-        peel_tally(q[S.r / P.npeel], S);
+        peel_tally(q[S.rec], S);
         ray_done();
         st = kPeelIdle;
       }
       continue;
     }
     if (st == kPeelWalk) {
-      st = peel_step(W, q[S.r / P.npeel], S);
+      st = peel_step(W, q[S.rec], S);
       if (st == kPeelIdle) ray_done();
     }
   }
