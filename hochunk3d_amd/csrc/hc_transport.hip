@@ -1059,7 +1059,10 @@ __device__ __forceinline__ void launch_star(Pkt<R, NA> &k, Mrg &rng, bool hotspo
   int ispotph = hotspot ? 1 : 0;
   k.s = Stokes4<R>{R(1.0), R(0.0), R(0.0), R(0.0)};
   if (!P.ips) {
-    for (;;) {
+    // a spot point by rejection (initpspot.f90:34-60); after kMaxTrials the
+    // last point is kept, a bound that a spot of any area reached by
+    // hc_set_sources' checks does not meet in practice
+    for (uint32_t ntry = 1;; ++ntry) {
       for (;;) {
         cosb = R(1.0) - R(2.0) * R(ran(rng));
         sinb = m_sqrt(R(1.0) - cosb * cosb);
@@ -1071,7 +1074,7 @@ __device__ __forceinline__ void launch_star(Pkt<R, NA> &k, Mrg &rng, bool hotspo
         const R d1 = R(P.scspot1) * sinb * cphi + R(P.ssspot1) * sinb * sphi + R(P.cspot1) * cosb;
         const R d2 = R(P.scspot2) * sinb * cphi + R(P.ssspot2) * sinb * sphi + R(P.cspot2) * cosb;
         const R cs = R(P.csprad);
-        if (P.nspot == 1 ? (d1 < cs) : (d1 < cs && d2 < cs)) continue;
+        if ((P.nspot == 1 ? (d1 < cs) : (d1 < cs && d2 < cs)) && ntry < kMaxTrials) continue;
       }
       break;
     }
@@ -2125,6 +2128,20 @@ int hc_set_sources(hc_ctx *ctx, const hc_sources *src) {
     return fail(ctx, HC_E_ARG,
                 "hc_set_sources: disk accretion needs 1 <= rddustmin < rmaxdmax, rmax > 1, "
                 "massc > 0, mdot >= 0");
+  if (src->ispot == 1) {
+    // spotset's geometry: cos/sin of the spot's colatitude and their
+    // products with cos/sin of its longitude; the launch's rejection loop
+    // only ends for a spot that lies on the star
+    auto unit = [](double c, double s, double sc, double ss) {
+      return std::fabs(c * c + s * s - 1.0) < 1e-6 && std::fabs(sc * sc + ss * ss - s * s) < 1e-6;
+    };
+    if (!(src->nspot == 1 || src->nspot == 2) || !(src->csprad > -1.0 && src->csprad < 1.0) ||
+        !unit(src->cspot1, src->sspot1, src->scspot1, src->ssspot1) ||
+        (src->nspot == 2 && !unit(src->cspot2, src->sspot2, src->scspot2, src->ssspot2)))
+      return fail(ctx, HC_E_ARG,
+                  "hc_set_sources: hot spots need nspot 1 or 2, -1 < csprad < 1 and spot "
+                  "directions with cspot^2 + sspot^2 = 1 and scspot^2 + ssspot^2 = sspot^2");
+  }
   HCK(hipSetDevice(ctx->device));
   std::vector<double> sini(d.nmu);
   for (int i = 0; i < d.nmu; ++i) sini[i] = std::sqrt(1.0 - src->u[i] * src->u[i]);

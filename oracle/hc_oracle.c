@@ -46,6 +46,7 @@
 #define GN 6.67428e-8                    /* constants.f90:19 */
 #define MSOL 1.989e33                    /* constants.f90:13 */
 #define HUGE_T DBL_MAX
+#define HCO_MAX_TRIALS (1u << 20)  /* the device's rejection-loop bound (kMaxTrials) */
 
 /* ------------------------------------------------------------------------ */
 /* RNG: ecuyer_cote_mod.f90 (L'Ecuyer & Cote 1991)                           */
@@ -1464,7 +1465,9 @@ static void launch_star(ctx_t *c, pkt_t *P, int hotspot) {
   P->sip = 1.0;
   P->sqp = P->sup = P->svp = 0.0;
   if (!s->ips) {
-    for (;;) {
+    /* the spot rejection ends after HCO_MAX_TRIALS tries (the device's
+     * kMaxTrials), keeping the last point */
+    for (unsigned ntry = 1;; ++ntry) {
       for (;;) {
         xran = hco_ran(&c->rng);
         cosb = 1.0 - 2.0 * xran;
@@ -1473,7 +1476,7 @@ static void launch_star(ctx_t *c, pkt_t *P, int hotspot) {
       }
       xran = hco_ran(&c->rng);
       lp = R2P * xran;
-      if (s->ispot == 1 && ispotph == 1) {
+      if (s->ispot == 1 && ispotph == 1 && ntry < HCO_MAX_TRIALS) {
         double cphi = HCO_COS(lp), sphi = HCO_SIN(lp);
         double dotsp1 = s->scspot1 * sinb * cphi + s->ssspot1 * sinb * sphi + s->cspot1 * cosb;
         double dotsp2 = s->scspot2 * sinb * cphi + s->ssspot2 * sinb * sphi + s->cspot2 * cosb;

@@ -164,3 +164,41 @@ def test_even_theta_walls_fractal_model_matches_oracle():
     cp = c.run(400, ipeel=1, trace=True, seg0=10_000)
     compare_traces(rp.trace, cp["trace"])
     rel_close(rp.peel, cp["peel"], rtol=1e-9, atol=1e-12)
+
+
+def test_hot_spot_launch_matches_oracle():
+    """Hot-spot packets (initpspot.f90 with two spots, newdisktherm.F90:256-273:
+    the spot's shock temperature and the X-ray half) beside stellar ones,
+    Lucy and peeled passes, traced packet by packet."""
+    from hochunk3d_amd._lib import Transport
+    from oracle.pyoracle import Oracle
+    m = build_model(config1(nximg=21, nfreq=50, accfrac2=0.3))
+    m.tdust[...] = synthetic_temperature(m)
+    d2r = np.pi / 180.0
+    th1, ph1, th2, ph2, rad = 30 * d2r, 45 * d2r, 140 * d2r, 200 * d2r, 20 * d2r
+    spot = dict(ispot=1, nspot=2, fspot=0.06, csprad=float(np.cos(rad)),
+                cspot1=float(np.cos(th1)), sspot1=float(np.sin(th1)),
+                scspot1=float(np.sin(th1) * np.cos(ph1)), ssspot1=float(np.sin(th1) * np.sin(ph1)),
+                cspot2=float(np.cos(th2)), sspot2=float(np.sin(th2)),
+                scspot2=float(np.sin(th2) * np.cos(ph2)), ssspot2=float(np.sin(th2) * np.sin(ph2)))
+    g = Transport(m)
+    try:
+        from hochunk3d_amd._lib import HcError
+        with pytest.raises(HcError):       # a spot direction that is not a unit vector
+            g.set_sources(**dict(spot, scspot1=0.0, ssspot1=0.0))
+        g.set_sources(**spot)
+        rg = g.run(8000, ipeel=0, trace=True)
+        rp = g.run(600, ipeel=1, trace=True, seg0=50_000)
+    finally:
+        g.close()
+    c = Oracle(m)
+    for k, v in spot.items():
+        setattr(c.setup, k, v)
+    rc = c.run(8000, ipeel=0, trace=True)
+    compare_traces(rg.trace, rc["trace"])
+    _check_tallies(rg, rc)
+    # the hot-spot packets' initial spectrum (slot 3) is populated, X-rays included
+    assert rg.si_init[:, 2].sum() > 0
+    cp = c.run(600, ipeel=1, trace=True, seg0=50_000)
+    compare_traces(rp.trace, cp["trace"])
+    rel_close(rp.peel, cp["peel"], rtol=1e-9, atol=1e-12)
