@@ -27,7 +27,7 @@ EXPORTS = (
     "hc_create", "hc_set_grid", "hc_set_temperature", "hc_get_temperature", "hc_set_dust",
     "hc_set_sources",
     "hc_run_iteration", "hc_temperature", "hc_run_lucy", "hc_set_option", "hc_destroy",
-    "hc_last_error", "hc_build_density", "hc_build_emissivity",
+    "hc_last_error", "hc_build_density", "hc_build_emissivity", "hc_fractal_counts",
     "hc_abi_version",
 )
 
@@ -127,6 +127,9 @@ def load() -> C.CDLL:
     lib.hc_build_density.argtypes = [C.POINTER(HcDensityModel), C.c_int, _dp, _dp, _dp]
     lib.hc_build_emissivity.argtypes = [C.c_int, C.c_int32, _dp, _dp, _dp, C.c_int32, _dp,
                                         C.c_int32, _dp, C.c_int32, _dp, _dp, _dp, _dp]
+    lib.hc_fractal_counts.argtypes = [C.c_int, C.c_int32, C.c_int32, C.c_int32, _dp, _dp, _dp,
+                                      _dp, C.c_int32, C.c_int32, _dp, C.c_double,
+                                      C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]
     lib.hc_destroy.argtypes = [C.c_void_p]
     lib.hc_last_error.argtypes = [C.c_void_p]
     lib.hc_last_error.restype = C.c_char_p
@@ -395,3 +398,23 @@ def build_emissivity(nud, kd, ad, lTint, nuint, xq, wq, device: int = 0):
     if rc != 0:
         raise HcError(f"hc_build_emissivity failed ({rc})")
     return kapB, kp
+
+
+def fractal_counts_device(rarr, mcos_t, cph, sph, n: int, levels: int, scale, scale_r: float,
+                          state: int, inc: int, device: int = 0) -> np.ndarray:
+    """hc_fractal_counts: points per cell (nrg,ntg,npg) F-order, as float64;
+    state/inc are the PCG64 state and increment (128-bit integers)."""
+    lib = load()
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (rarr, mcos_t, cph, sph)]
+    sc = np.ascontiguousarray(scale if len(scale) else [0.0], dtype=np.float64)
+    m64 = (1 << 64) - 1
+    pcg = np.array([state & m64, state >> 64, inc & m64, inc >> 64], dtype=np.uint64)
+    nrg, ntg, npg = len(a[0]), len(a[1]), len(a[2])
+    cnt = np.zeros((nrg, ntg, npg), dtype=np.uint32, order="F")
+    rc = lib.hc_fractal_counts(int(device), nrg, ntg, npg, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]),
+                               _ptr(a[3]), int(n), int(levels), _ptr(sc), float(scale_r),
+                               pcg.ctypes.data_as(C.POINTER(C.c_uint64)),
+                               cnt.ctypes.data_as(C.POINTER(C.c_uint32)))
+    if rc != 0:
+        raise HcError(f"hc_fractal_counts failed ({rc})")
+    return cnt.astype(np.float64, order="F")

@@ -14,7 +14,8 @@
 //
 // Also here: the dust emissivity tables of setemiss, one thread per
 // (temperature, frequency bin), then one per temperature row for the
-// cumulative sum.
+// cumulative sum; and the clump point counts of the envelope multiplier, one
+// thread per point.
 
 #include "../../include/hc.h"
 
@@ -211,7 +212,228 @@ __global__ __launch_bounds__(256) void emis_cumulate(Emis E) {
   E.bins[i + (size_t)E.nT * (E.nnu - 1)] = 1.0;
 }
 
+// ---------------------------------------------------------------------------
+// Clump points per cell for the envelope multiplier: the construction of
+// hochunk3d_amd/fractal.py, whose deviates come from NumPy's PCG64 (128-bit
+// LCG, XSL-RR output, doubles from the top 53 bits).  Each thread places one
+// leaf point: it jumps the stream to the point's draws (the LCG jumped by
+// 2^b steps, b = 0..63, is tabulated once), sums the parent coordinate and
+// the level offsets in the host's order and counts the point in its cell
+// with the host's comparisons.
+
+struct U128 {
+  uint64_t lo, hi;
+};
+
+__device__ __forceinline__ U128 mul128(U128 a, U128 b) {  // mod 2^128
+  U128 r;
+  r.lo = a.lo * b.lo;
+  r.hi = __umul64hi(a.lo, b.lo) + a.lo * b.hi + a.hi * b.lo;
+  return r;
+}
+
+__device__ __forceinline__ U128 add128(U128 a, U128 b) {
+  U128 r;
+  r.lo = a.lo + b.lo;
+  r.hi = a.hi + b.hi + (r.lo < a.lo ? 1u : 0u);
+  return r;
+}
+
+// jump[b] = (multiplier, increment) of 2^b LCG steps
+__global__ void pcg_jump_table(U128 mult, U128 inc, U128 *jump) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  U128 a = mult, c = inc;
+  const U128 one = {1u, 0u};
+  for (int b = 0; b < 64; ++b) {
+    jump[2 * b] = a;
+    jump[2 * b + 1] = c;
+    c = mul128(c, add128(a, one));
+    a = mul128(a, a);
+  }
+}
+
+__device__ __forceinline__ double pcg_double(U128 s) {
+  const unsigned rot = (unsigned)(s.hi >> 58);
+  const uint64_t x = s.hi ^ s.lo;
+  const uint64_t o = (x >> rot) | (x << ((64u - rot) & 63u));
+  return (double)(o >> 11) * 0x1.0p-53;
+}
+
+struct Frac {
+  int nrg, ntg, npg, n, levels;
+  uint64_t npts, nlev;                // n^(levels+1), n^levels
+  const double *rarr, *mcos_t, *cph, *sph;
+  double scale[8], scale_r;
+  uint64_t S;                         // draws per parent
+  uint64_t offk[9], div[9];           // first draw of level k; n^(levels-k)
+  U128 s0;
+  const U128 *jump;
+  uint32_t *counts;
+};
+
+// the draws i, i+1, i+2 of the stream that starts at s0
+__device__ __forceinline__ void draw3(const Frac &F, uint64_t i, double d[3]) {
+  U128 s = F.s0;
+  const uint64_t k = i + 1;
+  for (int b = 0; b < 64; ++b)
+    if ((k >> b) & 1u) s = add128(mul128(s, F.jump[2 * b]), F.jump[2 * b + 1]);
+  for (int c = 0; c < 3; ++c) {
+    d[c] = pcg_double(s);
+    s = add128(mul128(s, F.jump[0]), F.jump[1]);
+  }
+}
+
+// number of entries of a[0..n) below v (lower bound)
+__device__ __forceinline__ int count_below(const double *a, int n, double v) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] < v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void fractal_points(Frac F) {
+  extern __shared__ double walls[];  // rarr, mcos_t, cph, sph
+  double *rw = walls, *tw = walls + F.nrg, *cw = tw + F.ntg, *sw = cw + F.npg;
+  for (int i = threadIdx.x; i < F.nrg; i += blockDim.x) rw[i] = F.rarr[i];
+  for (int i = threadIdx.x; i < F.ntg; i += blockDim.x) tw[i] = F.mcos_t[i];
+  for (int i = threadIdx.x; i < F.npg; i += blockDim.x) {
+    cw[i] = F.cph[i];
+    sw[i] = F.sph[i];
+  }
+  __syncthreads();
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= F.npts) return;
+  const uint64_t parent = t / F.nlev, rem = t % F.nlev;
+  double d[3], x[3];
+  draw3(F, 3 * parent, d);
+  for (int c = 0; c < 3; ++c) x[c] = d[c] - 0.5;
+  for (int k = 1; k <= F.levels; ++k) {
+    const uint64_t node = rem / F.div[k];
+    draw3(F, 3 * (uint64_t)F.n + parent * F.S + F.offk[k] + 3 * node, d);
+    for (int c = 0; c < 3; ++c) x[c] = x[c] + (d[c] - 0.5) * F.scale[k - 1];
+  }
+  const double r = __builtin_sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  double cz = x[2] / r;
+  cz = cz < -1.0 ? -1.0 : (cz > 1.0 ? 1.0 : cz);
+  int ir = count_below(rw, F.nrg, r * F.scale_r);
+  if (ir == 0) ir = 1;
+  if (ir > F.nrg - 1) return;
+  int it = count_below(tw, F.ntg, -cz);
+  if (it < 1) it = 1;
+  const double px = x[0], py = x[1];
+  const bool lower = (py < 0.0) || (py == 0.0 && px < 0.0);
+  int lo = 0, hi = F.npg;  // first wall not below the point's azimuth
+  while (lo < hi) {
+    const int j = (lo + hi) >> 1;
+    const bool hw = 2 * j >= F.npg - 1;
+    const double cr = cw[j] * py - sw[j] * px;
+    const bool below = (hw != lower) ? (lower && !hw) : (cr > 0.0);
+    if (below)
+      lo = j + 1;
+    else
+      hi = j;
+  }
+  const int ip = lo < 1 ? 1 : lo;
+  const size_t idx = (size_t)(ir - 1) +
+                     (size_t)F.nrg * ((size_t)(it - 1) + (size_t)F.ntg * (size_t)(ip - 1));
+  atomicAdd(&F.counts[idx], 1u);
+}
+
 }  // namespace hc_grid
+
+extern "C" int hc_fractal_counts(int device, int32_t nrg, int32_t ntg, int32_t npg,
+                                 const double *rarr, const double *mcos_theta,
+                                 const double *cos_phi, const double *sin_phi, int32_t n,
+                                 int32_t levels, const double *scale, double scale_r,
+                                 const uint64_t *pcg, uint32_t *counts) {
+  using namespace hc_grid;
+  if (nrg < 2 || ntg < 2 || npg < 2 || n < 1 || levels < 0 || levels > 8 || !rarr ||
+      !mcos_theta || !cos_phi || !sin_phi || (levels > 0 && !scale) || !pcg || !counts)
+    return HC_E_ARG;
+  if ((pcg[2] & 1u) == 0) return HC_E_ARG;  // the PCG increment is odd
+  Frac F{};
+  F.nrg = nrg;
+  F.ntg = ntg;
+  F.npg = npg;
+  F.n = n;
+  F.levels = levels;
+  // n^(levels+1) points, at most 2^31
+  uint64_t p = 1;
+  for (int k = 0; k <= levels; ++k) {
+    p *= (uint64_t)n;
+    if (p > (1ull << 31)) return HC_E_ARG;
+  }
+  F.npts = p;
+  F.nlev = p / (uint64_t)n;
+  uint64_t nk = 1, acc = 0;
+  for (int k = 1; k <= levels; ++k) {
+    F.offk[k] = acc;  // 3 (n + ... + n^(k-1))
+    nk *= (uint64_t)n;
+    acc += 3 * nk;
+    F.scale[k - 1] = scale[k - 1];
+  }
+  F.S = acc;
+  for (int k = 1; k <= levels; ++k) {
+    uint64_t dv = 1;
+    for (int j = 0; j < levels - k; ++j) dv *= (uint64_t)n;
+    F.div[k] = dv;
+  }
+  F.scale_r = scale_r;
+  F.s0 = U128{pcg[0], pcg[1]};
+  const size_t lds = (size_t)(nrg + ntg + 2 * npg) * sizeof(double);
+  if (lds > 64 * 1024) return HC_E_UNSUPPORTED;
+  const size_t ncell = (size_t)nrg * ntg * npg;
+  if (hipSetDevice(device) != hipSuccess) return HC_E_HIP;
+  hipStream_t s = nullptr;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return HC_E_HIP;
+  double *d_w = nullptr;
+  U128 *d_jump = nullptr;
+  uint32_t *d_cnt = nullptr;
+  int rc = HC_OK;
+  auto ok = [&](hipError_t e) {
+    if (e != hipSuccess && rc == HC_OK) rc = HC_E_HIP;
+    return rc == HC_OK;
+  };
+  ok(hipMalloc((void **)&d_w, lds)) && ok(hipMalloc((void **)&d_jump, 128 * sizeof(U128))) &&
+      ok(hipMalloc((void **)&d_cnt, ncell * sizeof(uint32_t))) &&
+      ok(hipMemcpyAsync(d_w, rarr, nrg * sizeof(double), hipMemcpyHostToDevice, s)) &&
+      ok(hipMemcpyAsync(d_w + nrg, mcos_theta, ntg * sizeof(double), hipMemcpyHostToDevice, s)) &&
+      ok(hipMemcpyAsync(d_w + nrg + ntg, cos_phi, npg * sizeof(double), hipMemcpyHostToDevice,
+                        s)) &&
+      ok(hipMemcpyAsync(d_w + nrg + ntg + npg, sin_phi, npg * sizeof(double),
+                        hipMemcpyHostToDevice, s)) &&
+      ok(hipMemsetAsync(d_cnt, 0, ncell * sizeof(uint32_t), s));
+  if (rc == HC_OK) {
+    const U128 mult = {0x4385df649fccf645ull, 0x2360ed051fc65da4ull};  // PCG64's multiplier
+    hipLaunchKernelGGL(pcg_jump_table, dim3(1), dim3(64), 0, s, mult, U128{pcg[2], pcg[3]},
+                       d_jump);
+    ok(hipGetLastError());
+  }
+  if (rc == HC_OK) {
+    F.rarr = d_w;
+    F.mcos_t = d_w + nrg;
+    F.cph = d_w + nrg + ntg;
+    F.sph = d_w + nrg + ntg + npg;
+    F.jump = d_jump;
+    F.counts = d_cnt;
+    const uint64_t blocks = (F.npts + 255) / 256;
+    hipLaunchKernelGGL(fractal_points, dim3((unsigned)blocks), dim3(256), lds, s, F);
+    ok(hipGetLastError()) &&
+        ok(hipMemcpyAsync(counts, d_cnt, ncell * sizeof(uint32_t), hipMemcpyDeviceToHost, s)) &&
+        ok(hipStreamSynchronize(s));
+  }
+  (void)hipStreamSynchronize(s);
+  if (d_w) (void)hipFree(d_w);
+  if (d_jump) (void)hipFree(d_jump);
+  if (d_cnt) (void)hipFree(d_cnt);
+  (void)hipStreamDestroy(s);
+  return rc;
+}
 
 extern "C" int hc_build_emissivity(int device, int32_t nl, const double *nud, const double *kd,
                                    const double *ad, int32_t nT, const double *lTint,

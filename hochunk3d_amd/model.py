@@ -273,8 +273,8 @@ def _densities_host(cfg, rarr, thetarr, phiarr, rmincgs, autors, rmind, rmine, r
 
 def _densities_device(cfg, rarr, thetarr, phiarr, rmincgs, autors, rmind, rmine, rmax, rmaxd,
                       device):
-    """The same quantities from hc_build_density on GPU `device` (the clump
-    multiplier, a point count, stays on the host)."""
+    """The same quantities from hc_build_density on GPU `device`, with the
+    clump point counts from hc_fractal_counts there too."""
     from ._lib import build_density
     nrg, ntg, npg = len(rarr), len(thetarr), len(phiarr)
     disks = _disk_params(cfg, autors)
@@ -282,7 +282,7 @@ def _densities_device(cfg, rarr, thetarr, phiarr, rmincgs, autors, rmind, rmine,
     mult = None
     if cfg.fractal:
         from .fractal import fractal_multiplier
-        mult = fractal_multiplier(rarr, thetarr, phiarr, **cfg.fractal)
+        mult = fractal_multiplier(rarr, thetarr, phiarr, **cfg.fractal, device=device)
     fields = dict(
         nrg=nrg, ntg=ntg, npg=npg, ndisk=len(disks), rarr=rarr, thetarr=thetarr, phiarr=phiarr,
         fractal=mult, rmincgs=rmincgs, disk_id=[d[0] for d in disks],

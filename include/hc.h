@@ -233,6 +233,24 @@ int hc_build_emissivity(int device, int32_t nl, const double *nud, const double 
                         const double *nuint, int32_t nq, const double *xq, const double *wq,
                         double *kapBint, double *kappap);
 
+/* Points per cell of the clumpy-envelope construction (fractal_sph.f90:1-175
+ * as restated by hochunk3d_amd/fractal.py), the multiplier's input: n parents
+ * uniform in the unit cube, each with n children per level for `levels`
+ * levels, offset by (u - 0.5) scale[k-1] at level k; a point's radius is
+ * scaled by scale_r and the point counted in its cell.  The deviates are
+ * NumPy PCG64 doubles from the stream whose 128-bit state and increment are
+ * pcg = {state lo, state hi, inc lo, inc hi} (increment odd), drawn in the
+ * host construction's order, so the counts equal the host's.  Cells: radial
+ * walls rarr[nrg] below r scale_r, walls with mcos_theta[j] = -cos(theta_j)
+ * below -z/r, azimuth walls (uniform, cos_phi/sin_phi[npg]) below the point
+ * by half-plane and cross product.  counts: (nrg,ntg,npg) column-major.
+ * n^(levels+1) <= 2^31, levels <= 8.  No counterpart in the reference's
+ * loop: it replaces the host-side point placement of grid setup. */
+int hc_fractal_counts(int device, int32_t nrg, int32_t ntg, int32_t npg, const double *rarr,
+                      const double *mcos_theta, const double *cos_phi, const double *sin_phi,
+                      int32_t n, int32_t levels, const double *scale, double scale_r,
+                      const uint64_t *pcg, uint32_t *counts);
+
 int hc_destroy(hc_ctx *ctx);
 const char *hc_last_error(const hc_ctx *ctx);
 int hc_abi_version(void);

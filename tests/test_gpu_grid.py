@@ -2,7 +2,8 @@
 of the same formulas: flared disks with their mass normalisation, the Ulrich
 envelope (the streamline cubic), the polynomial cavity, the clump multiplier,
 cell volumes and (rho V)^(1/4).  The two differ only in the last bits of the
-elementary functions and of the disk-mass sums."""
+elementary functions and of the disk-mass sums.  The clump point counts
+(hc_fractal_counts) equal the host's exactly."""
 import time
 
 import numpy as np
@@ -71,3 +72,20 @@ def test_device_emissivity_tables_match_host(kappa_v, albedo_max):
     # a normalised cumulative table: non-decreasing, 1 at the last node
     kb = d.kapBint[:, :, 0]
     assert np.all(np.diff(kb, axis=1) >= 0.0) and np.all(kb[:, -1] == 1.0)
+
+
+@pytest.mark.parametrize("shape,n,levels", [((40, 41, 24), 8, 3), ((40, 41, 24), 32, 4),
+                                            ((30, 31, 1), 16, 3)])
+def test_device_clump_counts_equal_host(shape, n, levels):
+    """hc_fractal_counts jumps NumPy's PCG64 stream to each point's draws:
+    the counts per cell equal the host construction's exactly."""
+    from hochunk3d_amd.fractal import fractal_counts
+    nr, nt, nph = shape
+    m = build_model(config2(nr=nr, ntheta=nt, nphi=nph, fractal=None))
+    kw = dict(n=n, levels=levels, fl=3.792, seed=11)
+    h = fractal_counts(m.rarr, m.thetarr, m.phiarr, **kw)
+    t0 = time.perf_counter()
+    d = fractal_counts(m.rarr, m.thetarr, m.phiarr, **kw, device=0)
+    print(f"{n}^{levels + 1} points: device {time.perf_counter() - t0:.3f} s")
+    np.testing.assert_array_equal(d, h)
+    assert 0 < h.sum() <= n ** (levels + 1)      # points beyond rmax are dropped
