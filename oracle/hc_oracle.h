@@ -9,6 +9,18 @@
  * 0-based here.  Units follow the reference: lengths in stellar radii,
  * temperatures and photon energies in eV (T[K]/11605, nu = 1.2398/lambda[um]),
  * densities in g/cm^3, kapd = kappa/kappa_V * R*(cm) * kappa_V.
+ *
+ * How each part is pinned (tests/):
+ *   RNG, bbfreq, radface/phiface/closest_wall, locate, opacset, KapP,
+ *   lucydustfreq, dusttemp, select_dust, dustmat, findangle3: bit-exact
+ *   against the reference's own routines (golden vectors);
+ *   thetaface, the walk: closed-form ray/wall crossing counts;
+ *   stokes, stokespeel: closed forms and distributions fixed by the phase
+ *   matrix (polarisation degree, Cornette-Shanks moments, azimuth modulation);
+ *   the accretion launch: its sampled distributions against closed forms;
+ *   tfinal: dusttemp's golden vectors;
+ *   the star launch (initp_*): parity unpinned -- checked only through the
+ *   GPU-versus-oracle packet traces.
  */
 #ifndef HC_ORACLE_H
 #define HC_ORACLE_H
