@@ -1257,6 +1257,23 @@ static void initp_transform(pkt_t *P, double cosb, double sinb, double lp) {
   P->cosp = HCO_COS(P->phi);
 }
 
+/* Leaf entry for the tests: the emission direction of a packet leaving the
+ * stellar surface at (cosb, sinb, lp), initp_sample then initp_transform. */
+void hco_initp_direction(hco_rng *r, double cosb, double sinb, double lp, double *cost,
+                         double *sint, double *phi) {
+  ctx_t c;
+  pkt_t P;
+  memset(&c, 0, sizeof c);
+  memset(&P, 0, sizeof P);
+  c.rng = *r;
+  initp_sample(&c, &P);
+  initp_transform(&P, cosb, sinb, lp);
+  *r = c.rng;
+  *cost = P.cost;
+  *sint = P.sint;
+  *phi = P.phi;
+}
+
 static void bin_init(ctx_t *c, double nub, int i_orig_init) {
   const hco_setup *s = c->s;
   int inub;

@@ -19,8 +19,11 @@
  *   matrix (polarisation degree, Cornette-Shanks moments, azimuth modulation);
  *   the accretion launch: its sampled distributions against closed forms;
  *   tfinal: dusttemp's golden vectors;
- *   the star launch (initp_*): parity unpinned -- checked only through the
- *   GPU-versus-oracle packet traces.
+ *   the star launch's emission direction (initp_sample, initp_transform):
+ *   the cosine with the surface normal equals the sqrt of its draw, follows
+ *   F(mu) = mu^2, and the azimuth about the normal is uniform; the launch
+ *   point and spot selection of launch_star: parity unpinned -- checked only
+ *   through the GPU-versus-oracle packet traces.
  */
 #ifndef HC_ORACLE_H
 #define HC_ORACLE_H
@@ -167,6 +170,11 @@ void hco_stokespeel(double *I, double *Q, double *U, double *V, double *cost, do
 void hco_findangle3(double cost1, double sint1, double phi1, double cosp1, double sinp1,
                     double cost2, double sint2, double phi2, double cosp2, double sinp2,
                     double *costnew, double *sintnew, double *phinew, double *ri1);
+
+/* Emission direction at the stellar surface point (cosb, sinb, lp):
+ * initp_common.f90:1-94 (limb = 0). */
+void hco_initp_direction(hco_rng *r, double cosb, double sinb, double lp, double *cost,
+                         double *sint, double *phi);
 
 int hco_select_dust(const double *kapd, const double *dens, hco_rng *r);
 void hco_dustmat(const hco_opac *o, int id, double cost, double cost2, double *p1, double *p2,
