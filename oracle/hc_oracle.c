@@ -1474,6 +1474,58 @@ static void launch_acc(ctx_t *c, pkt_t *P) {
  * hot-spot branch of the photon loop (newdisktherm.F90:256-273): the packet
  * starts inside the spot when ispot=1, is emitted at tshock and half of such
  * packets are X-rays.  Stellar packets take the tstar blackbody. */
+/* The surface point of a stellar packet (initpspot.f90:3-120): uniform on
+ * the star, or, for a hot-spot packet with ispot=1, uniform within the spot
+ * caps by rejection; a stellar packet that lands in a cap becomes a spot
+ * packet.  Returns ispotph. */
+static int star_point(const hco_setup *s, hco_rng *r, int hotspot, double *cosb_out,
+                      double *sinb_out, double *lp_out) {
+  double cosb, sinb, lp, xran;
+  int ispotph = hotspot;
+  /* the spot rejection ends after HCO_MAX_TRIALS tries (the device's
+   * kMaxTrials), keeping the last point */
+  for (unsigned ntry = 1;; ++ntry) {
+    for (;;) {
+      xran = hco_ran(r);
+      cosb = 1.0 - 2.0 * xran;
+      sinb = sqrt(1.0 - cosb * cosb);
+      if (cosb != 0.0 && fabs(cosb) < 1.0) break;
+    }
+    xran = hco_ran(r);
+    lp = R2P * xran;
+    if (s->ispot == 1 && ispotph == 1 && ntry < HCO_MAX_TRIALS) {
+      double cphi = HCO_COS(lp), sphi = HCO_SIN(lp);
+      double dotsp1 = s->scspot1 * sinb * cphi + s->ssspot1 * sinb * sphi + s->cspot1 * cosb;
+      double dotsp2 = s->scspot2 * sinb * cphi + s->ssspot2 * sinb * sphi + s->cspot2 * cosb;
+      if (s->nspot == 1) {
+        if (dotsp1 < s->csprad) continue;
+      } else {
+        if (dotsp1 < s->csprad && dotsp2 < s->csprad) continue;
+      }
+    }
+    break;
+  }
+  if (s->ispot == 1 && s->fspot < 1.0) {
+    double dellon = (lp < PI) ? lp : R2P - lp;
+    double costhp = s->cspot1 * cosb + s->sspot1 * sinb * HCO_COS(dellon);
+    if (costhp > s->csprad) ispotph = 1;
+    if (s->nspot == 2) {
+      dellon = (lp < PI) ? PI - lp : lp - PI;
+      costhp = s->cspot2 * cosb + s->sspot2 * sinb * HCO_COS(dellon);
+      if (costhp > s->csprad) ispotph = 1;
+    }
+  }
+  *cosb_out = cosb;
+  *sinb_out = sinb;
+  *lp_out = lp;
+  return ispotph;
+}
+
+int hco_star_point(const hco_setup *s, hco_rng *r, int hotspot, double *cosb, double *sinb,
+                   double *lp) {
+  return star_point(s, r, hotspot, cosb, sinb, lp);
+}
+
 static void launch_star(ctx_t *c, pkt_t *P, int hotspot) {
   const hco_setup *s = c->s;
   const hco_grid *g = c->g;
@@ -1482,39 +1534,7 @@ static void launch_star(ctx_t *c, pkt_t *P, int hotspot) {
   P->sip = 1.0;
   P->sqp = P->sup = P->svp = 0.0;
   if (!s->ips) {
-    /* the spot rejection ends after HCO_MAX_TRIALS tries (the device's
-     * kMaxTrials), keeping the last point */
-    for (unsigned ntry = 1;; ++ntry) {
-      for (;;) {
-        xran = hco_ran(&c->rng);
-        cosb = 1.0 - 2.0 * xran;
-        sinb = sqrt(1.0 - cosb * cosb);
-        if (cosb != 0.0 && fabs(cosb) < 1.0) break;
-      }
-      xran = hco_ran(&c->rng);
-      lp = R2P * xran;
-      if (s->ispot == 1 && ispotph == 1 && ntry < HCO_MAX_TRIALS) {
-        double cphi = HCO_COS(lp), sphi = HCO_SIN(lp);
-        double dotsp1 = s->scspot1 * sinb * cphi + s->ssspot1 * sinb * sphi + s->cspot1 * cosb;
-        double dotsp2 = s->scspot2 * sinb * cphi + s->ssspot2 * sinb * sphi + s->cspot2 * cosb;
-        if (s->nspot == 1) {
-          if (dotsp1 < s->csprad) continue;
-        } else {
-          if (dotsp1 < s->csprad && dotsp2 < s->csprad) continue;
-        }
-      }
-      break;
-    }
-    if (s->ispot == 1 && s->fspot < 1.0) {
-      double dellon = (lp < PI) ? lp : R2P - lp;
-      double costhp = s->cspot1 * cosb + s->sspot1 * sinb * HCO_COS(dellon);
-      if (costhp > s->csprad) ispotph = 1;
-      if (s->nspot == 2) {
-        dellon = (lp < PI) ? PI - lp : lp - PI;
-        costhp = s->cspot2 * cosb + s->sspot2 * sinb * HCO_COS(dellon);
-        if (costhp > s->csprad) ispotph = 1;
-      }
-    }
+    ispotph = star_point(s, &c->rng, hotspot, &cosb, &sinb, &lp);
     initp_sample(c, P);
     initp_transform(P, cosb, sinb, lp);
   } else {

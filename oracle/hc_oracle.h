@@ -19,11 +19,10 @@
  *   matrix (polarisation degree, Cornette-Shanks moments, azimuth modulation);
  *   the accretion launch: its sampled distributions against closed forms;
  *   tfinal: dusttemp's golden vectors;
- *   the star launch's emission direction (initp_sample, initp_transform):
- *   the cosine with the surface normal equals the sqrt of its draw, follows
- *   F(mu) = mu^2, and the azimuth about the normal is uniform; the launch
- *   point and spot selection of launch_star: parity unpinned -- checked only
- *   through the GPU-versus-oracle packet traces.
+ *   the star launch: the surface point uniform on the star or within the
+ *   spot caps, the spot flag of stellar packets, and the emission direction
+ *   (its cosine with the normal equals the sqrt of its draw, its local
+ *   azimuth follows the second draw) -- geometry and distributions.
  */
 #ifndef HC_ORACLE_H
 #define HC_ORACLE_H
@@ -171,6 +170,10 @@ void hco_findangle3(double cost1, double sint1, double phi1, double cosp1, doubl
                     double cost2, double sint2, double phi2, double cosp2, double sinp2,
                     double *costnew, double *sintnew, double *phinew, double *ri1);
 
+/* The surface point of a stellar (hotspot=0) or hot-spot (hotspot=1) packet
+ * (initpspot.f90:3-120); returns 1 when it is emitted as a spot packet. */
+int hco_star_point(const hco_setup *s, hco_rng *r, int hotspot, double *cosb, double *sinb,
+                   double *lp);
 /* Emission direction at the stellar surface point (cosb, sinb, lp):
  * initp_common.f90:1-94 (limb = 0). */
 void hco_initp_direction(hco_rng *r, double cosb, double sinb, double lp, double *cost,

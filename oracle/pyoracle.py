@@ -146,6 +146,8 @@ def load(variant: str = "libm") -> C.CDLL:
         lib.hco_findangle3.argtypes = [C.c_double] * 10 + [_dp] * 4
         lib.hco_initp_direction.argtypes = [C.POINTER(Rng), C.c_double, C.c_double, C.c_double,
                                             _dp, _dp, _dp]
+        lib.hco_star_point.argtypes = [C.POINTER(Setup), C.POINTER(Rng), C.c_int, _dp, _dp, _dp]
+        lib.hco_star_point.restype = C.c_int
         lib.hco_mulmod.argtypes = [C.c_int32] * 3
         lib.hco_mulmod.restype = C.c_int32
         lib.hco_math.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int64]
