@@ -19,7 +19,8 @@ from __future__ import annotations
 from dataclasses import dataclass, field
 from typing import Callable, Dict, List, Optional, Sequence
 
-DEFAULTS = {"regen_lanes": 16, "regen_run_lanes": 16, "blocks_per_cu": 0, "peel_queue": 0}
+DEFAULTS = {"regen_lanes": 16, "regen_run_lanes": 16, "blocks_per_cu": 0, "peel_queue": 0,
+            "peel_waves": 0}
 
 LUCY_SPACE: Dict[str, Sequence[int]] = {
     "regen_lanes": (8, 12, 16, 24, 32),
@@ -27,6 +28,7 @@ LUCY_SPACE: Dict[str, Sequence[int]] = {
     "blocks_per_cu": (1, 0),
 }
 PEEL_SPACE: Dict[str, Sequence[int]] = {
+    "peel_waves": (8, 4),
     "peel_queue": (65, 80, 96, 128),
     "regen_lanes": (8, 16, 32),
 }
@@ -92,9 +94,14 @@ def tune_transport(tr, packets: int, seg0: int = 1 << 40, ipeel: int = 0,
     then leave the best options set on it."""
     space = space or (PEEL_SPACE if ipeel else LUCY_SPACE)
 
+    from ._lib import HcError
+
     def time_fn(opts):
         apply(tr, opts)
-        _, ms = tr.run_scalars(packets, ipeel=ipeel, seg0=seg0)
+        try:
+            _, ms = tr.run_scalars(packets, ipeel=ipeel, seg0=seg0)
+        except HcError:         # a launch shape this grid's LDS cannot hold
+            ms = float("inf")
         if log:
             log(f"autotune {opts}: {ms:.1f} ms")
         return ms

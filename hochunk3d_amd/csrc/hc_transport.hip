@@ -496,22 +496,35 @@ struct PeelOut {
 // What a peel-off ray needs of the packet at its emission or scattering
 // event.  Records wait in a per-wavefront queue in LDS; the wavefront's lanes
 // walk their (record, observer) rays (flush_peels).
+template <class R>
+struct Normal4 {  // the stellar surface normal at a launch point
+  R cosb, sinb, coslp, sinlp;
+};
+
+// A direct stellar record's Stokes vector is (1,0,0,0): every stellar launch
+// and re-emission is unpolarised with I = 1.  Its Stokes slots hold the
+// surface normal instead, which keeps the record at 168 B (FP64).
 template <class R, int NA>
 struct PeelRec {
   double x, y, z, rsq;
   double kapd[NA];
   Dir<R> d;
-  Stokes4<R> s;
+  union {
+    Stokes4<R> s;    // scattered or re-emitted by dust, accretion
+    Normal4<R> n;    // direct stellar emission
+  } u;
   R nub;
-  R cosb, sinb, coslp, sinlp;  // direct stellar emission: surface normal
   int ir, it, ip;
   int flags;                   // on_wall | iflag<<1 | i_orig<<2 | slot<<5 | direct<<9
   long long pid;
 };
 
+static_assert(sizeof(PeelRec<double, 4>) == 168, "FP64 peel record layout");
+static_assert(sizeof(PeelRec<float, 4>) == 128, "FP32 peel record layout");
+
 template <class R, int NA>
 __device__ __forceinline__ void make_rec(PeelRec<R, NA> &r, const Pkt<R, NA> &k, int slot,
-                                         bool direct, long long pid) {
+                                         bool direct, long long pid, const R *nrm) {
   r.x = k.x;
   r.y = k.y;
   r.z = k.z;
@@ -519,7 +532,10 @@ __device__ __forceinline__ void make_rec(PeelRec<R, NA> &r, const Pkt<R, NA> &k,
 #pragma unroll
   for (int s = 0; s < NA; ++s) r.kapd[s] = k.kapd[s];
   r.d = k.d;
-  r.s = k.s;
+  if (direct)
+    r.u.n = Normal4<R>{nrm[0], nrm[1], nrm[2], nrm[3]};
+  else
+    r.u.s = k.s;
   r.nub = k.nub;
   r.ir = k.ir;
   r.it = k.it;
@@ -562,17 +578,18 @@ __device__ __forceinline__ int peel_begin(const WallView &W, const PeelRec<R, NA
   const PeelRec<R, NA> &k = q[ri];
   S.r = -1;
   if ((k.flags >> 9) & 1) {  // direct stellar emission (newdisktherm.F90:342-377)
+    const Normal4<R> &nm = k.u.n;
     if (!P.ips) {
       const R cosnorm =
-          k.cosb * R(P.coste[obs]) +
-          (k.sinb * R(P.sinte[obs]) * (R(P.cospe[obs]) * k.coslp + R(P.sinpe[obs]) * k.sinlp));
+          nm.cosb * R(P.coste[obs]) +
+          (nm.sinb * R(P.sinte[obs]) * (R(P.cospe[obs]) * nm.coslp + R(P.sinpe[obs]) * nm.sinlp));
       if (!(cosnorm > R(0.0))) return kPeelIdle;
-      S.w = (double)(R(4.0) * k.s.I * cosnorm);
+      S.w = (double)(R(4.0) * R(1.0) * cosnorm);  // I = 1
     } else {
-      S.w = (double)(k.s.I * R(2.0));
+      S.w = (double)(R(1.0) * R(2.0));
     }
   } else {
-    S.w = (double)k.s.I;
+    S.w = (double)k.u.s.I;
   }
   S.r = r;
   S.rec = ri;
@@ -643,7 +660,9 @@ template <class R, int NA>
 __device__ __forceinline__ void peel_tally(const PeelRec<R, NA> &k, const PeelWalk &S) {
   const Params &P = c_P;
   const int peelid = S.obs;
-  Stokes4<R> s{R(S.w), k.s.Q, k.s.U, k.s.V};
+  const bool direct = ((k.flags >> 9) & 1) != 0;  // Q = U = V = 0
+  Stokes4<R> s{R(S.w), direct ? R(0.0) : k.u.s.Q, direct ? R(0.0) : k.u.s.U,
+               direct ? R(0.0) : k.u.s.V};
   const int iflag = (k.flags >> 1) & 1, i_orig = (k.flags >> 2) & 7, slot = (k.flags >> 5) & 15;
   const double cospe = P.cospe[peelid], sinpe = P.sinpe[peelid];
   const double coste = P.coste[peelid], sinte = P.sinte[peelid];
@@ -1374,7 +1393,7 @@ __device__ __forceinline__ void flush_peels(const WallView &W, const PeelRec<R, 
 #ifndef HC_BLOCK_LUCY
 #define HC_BLOCK_LUCY 256  // lanes per workgroup of the passes without peel-off
 #endif
-constexpr int kBlockPeel = 256;  // four wavefront peel queues per workgroup
+constexpr int kBlockPeel = 512;  // at most eight wavefront peel queues per workgroup
 constexpr int kBlockLucy = HC_BLOCK_LUCY;
 
 template <class R, int NA, bool TRACE, bool PEEL>
@@ -1439,11 +1458,7 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : kBlockLucy, HC_WAVES_PER_EU) vo
     if constexpr (PEEL) {
       const int qi = atomicAdd(&qcount[wave], 1);
       PeelRec<R, NA> &r = queue[qi];
-      make_rec(r, k, sp, direct, pid);
-      r.cosb = nrm[0];
-      r.sinb = nrm[1];
-      r.coslp = nrm[2];
-      r.sinlp = nrm[3];
+      make_rec(r, k, sp, direct, pid, nrm);
     }
   };
   Acc acc{};
@@ -1794,6 +1809,7 @@ struct hc_ctx {
   int regen_run_lanes = 16;
   int blocks_per_cu = 0;
   int peel_queue = 0;  // records per wavefront peel queue, 0 = 96 or what fits
+  int peel_waves = 0;  // wavefront queues per peel workgroup: 4 or 8 (0 = 4)
   size_t lds_per_block = 64 * 1024;  // the device's limit (hc_create)
   uint32_t max_work = 1u << 24;
 };
@@ -2392,18 +2408,23 @@ int prepare_pass(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int32_t 
   P.peel_base = (int)(walls + opac + emis);
   size_t shm = walls + opac + emis;
   P.peel_q = 0;
+  int block = pk ? 256 : kBlockLucy;
   if (pk) {
-    // two workgroups per compute unit when the queues leave room for them
+    // one queue per wavefront, four wavefronts per workgroup; eight on
+    // request share the workgroup's copy of the walls and tables (two
+    // wavefronts per SIMD at one workgroup per compute unit)
     const size_t budget = ctx->lds_per_block;
     const size_t avail = budget > shm + 64 ? budget - shm - 64 : 0;
     const size_t want = ctx->peel_queue > 0 ? (size_t)ctx->peel_queue : 96;
-    const size_t q = std::min<size_t>(want, avail / (4 * rec_bytes));
+    const size_t nq = ctx->peel_waves == 8 ? 8 : 4;
+    const size_t q = std::min<size_t>(want, avail / (nq * rec_bytes));
     if (q < 65)
       return fail(ctx, HC_E_UNSUPPORTED,
                   "transport pass: the peel-off queues do not fit the LDS left by this grid's "
                   "wall arrays and dust tables");
     P.peel_q = (int)q;
-    shm += 64 + 4 * q * rec_bytes;
+    shm += 64 + nq * q * rec_bytes;
+    block = 64 * (int)nq;
   }
   if (shm > ctx->lds_per_block)
     return fail(ctx, HC_E_UNSUPPORTED,
@@ -2414,7 +2435,6 @@ int prepare_pass(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int32_t 
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
   int dev_cus = 0, bpm = 0;
   HCK(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-  const int block = pk ? kBlockPeel : kBlockLucy;
   HCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpm, kern, block, shm));
   const int per_cu = ctx->blocks_per_cu > 0 ? std::min(ctx->blocks_per_cu, std::max(bpm, 1))
                                             : std::max(bpm, 1);
@@ -2722,6 +2742,10 @@ int hc_set_option(hc_ctx *ctx, const char *name, int64_t value) {
   } else if (n == "max_work") {
     if (value < 1 || value > 0x7fffffff) return fail(ctx, HC_E_ARG, "max_work must be in 1..2^31-1");
     ctx->max_work = (uint32_t)value;
+  } else if (n == "peel_waves") {
+    if (value != 0 && value != 4 && value != 8)
+      return fail(ctx, HC_E_ARG, "peel_waves must be 0, 4 or 8");
+    ctx->peel_waves = (int)value;
   } else if (n == "peel_queue") {
     if (value != 0 && (value < 65 || value > 1024))
       return fail(ctx, HC_E_ARG, "peel_queue must be 0 or in 65..1024");

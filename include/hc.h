@@ -184,8 +184,12 @@ int hc_run_lucy(hc_ctx *ctx, int32_t niter, int64_t np, int32_t seed1, int32_t s
  *   "peel_queue"    0 or 65..1024: peel-off records per wavefront queue in
  *                   LDS (0 = 96, or fewer when the LDS left by the grid's
  *                   walls and dust tables holds fewer);
- *   "blocks_per_cu" 0..64: workgroups of 256 lanes per compute unit, 0 =
- *                   as many as fit (default 0);
+ *   "peel_waves"    0, 4 or 8: wavefronts (each with its peel queue) per
+ *                   workgroup of a peel-off pass, 0 = 4 (default 0); 8 needs
+ *                   eight queues of at least 65 records beside the walls
+ *                   and tables in the LDS, else the pass is refused;
+ *   "blocks_per_cu" 0..64: workgroups per compute unit, 0 = as many as fit
+ *                   (default 0);
  *   "max_work"      1..2^31-1: events + cell crossings after which a packet
  *                   is dropped and counted in n_warn (default 2^24 on the
  *                   FP64 path and 2^21 on the FP32 path, both far above any

@@ -75,6 +75,25 @@ def test_peeled_pass_matches_oracle(pair):
         rel_close(getattr(rg, k), rc[k], rtol=1e-9, atol=1e-12)
 
 
+@pytest.mark.parametrize("waves", [4, 8])
+def test_peeled_pass_either_workgroup_shape_matches_oracle(pair, waves):
+    """Peel-off workgroups of four or eight wavefront queues (peel_waves)
+    give the same packets; only the order of the tally sums differs."""
+    g, c = pair
+    n = 1500
+    g.set_option("peel_waves", waves)
+    try:
+        rg = g.run(n, ipeel=1, trace=True, seg0=777)
+    finally:
+        g.set_option("peel_waves", 0)
+    rc = c.run(n, ipeel=1, trace=True, seg0=777)
+    compare_traces(rg.trace, rc["trace"])
+    _check_tallies(rg, rc)
+    rel_close(rg.peel, rc["peel"], rtol=1e-9, atol=1e-12)
+    for k in ("img_i", "img_q", "img_u", "img_v"):
+        rel_close(getattr(rg, k), rc[k], rtol=1e-9, atol=1e-12)
+
+
 def test_temperature_update_matches_oracle(pair, model):
     g, c = pair
     n = 20000
