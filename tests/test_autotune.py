@@ -3,7 +3,7 @@ from hochunk3d_amd.autotune import DEFAULTS, LUCY_SPACE, coordinate_search
 
 
 def test_finds_the_minimum_of_a_separable_surface():
-    best = {"regen_lanes": 24, "regen_run_lanes": 8, "blocks_per_cu": 1}
+    best = {"regen_lanes": 12, "regen_run_lanes": 8, "blocks_per_cu": 1}
 
     def ms(o):
         return (100.0 + 3 * abs(o["regen_lanes"] - best["regen_lanes"])
