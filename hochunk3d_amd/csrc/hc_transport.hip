@@ -420,6 +420,10 @@ __device__ __forceinline__ int walk_step(const WallView &W, Pkt<R, NA> &k, doubl
   tsum = tsum + dtau;
   face_step(face, k.ir, k.it, k.ip);
   load_dens(k, dm);
+  // the packet leaves the cell: its tallies go out now, after the next
+  // cell's density load is issued.  Loads and atomics share one in-order
+  // counter, so a load issued after the atomics would also wait for them.
+  lucy_flush(la);
   k.x = k.x + dt * k.ux;
   k.y = k.y + dt * k.uy;
   k.z = k.z + dt * k.uz;
