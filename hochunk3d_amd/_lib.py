@@ -179,6 +179,7 @@ class Transport:
     """One GPU context holding a model's grid, dust and sources."""
 
     def __init__(self, model, device: int = 0, precision: int = HC_FP64, imcube=None):
+        self.precision = precision
         self.lib = load()
         m = model
         self.model = m

@@ -20,8 +20,12 @@ from __future__ import annotations
 from dataclasses import dataclass, field
 from typing import Callable, Dict, List, Optional, Sequence
 
-DEFAULTS = {"regen_lanes": 16, "regen_run_lanes": 16, "blocks_per_cu": 0, "peel_queue": 0,
+# the library's defaults (include/hc.h): regen 8/8 on the FP64 path, 12/8 on
+# the FP32 path, the fastest of 8/8, 12/8 and 16/16 on full passes of the
+# 200x199x200-cell model
+DEFAULTS = {"regen_lanes": 8, "regen_run_lanes": 8, "blocks_per_cu": 0, "peel_queue": 0,
             "peel_waves": 0}
+DEFAULTS_FP32 = dict(DEFAULTS, regen_lanes=12)
 
 # regen_lanes 24 and 32 can win on the short probes, whose time is mostly
 # the longest packet, and then run a full 2.4e7-packet pass of the
@@ -110,6 +114,7 @@ def tune_transport(tr, packets: int, seg0: int = 1 << 40, ipeel: int = 0,
             log(f"autotune {opts}: {ms:.1f} ms")
         return ms
 
-    res = coordinate_search(time_fn, space, rounds=rounds)
+    start = DEFAULTS_FP32 if getattr(tr, "precision", 0) == 1 else DEFAULTS
+    res = coordinate_search(time_fn, space, start=start, rounds=rounds)
     apply(tr, res.options)
     return res

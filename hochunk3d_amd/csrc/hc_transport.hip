@@ -1809,8 +1809,8 @@ struct hc_ctx {
   double kappav[kMaxSpecies] = {0};
   hc_sources src;
   // launch shape
-  int regen_lanes = 16;
-  int regen_run_lanes = 16;
+  int regen_lanes = 8;  // 12 on the FP32 path (hc_create)
+  int regen_run_lanes = 8;
   int blocks_per_cu = 0;
   int peel_queue = 0;  // records per wavefront peel queue, 0 = 96 or what fits
   int peel_waves = 0;  // wavefront queues per peel workgroup: 4 or 8 (0 = 4)
@@ -1915,7 +1915,10 @@ int hc_create(const hc_dims *dims, int device, hc_ctx **out) {
   // FP32 packets very rarely lose their walk to rounding and circle (one in
   // ~2e6 on the 64x64x32-cell model); 2^21 events + crossings, still ~7x the
   // longest packet of the benchmark models, bounds the time such a lane holds
-  if (dims->precision == HC_FP32) ctx->max_work = 1u << 21;
+  if (dims->precision == HC_FP32) {
+    ctx->max_work = 1u << 21;
+    ctx->regen_lanes = 12;
+  }
   ctx->device = device;
   ctx->ncellw = (size_t)dims->nrg * dims->ntg * dims->npg;
   for (int i = 0; i < kMaxSpecies; ++i) ctx->slot[i] = -1;

@@ -177,10 +177,10 @@ int hc_run_lucy(hc_ctx *ctx, int32_t niter, int64_t np, int32_t seed1, int32_t s
  * loop is serial.  Names:
  *   "regen_lanes"   1..64: lanes of a wavefront that must wait for one event
  *                   kind before the wavefront stops walking and runs the
- *                   events (default 16);
+ *                   events (default 8 on the FP64 path, 12 on the FP32 path);
  *   "regen_run_lanes" 1..64: once the walk has stopped, every event kind
  *                   with at least this many waiting lanes runs (capped at
- *                   regen_lanes; default 16);
+ *                   regen_lanes; default 8);
  *   "peel_queue"    0 or 65..1024: peel-off records per wavefront queue in
  *                   LDS (0 = 96, or fewer when the LDS left by the grid's
  *                   walls and dust tables holds fewer);
