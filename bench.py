@@ -182,8 +182,9 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 path")
     ap.add_argument("--peel-np", type=float, default=500_000,
                     help="packets of the final peeled pass timed after the Lucy steps (0: skip)")
-    ap.add_argument("--autotune-np", type=float, default=2_000_000,
-                    help="packets per autotuning probe pass (0: default launch shape)")
+    ap.add_argument("--autotune-np", type=float, default=-1,
+                    help="packets per autotuning probe pass (-1: the step's packet count; "
+                         "0: default launch shape)")
     ap.add_argument("--batch-models", type=int, default=16,
                     help="parameter-grid models of the batched section (0: skip)")
     ap.add_argument("--batch-scale", type=float, default=0.05,
@@ -240,13 +241,16 @@ def main():
             f"T changed in {nch} cell-species")
     t_start = tr.get_temperature()  # where the timed steps start
     tuned = {}
-    if args.autotune_np > 0:
+    # probes as large as a step: a shorter pass is mostly its longest packet,
+    # whose walk the launch shape hardly changes
+    anp = np_ if args.autotune_np < 0 else int(args.autotune_np)
+    if anp > 0:
         from hochunk3d_amd.autotune import tune_transport
-        r = tune_transport(tr, int(args.autotune_np))
+        r = tune_transport(tr, anp)
         tuned["fp64"] = {"options": r.options, "gain_vs_default": round(r.gain, 3),
-                         "probes": len(r.trials)}
+                         "probes": len(r.trials), "probe_packets": anp}
         log(f"autotune fp64: {r.options}, x{r.gain:.3f} vs default on {len(r.trials)} probes "
-            f"of {int(args.autotune_np)} packets")
+            f"of {anp} packets")
 
     wall64, kms64, tot64, sed64 = timed_steps(tr, args.steps, np_, seg, "fp64")
     t64_end = tr.get_temperature()
@@ -258,7 +262,7 @@ def main():
     peel = None
     if args.peel_np > 0:
         pnp = int(args.peel_np)
-        if args.autotune_np > 0:
+        if anp > 0:
             r = tune_transport(tr, max(pnp // 5, 1000), ipeel=1)
             tuned["fp64_peeled"] = {"options": r.options, "gain_vs_default": round(r.gain, 3),
                                     "probes": len(r.trials)}
@@ -280,8 +284,8 @@ def main():
     if not args.no_fp32:
         model.tdust[...] = t_start
         tr32 = Transport(model, precision=HC_FP32)
-        if args.autotune_np > 0:
-            r = tune_transport(tr32, int(args.autotune_np))
+        if anp > 0:
+            r = tune_transport(tr32, anp)
             tuned["fp32"] = {"options": r.options, "gain_vs_default": round(r.gain, 3),
                              "probes": len(r.trials)}
             log(f"autotune fp32: {r.options}, x{r.gain:.3f} vs default")
