@@ -1630,8 +1630,18 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : kBlockLucy, HC_WAVES_PER_EU) vo
             o.pad = 0;
           }
         }
-        // take the next packet and launch it
-        const unsigned long long p = atomicAdd(P.next_packet, 1ull);
+        // take the next packet and launch it: the lanes of the wavefront that
+        // need one share one atomic on the packet counter (a block of packet
+        // numbers), each lane taking its rank among them
+        const unsigned long long need = __ballot(1);
+        const int leader = __ffsll((long long)need) - 1;
+        const int lane = (int)__lane_id();
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(P.next_packet, (unsigned long long)__popcll(need));
+        const unsigned blo = __shfl((unsigned)(base & 0xffffffffull), leader);
+        const unsigned bhi = __shfl((unsigned)(base >> 32), leader);
+        const unsigned long long p = (((unsigned long long)bhi << 32) | blo) +
+                                     (unsigned long long)__popcll(need & ((1ull << lane) - 1ull));
         if (p >= (unsigned long long)P.np) {
           st = ST_DONE;
         } else {
