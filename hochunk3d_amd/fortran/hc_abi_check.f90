@@ -6,7 +6,8 @@ program hc_abi_check
   type(hc_dims) :: d
   type(hc_sources) :: s
   type(hc_tallies) :: t
+  type(hc_density_model) :: m
   print '(A,I0)', 'abi ', hc_abi_version()
-  print '(A,I0,1X,I0,1X,I0)', 'sizes ', storage_size(d) / 8, storage_size(s) / 8, &
-       storage_size(t) / 8
+  print '(A,I0,1X,I0,1X,I0,1X,I0)', 'sizes ', storage_size(d) / 8, storage_size(s) / 8, &
+       storage_size(t) / 8, storage_size(m) / 8
 end program hc_abi_check

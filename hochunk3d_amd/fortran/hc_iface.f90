@@ -37,6 +37,19 @@ module hc_iface
     real(c_double) :: max_packet_work
   end type hc_tallies
 
+  ! the analytic densities of grid setup (hc_build_density)
+  type, bind(C) :: hc_density_model
+    integer(c_int32_t) :: nrg, ntg, npg, ndisk
+    type(c_ptr) :: rarr, thetarr, phiarr, fractal
+    real(c_double) :: rmincgs
+    integer(c_int32_t) :: disk_id(8)
+    real(c_double) :: disk_fmass(8), disk_z1(8), disk_b(8)
+    real(c_double) :: a_exp, rmind, rmaxd, massd
+    real(c_double) :: rc, rhoe0, rmine, rmax, rhoamb
+    integer(c_int32_t) :: env_id, cav_id
+    real(c_double) :: c1e, cavity_exp, rho_cavity, r1000
+  end type hc_density_model
+
   interface
     integer(c_int) function hc_create(dims, device, ctx) bind(C, name='hc_create')
       import :: c_int, c_ptr, hc_dims
@@ -136,5 +149,41 @@ module hc_iface
     integer(c_int) function hc_abi_version() bind(C, name='hc_abi_version')
       import :: c_int
     end function hc_abi_version
+
+    ! grid setup on the GPU (gridset.f90:1002-1083): densities, (rho V)^(1/4)
+    ! and cell volumes in grid_mod's layout
+    integer(c_int) function hc_build_density(model, device, densarr, massarr, vcell) &
+                                             bind(C, name='hc_build_density')
+      import :: c_int, c_double, hc_density_model
+      type(hc_density_model), intent(in) :: model
+      integer(c_int), value :: device
+      real(c_double), intent(out) :: densarr(*), massarr(*), vcell(*)
+    end function hc_build_density
+
+    ! setemiss for one species (dust.f90:394-474): kapBint(nT,nnu), kappap(nT)
+    integer(c_int) function hc_build_emissivity(device, nl, nud, kd, ad, nT, lTint, nnu, &
+                                                nuint, nq, xq, wq, kapBint, kappap) &
+                                                bind(C, name='hc_build_emissivity')
+      import :: c_int, c_int32_t, c_double
+      integer(c_int), value :: device
+      integer(c_int32_t), value :: nl, nT, nnu, nq
+      real(c_double), intent(in) :: nud(*), kd(*), ad(*), lTint(*), nuint(*), xq(*), wq(*)
+      real(c_double), intent(out) :: kapBint(*), kappap(*)
+    end function hc_build_emissivity
+
+    ! clump points per cell of the clumpy-envelope multiplier; pcg holds the
+    ! bit patterns of the PCG64 state and increment (lo, hi, lo, hi) and
+    ! counts the bit patterns of unsigned 32-bit counts
+    integer(c_int) function hc_fractal_counts(device, nrg, ntg, npg, rarr, mcos_theta, &
+                                              cos_phi, sin_phi, n, levels, scale, scale_r, &
+                                              pcg, counts) bind(C, name='hc_fractal_counts')
+      import :: c_int, c_int32_t, c_int64_t, c_double
+      integer(c_int), value :: device
+      integer(c_int32_t), value :: nrg, ntg, npg, n, levels
+      real(c_double), intent(in) :: rarr(*), mcos_theta(*), cos_phi(*), sin_phi(*), scale(*)
+      real(c_double), value :: scale_r
+      integer(c_int64_t), intent(in) :: pcg(4)
+      integer(c_int32_t), intent(out) :: counts(*)
+    end function hc_fractal_counts
   end interface
 end module hc_iface

@@ -53,6 +53,6 @@ def test_fortran_binding_matches_c_layout(tmp_path):
     assert r.returncode == 0, r.stderr
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60).stdout.split()
     assert out[out.index("abi") + 1] == "1"
-    sizes = [int(x) for x in out[out.index("sizes") + 1:out.index("sizes") + 4]]
+    sizes = [int(x) for x in out[out.index("sizes") + 1:out.index("sizes") + 5]]
     assert sizes == [ctypes.sizeof(_lib.HcDims), ctypes.sizeof(_lib.HcSources),
-                     ctypes.sizeof(_lib.HcTallies)]
+                     ctypes.sizeof(_lib.HcTallies), ctypes.sizeof(_lib.HcDensityModel)]
