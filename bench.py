@@ -114,9 +114,13 @@ def timed_steps(tr, steps: int, np_: int, seg: int, label: str):
         kernel_ms += ms
         for key, v in s.items():
             tot[key] = tot.get(key, 0.0) + v
+        # the pass cannot end before its longest packet's serial walk
+        tot["longest_packet_work"] = max(tot.get("longest_packet_work", 0.0),
+                                         s["max_packet_work"])
         log(f"{label} step {k}: kernel {ms:.0f} ms ({np_ / (ms / 1e3):.3e} packets/s), "
             f"{(s['nscat_ev'] + s['nabs_ev']) / np_:.1f} events and {s['ncross'] / np_:.0f} "
-            f"crossings per packet, errors wall={s['n_err_nowall']:.0f} "
+            f"crossings per packet, longest packet {s['max_packet_work']:.0f} events + "
+            f"crossings, errors wall={s['n_err_nowall']:.0f} "
             f"dust={s['n_err_dust']:.0f} warn={s['n_warn']:.0f}")
     wall = time.perf_counter() - t_start
     return wall, kernel_ms, tot, sed0
@@ -364,6 +368,11 @@ def main():
         },
         "roofline": roof,
         "cpu_baseline": cpu,
+        "longest_packet_fp64": {"events_plus_crossings": int(tot64["longest_packet_work"]),
+                                "mean_per_packet": round((tot64["nscat_ev"] + tot64["nabs_ev"]
+                                                          + tot64["ncross"]) / npk, 1),
+                                "note": "the longest packet of the timed steps: a pass "
+                                        "cannot end before its serial walk does"},
         "fp32_path": fp32,
         "peeled_pass_fp64": peel,
         "batched_grid": batch,
