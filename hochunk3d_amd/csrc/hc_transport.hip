@@ -735,41 +735,69 @@ __device__ __forceinline__ R bbfreq(Mrg &rng, R teff) {  // random.f90:49-77
 }
 
 // lucydustfreq.f90:19-157 with its guard chain; j clamped to [1,nT-1].
+// Split in two: ldf_begin draws the packet's number and issues the table
+// loads, ldf_end consumes them, so that a caller can place independent
+// arithmetic between the two while the loads are in flight.
 template <class R>
-__device__ __forceinline__ R lucydustfreq(double T0d, int id, Mrg &rng) {
+struct LdfState {
+  R T0, xi, f;
+  const double *row;
+  double a0, a1, e0, e1;
+  int g0a, g1a, g0b, g1b;
+};
+
+template <class R>
+__device__ __forceinline__ LdfState<R> ldf_begin(double T0d, int id, Mrg &rng) {
   const Params &P = c_P;
   const int nT = P.nT, nnu = P.nnu;
-  // lTint and nuint from LDS (staged by the transport kernel)
+  // lTint from LDS (staged by the transport kernel)
   const double *lT = reinterpret_cast<const double *>(hc_lds + P.emis_base);
-  const double *nuint = lT + nT;
   const double *kb = P.kapBint + (size_t)nT * nnu * id;
-  const R T0 = R(T0d);
-  const R lT0 = m_log(T0);
-  const R xi = R(ran(rng));
+  LdfState<R> L;
+  L.T0 = R(T0d);
+  const R lT0 = m_log(L.T0);
+  L.xi = R(ran(rng));
   int j = (int)((R)(nT - 1) * (lT0 - R(lT[0])) / (R(lT[nT - 1]) - R(lT[0]))) + 1;
   j = clampi(j, 1, nT - 1);
   R f = (lT0 - R(lT[j - 1])) / (R(lT[j]) - R(lT[j - 1]));
   if (f < R(0.0)) f = R(0.0);
   if (f > R(1.0)) f = R(1.0);
-  const double *row = kb + (j - 1);  // KB(j, i) = row[(i-1)*nT], KB(j+1, i) = row[(i-1)*nT + 1]
+  L.f = f;
+  L.row = kb + (j - 1);  // KB(j, i) = row[(i-1)*nT], KB(j+1, i) = row[(i-1)*nT + 1]
+  // the end nodes and the guide entries do not depend on each other: their
+  // loads are issued together before any of them is needed
+  const double *pe = L.row + (size_t)nT * (size_t)(nnu - 1);
+  L.a0 = L.row[0];
+  L.a1 = L.row[1];
+  L.e0 = pe[0];
+  L.e1 = pe[1];
+  const int bq = clampi((int)(L.xi * R(kCdfGuide)), 0, kCdfGuide - 1);
+  L.g0a = L.g1a = L.g0b = L.g1b = 0;
+  if (P.cdfguide) {
+    const uint16_t *g0 = P.cdfguide + ((size_t)id * nT + (size_t)(j - 1)) * (kCdfGuide + 1);
+    const uint16_t *g1 = g0 + (kCdfGuide + 1);
+    L.g0a = g0[bq];
+    L.g1a = g1[bq];
+    L.g0b = g0[bq + 1];
+    L.g1b = g1[bq + 1];
+  }
+  return L;
+}
+
+template <class R>
+__device__ __forceinline__ R ldf_end(const LdfState<R> &L) {
+  const Params &P = c_P;
+  const int nT = P.nT, nnu = P.nnu;
+  const double *nuint = reinterpret_cast<const double *>(hc_lds + P.emis_base) + nT;
+  const R T0 = L.T0, xi = L.xi;
+  R f = L.f;
+  const double *row = L.row;
   auto KB = [&](int ii) {
     const double *p = row + (size_t)nT * (size_t)(ii - 1);
     return (R(1.0) - f) * R(p[0]) + f * R(p[1]);
   };
-  // the end nodes and the guide entries do not depend on each other: their
-  // loads are issued together before any of them is needed
-  const double *pe = row + (size_t)nT * (size_t)(nnu - 1);
-  const double a0 = row[0], a1 = row[1], e0 = pe[0], e1 = pe[1];
-  const int bq = clampi((int)(xi * R(kCdfGuide)), 0, kCdfGuide - 1);
-  int g0a = 0, g1a = 0, g0b = 0, g1b = 0;
-  if (P.cdfguide) {
-    const uint16_t *g0 = P.cdfguide + ((size_t)id * nT + (size_t)(j - 1)) * (kCdfGuide + 1);
-    const uint16_t *g1 = g0 + (kCdfGuide + 1);
-    g0a = g0[bq];
-    g1a = g1[bq];
-    g0b = g0[bq + 1];
-    g1b = g1[bq + 1];
-  }
+  const double a0 = L.a0, a1 = L.a1, e0 = L.e0, e1 = L.e1;
+  const int g0a = L.g0a, g1a = L.g1a, g0b = L.g0b, g1b = L.g1b;
   int i0 = 1, i1 = nnu;
   R P0 = (R(1.0) - f) * R(a0) + f * R(a1);
   if (xi < P0) return (P0 > R(0.0)) ? T0 * xi * R(nuint[0]) / P0 : T0 * R(nuint[0]);
@@ -829,6 +857,11 @@ __device__ __forceinline__ R lucydustfreq(double T0d, int id, Mrg &rng) {
   else
     f = m_log(xi / P0) / m_log(P1 / P0);
   return T0 * R(nuint[i0 - 1]) * m_pow(R(nuint[i1 - 1]) / R(nuint[i0 - 1]), f);
+}
+
+template <class R>
+__device__ __forceinline__ R lucydustfreq(double T0d, int id, Mrg &rng) {
+  return ldf_end<R>(ldf_begin<R>(T0d, id, rng));
 }
 
 // opacset for all active species: kapd = kappa * R*(cm) * kappa_V and the
@@ -1573,12 +1606,17 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : kBlockLucy, HC_WAVES_PER_EU) vo
           if (s2 == slot) L.la.nabs[s2] += 1.0;
         k.i_orig = (idust2 == 0 || idust2 == 1 || idust2 == 4 || idust2 == 5) ? 2 : 3;
         k.s = Stokes4<R>{R(1.0), R(0.0), R(0.0), R(0.0)};
-        k.d.cost = R(1.0) - R(2.0) * R(ran(L.rng));
+        // the draws in their order (direction, direction, frequency); the
+        // frequency's table loads fly while the direction is computed
+        const R u1 = R(ran(L.rng));
+        const R u2 = R(ran(L.rng));
+        const LdfState<R> ldf = ldf_begin<R>(L.tdust, idust2, L.rng);
+        k.d.cost = R(1.0) - R(2.0) * u1;
         k.d.sint = m_sqrt(R(1.0) - m_pow(k.d.cost, R(2.0)));
-        k.d.phi = R(kR2P) * R(ran(L.rng));
+        k.d.phi = R(kR2P) * u2;
         k.d.cosp = m_cos(k.d.phi);
         k.d.sinp = m_sin(k.d.phi);
-        k.nub = lucydustfreq<R>(L.tdust, idust2, L.rng);
+        k.nub = ldf_end<R>(ldf);
         set_walk_dir(k);
         k.iflag = 0;
         set_opacity(k, false);
