@@ -807,6 +807,9 @@ __device__ __forceinline__ R ldf_end(const LdfState<R> &L) {
     if (P1 < R(1.0) && xi < R(1.0)) return T0 * R(nuint[nnu - 1]) * (R(1.0) - P1) / (R(1.0) - xi);
     return T0 * R(nuint[nnu - 1]);
   }
+  // a short bracket's interior nodes pb+1 .. pb+4, read with its ends
+  int pb = 0;
+  R q1 = R(0.0), q2 = R(0.0), q3 = R(0.0), q4 = R(0.0);
   if (P.cdfguide) {
     // The interpolated CDF lies between rows j and j+1, so the node counts
     // of both rows at the probability bin of xi bound the bracket; the
@@ -814,20 +817,32 @@ __device__ __forceinline__ R ldf_end(const LdfState<R> &L) {
     // the unique one the full bisection finds.
     const int lo = max(1, min(g0a, g1a));
     const int hi = min(nnu, max(g0b, g1b) + 1);
+    const int a = lo > i0 ? lo : i0, b = hi < i1 ? hi : i1;
+    // every node the bisection can visit lies inside (a, b): when there are
+    // at most four, all loads are issued before any value is used
+    const R Pa = KB(a), Pb = KB(b);
+    if (b - a > 1 && b - a <= 5) {
+      pb = a;
+      q1 = KB(min(a + 1, b));
+      q2 = KB(min(a + 2, b));
+      q3 = KB(min(a + 3, b));
+      q4 = KB(min(a + 4, b));
+    }
     if (lo > i0) {
       i0 = lo;
-      P0 = KB(i0);
+      P0 = Pa;
       if (P0 > R(1.0)) P0 = R(1.0);
     }
     if (hi < i1) {
       i1 = hi;
-      P1 = KB(i1);
+      P1 = Pb;
       if (P1 > R(1.0)) P1 = R(1.0);
     }
   }
   while ((i1 - i0) > 1) {
     const int i = (i0 + i1) / 2;
-    R Pm = KB(i);
+    const int d = i - pb;
+    R Pm = pb > 0 ? (d == 1 ? q1 : d == 2 ? q2 : d == 3 ? q3 : q4) : KB(i);
     if (Pm > R(1.0)) Pm = R(1.0);
     if (Pm < P0) {
       P0 = Pm;
