@@ -257,3 +257,34 @@ def test_run_outputs_of_an_oracle_pass(tmp_path):
     shape = tuple(np.frombuffer(rec[0], "<i4"))
     assert shape == (m.cfg.npeel, m.cfg.nximg, m.cfg.nximg, m.cfg.nfreq, NO)
     assert len(rec) == 5 and len(rec[1]) == 4 * int(np.prod(shape))
+
+
+def test_image_cube_normalisation_scalings(model):
+    """flux.f90:37,105-119: the cube's I is counts per unit frequency per
+    pixel solid angle: with the same counts, halving the pixel size (twice
+    nxhst) quadruples it, and bins with equal counts scale inversely with
+    their frequency width; Q, U, V become fractions of I."""
+    rng = np.random.default_rng(11)
+    nfreq = 20
+    sed = _sed(nfreq, 3, 2, 2)
+
+    def cube(nx):
+        ci = np.zeros((2, nx, nx, nfreq, NO))
+        ci[0, 1, 1, :, 0] = 8.0
+        cq = np.zeros((2, nx, nx, nfreq))
+        cq[0, 1, 1, :] = 2.0
+        cu = rng.uniform(-1, 1, size=cq.shape) * (ci[..., 0] > 0)
+        return ci, cq, cu, np.zeros_like(cq)
+
+    a = diskflux(model, 1000, sed, cube=cube(9))
+    b = diskflux(model, 1000, sed, cube=cube(18))
+    ia, ib = a.image_m_i[0, 1, 1, :, 0].astype(np.float64), b.image_m_i[0, 1, 1, :, 0]
+    np.testing.assert_allclose(ib / ia, 4.0, rtol=1e-6)
+    # equal counts in every bin: I times the bin's frequency width is constant
+    nu = 2.99792458e10 * 1e4 * (1.0 / 1.2398) * (1.2398 / 5000.0) * (5000.0 / 0.01) ** (
+        (np.arange(1, nfreq + 2) - 0.5) / nfreq)
+    width = np.diff(nu)
+    prod = ia * width
+    np.testing.assert_allclose(prod / prod[0], 1.0, rtol=1e-5)
+    np.testing.assert_allclose(a.image_m_q[0, 1, 1, :], 0.25, rtol=1e-7)
+    assert np.all(a.image_m_q[0, 0, 0, :] == 0.0)      # empty pixels stay zero
