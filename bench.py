@@ -326,7 +326,7 @@ def main():
                 "per-unit figures of hochunk3d_amd/roofline.py, over the kernel time measured "
                 "with HIP events on the library's stream; traffic = HBM bytes of one step "
                 "measured with rocprofv3 FETCH_SIZE/WRITE_SIZE (scaled by a 256 MiB copy's "
-                "readings) on this workload, 6.1x the algorithmic density bytes "
+                "readings) on this workload, 6.4x the algorithmic density bytes "
                 f"({rf.B_CROSS_FP64} B per crossing)",
         "kernel_ms_per_step": round(kms64 / args.steps, 3),
     }
