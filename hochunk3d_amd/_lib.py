@@ -27,8 +27,8 @@ EXPORTS = (
     "hc_create", "hc_set_grid", "hc_set_temperature", "hc_get_temperature", "hc_set_dust",
     "hc_set_sources",
     "hc_run_iteration", "hc_temperature", "hc_run_lucy", "hc_set_option", "hc_destroy",
-    "hc_last_error", "hc_build_density", "hc_build_emissivity", "hc_fractal_counts",
-    "hc_abi_version",
+    "hc_last_error", "hc_build_density", "hc_build_density_noalloc", "hc_build_emissivity",
+    "hc_fractal_counts", "hc_abi_version",
 )
 
 
@@ -125,6 +125,8 @@ def load() -> C.CDLL:
                                 C.POINTER(C.c_int64)]
     lib.hc_set_option.argtypes = [C.c_void_p, C.c_char_p, C.c_int64]
     lib.hc_build_density.argtypes = [C.POINTER(HcDensityModel), C.c_int, _dp, _dp, _dp]
+    lib.hc_build_density_noalloc.argtypes = [C.POINTER(HcDensityModel), C.c_int, C.c_void_p,
+                                             C.c_void_p, C.c_void_p, C.c_void_p]
     lib.hc_build_emissivity.argtypes = [C.c_int, C.c_int32, _dp, _dp, _dp, C.c_int32, _dp,
                                         C.c_int32, _dp, C.c_int32, _dp, _dp, _dp, _dp]
     lib.hc_fractal_counts.argtypes = [C.c_int, C.c_int32, C.c_int32, C.c_int32, _dp, _dp, _dp,
@@ -384,6 +386,41 @@ def build_density(fields: dict, nrg: int, ntg: int, npg: int, device: int = 0):
     if rc != 0:
         raise HcError(f"hc_build_density failed ({rc})")
     return dens, mass, vol
+
+
+def density_model_struct(fields: dict, arrays: Optional[dict] = None) -> HcDensityModel:
+    """An hc_density_model from `fields`; the array members (rarr, thetarr,
+    phiarr, fractal) are taken from `arrays` as raw addresses (ints) when
+    given, e.g. device addresses for hc_build_density_noalloc."""
+    m = HcDensityModel()
+    for k, v in fields.items():
+        if k in ("rarr", "thetarr", "phiarr", "fractal"):
+            continue
+        if k in ("disk_id", "disk_fmass", "disk_z1", "disk_b"):
+            arr = getattr(m, k)
+            for i, x in enumerate(v):
+                arr[i] = x
+        else:
+            setattr(m, k, v)
+    for k, addr in (arrays or {}).items():
+        if addr:
+            setattr(m, k, C.cast(C.c_void_p(int(addr)), _dp))
+    return m
+
+
+def build_density_noalloc(fields: dict, arrays: dict, densarr: int, massarr: int, vcell: int,
+                          device: int = 0, stream: int = 0) -> None:
+    """hc_build_density_noalloc: the densities of `fields` written into the
+    device arrays at addresses densarr, massarr, vcell; `arrays` maps rarr,
+    thetarr, phiarr (and optionally fractal) to device addresses.  Queued on
+    `stream` (0: the null stream); the caller synchronises."""
+    lib = load()
+    m = density_model_struct(fields, arrays)
+    rc = lib.hc_build_density_noalloc(C.byref(m), int(device), C.c_void_p(int(stream) or None),
+                                      C.c_void_p(int(densarr)), C.c_void_p(int(massarr)),
+                                      C.c_void_p(int(vcell)))
+    if rc != 0:
+        raise HcError(f"hc_build_density_noalloc failed ({rc})")
 
 
 def build_emissivity(nud, kd, ad, lTint, nuint, xq, wq, device: int = 0):

@@ -23,7 +23,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 namespace hc_grid {
@@ -501,16 +503,133 @@ extern "C" int hc_build_emissivity(int device, int32_t nl, const double *nud, co
   return rc;
 }
 
-extern "C" int hc_build_density(const hc_density_model *m, int device, double *densarr,
-                                double *massarr, double *vcell) {
-  using namespace hc_grid;
-  if (!m || !densarr || !massarr || !vcell) return HC_E_ARG;
+namespace hc_grid {
+
+// The kernel parameters of a density build whose arrays are device pointers
+// (model walls and multiplier in, densities, volumes out, mass sums scratch).
+static Dev density_params(const hc_density_model *m, const double *rarr, const double *thetarr,
+                          const double *phiarr, const double *mult, double *dens, double *vol,
+                          double *mass_sum) {
+  Dev D;
+  std::memset(&D, 0, sizeof D);
+  D.nrg = m->nrg;
+  D.ntg = m->ntg;
+  D.npg = m->npg;
+  D.ndisk = m->ndisk;
+  D.rarr = rarr;
+  D.thetarr = thetarr;
+  D.phiarr = phiarr;
+  D.mult = mult;
+  D.rmincgs = m->rmincgs;
+  for (int d = 0; d < m->ndisk; ++d) {
+    D.disk_id[d] = m->disk_id[d];
+    D.disk_z1[d] = m->disk_z1[d];
+    D.disk_b[d] = m->disk_b[d];
+    D.disk_norm[d] = m->disk_fmass[d] * m->massd * kMsol;
+  }
+  D.a_exp = m->a_exp;
+  D.rmind = m->rmind;
+  D.rmaxd = m->rmaxd;
+  D.rc = m->rc;
+  D.rhoe0 = m->rhoe0;
+  D.rmine = m->rmine;
+  D.rmax = m->rmax;
+  D.rhoamb = m->rhoamb;
+  D.env_id = m->env_id;
+  D.cav_id = m->cav_id;
+  D.c1e = m->c1e;
+  D.cavity_exp = m->cavity_exp;
+  D.rho_cavity = m->rho_cavity;
+  D.r1000 = m->r1000;
+  D.dens = dens;
+  D.vol = vol;
+  D.mass_sum = mass_sum;
+  return D;
+}
+
+static int check_density_model(const hc_density_model *m) {
+  if (!m) return HC_E_ARG;
   if (m->nrg < 2 || m->ntg < 2 || m->npg < 2 || m->ndisk < 0 || m->ndisk > 8 ||
       m->env_id < 0 || m->env_id > 7 || m->cav_id < 0 || m->cav_id > 7 || !m->rarr ||
       !m->thetarr || !m->phiarr)
     return HC_E_ARG;
   for (int d = 0; d < m->ndisk; ++d)
     if (m->disk_id[d] < 0 || m->disk_id[d] > 7) return HC_E_ARG;
+  return HC_OK;
+}
+
+__global__ void fill_zero(double *p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+// The disk-mass sums of hc_build_density_noalloc: one array per GPU that the
+// library holds until the process ends, so the call allocates nothing.  Host
+// code must not use this name as a device address (in host code it is a
+// stand-in); it uses the address hipGetSymbolAddress returns for the current
+// device instead (density_scratch()).
+constexpr int kDensityScratch = HC_DENSITY_SCRATCH;  // doubles: one mass sum per flared disk
+static __device__ __align__(256) double scratch[kDensityScratch];
+
+constexpr int kMaxDevices = 64;
+
+// The device address of `scratch` on `device` (which must be current),
+// looked up once per device; null if the lookup fails.
+static double *density_scratch(int device) {
+  static std::mutex mu;
+  static double *ptr[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[device]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+    ptr[device] = static_cast<double *>(p);
+  }
+  return ptr[device];
+}
+
+}  // namespace hc_grid
+
+extern "C" int hc_build_density_noalloc(const hc_density_model *m, int device, void *stream,
+                                        double *densarr, double *massarr, double *vcell) {
+  using namespace hc_grid;
+  if (!densarr || !massarr || !vcell) return HC_E_ARG;
+  if (m && (m->ndisk < 0 || m->ndisk > kDensityScratch)) {
+    std::fprintf(stderr,
+                 "hc_build_density_noalloc: %d disks exceed the %d mass sums of its device "
+                 "scratch array\n",
+                 m->ndisk, kDensityScratch);
+    return HC_E_ARG;
+  }
+  const int chk = check_density_model(m);
+  if (chk != HC_OK) return chk;
+  if (hipSetDevice(device) != hipSuccess) return HC_E_HIP;
+  double *sum = density_scratch(device);
+  if (!sum) return HC_E_HIP;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t wall = (size_t)m->nrg * m->ntg * m->npg;
+  const size_t nc = (size_t)(m->nrg - 1) * (m->ntg - 1) * (m->npg - 1);
+  const unsigned fill_blocks =
+      (unsigned)std::max<size_t>(1, std::min<size_t>((8 * wall + 255) / 256, 8192));
+  // outputs and mass sums to zero with kernels (no memset nodes when the
+  // caller captures the stream into a graph)
+  hipLaunchKernelGGL(fill_zero, dim3(fill_blocks), dim3(256), 0, s, densarr, 8 * wall);
+  hipLaunchKernelGGL(fill_zero, dim3(fill_blocks), dim3(256), 0, s, massarr, 8 * wall);
+  hipLaunchKernelGGL(fill_zero, dim3(fill_blocks), dim3(256), 0, s, vcell, wall);
+  hipLaunchKernelGGL(fill_zero, dim3(1), dim3(64), 0, s, sum, (size_t)kDensityScratch);
+  const Dev D = density_params(m, m->rarr, m->thetarr, m->phiarr, m->fractal, densarr, vcell, sum);
+  const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>((nc + 255) / 256, 8192));
+  hipLaunchKernelGGL(densities, dim3(blocks), dim3(256), 0, s, D);
+  hipLaunchKernelGGL(normalise, dim3(blocks), dim3(256), 0, s, D, massarr);
+  return hipGetLastError() == hipSuccess ? HC_OK : HC_E_HIP;
+}
+
+extern "C" int hc_build_density(const hc_density_model *m, int device, double *densarr,
+                                double *massarr, double *vcell) {
+  using namespace hc_grid;
+  if (!m || !densarr || !massarr || !vcell) return HC_E_ARG;
+  if (check_density_model(m) != HC_OK) return HC_E_ARG;
   const size_t wall = (size_t)m->nrg * m->ntg * m->npg;
   const size_t nc = (size_t)(m->nrg - 1) * (m->ntg - 1) * (m->npg - 1);
   if (hipSetDevice(device) != hipSuccess) return HC_E_HIP;
@@ -541,40 +660,7 @@ extern "C" int hc_build_density(const hc_density_model *m, int device, double *d
       ok(hipMemsetAsync(d_vol, 0, wall * sizeof(double), s)) &&
       ok(hipMemsetAsync(d_sum, 0, 8 * sizeof(double), s));
   if (rc == HC_OK) {
-    Dev D;
-    std::memset(&D, 0, sizeof D);
-    D.nrg = m->nrg;
-    D.ntg = m->ntg;
-    D.npg = m->npg;
-    D.ndisk = m->ndisk;
-    D.rarr = d_r;
-    D.thetarr = d_t;
-    D.phiarr = d_p;
-    D.mult = d_mult;
-    D.rmincgs = m->rmincgs;
-    for (int d = 0; d < m->ndisk; ++d) {
-      D.disk_id[d] = m->disk_id[d];
-      D.disk_z1[d] = m->disk_z1[d];
-      D.disk_b[d] = m->disk_b[d];
-      D.disk_norm[d] = m->disk_fmass[d] * m->massd * kMsol;
-    }
-    D.a_exp = m->a_exp;
-    D.rmind = m->rmind;
-    D.rmaxd = m->rmaxd;
-    D.rc = m->rc;
-    D.rhoe0 = m->rhoe0;
-    D.rmine = m->rmine;
-    D.rmax = m->rmax;
-    D.rhoamb = m->rhoamb;
-    D.env_id = m->env_id;
-    D.cav_id = m->cav_id;
-    D.c1e = m->c1e;
-    D.cavity_exp = m->cavity_exp;
-    D.rho_cavity = m->rho_cavity;
-    D.r1000 = m->r1000;
-    D.dens = d_dens;
-    D.vol = d_vol;
-    D.mass_sum = d_sum;
+    const Dev D = density_params(m, d_r, d_t, d_p, d_mult, d_dens, d_vol, d_sum);
     const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>((nc + 255) / 256, 8192));
     hipLaunchKernelGGL(densities, dim3(blocks), dim3(256), 0, s, D);
     ok(hipGetLastError());

@@ -271,19 +271,13 @@ def _densities_host(cfg, rarr, thetarr, phiarr, rmincgs, autors, rmind, rmine, r
     return densarr, vcell, massarr, fm8, z18, b8
 
 
-def _densities_device(cfg, rarr, thetarr, phiarr, rmincgs, autors, rmind, rmine, rmax, rmaxd,
-                      device):
-    """The same quantities from hc_build_density on GPU `device`, with the
-    clump point counts from hc_fractal_counts there too."""
-    from ._lib import build_density
+def _density_fields(cfg, rarr, thetarr, phiarr, rmincgs, autors, rmind, rmine, rmax, rmaxd,
+                    mult=None) -> dict:
+    """The hc_density_model members of the model (arrays as NumPy arrays)."""
     nrg, ntg, npg = len(rarr), len(thetarr), len(phiarr)
     disks = _disk_params(cfg, autors)
     rc, rhoe0, c1e = _envelope_params(cfg, rmincgs, autors, rmax)
-    mult = None
-    if cfg.fractal:
-        from .fractal import fractal_multiplier
-        mult = fractal_multiplier(rarr, thetarr, phiarr, **cfg.fractal, device=device)
-    fields = dict(
+    return dict(
         nrg=nrg, ntg=ntg, npg=npg, ndisk=len(disks), rarr=rarr, thetarr=thetarr, phiarr=phiarr,
         fractal=mult, rmincgs=rmincgs, disk_id=[d[0] for d in disks],
         disk_fmass=[d[1] for d in disks], disk_z1=[d[2] for d in disks],
@@ -291,6 +285,31 @@ def _densities_device(cfg, rarr, thetarr, phiarr, rmincgs, autors, rmind, rmine,
         massd=cfg.massd, rc=rc, rhoe0=rhoe0, rmine=rmine, rmax=rmax, rhoamb=cfg.rhoamb,
         env_id=2, cav_id=3, c1e=c1e, cavity_exp=cfg.cavity_exp, rho_cavity=cfg.rho_cavity,
         r1000=1000.0 * autors)
+
+
+def density_fields(model: "Model", mult=None) -> dict:
+    """The hc_density_model members that reproduce `model`'s densities
+    (arrays as NumPy arrays; `mult` an envelope multiplier of the cells)."""
+    cfg = model.cfg
+    rmincgs = cfg.rstar * RSOL
+    return _density_fields(cfg, model.rarr, model.thetarr, model.phiarr, rmincgs, model.autors,
+                           model.rddustmin, model.rddustmin, float(model.rarr[-1]),
+                           model.rmaxdmax, mult)
+
+
+def _densities_device(cfg, rarr, thetarr, phiarr, rmincgs, autors, rmind, rmine, rmax, rmaxd,
+                      device):
+    """The same quantities from hc_build_density on GPU `device`, with the
+    clump point counts from hc_fractal_counts there too."""
+    from ._lib import build_density
+    nrg, ntg, npg = len(rarr), len(thetarr), len(phiarr)
+    disks = _disk_params(cfg, autors)
+    mult = None
+    if cfg.fractal:
+        from .fractal import fractal_multiplier
+        mult = fractal_multiplier(rarr, thetarr, phiarr, **cfg.fractal, device=device)
+    fields = _density_fields(cfg, rarr, thetarr, phiarr, rmincgs, autors, rmind, rmine, rmax,
+                             rmaxd, mult)
     densarr, massarr, vcell = build_density(fields, nrg, ntg, npg, device)
     fm8, z18, b8 = [0.0] * NDG, [0.0] * NDG, [0.0] * NDG
     for id_, fm, z1, b in disks:

@@ -225,6 +225,21 @@ typedef struct {
 int hc_build_density(const hc_density_model *model, int device, double *densarr,
                      double *massarr, double *vcell);
 
+/* hc_build_density without allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  model's rarr,
+ * thetarr, phiarr and fractal, and densarr, massarr, vcell (same shapes and
+ * values as hc_build_density's) are DEVICE pointers of GPU `device`.  The
+ * kernels are queued on `stream` (a hipStream_t; NULL = the null stream) and
+ * the call returns without synchronising.  The disk-mass sums live in a
+ * device array of HC_DENSITY_SCRATCH doubles that the library holds on each
+ * GPU until the process ends (one sum per flared disk: ndisk above it is
+ * rejected with HC_E_ARG); every call on one GPU shares that array, so calls
+ * that overlap on one GPU race.  Calls no hipMalloc/hipFree. */
+#define HC_DENSITY_SCRATCH 8
+int hc_build_density_noalloc(const hc_density_model *model, int device, void *stream,
+                             double *densarr, double *massarr, double *vcell);
+
 /* setemiss for one species (dust.f90:394-474) on the GPU: kapBint (nT,nnu),
  * the cumulative emissivity kappa_abs B_nu over the frequency bins of nuint
  * (units of kT) normalised to 1 at the last node, and kappap[nT], its total.
