@@ -74,6 +74,18 @@ def cpu_baseline(model, seconds: float, threads: int, seg_base: int, np_total: i
     return sum(done) / wall, sum(done), wall
 
 
+def cpu_model_name() -> str:
+    """The host CPU's model name (lscpu's 'Model name')."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def sed_errors(s64: np.ndarray, s32: np.ndarray) -> dict:
     """SED differences of two passes over the same packets: per inclination
     bin (frequency-integrated, all photons, largest aperture) and the
@@ -100,16 +112,14 @@ def temperature_errors(model, t64: np.ndarray, t32: np.ndarray) -> dict:
 
 def timed_steps(tr, steps: int, np_: int, seg: int, label: str):
     """`steps` Lucy iterations; returns wall s, kernel ms, summed counters and
-    the SED of the first step."""
+    the SED summed over the steps."""
     kernel_ms = 0.0
     tot = {}
-    sed0 = None
+    sed_sum = None
     t_start = time.perf_counter()
     for k in range(steps):
-        if k == 0:
-            s, ms, sed0 = tr.run_sed(np_, seg0=seg + k * np_)
-        else:
-            s, ms = tr.run_scalars(np_, seg0=seg + k * np_)
+        s, ms, sed = tr.run_sed(np_, seg0=seg + k * np_)
+        sed_sum = sed if sed_sum is None else sed_sum + sed
         tr.temperature(np_, apply=True)
         kernel_ms += ms
         for key, v in s.items():
@@ -123,7 +133,7 @@ def timed_steps(tr, steps: int, np_: int, seg: int, label: str):
             f"crossings, errors wall={s['n_err_nowall']:.0f} "
             f"dust={s['n_err_dust']:.0f} warn={s['n_warn']:.0f}")
     wall = time.perf_counter() - t_start
-    return wall, kernel_ms, tot, sed0
+    return wall, kernel_ms, tot, sed_sum
 
 
 def batched_grid(args) -> dict:
@@ -195,7 +205,7 @@ def main():
                     help="fraction of each grid model's packet count run per Lucy iteration")
     ap.add_argument("--batch-iters", type=int, default=3)
     ap.add_argument("--batch-streams", type=int, default=4)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=30.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", type=float, default=0,
                     help="only copy this many bytes device-to-device once (a plain copy kernel "
@@ -301,7 +311,12 @@ def main():
         fp32 = {"value": round(npk / wall32, 2), "unit": "packets/s",
                 "speedup_vs_fp64": round(wall64 / wall32, 3),
                 "kernel_ms_per_step": round(kms32 / args.steps, 3),
-                "sed_error_vs_fp64_first_step": {k: round(v, 6) for k, v in err.items()},
+                "sed_error_vs_fp64_all_steps": {k: round(v, 6) for k, v in err.items()},
+                "dropped_packets": {"fp32": int(tot32["n_warn"]), "fp64": int(tot64["n_warn"]),
+                                    "note": "packets dropped at the per-packet work bound over "
+                                            "the timed steps (their energy is missing from "
+                                            "the tallies)"},
+                "longest_packet": int(tot32["longest_packet_work"]),
                 "temperature_error_vs_fp64_after_steps": {k: round(v, 6) for k, v in terr.items()},
                 "note": "same packets (L'Ecuyer-Cote segments) and starting temperatures as "
                         "the FP64 steps; SED error includes the Monte Carlo noise of histories "
@@ -313,35 +328,56 @@ def main():
     if args.batch_models > 0:
         batch = batched_grid(args)
 
+    # the binding roof of the Lucy pass: HBM (a crossing's density gather and
+    # the per-cell tally read-modify-writes) against FP64 VALU (its FLOPs)
+    kernel_s = kms64 / 1e3
+    nbytes = rf.pass_bytes(tot64)
     flops = rf.pass_flops(tot64, npk)
-    achieved = flops / (kms64 / 1e3) / 1e12
+    hbm_gbs = nbytes / kernel_s / 1e9
+    valu_tfs = flops / kernel_s / 1e12
+    hbm_frac = hbm_gbs / rf.PEAK_HBM_GBS
+    valu_frac = valu_tfs / rf.PEAK_FP64_TFLOPS
+    profiled = args.config == "config2" and np_ == rf.CONFIG2_STEP_PACKETS
     roof = {
-        "bound": "mfma", "achieved": round(achieved, 4), "peak": rf.PEAK_FP64_TFLOPS,
-        "unit": "TFLOP/s", "frac": round(achieved / rf.PEAK_FP64_TFLOPS, 5),
-        "traffic": (rf.CONFIG2_STEP_HBM_READ + rf.CONFIG2_STEP_HBM_WRITE
-                    if args.config == "config2" and np_ == rf.CONFIG2_STEP_PACKETS else None),
+        "bound": "hbm", "achieved": round(hbm_gbs, 2),
+        "peak": rf.PEAK_HBM_GBS, "unit": "GB/s", "frac": round(hbm_frac, 5),
+        "traffic": (rf.CONFIG2_STEP_HBM_READ + rf.CONFIG2_STEP_HBM_WRITE if profiled else None),
+        "traffic_source": (rf.CONFIG2_TRAFFIC_PROFILE if profiled else None),
         "kernel": "hc::transport<double, 4, false, false>",
-        "note": "FP64 vector ALU work (no MFMA instructions; MI355X's FP64 vector and matrix "
-                "peaks are both 78.6 TFLOP/s); algorithmic FLOPs = work counters of the pass x "
-                "per-unit figures of hochunk3d_amd/roofline.py, over the kernel time measured "
-                "with HIP events on the library's stream; traffic = HBM bytes of one step "
-                "measured with rocprofv3 FETCH_SIZE/WRITE_SIZE (scaled by a 256 MiB copy's "
-                "readings) on this workload, 6.4x the algorithmic density bytes "
-                f"({rf.B_CROSS_FP64} B per crossing)",
+        "algorithmic_bytes_per_step": round(nbytes / args.steps),
+        "per_crossing_bytes": round(nbytes / tot64["ncross"], 2),
+        "valu": {"achieved": round(valu_tfs, 4), "peak": rf.PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                 "frac": round(valu_frac, 5)},
+        "note": "HBM binds: a crossing moves the densities of its cell (4 x 8 B) and the "
+                "tally words its cell flush read-modify-writes (16 B per dtauabs/jmean/nabs/"
+                "dbig atomic, counted by the kernel), against 149 FP64 FLOPs "
+                "(hochunk3d_amd/roofline.py): 8 TB/s / ~70 B = 1.1e11 crossings/s against "
+                "78.6 TFLOP/s / 149 = 5.3e11.  achieved = those algorithmic bytes over the "
+                "kernel time measured with HIP events on the library's stream; traffic = HBM "
+                "bytes per step from the rocprofv3 FETCH_SIZE/WRITE_SIZE profile named in "
+                "traffic_source (a separate profiled run of this workload, scaled by a 256 MiB "
+                "copy's readings), null when the workload differs",
         "kernel_ms_per_step": round(kms64 / args.steps, 3),
     }
 
     cpu = None
     if args.cpu_seconds > 0:
         model.tdust[...] = t_start
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"cpu baseline: {threads} threads x {args.cpu_seconds:.0f}s")
+        avail = len(os.sched_getaffinity(0))
+        # the host cores this process may use, at most the 16-core share of
+        # one GPU on the GPU box
+        threads = args.cpu_threads or min(16, avail)
+        cpu_model = cpu_model_name()
+        log(f"cpu baseline: {threads} threads x {args.cpu_seconds:.0f}s on {cpu_model} "
+            f"({avail} cores available)")
         rate, n, w = cpu_baseline(model, args.cpu_seconds, threads, seg_base=seg + 10**10,
                                   np_total=np_)
         cpu = {"value": round(rate, 2), "unit": "packets/s", "cores": threads, "kind": "port",
+               "cpu": cpu_model, "cores_available": avail, "os_cpu_count": os.cpu_count(),
                "sample": f"{n} packets of the same model and starting temperatures in {w:.1f}s "
-                         f"wall, FP64 CPU restatement (oracle/, -O2, C library elementary "
-                         f"functions), one thread per core on disjoint packet segments"}
+                         f"wall, FP64 CPU restatement (oracle/, gcc -O3, C library elementary "
+                         f"functions), one independent seeded instance (thread with its own "
+                         f"packet segments and tallies) per core"}
 
     line = {
         "metric": "photon packets/s on 1 MI355X, FP32-batched vs FP64 plain, at <=1% SED error",

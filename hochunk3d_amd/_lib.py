@@ -26,7 +26,8 @@ _ip = C.POINTER(C.c_int32)
 EXPORTS = (
     "hc_create", "hc_set_grid", "hc_set_temperature", "hc_get_temperature", "hc_set_dust",
     "hc_set_sources",
-    "hc_run_iteration", "hc_temperature", "hc_run_lucy", "hc_set_option", "hc_destroy",
+    "hc_run_iteration", "hc_temperature", "hc_run_lucy", "hc_set_option",
+    "hc_dropped_packets", "hc_destroy",
     "hc_last_error", "hc_build_density", "hc_build_density_noalloc", "hc_build_emissivity",
     "hc_fractal_counts", "hc_abi_version",
 )
@@ -66,7 +67,7 @@ class HcTallies(C.Structure):
         ("n_err_nowall", C.c_int64), ("n_err_dust", C.c_int64), ("n_warn", C.c_int64),
         ("kernel_ms", C.c_double),
     ] + [(n, C.c_double) for n in ("ncross", "nscat_ev", "nabs_ev", "npeelray", "npeelcross",
-                                   "max_packet_work")]
+                                   "max_packet_work", "ntally")]
 
 
 class HcDensityModel(C.Structure):
@@ -91,7 +92,7 @@ TRACE_DTYPE = np.dtype([
 
 SCALARS = ("tot", "flux", "sflux", "nscat", "scount", "abflux", "aflux", "sumsub", "sumall",
            "hit", "htot", "n_err_nowall", "n_err_dust", "n_warn", "ncross", "nscat_ev",
-           "nabs_ev", "npeelray", "npeelcross", "max_packet_work")
+           "nabs_ev", "npeelray", "npeelcross", "max_packet_work", "ntally")
 
 
 def library_path() -> str:
@@ -124,6 +125,8 @@ def load() -> C.CDLL:
                                 C.c_int32, C.c_uint64, _dp, C.c_double, C.POINTER(HcTallies),
                                 C.POINTER(C.c_int64)]
     lib.hc_set_option.argtypes = [C.c_void_p, C.c_char_p, C.c_int64]
+    lib.hc_dropped_packets.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.c_int32,
+                                       C.POINTER(C.c_int64)]
     lib.hc_build_density.argtypes = [C.POINTER(HcDensityModel), C.c_int, _dp, _dp, _dp]
     lib.hc_build_density_noalloc.argtypes = [C.POINTER(HcDensityModel), C.c_int, C.c_void_p,
                                              C.c_void_p, C.c_void_p, C.c_void_p]
@@ -238,6 +241,15 @@ class Transport:
         """Launch-shape knob: regen_lanes, regen_run_lanes, peel_queue,
         peel_waves, blocks_per_cu or max_work (include/hc.h documents each)."""
         self._check(self.lib.hc_set_option(self.h, name.encode(), int(value)), "hc_set_option")
+
+    def dropped_packets(self):
+        """(count, packet numbers) of the last pass's packets dropped at
+        max_work (at most 16 numbers)."""
+        pids = (C.c_int64 * 16)()
+        n = C.c_int64(0)
+        self._check(self.lib.hc_dropped_packets(self.h, pids, 16, C.byref(n)),
+                    "hc_dropped_packets")
+        return int(n.value), [int(pids[i]) for i in range(min(int(n.value), 16))]
 
     def _check(self, rc, what):
         if rc != 0:

@@ -109,8 +109,15 @@ __device__ __forceinline__ const Params &kernel_params() {
 enum Scalar {
   S_TOT = 0, S_FLUX, S_SFLUX, S_NSCAT, S_SCOUNT, S_ABFLUX, S_AFLUX, S_SUMSUB, S_SUMALL,
   S_HIT, S_HTOT, S_ERR_WALL, S_ERR_DUST, S_WARN, S_NCROSS, S_NSCAT_EV, S_NABS_EV, S_NPEELRAY,
-  S_NPEELCROSS, S_COUNT
+  S_NPEELCROSS, S_NTALLY, S_COUNT
 };
+
+// Device counters of a pass (Params::next_packet): the next packet number,
+// the longest packet's work, the number of packets dropped at max_work and,
+// from kCtrDropped on, the first kDropMax of their packet numbers.
+constexpr int kCtrNext = 0, kCtrLongest = 1, kCtrNDropped = 2, kCtrDropped = 4;
+constexpr int kDropMax = 16;
+constexpr int kCounters = kCtrDropped + kDropMax;
 
 // ---------------------------------------------------------------------------
 // packet state (registers)
@@ -146,6 +153,7 @@ struct LucyAcc {
   double len;
   double nabs[NA];  // absorptions per species (nabs)
   int nabs_any;     // absorptions of any species (dbig)
+  uint32_t ntally;  // per-cell tally atomics issued by this lane (all its packets)
 };
 
 struct Trace {
@@ -264,15 +272,25 @@ __device__ __forceinline__ void lucy_flush(LucyAcc<R, NA> &la) {
     double *ta = P.dtauabs + (size_t)la.cell * (size_t)P.nact;
 #pragma unroll
     for (int s = 0; s < NA; ++s)
-      if (s < P.nact && la.tau[s] != 0.0) atomic_add(&ta[s], la.tau[s]);
+      if (s < P.nact && la.tau[s] != 0.0) {
+        atomic_add(&ta[s], la.tau[s]);
+        la.ntally++;
+      }
   }
-  if (P.jmean && la.len != 0.0) atomic_add(&P.jmean[la.cell], la.len * P.rmincgs);
+  if (P.jmean && la.len != 0.0) {
+    atomic_add(&P.jmean[la.cell], la.len * P.rmincgs);
+    la.ntally++;
+  }
   if (la.nabs_any) {
     double *na = P.nabs + (size_t)la.cell * (size_t)P.nact;
 #pragma unroll
     for (int s = 0; s < NA; ++s)
-      if (s < P.nact && la.nabs[s] != 0.0) atomic_add(&na[s], la.nabs[s]);
+      if (s < P.nact && la.nabs[s] != 0.0) {
+        atomic_add(&na[s], la.nabs[s]);
+        la.ntally++;
+      }
     atomicAdd(&P.dbig[la.cell], la.nabs_any);
+    la.ntally++;
   }
   la.cell = -1;
 }
@@ -1515,6 +1533,7 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : kBlockLucy, HC_WAVES_PER_EU) vo
   };
   Acc acc{};
   Lane<R, NA> L;
+  L.la.ntally = 0;
   Trace tr{0.0, 0.0, 0, 0, 0, 0, 0, 14695981039346656037ull};
   R nrm[4] = {R(0.0), R(0.0), R(0.0), R(0.0)};
   double tau = 0.0, tsum = 0.0;
@@ -1536,7 +1555,11 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : kBlockLucy, HC_WAVES_PER_EU) vo
       return (int)ST_END;
     }
     if (s == ST_ERROR || s == ST_ABORT) {
-      if (s == ST_ABORT) acc.warn++;
+      if (s == ST_ABORT) {
+        acc.warn++;
+        const unsigned long long i = atomicAdd(P.next_packet + kCtrNDropped, 1ull);
+        if (i < (unsigned long long)kDropMax) P.next_packet[kCtrDropped + i] = (unsigned long long)L.pid;
+      }
       status = 2;
       return (int)ST_END;
     }
@@ -1690,7 +1713,8 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : kBlockLucy, HC_WAVES_PER_EU) vo
         const int leader = __ffsll((long long)need) - 1;
         const int lane = (int)__lane_id();
         unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd(P.next_packet, (unsigned long long)__popcll(need));
+        if (lane == leader)
+          base = atomicAdd(P.next_packet + kCtrNext, (unsigned long long)__popcll(need));
         const unsigned blo = __shfl((unsigned)(base & 0xffffffffull), leader);
         const unsigned bhi = __shfl((unsigned)(base >> 32), leader);
         const unsigned long long p = (((unsigned long long)bhi << 32) | blo) +
@@ -1748,7 +1772,8 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : kBlockLucy, HC_WAVES_PER_EU) vo
                              (double)acc.hit,    (double)acc.htot,     (double)acc.err_wall,
                              (double)acc.err_dust, (double)acc.warn,
                              (double)acc.ncross, (double)acc.nscat_ev, (double)acc.nabs_ev,
-                             (double)acc.npeelray, (double)acc.npeelcross};
+                             (double)acc.npeelray, (double)acc.npeelcross,
+                             (double)L.la.ntally};
 #pragma unroll
   for (int i = 0; i < S_COUNT; ++i) {
     const double w = wave_sum(v[i]);
@@ -1756,7 +1781,7 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : kBlockLucy, HC_WAVES_PER_EU) vo
   }
   unsigned mw = acc.maxwork;
   for (int off = 32; off > 0; off >>= 1) mw = max(mw, (unsigned)__shfl_xor((int)mw, off, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(P.next_packet + 1, (unsigned long long)mw);
+  if ((threadIdx.x & 63) == 0) atomicMax(P.next_packet + kCtrLongest, (unsigned long long)mw);
 }
 
 // ---------------------------------------------------------------------------
@@ -2010,7 +2035,7 @@ int hc_create(const hc_dims *dims, int device, hc_ctx **out) {
   int rc = upload(ctx, &ctx->pow1, p1.data(), 64);
   if (!rc) rc = upload(ctx, &ctx->pow2, p2.data(), 64);
   if (!rc) rc = dalloc(ctx, &ctx->scal, (size_t)S_COUNT);
-  if (!rc) rc = dalloc(ctx, &ctx->counter, 2);  // next packet, longest packet
+  if (!rc) rc = dalloc(ctx, &ctx->counter, kCounters);
   double z[146];
   z[0] = 0.0;
   for (int i = 1; i <= 145; ++i) z[i] = 0.0;
@@ -2539,7 +2564,7 @@ int zero_tallies(hc_ctx *ctx, const PassLaunch &L, int64_t np, bool trace) {
   z(ctx->peel, L.npeelsed * sizeof(double));
   z(ctx->si_init, (size_t)d.nfreq * 5 * sizeof(double));
   z(ctx->scal, S_COUNT * sizeof(double));
-  z(ctx->counter, 2 * sizeof(unsigned long long));
+  z(ctx->counter, kCounters * sizeof(unsigned long long));
   if (trace) z(ctx->trace, (size_t)np * sizeof(hc_trace));
   if (L.cube) {
     z(ctx->img_i, L.nimg * kNo * sizeof(double));
@@ -2573,6 +2598,7 @@ int download_scalars(hc_ctx *ctx, hc_tallies *out) {
   out->nabs_ev = scal[S_NABS_EV];
   out->npeelray = scal[S_NPEELRAY];
   out->npeelcross = scal[S_NPEELCROSS];
+  out->ntally = scal[S_NTALLY];
   return HC_OK;
 }
 
@@ -2797,6 +2823,17 @@ int hc_run_lucy(hc_ctx *ctx, int32_t niter, int64_t np, int32_t seed1, int32_t s
   HC_D2H(counters, ctx->counter, sizeof counters);
   if (nchanged) *nchanged = (int64_t)counters[0];
   out->max_packet_work = (double)counters[1];
+  return HC_OK;
+}
+
+int hc_dropped_packets(hc_ctx *ctx, int64_t *pids, int32_t cap, int64_t *ndropped) {
+  if (!ctx || !ndropped || cap < 0 || (cap > 0 && !pids)) return HC_E_ARG;
+  if (!ctx->counter) return fail(ctx, HC_E_STATE, "hc_dropped_packets: no pass has run");
+  unsigned long long c[kCounters];
+  HC_D2H(c, ctx->counter, sizeof c);
+  *ndropped = (int64_t)c[kCtrNDropped];
+  const int n = (int)std::min<unsigned long long>(c[kCtrNDropped], (unsigned long long)kDropMax);
+  for (int i = 0; i < n && i < cap; ++i) pids[i] = (int64_t)c[kCtrDropped + i];
   return HC_OK;
 }
 

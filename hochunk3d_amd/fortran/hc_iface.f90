@@ -35,6 +35,7 @@ module hc_iface
     real(c_double) :: kernel_ms
     real(c_double) :: ncross, nscat_ev, nabs_ev, npeelray, npeelcross
     real(c_double) :: max_packet_work
+    real(c_double) :: ntally
   end type hc_tallies
 
   ! the analytic densities of grid setup (hc_build_density)
@@ -135,6 +136,15 @@ module hc_iface
       character(kind=c_char), intent(in) :: name(*)   ! NUL-terminated
       integer(c_int64_t), value :: value
     end function hc_set_option
+
+    integer(c_int) function hc_dropped_packets(ctx, pids, cap, ndropped) &
+        bind(C, name='hc_dropped_packets')
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), intent(out) :: pids(*)
+      integer(c_int32_t), value :: cap
+      integer(c_int64_t), intent(out) :: ndropped
+    end function hc_dropped_packets
 
     integer(c_int) function hc_destroy(ctx) bind(C, name='hc_destroy')
       import :: c_int, c_ptr

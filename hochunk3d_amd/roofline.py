@@ -31,8 +31,12 @@ envelope + cavity models):
                              weighting for scattered packets -> 250 (mean)
 
 Algorithmic HBM bytes: a transport crossing reads the masked densities of
-its cell, NA doubles (32 B at NA = 4, FP64) or floats (16 B, FP32); the walls
-and dust tables are small and cache resident.  Tally atomics go to L2.
+its cell, NA doubles (32 B at NA = 4, FP64) or floats (16 B, FP32); when the
+packet leaves a cell, the cell's Lucy tallies it accumulated (dtauabs of each
+species with opacity, jmean, and nabs/dbig after absorptions) are added with
+one atomic each, an 8 B read plus an 8 B write of the tally word (B_TALLY
+per atomic; the kernel counts them).  The walls and dust tables are small
+and cache resident.
 
 MI355X peaks (vector): FP64 78.6 TFLOP/s, FP32 157.3 TFLOP/s; HBM3E 8 TB/s.
 """
@@ -47,6 +51,7 @@ F_LAUNCH = 450
 F_PEEL_RAY = 250
 B_CROSS_FP64 = 32
 B_CROSS_FP32 = 16
+B_TALLY = 16
 
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
@@ -62,8 +67,11 @@ def pass_flops(s: dict, npackets: int) -> float:
 
 
 def pass_bytes(s: dict, fp32: bool = False) -> float:
+    """Algorithmic HBM bytes of one pass from its work counters: the density
+    gathers of transport and peel-ray crossings and the tally
+    read-modify-writes."""
     b = B_CROSS_FP32 if fp32 else B_CROSS_FP64
-    return b * (s["ncross"] + s["npeelcross"])
+    return b * (s["ncross"] + s["npeelcross"]) + B_TALLY * s.get("ntally", 0.0)
 
 
 # Measured HBM traffic of one plain FP64 Lucy pass of the 200x199x200-cell
@@ -74,5 +82,6 @@ def pass_bytes(s: dict, fp32: bool = False) -> float:
 # density bytes: the reads are 64-128 B lines for 32 B of density, the
 # writes evictions of the per-cell tally lines the atomics dirty.
 CONFIG2_STEP_PACKETS = 23_880_000
+CONFIG2_TRAFFIC_PROFILE = "profiles/transport_fp64_config2_pmc.md"
 CONFIG2_STEP_HBM_READ = 1.74109e12
 CONFIG2_STEP_HBM_WRITE = 1.26454e12

@@ -99,6 +99,9 @@ typedef struct {
   /* the longest packet of the pass: its events + transport-ray crossings
    * (the pass cannot end before that packet's serial walk does) */
   double max_packet_work;
+  /* per-cell tally atomics of the pass (dtauabs, jmean, nabs, dbig): each
+   * is one 8 B read-modify-write of a tally word */
+  double ntally;
 } hc_tallies;
 
 /* Per-packet record of the deterministic-trace mode. */
@@ -198,6 +201,13 @@ int hc_run_lucy(hc_ctx *ctx, int32_t niter, int64_t np, int32_t seed1, int32_t s
  * Apart from max_work, which drops the packets that exceed it, results do
  * not depend on them beyond the order of the floating-point tally sums. */
 int hc_set_option(hc_ctx *ctx, const char *name, int64_t value);
+
+/* Packets of the last pass dropped at max_work (counted in n_warn): their
+ * number in *ndropped and the first min(cap, 16) packet numbers (0-based,
+ * draw segment seg0 + pid) in pids, so that a caller can run them again
+ * alone, e.g. in the trace mode.  No counterpart in the reference, whose
+ * serial loop has no such bound. */
+int hc_dropped_packets(hc_ctx *ctx, int64_t *pids, int32_t cap, int64_t *ndropped);
 
 /* Analytic densities of the benchmark models, for grid construction on the
  * GPU: the density part of gridset (gridset.f90:1002-1083) with flared disks
