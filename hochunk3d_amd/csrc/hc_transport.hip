@@ -395,7 +395,7 @@ __device__ __forceinline__ int walk_step(const WallView &W, Pkt<R, NA> &k, doubl
     load_dens(k, dm);
   }
   const size_t cell = cell_of(k.ir, k.it, k.ip);
-  auto interact = [&](double ds) {
+  auto interact = [&](double ds) __attribute__((always_inline)) {
     lucy_add(la, k, dm, cell, ds);
     k.x = k.x + ds * k.ux;
     k.y = k.y + ds * k.uy;
@@ -810,7 +810,7 @@ __device__ __forceinline__ R ldf_end(const LdfState<R> &L) {
   const R T0 = L.T0, xi = L.xi;
   R f = L.f;
   const double *row = L.row;
-  auto KB = [&](int ii) {
+  auto KB = [&](int ii) __attribute__((always_inline)) {
     const double *p = row + (size_t)nT * (size_t)(ii - 1);
     return (R(1.0) - f) * R(p[0]) + f * R(p[1]);
   };
@@ -1140,10 +1140,19 @@ __device__ __forceinline__ int launch_acc(const WallView &W, Pkt<R, NA> &k, Mrg 
 
 // Star / hot-spot launch (initpspot.f90:3-175 or initp_ps.f90:3-79, then
 // newdisktherm.F90:256-377).
+// The launch point is FP64 on both paths (cos b, sin b and the longitude lp
+// from the FP64 draws, the position from FP64 sin/cos), as the positions of
+// the walk are: an FP32 longitude is quantised to 2^-24 of a turn, and the
+// native FP32 sine of a longitude that lands on a multiple of pi/2 is exactly
+// zero, which put such a packet on a phi wall and on the stellar surface at
+// once -- two walls that the walk's single on-wall flag cannot both skip, so
+// it stepped between the two phi cells without moving (about one packet in
+// 10^6).  The direction frame and the event arithmetic stay in R.
 template <class R, int NA>
 __device__ __forceinline__ void launch_star(Pkt<R, NA> &k, Mrg &rng, bool hotspot, R *nrm) {
   const Params &P = c_P;
   const R kpi = R(kPi), kr2p = R(kR2P);
+  double cosb_d, sinb_d, lp_d;
   R cosb, sinb, lp;
   int ispotph = hotspot ? 1 : 0;
   k.s = Stokes4<R>{R(1.0), R(0.0), R(0.0), R(0.0)};
@@ -1153,11 +1162,14 @@ __device__ __forceinline__ void launch_star(Pkt<R, NA> &k, Mrg &rng, bool hotspo
     // hc_set_sources' checks does not meet in practice
     for (uint32_t ntry = 1;; ++ntry) {
       for (;;) {
-        cosb = R(1.0) - R(2.0) * R(ran(rng));
-        sinb = m_sqrt(R(1.0) - cosb * cosb);
-        if (cosb != R(0.0) && fabs(cosb) < R(1.0)) break;
+        cosb_d = 1.0 - 2.0 * ran(rng);
+        sinb_d = sqrt(1.0 - cosb_d * cosb_d);
+        if (cosb_d != 0.0 && fabs(cosb_d) < 1.0) break;
       }
-      lp = kr2p * R(ran(rng));
+      lp_d = kR2P * ran(rng);
+      cosb = R(cosb_d);
+      sinb = R(sinb_d);
+      lp = R(lp_d);
       if (P.ispot == 1 && ispotph == 1) {
         const R cphi = m_cos(lp), sphi = m_sin(lp);
         const R d1 = R(P.scspot1) * sinb * cphi + R(P.ssspot1) * sinb * sphi + R(P.cspot1) * cosb;
@@ -1179,13 +1191,16 @@ __device__ __forceinline__ void launch_star(Pkt<R, NA> &k, Mrg &rng, bool hotspo
     initp_transform(k, cosb, sinb, lp);
   } else {
     for (;;) {
-      cosb = clamp1(R(1.0) - R(2.0) * R(ran(rng)));
-      sinb = clamp1(m_sqrt(R(1.0) - cosb * cosb));
-      if (cosb != R(0.0)) break;
+      cosb_d = clamp1(1.0 - 2.0 * ran(rng));
+      sinb_d = clamp1(sqrt(1.0 - cosb_d * cosb_d));
+      if (cosb_d != 0.0) break;
     }
+    cosb = R(cosb_d);
+    sinb = R(sinb_d);
     k.d.cost = cosb;
     k.d.sint = sinb;
-    lp = kr2p * R(ran(rng));
+    lp_d = kR2P * ran(rng);
+    lp = R(lp_d);
     k.d.phi = lp;
     k.d.sinp = clamp1(m_sin(k.d.phi));
     k.d.cosp = clamp1(m_cos(k.d.phi));
@@ -1200,37 +1215,22 @@ __device__ __forceinline__ void launch_star(Pkt<R, NA> &k, Mrg &rng, bool hotspo
   }
   k.i_orig = 1;
   set_opacity(k, true);
-  const R coslp = m_cos(lp), sinlp = m_sin(lp);
-  k.z = (double)cosb * 1.0;
-  const double rp = (double)sinb * 1.0;
-  k.x = rp * (double)coslp;
-  k.y = rp * (double)sinlp;
-  if constexpr (sizeof(R) == sizeof(float)) {
-    // FP32 angles put the point ~1e-7 off the stellar surface: back onto it
-    const double inv = 1.0 / sqrt(k.x * k.x + k.y * k.y + k.z * k.z);
-    k.x *= inv;
-    k.y *= inv;
-    k.z *= inv;
-    k.rsq = k.x * k.x + k.y * k.y + k.z * k.z;
-  } else {
-    k.rsq = k.z * k.z + rp * rp;
-  }
+  const double coslp = hcm_cos(lp_d), sinlp = hcm_sin(lp_d);
+  k.z = cosb_d * 1.0;
+  const double rp = sinb_d * 1.0;
+  k.x = rp * coslp;
+  k.y = rp * sinlp;
+  k.rsq = k.z * k.z + rp * rp;
   set_walk_dir(k);
   k.iflag = 0;
   k.ir = 1;
   k.on_wall = true;
-  if constexpr (sizeof(R) == sizeof(double)) {
-    k.it = locate(P.thetarr, P.ntg, (double)m_acos(cosb));
-    k.ip = locate(P.phiarr, P.npg, (double)lp);
-  } else {
-    // the FP64 position carries the FP32 angles' rounding; its cell is
-    // located from the position itself, so the walk starts consistent
-    locate_cell_angles(k);
-  }
+  k.it = locate(P.thetarr, P.ntg, hcm_acos(cosb_d));
+  k.ip = locate(P.phiarr, P.npg, lp_d);
   nrm[0] = cosb;
   nrm[1] = sinb;
-  nrm[2] = coslp;
-  nrm[3] = sinlp;
+  nrm[2] = R(coslp);
+  nrm[3] = R(sinlp);
 }
 
 // Star re-emission of a packet that reached the stellar surface.
@@ -1402,7 +1402,7 @@ __device__ __forceinline__ void flush_peels(const WallView &W, const PeelRec<R, 
   PeelWalk S;
   S.r = -1;
   int st = kPeelIdle;
-  auto ray_done = [&]() {
+  auto ray_done = [&]() __attribute__((always_inline)) {
     acc.npeelray++;
     acc.npeelcross += (uint32_t)S.ncross;
     acc.err_wall += (uint32_t)S.err;
@@ -1524,7 +1524,8 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : kBlockLucy, HC_WAVES_PER_EU) vo
     if ((threadIdx.x & 63) == 0) qcount[wave] = 0;
   }
   __syncthreads();
-  auto push = [&](const Pkt<R, NA> &k, int sp, bool direct, long long pid, const R *nrm) {
+  auto push = [&](const Pkt<R, NA> &k, int sp, bool direct, long long pid,
+                  const R *nrm) __attribute__((always_inline)) {
     if constexpr (PEEL) {
       const int qi = atomicAdd(&qcount[wave], 1);
       PeelRec<R, NA> &r = queue[qi];
@@ -1547,7 +1548,7 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : kBlockLucy, HC_WAVES_PER_EU) vo
   // After a walk step: an interaction draws its species and the
   // scatter/absorb test (select_dust.f90, propagate.f90:60-104); the packet
   // ends when it escaped, was lost, or interacted iintmax times in one cell.
-  auto after_step = [&](int s) {
+  auto after_step = [&](int s) __attribute__((always_inline)) {
     Pkt<R, NA> &k = L.k;
     if (s == ST_WALK || s == ST_STAR) return s;
     if (s == ST_EXIT) {
@@ -1591,7 +1592,7 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : kBlockLucy, HC_WAVES_PER_EU) vo
   };
   // the walk after an event: new optical depth, entry into the cell
   // reload: the event moved the packet to another cell (launch, star)
-  auto start_walk = [&](bool reload) {
+  auto start_walk = [&](bool reload) __attribute__((always_inline)) {
     Pkt<R, NA> &k = L.k;
     tau = draw_tau<R>(L.rng);
     L.xpold = k.x;
