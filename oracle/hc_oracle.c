@@ -1881,6 +1881,82 @@ int hco_run_pass(const hco_grid *g, const hco_dust *d, const hco_setup *s, hco_t
   return 0;
 }
 
+/* Leaf entries of tauint and peeloff for the closed-form tests: a packet
+ * state built by the caller instead of a launch. */
+static void leaf_ctx(ctx_t *c, const hco_grid *g, const hco_setup *s, hco_tallies *t) {
+  memset(c, 0, sizeof *c);
+  c->g = g;
+  c->s = s;
+  c->t = t;
+  c->rmincgs = s->rstar * RSOL;
+  c->tsub_ev = g->tsub / 11605.0;
+  c->ncell = (size_t)g->nrg * g->ntg * g->npg;
+}
+
+static void leaf_packet(pkt_t *P, const double *xyz, const double *u, const int *cell,
+                        int on_wall, const double *kapd, const double *albedo) {
+  int id;
+  memset(P, 0, sizeof *P);
+  P->xp = xyz[0];
+  P->yp = xyz[1];
+  P->zp = xyz[2];
+  P->ux = u[0];
+  P->uy = u[1];
+  P->uz = u[2];
+  P->rsq = P->xp * P->xp + P->yp * P->yp + P->zp * P->zp;
+  P->rtot = sqrt(P->rsq);
+  P->ir = cell[0];
+  P->it = cell[1];
+  P->ip = cell[2];
+  P->on_wall = on_wall;
+  for (id = 0; id < HCO_NDG; id++) {
+    P->kapd[id] = kapd[id];
+    P->op.albedo[id] = albedo[id];
+  }
+}
+
+int hco_tauint_ray(const hco_grid *g, const hco_setup *s, hco_tallies *t, const double *xyz,
+                   const double *u, int *cell, int on_wall, const double *kapd,
+                   const double *albedo, double tau, double *xyz_out) {
+  ctx_t c;
+  pkt_t P;
+  int st;
+  leaf_ctx(&c, g, s, t);
+  leaf_packet(&P, xyz, u, cell, on_wall, kapd, albedo);
+  st = tauint(&c, &P, tau);
+  xyz_out[0] = P.xp;
+  xyz_out[1] = P.yp;
+  xyz_out[2] = P.zp;
+  cell[0] = P.ir;
+  cell[1] = P.it;
+  cell[2] = P.ip;
+  return st;
+}
+
+void hco_peel_ray(const hco_grid *g, const hco_setup *s, hco_tallies *t, const double *xyz,
+                  const int *cell, int on_wall, const double *kapd, const double *stokes,
+                  const double *dir, double nub, int iflag, int i_orig, int peelid) {
+  ctx_t c;
+  pkt_t P;
+  const double u0[3] = {0.0, 0.0, 1.0};
+  static const double alb0[HCO_NDG] = {0};
+  int cl[3] = {cell[0], cell[1], cell[2]};
+  leaf_ctx(&c, g, s, t);
+  leaf_packet(&P, xyz, u0, cl, on_wall, kapd, alb0);
+  P.sqp = stokes[1];
+  P.sup = stokes[2];
+  P.svp = stokes[3];
+  P.cost = dir[0];
+  P.sint = dir[1];
+  P.phi = dir[2];
+  P.cosp = HCO_COS(dir[2]);
+  P.sinp = HCO_SIN(dir[2]);
+  P.nub = nub;
+  P.iflag = iflag;
+  P.i_orig = i_orig;
+  peeloff(&c, &P, stokes[0], 0, peelid);
+}
+
 /* Elementary-function probe for the accuracy tests: fn 0 log, 1 exp, 2 sin,
  * 3 cos, 4 acos, 5 pow(x,y), 6 atan2(x,y). */
 void hco_math(int fn, const double *x, const double *y, double *out, int64_t n) {

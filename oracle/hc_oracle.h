@@ -18,7 +18,22 @@
  *   stokes, stokespeel: closed forms and distributions fixed by the phase
  *   matrix (polarisation degree, Cornette-Shanks moments, azimuth modulation);
  *   the accretion launch: its sampled distributions against closed forms;
- *   tfinal: dusttemp's golden vectors;
+ *   tauint (reconstructed): the absorbed optical depth and mean-intensity
+ *   path of uniform shells equal kappa (1 - albedo) rho times the chords of
+ *   straight lines, the per-cell split equals brute-force point location,
+ *   the interaction lands at tau / opacity (hco_tauint_ray);
+ *   peeloff: the weight exp(-kappa rho s) over the exit distance s, the
+ *   sublimation mask, the star and tau > 25 cuts, the image pixel of the
+ *   sky projection (hco_peel_ray);
+ *   exit binning: uniform cos(theta) and phi bins, Planck frequency bins and
+ *   the a^2 aperture flux of direct stellar packets;
+ *   the direct stellar peel weight: the moments of 4 max(n.o, 0);
+ *   tfinal: dusttemp's golden vectors, and optically thin grey dust at the
+ *   temperature of the volume-mean dilution factor;
+ *   star re-emission: energy conservation and the stellar spectrum of a
+ *   purely scattering shell;
+ *   tests/test_oracle_plants.py plants a mistake in each of these and
+ *   requires the closed-form tests to fail;
  *   the star launch: the surface point uniform on the star or within the
  *   spot caps, the spot flag of stellar packets, and the emission direction
  *   (its cosine with the normal equals the sqrt of its draw, its local
@@ -178,6 +193,25 @@ int hco_star_point(const hco_setup *s, hco_rng *r, int hotspot, double *cosb, do
  * initp_common.f90:1-94 (limb = 0). */
 void hco_initp_direction(hco_rng *r, double cosb, double sinb, double lp, double *cost,
                          double *sint, double *phi);
+
+/* tauint (reconstructed, see hc_oracle.c) for a packet state given by the
+ * caller: from xyz in cell[3] = (ir,it,ip) (on a wall when on_wall) along
+ * the unit vector u to optical depth tau, with per-species opacities kapd
+ * (kappa R* kappa_V) and albedos; dtauabs and jmean of t accumulate.
+ * Returns 0 (interaction), 1 (escape), 2 (star) or 3 (no wall); the end
+ * point goes to xyz_out and its cell to cell. */
+int hco_tauint_ray(const hco_grid *g, const hco_setup *s, hco_tallies *t, const double *xyz,
+                   const double *u, int *cell, int on_wall, const double *kapd,
+                   const double *albedo, double tau, double *xyz_out);
+/* peeloff_3d.f90:5-288 towards observer peelid for a packet at xyz in cell
+ * (on a wall when on_wall) with Stokes vector stokes[4] (stokes[0] = the
+ * peel weight sip), direction dir = (cost, sint, phi) before the scattering
+ * when iflag = 1, frequency nub (eV) and origin i_orig; the peeled SED and
+ * image cube of t accumulate.  Species 1's phase matrix applies (opacset
+ * left at zero: isotropic, unpolarised scattering). */
+void hco_peel_ray(const hco_grid *g, const hco_setup *s, hco_tallies *t, const double *xyz,
+                  const int *cell, int on_wall, const double *kapd, const double *stokes,
+                  const double *dir, double nub, int iflag, int i_orig, int peelid);
 
 int hco_select_dust(const double *kapd, const double *dens, hco_rng *r);
 void hco_dustmat(const hco_opac *o, int id, double cost, double cost2, double *p1, double *p2,

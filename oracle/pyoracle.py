@@ -102,11 +102,13 @@ def build():
 
 def load(variant: str = "libm") -> C.CDLL:
     """variant "libm": elementary functions of the C library (as the
-    reference's own build); "dm": the deterministic set the GPU uses."""
+    reference's own build); "dm": the deterministic set the GPU uses; or the
+    path of another build of hc_oracle.c (a .so file)."""
     if variant not in _libs:
-        if not os.path.exists(LIBS[variant]):
+        path = variant if variant.endswith(".so") else LIBS[variant]
+        if not os.path.exists(path):
             build()
-        lib = C.CDLL(LIBS[variant])
+        lib = C.CDLL(path)
         lib.hco_run_pass.argtypes = [C.POINTER(Grid), C.POINTER(Dust), C.POINTER(Setup),
                                      C.POINTER(Tallies), C.c_int64, C.c_int64, C.c_int64,
                                      C.c_int, C.c_int32, C.c_int32, C.c_int, C.c_uint64,
@@ -148,6 +150,12 @@ def load(variant: str = "libm") -> C.CDLL:
                                             _dp, _dp, _dp]
         lib.hco_star_point.argtypes = [C.POINTER(Setup), C.POINTER(Rng), C.c_int, _dp, _dp, _dp]
         lib.hco_star_point.restype = C.c_int
+        lib.hco_tauint_ray.argtypes = [C.POINTER(Grid), C.POINTER(Setup), C.POINTER(Tallies),
+                                       _dp, _dp, _ip, C.c_int, _dp, _dp, C.c_double, _dp]
+        lib.hco_tauint_ray.restype = C.c_int
+        lib.hco_peel_ray.argtypes = [C.POINTER(Grid), C.POINTER(Setup), C.POINTER(Tallies), _dp,
+                                     _ip, C.c_int, _dp, _dp, _dp, C.c_double, C.c_int, C.c_int,
+                                     C.c_int]
         lib.hco_mulmod.argtypes = [C.c_int32] * 3
         lib.hco_mulmod.restype = C.c_int32
         lib.hco_math.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int64]
@@ -222,6 +230,53 @@ class Oracle:
                                   float(uy), float(uz), C.byref(cir), C.byref(cit),
                                   C.byref(cip), int(on_wall))
         return n, cir.value, cit.value, cip.value
+
+    def _leaf_tallies(self):
+        m = self.model
+        c = m.cfg
+        out = {
+            "dtauabs": np.zeros((m.nrg, m.ntg, m.npg, NDG), order="F"),
+            "jmean": np.zeros((m.nrg, m.ntg, m.npg), order="F"),
+            "peel": np.zeros((c.nfreq, max(c.npeel, 1), c.nap, NO, 9), order="F"),
+            "img_i": np.zeros((max(c.npeel, 1), c.nximg, c.nximg, c.nfreq, NO), order="F"),
+        }
+        for k in ("img_q", "img_u", "img_v"):
+            out[k] = np.zeros((max(c.npeel, 1), c.nximg, c.nximg, c.nfreq), order="F")
+        t = Tallies()
+        t.dtauabs, t.jmean, t.peel = _p(out["dtauabs"]), _p(out["jmean"]), _p(out["peel"])
+        t.img_i, t.img_q, t.img_u, t.img_v = (_p(out[k]) for k in ("img_i", "img_q", "img_u",
+                                                                     "img_v"))
+        out["_struct"] = t
+        return out
+
+    def tauint_ray(self, xyz, u, cell, tau, kapd, albedo, on_wall=False, tallies=None):
+        """hco_tauint_ray: (status, end point, end cell, tallies)."""
+        out = tallies if tallies is not None else self._leaf_tallies()
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        uu = np.ascontiguousarray(u, dtype=np.float64)
+        cl = np.ascontiguousarray(cell, dtype=np.int32)
+        kd = np.ascontiguousarray(kapd, dtype=np.float64)
+        al = np.ascontiguousarray(albedo, dtype=np.float64)
+        end = np.zeros(3)
+        st = self.lib.hco_tauint_ray(C.byref(self.grid), C.byref(self.setup),
+                                     C.byref(out["_struct"]), _p(x), _p(uu), _p(cl, C.c_int32),
+                                     int(on_wall), _p(kd), _p(al), float(tau), _p(end))
+        return st, end, tuple(int(v) for v in cl), out
+
+    def peel_ray(self, xyz, cell, kapd, stokes, peelid, nub, iflag=0, i_orig=1, direction=None,
+                 on_wall=False, tallies=None):
+        """hco_peel_ray towards observer peelid; returns the tallies."""
+        out = tallies if tallies is not None else self._leaf_tallies()
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        cl = np.ascontiguousarray(cell, dtype=np.int32)
+        kd = np.ascontiguousarray(kapd, dtype=np.float64)
+        sv = np.ascontiguousarray(stokes, dtype=np.float64)
+        d = np.ascontiguousarray(direction if direction is not None else (1.0, 0.0, 0.0),
+                                 dtype=np.float64)
+        self.lib.hco_peel_ray(C.byref(self.grid), C.byref(self.setup), C.byref(out["_struct"]),
+                              _p(x), _p(cl, C.c_int32), int(on_wall), _p(kd), _p(sv), _p(d),
+                              float(nub), int(iflag), int(i_orig), int(peelid))
+        return out
 
     def run(self, np_: int, p_begin: int = 0, p_end: Optional[int] = None, ipeel: int = 0,
             seed1: int = 12345678, seed2: int = 87654321, gen: int = 1, seg0: int = 0,
