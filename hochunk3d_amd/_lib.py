@@ -25,7 +25,7 @@ _ip = C.POINTER(C.c_int32)
 # Every symbol include/hc.h declares.
 EXPORTS = (
     "hc_create", "hc_set_grid", "hc_set_temperature", "hc_get_temperature", "hc_set_dust",
-    "hc_set_sources",
+    "hc_set_sources", "hc_set_atmosphere", "hc_set_limb_table",
     "hc_run_iteration", "hc_temperature", "hc_run_lucy", "hc_set_option",
     "hc_dropped_packets", "hc_destroy",
     "hc_last_error", "hc_build_density", "hc_build_density_noalloc", "hc_build_emissivity",
@@ -117,6 +117,8 @@ def load() -> C.CDLL:
     lib.hc_get_temperature.argtypes = [C.c_void_p, _dp]
     lib.hc_set_dust.argtypes = [C.c_void_p, _ip] + [_dp] * 10
     lib.hc_set_sources.argtypes = [C.c_void_p, C.POINTER(HcSources)]
+    lib.hc_set_atmosphere.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, _dp, _dp]
+    lib.hc_set_limb_table.argtypes = [C.c_void_p, C.c_int32, _dp, _dp]
     lib.hc_run_iteration.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
                                      C.c_int32, C.c_uint64, C.POINTER(HcTallies), C.c_void_p]
     lib.hc_temperature.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, _dp,
@@ -227,6 +229,7 @@ class Transport:
         s.ltot = m.ltot
         s.iintmax = 10000
         s.nspot = 1
+        s.limb = int(m.cfg.limb)
         s.accfrac = m.accfrac
         s.rddustmin, s.rmaxdmax, s.rmax = m.rddustmin, m.rmaxdmax, m.rarr[-1]
         s.massc, s.mdot = m.cfg.massc, m.mdot
@@ -235,7 +238,23 @@ class Transport:
         for k, v in over.items():
             setattr(s, k, v)
         s.u, s.aperture2, s.coste, s.sinte, s.cospe, s.sinpe = [_ptr(a) for a in arrs]
+        if m.limb_tab is not None:
+            xmu, prob = (_f64(a) for a in m.limb_tab)
+            self._check(self.lib.hc_set_limb_table(self.h, xmu.size, _ptr(xmu), _ptr(prob)),
+                        "hc_set_limb_table")
         self._check(self.lib.hc_set_sources(self.h, C.byref(s)), "hc_set_sources")
+        self.set_atmosphere(m.atmosphere)
+
+    def set_atmosphere(self, atm):
+        """The stellar spectrum table (hochunk3d_amd.atmos.Atmosphere), or
+        None for the Planck function at tstar."""
+        if atm is None:
+            self._check(self.lib.hc_set_atmosphere(self.h, 0, None, None, None, None),
+                        "hc_set_atmosphere")
+            return
+        a = [_f64(v) for v in (atm.atmosnu, atm.hnuint, atm.lognu, atm.loghnu)]
+        self._check(self.lib.hc_set_atmosphere(self.h, a[0].size, *(_ptr(v) for v in a)),
+                    "hc_set_atmosphere")
 
     def set_option(self, name: str, value: int):
         """Launch-shape knob: regen_lanes, regen_run_lanes, peel_queue,

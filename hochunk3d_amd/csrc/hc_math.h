@@ -22,6 +22,10 @@
  *   atan2 angle from the x axis as asin(|y|/r) (|y|/r <= 1/2) or acos(|x|/r),
  *         r = sqrt(x^2 + y^2), placed in its quadrant.
  *   pow   x^0.25 = sqrt(sqrt(x)), x^2 = x x, else exp(y log x) (x > 0).
+ *   log10 log(x) times 1/ln 10.
+ *   exp10 10^y = e^p (1 + e) with p = y ln 10 rounded, e its rounding error
+ *         (a fused multiply-add, correctly rounded on every target) plus
+ *         y times the tail of ln 10.
  * Accuracy is within a few ulp of the correctly rounded result over the
  * arguments the transport path uses; tests/test_hc_math.py measures it.
  *
@@ -236,6 +240,19 @@ HCM_FN double hcm_pow(double x, double y) {
   if (x == 0.0) return y > 0.0 ? 0.0 : 1.0 / 0.0;
   if (x == 1.0) return 1.0;
   return hcm_exp(y * hcm_log(x));
+}
+
+HCM_FN double hcm_log10(double x) { return hcm_log(x) * 0.43429448190325182765; }
+
+HCM_FN double hcm_exp10(double y) {
+  const double l_hi = 2.302585092994046;       /* ln 10 rounded to double */
+  const double l_lo = -2.1707562233822494e-16; /* ln 10 - l_hi */
+  const double p = y * l_hi;
+  /* the rounding error of p (a correctly rounded fused multiply-add on
+   * every target) plus the tail of ln 10: 10^y = e^p (1 + e) */
+  const double e = __builtin_fma(y, l_hi, -p) + y * l_lo;
+  const double x = hcm_exp(p);
+  return x + x * e;
 }
 
 #endif

@@ -33,6 +33,8 @@ namespace hc {
 
 constexpr int kGuide = 512;  // bins of the opacity-table guide
 constexpr int kCdfGuide = 1024;  // probability bins of the emissivity-CDF guide
+constexpr int kAtmGuide = 4096;  // probability bins of the stellar-spectrum CDF guide (2^12:
+                                 // floor(xi kAtmGuide) and b / kAtmGuide are exact)
 
 struct Params {
   // grid
@@ -87,6 +89,15 @@ struct Params {
   double fmass[kMaxSpecies], z1[kMaxSpecies], bflare[kMaxSpecies];
   int slot_of[kMaxSpecies];  // active slot of reference species id, -1 if none
   uint32_t max_work;  // events + crossings after which a packet is dropped
+  // stellar spectrum (atmosinit/atmosfreq): nhnu = 0 samples the Planck
+  // function at tstar instead
+  int nhnu;
+  const double *atm;          // atmosnu, hnuint, lognu, loghnu [nhnu each]
+  const int32_t *atm_guide;   // [kAtmGuide + 1]: table nodes with hnuint below b / kAtmGuide
+  // Eddington limb darkening (samptabl.f90): xmu[ntab], prob[ntab]
+  int limb, ntab;
+  const double *limb_tab;
+  double delmu;
 };
 
 // Launch parameters are the transport kernel's argument: every wave reads
@@ -606,7 +617,10 @@ __device__ __forceinline__ int peel_begin(const WallView &W, const PeelRec<R, NA
           nm.cosb * R(P.coste[obs]) +
           (nm.sinb * R(P.sinte[obs]) * (R(P.cospe[obs]) * nm.coslp + R(P.sinpe[obs]) * nm.sinlp));
       if (!(cosnorm > R(0.0))) return kPeelIdle;
-      S.w = (double)(R(4.0) * R(1.0) * cosnorm);  // I = 1
+      // I = 1; energy per sr ~ mu for isotropic intensity, and another
+      // factor mu under Eddington limb darkening (newdisktherm.F90:349-355)
+      S.w = P.limb ? (double)(R(12.0 / 5.0) * R(1.0) * (cosnorm + cosnorm * cosnorm))
+                   : (double)(R(4.0) * R(1.0) * cosnorm);
     } else {
       S.w = (double)(R(1.0) * R(2.0));
     }
@@ -915,9 +929,47 @@ __device__ __forceinline__ void set_opacity(Pkt<R, NA> &k, bool launch_order) {
   }
 }
 
+// darkening (samptabl.f90:29-66): the draw located in the cumulative
+// probability of the emission cosine, mu interpolated linearly from the
+// bin's upper node (the table's first node is mu = 0, probability 0).
+__device__ __forceinline__ double darkening(double xi) {
+  const Params &P = c_P;
+  const double *xmu = P.limb_tab, *prob = P.limb_tab + P.ntab;
+  const int n = clampi(locate(prob, P.ntab, xi), 1, P.ntab - 1);
+  return xmu[n] - (prob[n] - xi) / (prob[n] - prob[n - 1]) * P.delmu;
+}
+
+// atmosfreq (starbarb.f90:120-156): the bracket hnuint(i) < xi <= hnuint(i+1)
+// of locate2's bisection is the number i of nodes below the draw, found from
+// the guide and a forward scan; then log-log interpolation, linear in the
+// first bin (hnuint(1) = 0).
+__device__ __forceinline__ double atmosfreq(Mrg &rng) {
+  const Params &P = c_P;
+  const int n = P.nhnu;
+  const double *nu = P.atm, *h = P.atm + n, *lnu = P.atm + 2 * n, *lh = P.atm + 3 * n;
+  const double xi = ran(rng);
+  int i = P.atm_guide[clampi((int)(xi * (double)kAtmGuide), 0, kAtmGuide)];
+  while (i < n && h[i] < xi) ++i;
+  i = clampi(i, 1, n - 1);
+  if (i > 1) {
+    const double logatm = lnu[i - 1] + (lnu[i] - lnu[i - 1]) * (hcm_log10(xi) - lh[i - 1]) /
+                                           (lh[i] - lh[i - 1]);
+    return hcm_exp10(logatm);
+  }
+  return nu[0] + (nu[1] - nu[0]) * (xi - h[0]) / (h[1] - h[0]);
+}
+
+// The photon energy of stellar emission: the atmosphere table when one is
+// set (the reference's stellar photons), else the Planck function at tstar.
+template <class R>
+__device__ __forceinline__ R star_freq(Mrg &rng) {
+  return c_P.nhnu > 0 ? R(atmosfreq(rng)) : bbfreq<R>(rng, R(c_P.tstar));
+}
+
 template <class R, int NA>
 __device__ __forceinline__ void initp_sample(Pkt<R, NA> &k, Mrg &rng) {  // initp_common.f90:1-29
-  k.d.cost = m_sqrt(R(ran(rng)));
+  // isotropic intensity: mu = sqrt(xi); Eddington limb darkening: the table
+  k.d.cost = c_P.limb ? R(darkening(ran(rng))) : m_sqrt(R(ran(rng)));
   k.d.sint = m_sqrt(R(1.0) - k.d.cost * k.d.cost);
   k.d.phi = R(kR2P) * R(ran(rng));
   k.d.cosp = m_cos(k.d.phi);
@@ -1205,7 +1257,17 @@ __device__ __forceinline__ void launch_star(Pkt<R, NA> &k, Mrg &rng, bool hotspo
     k.d.sinp = clamp1(m_sin(k.d.phi));
     k.d.cosp = clamp1(m_cos(k.d.phi));
   }
-  k.nub = bbfreq<R>(rng, R(ispotph ? P.tshock : P.tstar));
+  // initpspot.f90:147-159 / initp_ps.f90:59-69: hot-spot photons
+  // (iplanckst = 1) take the Planck function at tshock inside a spot and at
+  // tstar outside it; stellar photons (iplanckst = 0) take the stellar
+  // spectrum outside a spot (everywhere when fspot = 1) and tshock's Planck
+  // function inside one
+  if (hotspot)
+    k.nub = bbfreq<R>(rng, R(((P.fspot == 1.0 && !P.ips) || ispotph) ? P.tshock : P.tstar));
+  else if (ispotph && (P.ips || P.fspot != 1.0))
+    k.nub = bbfreq<R>(rng, R(P.tshock));
+  else
+    k.nub = star_freq<R>(rng);
   if (hotspot) {
     if (ran(rng) < 0.5)
       k.nub = R(1.2398 / 0.06) + R(ran(rng)) * R(1.2398 / 0.015 - 1.2398 / 0.06);
@@ -1245,7 +1307,7 @@ __device__ __forceinline__ void star_reemit(Pkt<R, NA> &k, Mrg &rng, Acc &acc, R
   k.s = Stokes4<R>{R(1.0), R(0.0), R(0.0), R(0.0)};
   initp_sample(k, rng);
   initp_transform(k, cosb, sinb, lp);
-  k.nub = bbfreq<R>(rng, R(P.tstar));
+  k.nub = star_freq<R>(rng);
   k.i_orig = 1;
   k.iflag = 0;
   set_opacity(k, true);
@@ -1891,6 +1953,13 @@ struct hc_ctx {
          *tdust2 = nullptr;
   unsigned long long *counter = nullptr;
   unsigned long long *segdev = nullptr;  // segment base of graph-replayed passes
+  // stellar spectrum table (hc_set_atmosphere) and limb-darkening table
+  // (hc_set_limb_table)
+  int nhnu = 0;
+  double *atm = nullptr;
+  int32_t *atm_guide = nullptr;
+  int ntab = 0;
+  double *limb_tab = nullptr;
   hc_trace *trace = nullptr;
   size_t trace_cap = 0;
   // host copies of scalars
@@ -2231,7 +2300,8 @@ int hc_set_sources(hc_ctx *ctx, const hc_sources *src) {
   if (!src->u || !src->aperture2 || (d.npeel > 0 && (!src->coste || !src->sinte || !src->cospe ||
                                                      !src->sinpe)))
     return fail(ctx, HC_E_ARG, "hc_set_sources: null array");
-  if (src->limb != 0) return fail(ctx, HC_E_UNSUPPORTED, "limb-darkened launch not supported");
+  if (src->limb != 0 && src->limb != 1)
+    return fail(ctx, HC_E_ARG, "hc_set_sources: limb must be 0 (isotropic) or 1 (Eddington)");
   if (src->accfrac < 0.0 || src->accfrac >= 1.0 || src->accfrac2 < 0.0 || src->accfrac2 >= 1.0)
     return fail(ctx, HC_E_ARG, "hc_set_sources: accfrac and accfrac2 must lie in [0, 1)");
   if (src->accfrac > 0.0 &&
@@ -2310,6 +2380,8 @@ int prepare_pass(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int32_t 
                  int32_t gen, uint64_t seg0, bool trace, PassLaunch &L) {
   if (!ctx->have_grid || !ctx->have_dust || !ctx->have_src)
     return fail(ctx, HC_E_STATE, "transport pass: grid, dust and sources must be set");
+  if (ctx->src.limb && ctx->ntab == 0)
+    return fail(ctx, HC_E_STATE, "transport pass: limb darkening needs hc_set_limb_table");
   if (np < 0 || gen < 1 || gen > 32 || seed1 <= 0 || seed1 >= 2147483563 || seed2 <= 0 ||
       seed2 >= 2147483399)
     return fail(ctx, HC_E_ARG, "transport pass: bad np or seeds");
@@ -2462,6 +2534,13 @@ int prepare_pass(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int32_t 
     P.bflare[i] = i < 8 ? S.bflare[i] : 0.0;
     P.slot_of[i] = ctx->slot[i];
   }
+  P.nhnu = ctx->nhnu;
+  P.atm = ctx->atm;
+  P.atm_guide = ctx->atm_guide;
+  P.limb = S.limb;
+  P.ntab = ctx->ntab;
+  P.limb_tab = ctx->limb_tab;
+  P.delmu = ctx->ntab > 0 ? 1.0 / (double)ctx->ntab : 0.0;
 
   void (*&kern)(Params) = L.kern;
   size_t rec_bytes = 0;
@@ -2827,6 +2906,67 @@ int hc_run_lucy(hc_ctx *ctx, int32_t niter, int64_t np, int32_t seed1, int32_t s
   return HC_OK;
 }
 
+int hc_set_atmosphere(hc_ctx *ctx, int32_t nhnu, const double *atmosnu, const double *hnuint,
+                      const double *lognu, const double *loghnu) {
+  if (!ctx) return HC_E_ARG;
+  HCK(hipSetDevice(ctx->device));
+  if (nhnu == 0) {  // back to the Planck function at tstar
+    ctx->nhnu = 0;
+    return HC_OK;
+  }
+  if (nhnu < 2 || nhnu > 60000 || !atmosnu || !hnuint || !lognu || !loghnu)
+    return fail(ctx, HC_E_ARG, "hc_set_atmosphere: need 2 <= nhnu <= 60000 and four arrays");
+  for (int i = 1; i < nhnu; ++i)
+    if (!(atmosnu[i] > atmosnu[i - 1]) || hnuint[i] < hnuint[i - 1])
+      return fail(ctx, HC_E_ARG,
+                  "hc_set_atmosphere: atmosnu must increase and hnuint must not decrease");
+  if (!(hnuint[0] == 0.0 && hnuint[nhnu - 1] == 1.0))
+    return fail(ctx, HC_E_ARG, "hc_set_atmosphere: hnuint must run from 0 to 1");
+  std::vector<double> tab(4 * (size_t)nhnu);
+  std::copy(atmosnu, atmosnu + nhnu, tab.begin());
+  std::copy(hnuint, hnuint + nhnu, tab.begin() + nhnu);
+  std::copy(lognu, lognu + nhnu, tab.begin() + 2 * (size_t)nhnu);
+  std::copy(loghnu, loghnu + nhnu, tab.begin() + 3 * (size_t)nhnu);
+  // guide: the nodes with hnuint below b / kAtmGuide
+  std::vector<int32_t> guide(kAtmGuide + 1);
+  int cnt = 0;
+  for (int b = 0; b <= kAtmGuide; ++b) {
+    const double v = (double)b / (double)kAtmGuide;
+    while (cnt < nhnu && hnuint[cnt] < v) ++cnt;
+    guide[b] = cnt;
+  }
+  for (void *b : {(void *)ctx->atm, (void *)ctx->atm_guide})
+    if (b) (void)hipFree(b);
+  ctx->atm = nullptr;
+  ctx->atm_guide = nullptr;
+  ctx->nhnu = 0;
+  int rc = upload(ctx, &ctx->atm, tab.data(), tab.size());
+  if (!rc) rc = upload(ctx, &ctx->atm_guide, guide.data(), guide.size());
+  if (rc) return rc;
+  ctx->nhnu = nhnu;
+  return HC_OK;
+}
+
+int hc_set_limb_table(hc_ctx *ctx, int32_t ntab, const double *xmu, const double *prob) {
+  if (!ctx || ntab < 3 || !xmu || !prob) return HC_E_ARG;
+  for (int i = 1; i < ntab; ++i)
+    if (!(prob[i] > prob[i - 1]) || !(xmu[i] > xmu[i - 1]))
+      return fail(ctx, HC_E_ARG, "hc_set_limb_table: xmu and prob must increase");
+  if (prob[ntab - 1] != 1.0)
+    return fail(ctx, HC_E_ARG, "hc_set_limb_table: prob must end at 1");
+  HCK(hipSetDevice(ctx->device));
+  std::vector<double> tab(2 * (size_t)ntab);
+  std::copy(xmu, xmu + ntab, tab.begin());
+  std::copy(prob, prob + ntab, tab.begin() + ntab);
+  if (ctx->limb_tab) (void)hipFree(ctx->limb_tab);
+  ctx->limb_tab = nullptr;
+  ctx->ntab = 0;
+  const int rc = upload(ctx, &ctx->limb_tab, tab.data(), tab.size());
+  if (rc) return rc;
+  ctx->ntab = ntab;
+  return HC_OK;
+}
+
 int hc_dropped_packets(hc_ctx *ctx, int64_t *pids, int32_t cap, int64_t *ndropped) {
   if (!ctx || !ndropped || cap < 0 || (cap > 0 && !pids)) return HC_E_ARG;
   if (!ctx->counter) return fail(ctx, HC_E_STATE, "hc_dropped_packets: no pass has run");
@@ -2878,7 +3018,8 @@ int hc_destroy(hc_ctx *ctx) {
                   ctx->sinpe,  ctx->pow1,    ctx->pow2,    ctx->dtauabs, ctx->nabs,   ctx->jmean,
                   ctx->killed, ctx->dbig,    ctx->sed,     ctx->peel,    ctx->img_i,  ctx->img_q,
                   ctx->img_u,  ctx->img_v,   ctx->si_init, ctx->scal,    ctx->mass4,  ctx->counter,
-                  ctx->trace,  ctx->guide,   ctx->tdust2,  ctx->cdfguide, ctx->segdev};
+                  ctx->trace,  ctx->guide,   ctx->tdust2,  ctx->cdfguide, ctx->segdev,
+                  ctx->atm,    ctx->atm_guide, ctx->limb_tab};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);

@@ -137,6 +137,25 @@ module hc_iface
       integer(c_int64_t), value :: value
     end function hc_set_option
 
+    ! the stellar spectrum of atmosinit (starbarb.f90:3-118): the atmos_interp
+    ! module arrays, e.g. hc_set_atmosphere(hc, nhnu, atmosnu, hnuint, lognu, loghnu)
+    integer(c_int) function hc_set_atmosphere(ctx, nhnu, atmosnu, hnuint, lognu, loghnu) &
+        bind(C, name='hc_set_atmosphere')
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nhnu
+      real(c_double), intent(in) :: atmosnu(*), hnuint(*), lognu(*), loghnu(*)
+    end function hc_set_atmosphere
+
+    ! the limb-darkening table of `table` (samptabl.f90:1-22): xmu(ntab), prob(ntab)
+    integer(c_int) function hc_set_limb_table(ctx, ntab, xmu, prob) &
+        bind(C, name='hc_set_limb_table')
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ntab
+      real(c_double), intent(in) :: xmu(*), prob(*)
+    end function hc_set_limb_table
+
     integer(c_int) function hc_dropped_packets(ctx, pids, cap, ndropped) &
         bind(C, name='hc_dropped_packets')
       import :: c_int, c_ptr, c_int32_t, c_int64_t

@@ -85,6 +85,14 @@ class ModelConfig:
     mdot_disk: float = 0.0         # disk accretion rate, Msun/yr (0: no accretion photons)
     fractal: Optional[dict] = None
     dust_table: Optional[np.ndarray] = field(default=None, repr=False)
+    # stellar spectrum: None samples the Planck function at tstar (bbfreq);
+    # (wavelength um, F_nu) rows of a model atmosphere sample atmosinit's
+    # table with atmosfreq, as the reference's stellar photons do, the
+    # fluxes replaced by the Planck function at tstar when cplanckst
+    atmosphere: Optional[tuple] = field(default=None, repr=False)
+    cplanckst: bool = False
+    limb: int = 0                  # CLIMB: 0 isotropic, 1 Eddington limb darkening
+    ntab: int = 1000               # nodes of the limb-darkening table
 
 
 @dataclass
@@ -128,6 +136,9 @@ class Model:
     fmass8: tuple = (0.0,) * 8
     z1_8: tuple = (0.0,) * 8
     bflare8: tuple = (0.0,) * 8
+    # stellar spectrum table (atmosinit) and limb-darkening table (xmu, prob)
+    atmosphere: Optional[object] = None
+    limb_tab: Optional[tuple] = None
 
     @property
     def ncells(self) -> int:
@@ -394,11 +405,19 @@ def build_model(cfg: ModelConfig, device: Optional[int] = None) -> Model:
     if np.any(np.min(np.abs(thete[:, None] - thetarr[None, :]), axis=1) == 0.0):
         raise ValueError("a peel-off angle equals a theta wall (gridset.f90:569-577)")
     phie = np.full_like(thete, np.deg2rad(cfg.phie_deg))
-    return Model(cfg, nrg, ntg, npg, rarr, r2arr, thetarr, tan2arr, phiarr, aarr, barr, densarr,
+    from .atmos import atmosinit, limb_table
+    atm = None
+    if cfg.atmosphere is not None:
+        atm = atmosinit(cfg.atmosphere[0], cfg.atmosphere[1], tstar_ev, cfg.cplanckst)
+    ltab = limb_table(cfg.ntab) if cfg.limb else None
+    model = Model(cfg, nrg, ntg, npg, rarr, r2arr, thetarr, tan2arr, phiarr, aarr, barr, densarr,
                  vcell, massarr, tdust, dust, cfg.rstar, tstar_ev, cfg.tshock / 11605.0, ltot,
                  numin, float(lnurat), u, aperture2, np.cos(thete), np.sin(thete), np.cos(phie),
                  np.sin(phie), cfg.rmaxi_au * autors, autors, accfrac, mdot, rmind, rmaxd,
                  tuple(fm8), tuple(z18), tuple(b8))
+    model.atmosphere = atm
+    model.limb_tab = ltab
+    return model
 
 
 def config1(**over) -> ModelConfig:

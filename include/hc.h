@@ -55,7 +55,7 @@ typedef struct {
 typedef struct {
   double rstar;            /* solar radii */
   double tstar, tshock;    /* eV (K / 11605) */
-  int32_t limb, ips, ispot, nspot;
+  int32_t limb, ips, ispot, nspot;  /* limb: 0 isotropic, 1 Eddington (hc_set_limb_table) */
   double fspot, csprad, cspot1, sspot1, scspot1, ssspot1, cspot2, sspot2, scspot2, ssspot2;
   double accfrac, accfrac2;
   double numin, lnurat;    /* SED frequency binning (setup.f90:36-40) */
@@ -141,6 +141,25 @@ int hc_set_dust(hc_ctx *ctx, const int32_t *nlambda, const double *nudust, const
                 const double *kapBint, const double *kappap);
 
 int hc_set_sources(hc_ctx *ctx, const hc_sources *src);
+
+/* The stellar spectrum of atmosinit (starbarb.f90:3-118), the atmos_interp
+ * module arrays in increasing frequency: atmosnu (eV), the normalised
+ * cumulative flux hnuint (0 first, 1 last), lognu = log10(atmosnu), loghnu =
+ * log10(hnuint) [nhnu <= 60000].  Stellar (non-spot) photons then take
+ * atmosfreq (starbarb.f90:120-156), as the reference's photon loop does for
+ * every stellar photon (iplanckst = 0, newdisktherm.F90:283-287); star
+ * re-emission too.  nhnu = 0 returns to the Planck function at tstar
+ * (bbfreq), the default. */
+int hc_set_atmosphere(hc_ctx *ctx, int32_t nhnu, const double *atmosnu, const double *hnuint,
+                      const double *lognu, const double *loghnu);
+
+/* The limb-darkening table of `table` (samptabl.f90:1-22): xmu[ntab],
+ * prob[ntab] (the cumulative probability (mu^3 + mu^2)/2, increasing to 1;
+ * delmu = 1/ntab).  Used by the stellar launch and star re-emission when
+ * hc_sources.limb = 1 (initp_common.f90:1-29, darkening samptabl.f90:29-66),
+ * with the direct peel weight 12/5 (mu + mu^2) (newdisktherm.F90:349-355).
+ * The reference's tabl_mod, which sets ntab, is not in its source tree. */
+int hc_set_limb_table(hc_ctx *ctx, int32_t ntab, const double *xmu, const double *prob);
 
 /* One transport pass (the body of disk(), newdisktherm.F90:194-547): np
  * packets, peel-off when ipeel=1.  Packet p (0-based) draws from segment

@@ -29,6 +29,8 @@
 #define HCO_ACOS hcm_acos
 #define HCO_POW hcm_pow
 #define HCO_ATAN2 hcm_atan2
+#define HCO_LOG10 hcm_log10
+#define HCO_EXP10 hcm_exp10
 #else
 #define HCO_LOG log
 #define HCO_EXP exp
@@ -37,6 +39,8 @@
 #define HCO_ACOS acos
 #define HCO_POW pow
 #define HCO_ATAN2 atan2
+#define HCO_LOG10 log10
+#define HCO_EXP10(y) pow(10.0, (y))
 #endif
 
 #define PI 3.1415926535897932384626433  /* constants.f90:5 */
@@ -1210,9 +1214,22 @@ static void set_kapd_reemit(ctx_t *c, pkt_t *P) {  /* propagate.f90:149-151 */
 }
 
 /* initp_sample with limb=0 (initp_common.f90:1-29). */
+/* darkening (samptabl.f90:29-66): locate the draw in the table of the
+ * cumulative probability prob(mu) = (mu^3 + mu^2) / 2, then interpolate
+ * linearly in mu from the bin's upper node (xmu(1) = 0, xmu(i) = i delmu
+ * otherwise, as `table` fills it). */
+double hco_darkening(const hco_setup *s, double xi) {
+  int n = hco_locate(s->prob, s->ntab, xi);
+  if (n < 1) n = 1;
+  if (n > s->ntab - 1) n = s->ntab - 1;  /* a draw of 1 would read past the table */
+  return s->xmu[n] - (s->prob[n] - xi) / (s->prob[n] - s->prob[n - 1]) * s->delmu;
+}
+
+/* initp_sample (initp_common.f90:1-29): mu = sqrt(xi) for isotropic
+ * intensity, or from the limb-darkening table. */
 static void initp_sample(ctx_t *c, pkt_t *P) {
   double xran = hco_ran(&c->rng);
-  P->cost = sqrt(xran);
+  P->cost = (c->s && c->s->limb) ? hco_darkening(c->s, xran) : sqrt(xran);
   P->sint = sqrt(1.0 - P->cost * P->cost);
   xran = hco_ran(&c->rng);
   P->phi = R2P * xran;
@@ -1292,7 +1309,10 @@ static void peel_direct(ctx_t *c, pkt_t *P, double cosb, double sinb, double cos
     if (!s->ips) {
       double cosnorm = cosb * s->coste[k] +
                        (sinb * s->sinte[k] * (s->cospe[k] * coslp + s->sinpe[k] * sinlp));
-      double sipnew = 4.0 * P->sip * cosnorm;
+      /* isotropic intensity: energy per sr ~ mu; Eddington limb darkening:
+       * intensity ~ (1 + 3/2 mu), another factor mu (newdisktherm.F90:349-355) */
+      double sipnew = s->limb ? 12.0 / 5.0 * P->sip * (cosnorm + cosnorm * cosnorm)
+                              : 4.0 * P->sip * cosnorm;
       if (cosnorm > 0.0) peeloff(c, P, sipnew, 0, k);
     } else {
       peeloff(c, P, P->sip * 2.0, 0, k);
@@ -1473,7 +1493,8 @@ static void launch_acc(ctx_t *c, pkt_t *P) {
  * point-source mode), then newdisktherm.F90:256-337.  hotspot selects the
  * hot-spot branch of the photon loop (newdisktherm.F90:256-273): the packet
  * starts inside the spot when ispot=1, is emitted at tshock and half of such
- * packets are X-rays.  Stellar packets take the tstar blackbody. */
+ * packets are X-rays.  Stellar packets take the stellar spectrum: the
+ * atmosphere table when one is set (atmosfreq), else the tstar blackbody. */
 /* The surface point of a stellar packet (initpspot.f90:3-120): uniform on
  * the star, or, for a hot-spot packet with ispot=1, uniform within the spot
  * caps by rejection; a stellar packet that lands in a cap becomes a spot
@@ -1526,6 +1547,32 @@ int hco_star_point(const hco_setup *s, hco_rng *r, int hotspot, double *cosb, do
   return star_point(s, r, hotspot, cosb, sinb, lp);
 }
 
+/* atmosfreq (starbarb.f90:120-156): the bracket hnuint(i) < xi <= hnuint(i+1)
+ * by locate2's bisection, then log-log interpolation (linear in the first
+ * bin, where hnuint(1) = 0). */
+double hco_atmosfreq_xi(const hco_setup *s, double xi) {
+  int i = hco_locate(s->hnuint, s->nhnu, xi);
+  if (i < 1) i = 1;
+  if (i > s->nhnu - 1) i = s->nhnu - 1;  /* xi < 1 = hnuint(nhnu): never taken */
+  if (i > 1) {
+    double logatm = s->lognu[i - 1] + (s->lognu[i] - s->lognu[i - 1]) *
+                                          (HCO_LOG10(xi) - s->loghnu[i - 1]) /
+                                          (s->loghnu[i] - s->loghnu[i - 1]);
+    return HCO_EXP10(logatm);
+  }
+  return s->atmosnu[i - 1] +
+         (s->atmosnu[i] - s->atmosnu[i - 1]) * (xi - s->hnuint[i - 1]) /
+             (s->hnuint[i] - s->hnuint[i - 1]);
+}
+
+/* The photon energy of stellar emission: the atmosphere table when one is
+ * set (the reference's stellar photons, iplanckst = 0), else the Planck
+ * function at tstar. */
+static double star_freq(ctx_t *c) {
+  if (c->s->nhnu > 0) return hco_atmosfreq_xi(c->s, hco_ran(&c->rng));
+  return hco_bbfreq(&c->rng, c->s->tstar);
+}
+
 static void launch_star(ctx_t *c, pkt_t *P, int hotspot) {
   const hco_setup *s = c->s;
   const hco_grid *g = c->g;
@@ -1552,7 +1599,18 @@ static void launch_star(ctx_t *c, pkt_t *P, int hotspot) {
     P->sinp = clamp1(HCO_SIN(P->phi));
     P->cosp = clamp1(HCO_COS(P->phi));
   }
-  P->nub = hco_bbfreq(&c->rng, ispotph ? s->tshock : s->tstar);
+  /* initpspot.f90:147-159 / initp_ps.f90:59-69: hot-spot photons
+   * (iplanckst = 1) take the Planck function at tshock inside a spot and at
+   * tstar outside it; stellar photons (iplanckst = 0) take the stellar
+   * spectrum outside a spot (and everywhere when fspot = 1) and tshock's
+   * Planck function inside one */
+  if (hotspot)
+    P->nub = hco_bbfreq(&c->rng, ((s->fspot == 1.0 && !s->ips) || ispotph) ? s->tshock
+                                                                           : s->tstar);
+  else if (ispotph && (s->ips || s->fspot != 1.0))
+    P->nub = hco_bbfreq(&c->rng, s->tshock);
+  else
+    P->nub = star_freq(c);
   if (hotspot) {
     if (hco_ran(&c->rng) < 0.5)
       P->nub = 1.2398 / 0.06 + hco_ran(&c->rng) * (1.2398 / 0.015 - 1.2398 / 0.06);
@@ -1599,7 +1657,7 @@ static void star_reemit(ctx_t *c, pkt_t *P) {
   P->sqp = P->sup = P->svp = 0.0;
   initp_sample(c, P);
   initp_transform(P, cosb, sinb, lp);
-  P->nub = hco_bbfreq(&c->rng, c->s->tstar);
+  P->nub = star_freq(c);
   P->i_orig = 1;
   P->iflag = 0;
   hco_opacset(c->d, P->nub, &P->op);
@@ -1958,7 +2016,7 @@ void hco_peel_ray(const hco_grid *g, const hco_setup *s, hco_tallies *t, const d
 }
 
 /* Elementary-function probe for the accuracy tests: fn 0 log, 1 exp, 2 sin,
- * 3 cos, 4 acos, 5 pow(x,y), 6 atan2(x,y). */
+ * 3 cos, 4 acos, 5 pow(x,y), 6 atan2(x,y), 7 log10, 8 10^x. */
 void hco_math(int fn, const double *x, const double *y, double *out, int64_t n) {
   int64_t i;
   for (i = 0; i < n; i++) {
@@ -1969,6 +2027,8 @@ void hco_math(int fn, const double *x, const double *y, double *out, int64_t n) 
       case 3: out[i] = HCO_COS(x[i]); break;
       case 4: out[i] = HCO_ACOS(x[i]); break;
       case 5: out[i] = HCO_POW(x[i], y[i]); break;
+      case 7: out[i] = HCO_LOG10(x[i]); break;
+      case 8: out[i] = HCO_EXP10(x[i]); break;
       default: out[i] = HCO_ATAN2(x[i], y[i]); break;
     }
   }

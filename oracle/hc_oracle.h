@@ -82,7 +82,7 @@ typedef struct {
   double rstar;                 /* solar radii */
   double tstar;                 /* eV (already divided by 11605) */
   double tshock;                /* eV */
-  int limb;                     /* 0: isotropic intensity (only mode restated) */
+  int limb;                     /* 0: isotropic intensity, 1: Eddington limb darkening */
   int ips;                      /* point-source stellar launch (initp_ps) */
   int iplanckst;                /* 1: blackbody star */
   /* hot spot */
@@ -104,6 +104,17 @@ typedef struct {
   double rddustmin, rmaxdmax, rmax;  /* stellar radii */
   double massc, mdot;                /* Msun, g/s */
   double fmass[HCO_NDG], z1[HCO_NDG], bflare[HCO_NDG];
+  /* stellar spectrum of atmosinit (starbarb.f90:3-118), the atmos_interp
+   * arrays in increasing frequency: with nhnu > 0 stellar (non-spot)
+   * photons take atmosfreq (starbarb.f90:120-156), as the reference's loop
+   * does for every stellar photon (iplanckst = 0, newdisktherm.F90:286);
+   * with nhnu = 0 they take bbfreq(tstar) */
+  int nhnu;
+  const double *atmosnu, *hnuint, *lognu, *loghnu;  /* [nhnu] */
+  /* limb-darkening table of `table` (samptabl.f90:1-22), used when limb = 1 */
+  int ntab;
+  const double *xmu, *prob;                        /* [ntab] */
+  double delmu;
 } hco_setup;
 
 typedef struct {
@@ -212,6 +223,11 @@ int hco_tauint_ray(const hco_grid *g, const hco_setup *s, hco_tallies *t, const 
 void hco_peel_ray(const hco_grid *g, const hco_setup *s, hco_tallies *t, const double *xyz,
                   const int *cell, int on_wall, const double *kapd, const double *stokes,
                   const double *dir, double nub, int iflag, int i_orig, int peelid);
+
+/* atmosfreq (starbarb.f90:120-156) for the draw xi: the frequency (eV) */
+double hco_atmosfreq_xi(const hco_setup *s, double xi);
+/* darkening (samptabl.f90:29-66) for the draw xi: the emission cosine */
+double hco_darkening(const hco_setup *s, double xi);
 
 int hco_select_dust(const double *kapd, const double *dens, hco_rng *r);
 void hco_dustmat(const hco_opac *o, int id, double cost, double cost2, double *p1, double *p2,

@@ -51,7 +51,9 @@ class Setup(C.Structure):
         ("ltot", C.c_double),
         ("rddustmin", C.c_double), ("rmaxdmax", C.c_double), ("rmax", C.c_double),
         ("massc", C.c_double), ("mdot", C.c_double),
-        ("fmass", C.c_double * NDG), ("z1", C.c_double * NDG), ("bflare", C.c_double * NDG)]
+        ("fmass", C.c_double * NDG), ("z1", C.c_double * NDG), ("bflare", C.c_double * NDG),
+        ("nhnu", C.c_int), ("atmosnu", _dp), ("hnuint", _dp), ("lognu", _dp), ("loghnu", _dp),
+        ("ntab", C.c_int), ("xmu", _dp), ("prob", _dp), ("delmu", C.c_double)]
 
 
 class Tallies(C.Structure):
@@ -156,6 +158,10 @@ def load(variant: str = "libm") -> C.CDLL:
         lib.hco_peel_ray.argtypes = [C.POINTER(Grid), C.POINTER(Setup), C.POINTER(Tallies), _dp,
                                      _ip, C.c_int, _dp, _dp, _dp, C.c_double, C.c_int, C.c_int,
                                      C.c_int]
+        lib.hco_atmosfreq_xi.argtypes = [C.POINTER(Setup), C.c_double]
+        lib.hco_atmosfreq_xi.restype = C.c_double
+        lib.hco_darkening.argtypes = [C.POINTER(Setup), C.c_double]
+        lib.hco_darkening.restype = C.c_double
         lib.hco_mulmod.argtypes = [C.c_int32] * 3
         lib.hco_mulmod.restype = C.c_int32
         lib.hco_math.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int64]
@@ -217,6 +223,18 @@ class Oracle:
         s.imcube = int(c.imcube)
         s.iintmax = 10000
         s.ltot = m.ltot
+        s.limb = int(c.limb)
+        if getattr(m, "atmosphere", None) is not None:
+            a = m.atmosphere
+            for k in ("atmosnu", "hnuint", "lognu", "loghnu"):
+                self._arrs[k] = f(getattr(a, k))
+                setattr(s, k, _p(self._arrs[k]))
+            s.nhnu = a.atmosnu.size
+        if getattr(m, "limb_tab", None) is not None:
+            self._arrs["xmu"], self._arrs["prob"] = (f(v) for v in m.limb_tab)
+            s.ntab = self._arrs["xmu"].size
+            s.xmu, s.prob = _p(self._arrs["xmu"]), _p(self._arrs["prob"])
+            s.delmu = 1.0 / float(s.ntab)
         self.setup = s
 
     def set_temperature(self, tdust):

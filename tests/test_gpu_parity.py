@@ -221,3 +221,48 @@ def test_hot_spot_launch_matches_oracle():
     cp = c.run(600, ipeel=1, trace=True, seg0=50_000)
     compare_traces(rp.trace, cp["trace"])
     rel_close(rp.peel, cp["peel"], rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("cplanckst,limb,spots", [(False, 1, False), (True, 0, True)],
+                         ids=["kurucz-limb", "cplanckst-spots"])
+def test_atmosphere_and_limb_launch_matches_oracle(cplanckst, limb, spots):
+    """Stellar photons from the atmosphere table (atmosfreq, starbarb.f90:120-156;
+    the reference's T = 4000 K Kurucz fluxes, or their CPLANCKST replacement by
+    the Planck function) with Eddington limb darkening (samptabl.f90,
+    initp_common.f90) or beside hot-spot packets at the shock temperature,
+    Lucy and peeled passes (direct peel weight 12/5 (mu + mu^2) under limb
+    darkening), traced packet by packet."""
+    import os
+
+    from hochunk3d_amd._lib import Transport
+    from oracle.pyoracle import Oracle
+    fix = np.load(os.path.join(os.path.dirname(__file__), "golden", "kt04000g+4.0z-2.0.npz"))
+    m = build_model(config1(nximg=21, nfreq=50, accfrac2=0.2 if spots else 0.0,
+                            atmosphere=(fix["wave"], fix["flux"]), cplanckst=cplanckst,
+                            limb=limb))
+    m.tdust[...] = synthetic_temperature(m)
+    spot = {}
+    if spots:
+        th, ph, rad = 0.4, 1.0, 0.3
+        spot = dict(ispot=1, nspot=1, fspot=0.05, csprad=float(np.cos(rad)),
+                    cspot1=float(np.cos(th)), sspot1=float(np.sin(th)),
+                    scspot1=float(np.sin(th) * np.cos(ph)), ssspot1=float(np.sin(th) * np.sin(ph)))
+    print(f"atmosphere {m.atmosphere.nhnu} nodes, cplanckst={cplanckst}, limb={limb}, "
+          f"spots={spots}", flush=True)
+    g = Transport(m)
+    try:
+        if spot:
+            g.set_sources(**spot)
+        rg = g.run(6000, ipeel=0, trace=True)
+        rp = g.run(400, ipeel=1, trace=True, seg0=70_000)
+    finally:
+        g.close()
+    c = Oracle(m)
+    for k, v in spot.items():
+        setattr(c.setup, k, v)
+    rc = c.run(6000, ipeel=0, trace=True)
+    compare_traces(rg.trace, rc["trace"])
+    _check_tallies(rg, rc)
+    cp = c.run(400, ipeel=1, trace=True, seg0=70_000)
+    compare_traces(rp.trace, cp["trace"])
+    rel_close(rp.peel, cp["peel"], rtol=1e-9, atol=1e-12)

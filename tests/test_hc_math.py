@@ -8,7 +8,8 @@ import pytest
 
 from oracle import pyoracle as po
 
-FN = {"log": 0, "exp": 1, "sin": 2, "cos": 3, "acos": 4, "pow": 5, "atan2": 6}
+FN = {"log": 0, "exp": 1, "sin": 2, "cos": 3, "acos": 4, "pow": 5, "atan2": 6, "log10": 7,
+      "exp10": 8}
 
 
 def _eval(name, x, y=None):
@@ -38,6 +39,10 @@ CASES = [
     ("acos", rng.uniform(-1, 1, 200000), None, np.arccos, 2.0),
     ("pow", rng.uniform(0.9, 1.2, 200000), rng.uniform(0, 1, 200000), np.power, 2.0),
     ("pow", 10.0 ** rng.uniform(-30, 30, 100000), np.full(100000, 0.25), np.power, 1.0),
+    ("log10", 10.0 ** rng.uniform(-300, 300, 200000), None, np.log10, 3.0),
+    ("log10", rng.uniform(1e-12, 1.0, 200000), None, np.log10, 3.0),
+    # photon energies in eV: 10^y for |y| < 3
+    ("exp10", rng.uniform(-3, 3, 200000), None, lambda v: 10.0 ** v, 4.0),
 ]
 
 

@@ -10,6 +10,7 @@ import subprocess
 
 import pytest
 
+import test_atmosphere as at
 import test_oracle_closed_forms as cf
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -34,22 +35,27 @@ PLANTS = [
      "if (c->g->tdust[k] < c->tsub_ev) opac", "if (1) opac", ["mask"]),
     ("image axes swapped",
      "ximage = yp1 * cospe - xp1 * sinpe;", "ximage = xp1 * cospe - yp1 * sinpe;", ["image"]),
-    ("direct peel weight 2 cos", "double sipnew = 4.0 * P->sip * cosnorm;",
-     "double sipnew = 2.0 * P->sip * cosnorm;", ["direct"]),
+    ("direct peel weight 2 cos", ": 4.0 * P->sip * cosnorm;", ": 2.0 * P->sip * cosnorm;",
+     ["direct"]),
     ("cos(theta) bin of 1 - cos^2",
      "it = (int)((double)s->nmu * (1.0 - P->cost) / 2.0) + 1;",
      "it = (int)((double)s->nmu * (1.0 - P->cost * fabs(P->cost)) / 2.0) + 1;", ["bins"]),
-    ("emission mu = xi instead of sqrt(xi)", "P->cost = sqrt(xran);", "P->cost = xran;",
+    ("emission mu = xi instead of sqrt(xi)", ": sqrt(xran);", ": xran;",
      ["dilution", "aperture"]),
     ("tfinal with 2 sigma T^4", "(na * ltot) / (4.0 * SIGT", "(na * ltot) / (2.0 * SIGT",
      ["dilution"]),
     ("star re-emission at half weight",
      "if (lp < 0.0) lp = lp + R2P;\n  P->sip = 1.0;\n  P->sqp = P->sup = P->svp = 0.0;\n"
-     "  initp_sample(c, P);\n  initp_transform(P, cosb, sinb, lp);\n  P->nub = "
-     "hco_bbfreq(&c->rng, c->s->tstar);",
+     "  initp_sample(c, P);\n  initp_transform(P, cosb, sinb, lp);\n  P->nub = star_freq(c);",
      "if (lp < 0.0) lp = lp + R2P;\n  P->sip = 0.5;\n  P->sqp = P->sup = P->svp = 0.0;\n"
-     "  initp_sample(c, P);\n  initp_transform(P, cosb, sinb, lp);\n  P->nub = "
-     "hco_bbfreq(&c->rng, c->s->tstar);", ["reemit"]),
+     "  initp_sample(c, P);\n  initp_transform(P, cosb, sinb, lp);\n  P->nub = star_freq(c);",
+     ["reemit"]),
+    ("atmosfreq interpolates from the wrong node",
+     "(HCO_LOG10(xi) - s->loghnu[i - 1])", "(HCO_LOG10(xi) - s->loghnu[i])", ["atmos"]),
+    ("darkening steps two bins", "(s->prob[n] - s->prob[n - 1]) * s->delmu;",
+     "(s->prob[n] - s->prob[n - 1]) * 2.0 * s->delmu;", ["limb"]),
+    ("limb peel weight 2 (mu + mu^2)", "? 12.0 / 5.0 * P->sip", "? 2.0 * P->sip",
+     ["limbpeel"]),
 ]
 
 
@@ -68,6 +74,10 @@ def _checks():
         "aperture": [transparent(cf.test_aperture_holds_the_disk_flux_inside_its_radius)],
         "dilution": [cf.test_optically_thin_grey_shell_reaches_the_dilution_temperature],
         "reemit": [cf.test_scattering_shell_returns_every_packet_and_the_stellar_spectrum],
+        "atmos": [lambda: at.test_atmosfreq_maps_nodes_to_nodes_and_is_monotone(at.ORACLE),
+                  at.test_atmosfreq_fills_brackets_with_their_probability_and_log_law],
+        "limb": [at.test_limb_darkening_cosine_law],
+        "limbpeel": [at.test_limb_darkened_disk_flux_and_peel_weight],
     }
 
 
@@ -82,7 +92,7 @@ def test_planted_mistake_is_caught(tmp_path, name, orig, planted, checks):
                                        "-lm"], check=True)
     table = _checks()
     old = cf.ORACLE
-    cf.ORACLE = str(so)
+    cf.ORACLE = at.ORACLE = str(so)
     caught = []
     try:
         for key in checks:
@@ -92,5 +102,5 @@ def test_planted_mistake_is_caught(tmp_path, name, orig, planted, checks):
                 except AssertionError:
                     caught.append(key)
     finally:
-        cf.ORACLE = old
+        cf.ORACLE = at.ORACLE = old
     assert caught, f"planted mistake '{name}' passed every check {checks}"
