@@ -183,6 +183,57 @@ def batched_grid(args) -> dict:
     }
 
 
+def config5_section(args) -> dict:
+    """The large hydro-like model (400 x 300 x 256 cells, log-normal
+    densities, 20 observers x 500^2 pixels x 200 frequencies): a Lucy pass
+    on the FP32 path against the plain FP64 path on the same packets and
+    temperatures, then a peeled pass with the FP32 image cube in HBM."""
+    from hochunk3d_amd import build_model, config5
+    from hochunk3d_amd._lib import HC_FP32, HC_FP64, Transport
+    from hochunk3d_amd.testing import synthetic_temperature
+
+    t0 = time.perf_counter()
+    m = build_model(config5(), device=0)
+    m.tdust[...] = synthetic_temperature(m)
+    build_s = time.perf_counter() - t0
+    n = int(args.config5_np)
+    log(f"config5: {m.ncells} cells built in {build_s:.1f}s (device densities, log-normal "
+        f"multiplier); Lucy passes of {n} packets")
+    out = {"workload": f"config5: {m.nrg - 1}x{m.ntg - 1}x{m.npg - 1} cells, disk+envelope x "
+                       f"log-normal (sigma 1.5, k^-11/3, seed 11), synthetic temperatures; "
+                       f"one Lucy pass of {n} packets per path, then a peeled pass of "
+                       f"{int(args.config5_peel_np)} packets with 20 observers x 500^2 pixels "
+                       f"x 200 frequencies (FP32 IQUV cube, 44 GB, in HBM)",
+           "build_s": round(build_s, 2)}
+    for prec, name in ((HC_FP32, "fp32"), (HC_FP64, "fp64")):
+        tr = Transport(m, precision=prec, cube_fp32=True)
+        try:
+            t1 = time.perf_counter()
+            sc, ms = tr.run_scalars(n, seg0=0)
+            wall = time.perf_counter() - t1
+            out[name] = {"value": round(n / wall, 2), "unit": "packets/s",
+                         "kernel_ms": round(ms, 1),
+                         "longest_packet": int(sc["max_packet_work"]),
+                         "crossings_per_packet": round(sc["ncross"] / n, 1),
+                         "events_per_packet": round((sc["nscat_ev"] + sc["nabs_ev"]) / n, 1),
+                         "dropped": int(sc["n_warn"]), "lost": int(sc["n_err_nowall"])}
+            log(f"config5 {name}: {n / wall:.3e} packets/s, longest packet "
+                f"{sc['max_packet_work']:.0f}, dropped {sc['n_warn']:.0f}")
+            if prec == HC_FP32 and args.config5_peel_np > 0:
+                pn = int(args.config5_peel_np)
+                t1 = time.perf_counter()
+                sp, pms = tr.run_scalars(pn, ipeel=1, seg0=n)
+                pw = time.perf_counter() - t1
+                out["fp32_peeled"] = {"value": round(pn / pw, 2), "unit": "packets/s",
+                                      "kernel_ms": round(pms, 1),
+                                      "peel_rays_per_packet": round(sp["npeelray"] / pn, 1)}
+                log(f"config5 fp32 peeled: {pn / pw:.3e} packets/s")
+        finally:
+            tr.close()
+    out["fp32_speedup_vs_fp64"] = round(out["fp32"]["value"] / out["fp64"]["value"], 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,6 +256,9 @@ def main():
                     help="fraction of each grid model's packet count run per Lucy iteration")
     ap.add_argument("--batch-iters", type=int, default=3)
     ap.add_argument("--batch-streams", type=int, default=4)
+    ap.add_argument("--config5-np", type=float, default=4_000_000,
+                    help="packets of the config-5 Lucy passes (0: skip the section)")
+    ap.add_argument("--config5-peel-np", type=float, default=20_000)
     ap.add_argument("--cpu-seconds", type=float, default=30.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", type=float, default=0,
@@ -327,6 +381,7 @@ def main():
     batch = None
     if args.batch_models > 0:
         batch = batched_grid(args)
+    c5 = config5_section(args) if args.config5_np > 0 else None
 
     # the binding roof of the Lucy pass: HBM (a crossing's density gather and
     # the per-cell tally read-modify-writes) against FP64 VALU (its FLOPs)
@@ -412,6 +467,7 @@ def main():
         "fp32_path": fp32,
         "peeled_pass_fp64": peel,
         "batched_grid": batch,
+        "config5": c5,
         "autotune": tuned or None,
     }
     print(json.dumps(line), flush=True)

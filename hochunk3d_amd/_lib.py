@@ -185,16 +185,21 @@ class HcError(RuntimeError):
 class Transport:
     """One GPU context holding a model's grid, dust and sources."""
 
-    def __init__(self, model, device: int = 0, precision: int = HC_FP64, imcube=None):
+    def __init__(self, model, device: int = 0, precision: int = HC_FP64, imcube=None,
+                 cube_fp32: bool = False):
+        """cube_fp32: accumulate the image cube in FP32 (hc_dims.imcube = 2),
+        returned as float32 arrays, as the reference's real*4 images."""
         self.precision = precision
         self.lib = load()
         m = model
         self.model = m
         cfg = m.cfg
         self.imcube = bool(cfg.imcube if imcube is None else imcube)
+        self.cube_fp32 = bool(cube_fp32 and self.imcube)
         self.dims = HcDims(m.nrg, m.ntg, m.npg, 8, cfg.nfreq, cfg.nmu, cfg.nph, cfg.nap,
                            cfg.npeel, cfg.nximg, m.dust.nudust.shape[0], m.dust.lTint.size,
-                           m.dust.nuint.size, precision, int(self.imcube))
+                           m.dust.nuint.size, precision,
+                           (2 if self.cube_fp32 else 1) if self.imcube else 0)
         h = C.c_void_p()
         rc = self.lib.hc_create(C.byref(self.dims), device, C.byref(h))
         if rc != 0:
@@ -323,17 +328,19 @@ class Transport:
         peel = np.zeros((cfg.nfreq, max(cfg.npeel, 1), cfg.nap, NO, 9), order="F") if ipeel else None
         cube = bool(ipeel and self.imcube and images)
         nimg = (max(cfg.npeel, 1), cfg.nximg, cfg.nximg, cfg.nfreq)
-        img_i = np.zeros(nimg + (NO,), order="F") if cube else None
-        img_q = np.zeros(nimg, order="F") if cube else None
-        img_u = np.zeros(nimg, order="F") if cube else None
-        img_v = np.zeros(nimg, order="F") if cube else None
+        ct = np.float32 if self.cube_fp32 else np.float64
+        img_i = np.zeros(nimg + (NO,), ct, order="F") if cube else None
+        img_q = np.zeros(nimg, ct, order="F") if cube else None
+        img_u = np.zeros(nimg, ct, order="F") if cube else None
+        img_v = np.zeros(nimg, ct, order="F") if cube else None
         si_init = np.zeros((cfg.nfreq, 5), order="F")
         t = HcTallies()
         t.dtauabs, t.nabs, t.jmean = _ptr(dtauabs), _ptr(nabs), _ptr(jmean)
         t.killed, t.dbig = _ptr(killed, C.c_int32), _ptr(dbig, C.c_int32)
         t.sed = _ptr(sed)
         t.peel = _ptr(peel)
-        t.img_i, t.img_q, t.img_u, t.img_v = (_ptr(a) for a in (img_i, img_q, img_u, img_v))
+        t.img_i, t.img_q, t.img_u, t.img_v = (
+            None if a is None else C.cast(a.ctypes.data, _dp) for a in (img_i, img_q, img_u, img_v))
         t.si_init = _ptr(si_init)
         tr = np.zeros(np_, TRACE_DTYPE) if trace else None
         rc = self.lib.hc_run_iteration(self.h, int(np_), int(ipeel), int(seed1), int(seed2),

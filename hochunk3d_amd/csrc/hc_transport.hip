@@ -58,7 +58,7 @@ struct Params {
   double fspot, csprad, cspot1, sspot1, scspot1, ssspot1, cspot2, sspot2, scspot2, ssspot2;
   long long np, npacc, npaccstar;
   // binning
-  int nfreq, nmu, nph, nap, npeel, nxhst, imcube, iintmax;
+  int nfreq, nmu, nph, nap, npeel, nxhst, imcube, iintmax;  // imcube 2: FP32 cube words
   double numin, lnurat, rmaxi, fractxh;
   const double *u, *sini, *aperture2, *coste, *sinte, *cospe, *sinpe;
   // rng
@@ -747,11 +747,20 @@ __device__ __forceinline__ void peel_tally(const PeelRec<R, NA> &k, const PeelWa
                      (size_t)P.npeel * ((size_t)(ix - 1) + nx * ((size_t)(iy - 1) +
                                                                   nx * (size_t)(inub - 1)));
     const size_t n4 = (size_t)P.npeel * nx * nx * P.nfreq;
+    if (P.imcube == 2) {  // the reference's real*4 image arrays
+      float *ci = reinterpret_cast<float *>(P.img_i);
 #pragma unroll
-    for (int j = 0; j < 3; ++j) atomic_add(&P.img_i[q + n4 * (size_t)(indx[j] - 1)], s.I);
-    atomic_add(&P.img_q[q], s.Q);
-    atomic_add(&P.img_u[q], s.U);
-    atomic_add(&P.img_v[q], s.V);
+      for (int j = 0; j < 3; ++j) unsafeAtomicAdd(&ci[q + n4 * (size_t)(indx[j] - 1)], (float)s.I);
+      unsafeAtomicAdd(&reinterpret_cast<float *>(P.img_q)[q], (float)s.Q);
+      unsafeAtomicAdd(&reinterpret_cast<float *>(P.img_u)[q], (float)s.U);
+      unsafeAtomicAdd(&reinterpret_cast<float *>(P.img_v)[q], (float)s.V);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) atomic_add(&P.img_i[q + n4 * (size_t)(indx[j] - 1)], s.I);
+      atomic_add(&P.img_q[q], s.Q);
+      atomic_add(&P.img_u[q], s.U);
+      atomic_add(&P.img_v[q], s.V);
+    }
   }
 }
 
@@ -2068,15 +2077,10 @@ int hc_create(const hc_dims *dims, int device, hc_ctx **out) {
       dims->nnu < 2)
     return HC_E_ARG;
   if (dims->precision != HC_FP64 && dims->precision != HC_FP32) return HC_E_UNSUPPORTED;
+  if (dims->imcube < 0 || dims->imcube > 2) return HC_E_ARG;
   hc_ctx *ctx = new hc_ctx();
   ctx->dims = *dims;
-  // FP32 packets very rarely lose their walk to rounding and circle (one in
-  // ~2e6 on the 64x64x32-cell model); 2^21 events + crossings, still ~7x the
-  // longest packet of the benchmark models, bounds the time such a lane holds
-  if (dims->precision == HC_FP32) {
-    ctx->max_work = 1u << 21;
-    ctx->regen_lanes = 12;
-  }
+  if (dims->precision == HC_FP32) ctx->regen_lanes = 12;
   ctx->device = device;
   ctx->ncellw = (size_t)dims->nrg * dims->ntg * dims->npg;
   for (int i = 0; i < kMaxSpecies; ++i) ctx->slot[i] = -1;
@@ -2401,10 +2405,13 @@ int prepare_pass(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int32_t 
   rc = rc ? rc : dalloc(ctx, &ctx->peel, npeelsed);
   rc = rc ? rc : dalloc(ctx, &ctx->si_init, (size_t)d.nfreq * 5);
   if (d.imcube && ipeel) {
-    rc = rc ? rc : dalloc(ctx, &ctx->img_i, nimg * kNo);
-    rc = rc ? rc : dalloc(ctx, &ctx->img_q, nimg);
-    rc = rc ? rc : dalloc(ctx, &ctx->img_u, nimg);
-    rc = rc ? rc : dalloc(ctx, &ctx->img_v, nimg);
+    // the FP32 cube (imcube 2) takes half the words of the FP64 one
+    const size_t wi = d.imcube == 2 ? (nimg * kNo + 1) / 2 : nimg * kNo;
+    const size_t wq = d.imcube == 2 ? (nimg + 1) / 2 : nimg;
+    rc = rc ? rc : dalloc(ctx, &ctx->img_i, wi);
+    rc = rc ? rc : dalloc(ctx, &ctx->img_q, wq);
+    rc = rc ? rc : dalloc(ctx, &ctx->img_u, wq);
+    rc = rc ? rc : dalloc(ctx, &ctx->img_v, wq);
   }
   if (rc) return rc;
   if (trace && (size_t)np > ctx->trace_cap) {
@@ -2482,7 +2489,7 @@ int prepare_pass(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int32_t 
   P.nap = d.nap;
   P.npeel = d.npeel;
   P.nxhst = d.nxhst;
-  P.imcube = cube ? 1 : 0;
+  P.imcube = cube ? d.imcube : 0;
   P.iintmax = S.iintmax > 0 ? S.iintmax : 10000;
   P.numin = S.numin;
   P.lnurat = S.lnurat;
@@ -2511,7 +2518,7 @@ int prepare_pass(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int32_t 
   P.sed = ctx->sed;
   P.peel = P.ipeel ? ctx->peel : nullptr;
   P.img_i = cube ? ctx->img_i : nullptr;
-  P.imcube = cube ? 1 : 0;
+  P.imcube = cube ? d.imcube : 0;
   P.img_q = ctx->img_q;
   P.img_u = ctx->img_u;
   P.img_v = ctx->img_v;
@@ -2647,10 +2654,11 @@ int zero_tallies(hc_ctx *ctx, const PassLaunch &L, int64_t np, bool trace) {
   z(ctx->counter, kCounters * sizeof(unsigned long long));
   if (trace) z(ctx->trace, (size_t)np * sizeof(hc_trace));
   if (L.cube) {
-    z(ctx->img_i, L.nimg * kNo * sizeof(double));
-    z(ctx->img_q, L.nimg * sizeof(double));
-    z(ctx->img_u, L.nimg * sizeof(double));
-    z(ctx->img_v, L.nimg * sizeof(double));
+    const size_t w = d.imcube == 2 ? sizeof(float) : sizeof(double);
+    z(ctx->img_i, L.nimg * kNo * w);
+    z(ctx->img_q, L.nimg * w);
+    z(ctx->img_u, L.nimg * w);
+    z(ctx->img_v, L.nimg * w);
   }
   return rc;
 }
@@ -2809,14 +2817,16 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
   if (out->si_init)
     HC_D2H(out->si_init, ctx->si_init, (size_t)d.nfreq * 5 * sizeof(double));
   if (cube) {
+    // FP32 cube (imcube 2): the caller's img arrays are real*4
+    const size_t w = d.imcube == 2 ? sizeof(float) : sizeof(double);
     if (out->img_i)
-      HC_D2H(out->img_i, ctx->img_i, nimg * kNo * sizeof(double));
+      HC_D2H(out->img_i, ctx->img_i, nimg * kNo * w);
     if (out->img_q)
-      HC_D2H(out->img_q, ctx->img_q, nimg * sizeof(double));
+      HC_D2H(out->img_q, ctx->img_q, nimg * w);
     if (out->img_u)
-      HC_D2H(out->img_u, ctx->img_u, nimg * sizeof(double));
+      HC_D2H(out->img_u, ctx->img_u, nimg * w);
     if (out->img_v)
-      HC_D2H(out->img_v, ctx->img_v, nimg * sizeof(double));
+      HC_D2H(out->img_v, ctx->img_v, nimg * w);
   }
   if (trace && np > 0)
     HC_D2H(trace, ctx->trace, (size_t)np * sizeof(hc_trace));

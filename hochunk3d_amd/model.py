@@ -93,6 +93,13 @@ class ModelConfig:
     cplanckst: bool = False
     limb: int = 0                  # CLIMB: 0 isotropic, 1 Eddington limb darkening
     ntab: int = 1000               # nodes of the limb-darkening table
+    # log-normal density multiplier (hochunk3d_amd.hydro.lognormal_field):
+    # {"sigma", "slope", "seed"} and "species" (0-based ids it multiplies,
+    # default the disks and the envelope)
+    lognormal: Optional[dict] = None
+    # DENSINPUT=YES: a densarr.unf whose walls and disk species 1..2 replace
+    # the computed ones (gridsethydro.f90:346-362)
+    densinput: Optional[str] = None
 
 
 @dataclass
@@ -193,6 +200,22 @@ def theta_walls(ntg: int) -> np.ndarray:
     assert np.all(np.diff(th) > 0.0), "theta walls not increasing"
     assert ncell == len(th) - 1
     return th
+
+
+def _tan2(thetarr: np.ndarray) -> np.ndarray:
+    """tan^2 of the theta walls with the poles at 0 and the midplane wall at
+    -1 (gridset.f90:581-601); an even wall count has no midplane wall, its
+    two central walls keep tan^2 of the wall above the midplane."""
+    ntg = thetarr.size
+    tan2arr = np.tan(thetarr) ** 2
+    tan2arr[0] = 0.0
+    tan2arr[-1] = 0.0
+    if ntg % 2 == 1:
+        tan2arr[ntg // 2] = -1.0
+    else:
+        nh = ntg // 2
+        tan2arr[nh] = tan2arr[nh - 1] = np.tan(thetarr[nh]) ** 2
+    return tan2arr
 
 
 def _disk_params(cfg: ModelConfig, autors: float):
@@ -353,17 +376,19 @@ def build_model(cfg: ModelConfig, device: Optional[int] = None) -> Model:
     r2arr = rarr ** 2
 
     thetarr = theta_walls(ntg)
-    tan2arr = np.tan(thetarr) ** 2
-    tan2arr[0] = 0.0
-    tan2arr[-1] = 0.0
-    if ntg % 2 == 1:
-        tan2arr[ntg // 2] = -1.0
-    else:
-        nh = ntg // 2
-        tan2arr[nh] = tan2arr[nh - 1] = np.tan(thetarr[nh]) ** 2
+    tan2arr = _tan2(thetarr)
 
     dp = R2P / float(npg - 1)
     phiarr = dp * np.arange(npg, dtype=float)
+    filedens = None
+    if cfg.densinput:
+        from .hydro import read_densarr_unf
+        frarr, fthet, fphi, filedens = read_densarr_unf(cfg.densinput)
+        if (frarr.size, fthet.size, fphi.size) != (nrg, ntg, npg):
+            raise ValueError("densarr.unf walls do not match the model's wall counts")
+        rarr, thetarr, phiarr = frarr, fthet, fphi
+        r2arr = rarr ** 2
+        tan2arr = _tan2(thetarr)
     aarr = np.sin(phiarr)
     barr = -np.cos(phiarr)
 
@@ -373,6 +398,25 @@ def build_model(cfg: ModelConfig, device: Optional[int] = None) -> Model:
     else:
         densarr, vcell, massarr, fm8, z18, b8 = _densities_device(
             cfg, rarr, thetarr, phiarr, rmincgs, autors, rmind, rmine, rmax, rmaxd, device)
+    if cfg.lognormal:
+        from .hydro import lognormal_field
+        ln = dict(cfg.lognormal)
+        ids = tuple(ln.pop("species", (0, 1, 2)))
+        mult = lognormal_field((nrg - 1, ntg - 1, npg - 1), **ln)
+        for id_ in ids:
+            densarr[:-1, :-1, :-1, id_] *= mult
+        del mult
+    else:
+        ids = ()
+    if filedens is not None:
+        # gridsethydro.f90:361: the disk species come from the file as they are
+        ids = tuple(sorted(set(ids) | {0, 1}))
+        densarr[..., :2] = filedens[..., :2]
+        densarr[-1, :, :, :2] = 0.0
+        densarr[:, -1, :, :2] = 0.0
+        densarr[:, :, -1, :2] = 0.0
+    for id_ in ids:
+        massarr[..., id_] = (densarr[..., id_] * vcell) ** 0.25
     tdust = np.full((nrg, ntg, npg, NDG), 0.1 / 11605.0, order="F")
 
     table = cfg.dust_table if cfg.dust_table is not None else synthetic_dust_table()
@@ -425,6 +469,18 @@ def config1(**over) -> ModelConfig:
     64 x 64 x 32 cells, 200 wavelengths, 10 peel-off inclinations, 101^2
     IQUV images."""
     c = ModelConfig()
+    for k, v in over.items():
+        setattr(c, k, v)
+    return c
+
+
+def config5(**over) -> ModelConfig:
+    """The large hydro-like 3-D model: the disk + envelope + star of config1
+    on 400 x 300 x 256 cells (~31M) with a log-normal density multiplier
+    (sigma_lnrho = 1.5, P(k) ~ k^-11/3, seed 11) on the disks and the
+    envelope; 20 peel-off inclinations, 500^2 IQUV pixels, 200 wavelengths."""
+    c = ModelConfig(nr=400, ntheta=300, nphi=256, npeel=20, nximg=500, nfreq=200,
+                    lognormal={"sigma": 1.5, "slope": -11.0 / 3.0, "seed": 11})
     for k, v in over.items():
         setattr(c, k, v)
     return c

@@ -46,7 +46,10 @@ typedef struct {
   int32_t nfreq, nmu, nph, nap, npeel, nxhst;
   int32_t nlammax, nT, nnu;
   int32_t precision;       /* HC_FP64 or HC_FP32 packet state */
-  int32_t imcube;          /* allocate the (npeel,nx,nx,nfreq,no) image cube */
+  int32_t imcube;          /* the (npeel,nx,nx,nfreq,no) image cube: 0 none, 1 FP64
+                            * accumulators, 2 FP32 accumulators returned in the
+                            * reference's real*4 arrays (hc_tallies img_* then point
+                            * to float) */
 } hc_dims;
 
 /* Source and binning scalars: the tts_mod / spot_mod / stokesvar_mod values
@@ -83,7 +86,9 @@ typedef struct {
  * (nfreq,npeel,nap,8) arrays (peeloff_3d.f90:247-262); img_i is
  * (npeel,nx,nx,nfreq,8), img_q/u/v are (npeel,nx,nx,nfreq)
  * (peeloff_3d.f90:270-277).  All accumulators are FP64 (the reference's
- * SED/image arrays are FP32; the caller may narrow them). */
+ * SED/image arrays are FP32; the caller may narrow them), except the image
+ * cube of a context created with imcube = 2, which accumulates and returns
+ * FP32 (img_i..img_v are then float arrays). */
 typedef struct {
   double *dtauabs, *nabs, *jmean;
   int32_t *killed, *dbig, *dsg;
@@ -213,10 +218,11 @@ int hc_run_lucy(hc_ctx *ctx, int32_t niter, int64_t np, int32_t seed1, int32_t s
  *   "blocks_per_cu" 0..64: workgroups per compute unit, 0 = as many as fit
  *                   (default 0);
  *   "max_work"      1..2^31-1: events + cell crossings after which a packet
- *                   is dropped and counted in n_warn (default 2^24 on the
- *                   FP64 path and 2^21 on the FP32 path, both far above any
- *                   packet of the reference's models, ~3e5; it bounds the
- *                   kernel's run time against runaway packets).
+ *                   is dropped and counted in n_warn (default 2^24, above the
+ *                   longest packets of the benchmark models: ~4e5 on the 8M-
+ *                   cell fractal model, ~3.4e6 in the optically thick clumps
+ *                   of the 31M-cell log-normal model; it bounds the kernel's
+ *                   run time against runaway packets).
  * Apart from max_work, which drops the packets that exceed it, results do
  * not depend on them beyond the order of the floating-point tally sums. */
 int hc_set_option(hc_ctx *ctx, const char *name, int64_t value);
