@@ -1,0 +1,168 @@
+"""Full runs of the 64 x 64 x 32-cell model (200 frequencies, 10 observers,
+101^2 IQUV images): five Lucy iterations of 1e6 packets from the initial
+dust temperatures, then the final passes (newtts.F90:304-349), compared
+within Monte Carlo noise.
+
+The plain FP64 path equals the oracle packet by packet (tests/
+test_gpu_parity.py), so FP64 run A stands for the oracle: a 1e6-packet
+oracle run would take ~9 core-minutes.  FP64 run B on independent packets
+is the control; the FP32 path is the form under test, on independent
+packets (run F) and on A's own packets (run G).
+
+* Temperatures: run A, the FP32 path on A's packets, and the control B each
+  iterate five times from 0.1 K.  The model is axisymmetric, so T(r, theta)
+  is the azimuthal mean over the 32 phi cells (mass-weighted).  Over the
+  (r, theta, species) whose absorbed energy is well sampled, FP32 is within
+  1 % of A in 99 % of them and within 0.5 % mass-weighted (FP32 on A's
+  packets: the precision effect; the independent control shows the Monte
+  Carlo noise of five iterations from 0.1 K).
+* SEDs and images: final passes of 1e6 packets at A's temperatures for all
+  four runs (so only packets and precision differ), each as 10 batches of 1e5
+  packets on independent segments.  Per bin, z = (mean_A - mean_X) /
+  sqrt((s_A^2 + s_X^2) / 10) with the batch variances s^2, which hold the
+  correlation of one packet's several rays in a bin; z follows Student's t
+  with ~18 degrees of freedom (E z^2 = 18/16).  For the peeled SED per
+  (frequency, observer), the exit SED per (frequency, inclination) and the
+  frequency-integrated images in 4 x 4-pixel blocks, the mean z^2 of the
+  control and of F is that of t(18) within five times its sampling error,
+  and FP32 on A's packets (G) differs from A by a small fraction of the
+  noise (mean z^2 below 0.05).  The bins are not independent (one long
+  random walk adds peel rays to many of them), so no test across bins
+  assumes they are."""
+import numpy as np
+import pytest
+
+from hochunk3d_amd import build_model, config1
+
+pytestmark = pytest.mark.gpu
+
+NP = 1_000_000
+NITER = 5
+K = 10                 # batches of the final passes
+NB = NP // K
+
+
+def _iterate(m, precision, seg0):
+    from hochunk3d_amd._lib import Transport
+    g = Transport(m, precision=precision)
+    try:
+        g.run_lucy(NITER, NP, seg0=seg0)
+        return g.get_temperature()
+    finally:
+        g.close()
+
+
+def _blocks(img):
+    b = img[:, :100, :100, :, 0].sum(axis=3)                  # (npeel, 100, 100)
+    return b.reshape(b.shape[0], 25, 4, 25, 4).sum(axis=(2, 4))
+
+
+def _final(m, tdust, precision, seg0):
+    """Per-batch peeled SED (I, counts), image blocks and exit SED of the
+    final passes at temperatures tdust."""
+    from hochunk3d_amd._lib import Transport
+    g = Transport(m, precision=precision)
+    out = {"peel": [], "pcount": [], "img": [], "sed": [], "scount": []}
+    try:
+        g.set_temperature(tdust)
+        for b in range(K):
+            r = g.run(NB, ipeel=1, seg0=seg0 + b * NB)
+            out["peel"].append(r.peel[0, :, :, -1, 0].copy())
+            out["pcount"].append(r.peel[8, :, :, -1, 0].copy())
+            out["img"].append(_blocks(r.img_i))
+            s, _, sed = g.run_sed(NB, seg0=seg0 + (K + b) * NB)
+            out["sed"].append(sed[0, :, :, 0, -1, 0].copy())
+            out["scount"].append(sed[8, :, :, 0, -1, 0].copy())
+    finally:
+        g.close()
+    return {k: np.array(v) for k, v in out.items()}
+
+
+@pytest.fixture(scope="module")
+def runs():
+    import time
+
+    from hochunk3d_amd._lib import HC_FP32, HC_FP64
+    m = build_model(config1())
+    t0 = time.time()
+    T = {"A": _iterate(m, HC_FP64, 0), "F": _iterate(m, HC_FP32, 0),
+         "B": _iterate(m, HC_FP64, 10**9)}
+    print(f"Lucy iterations of three runs: {time.time() - t0:.1f}s", flush=True)
+    t0 = time.time()
+    fin = {"A": _final(m, T["A"], HC_FP64, 10**10), "B": _final(m, T["A"], HC_FP64, 2 * 10**10),
+           "F": _final(m, T["A"], HC_FP32, 3 * 10**10), "G": _final(m, T["A"], HC_FP32, 10**10)}
+    print(f"final passes (4 x {K} batches): {time.time() - t0:.1f}s", flush=True)
+    return m, T, fin
+
+
+def _z(xa, xb, keep):
+    ma, mb = xa.mean(axis=0), xb.mean(axis=0)
+    va, vb = xa.var(axis=0, ddof=1), xb.var(axis=0, ddof=1)
+    ok = keep & (va + vb > 0)
+    return ((ma - mb) / np.sqrt((va + vb) / K))[ok]
+
+
+def _check(name, zb, zf, zg):
+    """zb: A vs the FP64 control B; zf: A vs FP32 on independent packets;
+    zg: A vs FP32 on A's packets."""
+    nu = 2 * (K - 1)
+    ez2 = nu / (nu - 2.0)
+    # the sampling error of the mean z^2 over n independent bins (t(nu) has
+    # Var z^2 = 2 nu^2 (nu - 1) / ((nu - 2)^2 (nu - 4)))
+    sd = np.sqrt(2 * nu * nu * (nu - 1) / ((nu - 2) ** 2 * (nu - 4)))
+    for lab, z in (("control", zb), ("FP32", zf)):
+        m2 = float(np.mean(z * z))
+        tol = 5 * sd / np.sqrt(z.size)
+        print(f"{name} {lab}: mean z^2 {m2:.3f} over {z.size} bins "
+              f"(t({nu}): {ez2:.3f} +- {tol:.3f})")
+        assert z.size > 200
+        assert abs(m2 - ez2) < tol, (name, lab, m2)
+    g2 = float(np.mean(zg * zg))
+    print(f"{name} FP32 on the oracle run's packets: mean z^2 {g2:.4f}")
+    assert g2 < 0.05
+
+
+def _three(f, key, keep):
+    return [_z(f["A"][key], f[o][key], keep) for o in ("B", "F", "G")]
+
+
+def test_peeled_sed_within_monte_carlo_noise(runs):
+    _, _, f = runs
+    _check("peeled SED", *_three(f, "peel", f["A"]["pcount"].sum(axis=0) >= 100))
+
+
+def test_exit_sed_within_monte_carlo_noise(runs):
+    _, _, f = runs
+    _check("exit SED", *_three(f, "sed", f["A"]["scount"].sum(axis=0) >= 100))
+
+
+def test_images_within_monte_carlo_noise(runs):
+    _, _, f = runs
+    _check("images (4x4 blocks)", *_three(f, "img", f["A"]["img"].min(axis=0) > 0))
+
+
+def test_temperatures_within_one_percent(runs):
+    m, T, _ = runs
+    mass = m.densarr[:-1, :-1, :-1, :] * m.vcell[:-1, :-1, :-1, None]
+    msum = mass.sum(axis=2)                                   # (nr, ntheta, species)
+    dusty = msum > 0
+
+    def trt(t):  # mass-weighted azimuthal mean of T
+        return np.where(dusty, (t[:-1, :-1, :-1, :] * mass).sum(axis=2) / np.where(
+            dusty, msum, 1.0), 0.0)
+
+    ta = trt(T["A"])
+    # well sampled: the ring absorbs a fair share of the energy (the
+    # optically thick interior of the inner disk sees few packets)
+    absorbed = ta ** 4 * msum
+    well = dusty & (absorbed > 1e-3 * absorbed.max())
+    assert well.sum() > 500
+    w = msum[well]
+    for name in ("F", "B"):
+        rel = np.abs(trt(T[name])[well] / ta[well] - 1.0)
+        wmean = float(np.sum(w * rel) / np.sum(w))
+        print(f"T(r, theta) A vs {name}: mass-weighted {wmean:.4f}, median {np.median(rel):.4f}, "
+              f"p99 {np.percentile(rel, 99):.4f} over {well.sum()} rings")
+        if name == "F":
+            assert wmean < 0.005
+            assert np.percentile(rel, 99) < 0.01
