@@ -137,9 +137,15 @@ def timed_steps(tr, steps: int, np_: int, seg: int, label: str):
 
 
 def batched_grid(args) -> dict:
-    """Parameter-grid models: the FP32 path with `batch_streams` models in
-    flight against the plain FP64 path one model at a time, the same models,
-    packets and iterations (from each model's initial temperatures)."""
+    """Parameter-grid models, the same models, packets and iterations (from
+    each model's initial temperatures) run four ways:
+      - fp32_batched: the FP32 path, every model's passes in one launch
+        (hc_run_batch, each iteration of the grid one replayed graph);
+      - fp64_batched: the plain FP64 path in the same single launch, which
+        separates the precision's gain from the batching's;
+      - fp32_streams: the FP32 path with `batch_streams` models on their own
+        contexts and streams, one graph-replayed Lucy loop each;
+      - fp64_plain: the plain FP64 path, one model at a time."""
     from hochunk3d_amd._lib import HC_FP32, HC_FP64
     from hochunk3d_amd.batch import build_models, parameter_grid, run_batch
 
@@ -150,37 +156,50 @@ def batched_grid(args) -> dict:
     log(f"batch: {len(models)} grid models, {cells} cells, "
         f"{sum(m.packets for m in models)} packets per iteration, built in "
         f"{time.perf_counter() - t0:.1f}s")
-    b32 = run_batch(models, args.batch_iters, streams=args.batch_streams, precision=HC_FP32,
-                    built=built, sed_last=True, graph=True)
-    log(f"batch fp32, {args.batch_streams} streams: {b32.rate:.3e} packets/s "
-        f"({b32.wall_s:.2f}s)")
-    b64 = run_batch(models, args.batch_iters, streams=1, precision=HC_FP64, built=built,
-                    sed_last=True)
-    log(f"batch fp64, one model at a time: {b64.rate:.3e} packets/s ({b64.wall_s:.2f}s)")
-    errs = [sed_errors(a["sed"], b["sed"]) for a, b in zip(b64.per_model, b32.per_model)]
+    runs = {}
+    runs["fp32_batched"] = run_batch(models, args.batch_iters, precision=HC_FP32, built=built,
+                                     sed_last=True, fused=True)
+    runs["fp64_batched"] = run_batch(models, args.batch_iters, precision=HC_FP64, built=built,
+                                     sed_last=True, fused=True)
+    runs["fp32_streams"] = run_batch(models, args.batch_iters, streams=args.batch_streams,
+                                     precision=HC_FP32, built=built, sed_last=True, graph=True)
+    runs["fp64_plain"] = run_batch(models, args.batch_iters, streams=1, precision=HC_FP64,
+                                   built=built, sed_last=True)
+    for k, r in runs.items():
+        log(f"batch {k}: {r.rate:.3e} packets/s ({r.wall_s:.2f}s)")
+    b64 = runs["fp64_plain"]
+    errs = [sed_errors(a["sed"], b["sed"])
+            for a, b in zip(b64.per_model, runs["fp32_batched"].per_model)]
     lw = np.array([e["luminosity_weighted"] for e in errs])
     inc = np.array([e["per_inclination_max"] for e in errs])
-    return {
+    out = {
         "workload": (f"{len(models)} parameter-grid models (nr 50-250, ntheta 60-200, nphi "
                      f"1-128, {cells} cells in all), {args.batch_scale:g} x their 1e6-2e7 "
                      f"packets = {sum(m.packets for m in models)} packets per Lucy iteration, "
                      f"{args.batch_iters} iterations from the initial temperatures; grids "
                      f"resident in HBM before the timed region"),
-        "fp32_batched": {"value": round(b32.rate, 2), "unit": "packets/s",
-                         "wall_s": round(b32.wall_s, 3), "streams": args.batch_streams,
-                         "lucy_iterations": "replayed from one captured HIP graph per model "
-                                            "(the SED-downloading last one called directly)"},
-        "fp64_plain": {"value": round(b64.rate, 2), "unit": "packets/s",
-                       "wall_s": round(b64.wall_s, 3), "streams": 1},
-        "speedup": round(b64.wall_s / b32.wall_s, 3),
-        "sed_error_last_iteration": {
-            "luminosity_weighted_median": round(float(np.median(lw)), 6),
-            "luminosity_weighted_max": round(float(lw.max()), 6),
-            "per_inclination_max_median": round(float(np.median(inc)), 6),
-            "note": "FP32 vs FP64 SED of each model's last iteration: same packets, "
-                    "temperatures iterated separately; includes the Monte Carlo noise of "
-                    "histories that diverge"},
     }
+    how = {"fp32_batched": "one launch for all models per iteration (hc_run_batch), the "
+                           "iterations replayed from one captured graph",
+           "fp64_batched": "as fp32_batched on the plain FP64 path",
+           "fp32_streams": f"{args.batch_streams} models at a time on their own streams, each "
+                           f"model's Lucy loop replayed from its own graph",
+           "fp64_plain": "one model at a time, one pass per call"}
+    for k, r in runs.items():
+        out[k] = {"value": round(r.rate, 2), "unit": "packets/s", "wall_s": round(r.wall_s, 3),
+                  "how": how[k]}
+    out["speedup_fp32_batched_vs_fp64_plain"] = round(b64.wall_s / runs["fp32_batched"].wall_s, 3)
+    out["speedup_fp64_batched_vs_fp64_plain"] = round(b64.wall_s / runs["fp64_batched"].wall_s, 3)
+    out["speedup_fp32_vs_fp64_both_batched"] = round(runs["fp64_batched"].wall_s
+                                                     / runs["fp32_batched"].wall_s, 3)
+    out["sed_error_last_iteration"] = {
+        "luminosity_weighted_median": round(float(np.median(lw)), 6),
+        "luminosity_weighted_max": round(float(lw.max()), 6),
+        "per_inclination_max_median": round(float(np.median(inc)), 6),
+        "note": "fp32_batched vs fp64_plain SED of each model's last iteration: same packets, "
+                "temperatures iterated separately; includes the Monte Carlo noise of histories "
+                "that diverge"}
+    return out
 
 
 def config5_section(args) -> dict:
@@ -250,9 +269,9 @@ def main():
     ap.add_argument("--autotune-np", type=float, default=-1,
                     help="packets per autotuning probe pass (-1: the step's packet count; "
                          "0: default launch shape)")
-    ap.add_argument("--batch-models", type=int, default=16,
+    ap.add_argument("--batch-models", type=int, default=64,
                     help="parameter-grid models of the batched section (0: skip)")
-    ap.add_argument("--batch-scale", type=float, default=0.05,
+    ap.add_argument("--batch-scale", type=float, default=0.02,
                     help="fraction of each grid model's packet count run per Lucy iteration")
     ap.add_argument("--batch-iters", type=int, default=3)
     ap.add_argument("--batch-streams", type=int, default=4)

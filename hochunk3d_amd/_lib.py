@@ -26,7 +26,7 @@ _ip = C.POINTER(C.c_int32)
 EXPORTS = (
     "hc_create", "hc_set_grid", "hc_set_temperature", "hc_get_temperature", "hc_set_dust",
     "hc_set_sources", "hc_set_atmosphere", "hc_set_limb_table",
-    "hc_run_iteration", "hc_temperature", "hc_run_lucy", "hc_set_option",
+    "hc_run_iteration", "hc_temperature", "hc_run_lucy", "hc_run_batch", "hc_set_option",
     "hc_dropped_packets", "hc_destroy",
     "hc_last_error", "hc_build_density", "hc_build_density_noalloc", "hc_build_emissivity",
     "hc_fractal_counts", "hc_abi_version",
@@ -126,6 +126,11 @@ def load() -> C.CDLL:
     lib.hc_run_lucy.argtypes = [C.c_void_p, C.c_int32, C.c_int64, C.c_int32, C.c_int32,
                                 C.c_int32, C.c_uint64, _dp, C.c_double, C.POINTER(HcTallies),
                                 C.POINTER(C.c_int64)]
+    lib.hc_run_batch.argtypes = [C.c_int32, C.POINTER(C.c_void_p), C.c_int32,
+                                 C.POINTER(C.c_int64), C.c_int32, C.c_int32, C.c_int32,
+                                 C.POINTER(C.c_uint64), C.POINTER(_dp), _dp, C.c_int32,
+                                 C.POINTER(HcTallies), C.POINTER(C.c_int64),
+                                 C.POINTER(C.c_double)]
     lib.hc_set_option.argtypes = [C.c_void_p, C.c_char_p, C.c_int64]
     lib.hc_dropped_packets.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.c_int32,
                                        C.POINTER(C.c_int64)]
@@ -397,6 +402,52 @@ class Transport:
             self.close()
         except Exception:
             pass
+
+
+def run_batch_lucy(transports, niter: int, packets, seg0=None, seed1: int = 12345678,
+                   seed2: int = 87654321, gen: int = 1, oversubscribe: int = 0,
+                   sed: bool = False):
+    """hc_run_batch: niter Lucy iterations of every Transport's model with all
+    passes in one launch per kernel variant.  Returns (per-model list of
+    (scalars, changed cell-species, SED or None), device ms of all
+    iterations); sed downloads each model's last-pass SED
+    [9, nfreq, nmu, nph, nap, 8]."""
+    n = len(transports)
+    lib = load()
+    hs = (C.c_void_p * n)(*[t.h.value for t in transports])
+    npk = (C.c_int64 * n)(*[int(p) for p in packets])
+    sg = (C.c_uint64 * n)(*[int(x) for x in (seg0 if seg0 is not None else [0] * n)])
+    mass_arrays = []
+    mass = (_dp * n)()
+    for i, t in enumerate(transports):
+        if not getattr(t, "_mass_sent", False):
+            a = _f64(t.model.massarr)
+            mass_arrays.append(a)
+            mass[i] = _ptr(a)
+            t._mass_sent = True
+    ltot = np.array([float(t.model.ltot) for t in transports])
+    outs = (HcTallies * n)()
+    seds = []
+    for i, t in enumerate(transports):
+        if sed:
+            cfg = t.model.cfg
+            a = np.zeros((cfg.nfreq, cfg.nmu, cfg.nph, cfg.nap, NO, 9), order="F")
+            outs[i].sed = _ptr(a)
+            seds.append(a)
+        else:
+            seds.append(None)
+    nch = (C.c_int64 * n)()
+    ms = C.c_double(0.0)
+    rc = lib.hc_run_batch(n, hs, int(niter), npk, int(seed1), int(seed2), int(gen), sg, mass,
+                          _ptr(ltot), int(oversubscribe), outs, nch, C.byref(ms))
+    if rc != 0:
+        msg = lib.hc_last_error(transports[0].h)
+        raise HcError(f"hc_run_batch failed ({rc}): {msg.decode() if msg else ''}")
+    res = []
+    for i in range(n):
+        scal = {k: getattr(outs[i], k) for k in SCALARS}
+        res.append((scal, int(nch[i]), None if seds[i] is None else np.moveaxis(seds[i], -1, 0)))
+    return res, ms.value
 
 
 def build_density(fields: dict, nrg: int, ntg: int, npg: int, device: int = 0):

@@ -9,6 +9,12 @@ pass run beside another model's tail.  Every model's results are those of
 running it alone (its packets draw from their own generator segments); only
 the order of the floating-point tally sums may differ.
 
+With fused=True all models' passes run in one launch instead
+(hc_run_batch): workgroups are dealt to the models in proportion to their
+packets, and every iteration of the whole grid is one replayed graph, so the
+models' longest packets overlap each other rather than each ending its own
+pass.
+
 The model parameters follow the parameter-grid configuration of the build's
 benchmark set: radial cells uniform in 50-250, theta cells 60-200, phi cells
 1-128, disk mass log-uniform 1e-4-0.1 Msun, infall rate log-uniform
@@ -109,7 +115,8 @@ def _iterate(tr, packets: int, iterations: int, sed_last: bool, graph: bool):
 def run_batch(models: List[GridModel], iterations: int = 3, streams: int = 4,
               precision: int = 0, options: Optional[dict] = None,
               built: Optional[list] = None, sed_last: bool = False,
-              graph: bool = False) -> BatchResult:
+              graph: bool = False, fused: bool = False,
+              oversubscribe: int = 0) -> BatchResult:
     """Lucy iterations of every model from its initial temperatures.
 
     Every model is built (unless `built` holds build_models' result) and
@@ -119,8 +126,10 @@ def run_batch(models: List[GridModel], iterations: int = 3, streams: int = 4,
     iterations only.  sed_last downloads the SED of every model's last
     iteration (per_model[i]["sed"]).  graph runs the iterations through
     hc_run_lucy (one captured HIP graph per model, replayed); the scalars
-    then come from the last replay only."""
-    from ._lib import Transport
+    then come from the last replay only.  fused runs every model's passes in
+    one launch (hc_run_batch: one captured graph for all models, replayed;
+    `streams` is then unused)."""
+    from ._lib import Transport, run_batch_lucy
     t0 = time.perf_counter()
     if built is None:
         built = build_models(models)
@@ -131,6 +140,15 @@ def run_batch(models: List[GridModel], iterations: int = 3, streams: int = 4,
     setup = time.perf_counter() - t0
     order = sorted(range(len(models)), key=lambda i: -models[i].packets)
     try:
+        if fused:
+            t1 = time.perf_counter()
+            res, ms = run_batch_lucy(trs, iterations, [m.packets for m in models],
+                                     oversubscribe=oversubscribe, sed=sed_last)
+            wall = time.perf_counter() - t1
+            per = [{"packets": models[i].packets * iterations, "kernel_ms": ms,
+                    "scalars": [res[i][0]], "sed": res[i][2],
+                    "tdust": trs[i].get_temperature()} for i in range(len(models))]
+            return BatchResult(sum(p["packets"] for p in per), wall, setup, ms, per)
         t1 = time.perf_counter()
         with cf.ThreadPoolExecutor(max_workers=max(1, streams)) as pool:
             futs = {i: pool.submit(_iterate, trs[i], models[i].packets, iterations, sed_last,

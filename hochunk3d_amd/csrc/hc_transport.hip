@@ -107,10 +107,27 @@ struct Params {
 // a function the compiler emits as a call does not receive the segment's
 // address (the builtin yields a null pointer there), so every function marked
 // noinline takes what it needs as arguments.
-static_assert(sizeof(Params) <= 4096, "launch parameters exceed the kernel-argument limit");
+//
+// A batched launch runs the passes of several models in one grid: the
+// kernel's argument then also points at a device table of the models'
+// Params and at the model of each workgroup, and every wave reads its
+// workgroup's entry, again with scalar loads (the table is read through the
+// constant address space, which the kernel never writes).
+struct KernArgs {
+  Params p;                     // single-model launch
+  const Params *table;          // batched launch: the models' parameters, else null
+  const uint32_t *block_model;  // batched launch: model of each workgroup
+};
+static_assert(sizeof(KernArgs) <= 4096, "launch parameters exceed the kernel-argument limit");
+typedef const __attribute__((address_space(4))) Params ConstParams;
+typedef const __attribute__((address_space(4))) uint32_t ConstU32;
 __device__ __forceinline__ const Params &kernel_params() {
 #if defined(__HIP_DEVICE_COMPILE__)
-  return *reinterpret_cast<const Params *>(__builtin_amdgcn_kernarg_segment_ptr());
+  const KernArgs &ka = *reinterpret_cast<const KernArgs *>(__builtin_amdgcn_kernarg_segment_ptr());
+  if (ka.table == nullptr) return ka.p;
+  ConstU32 *bm = (ConstU32 *)ka.block_model;
+  ConstParams *t = (ConstParams *)ka.table + bm[blockIdx.x];
+  return *(const Params *)t;
 #else
   __builtin_unreachable();  // the host pass never runs device code
 #endif
@@ -1539,7 +1556,7 @@ constexpr int kBlockLucy = HC_BLOCK_LUCY;
 
 template <class R, int NA, bool TRACE, bool PEEL>
 __global__ __launch_bounds__(PEEL ? kBlockPeel : kBlockLucy, HC_WAVES_PER_EU) void transport(
-    Params /* read through kernel_params() */) {
+    Params, const Params *, const uint32_t * /* read through kernel_params() */) {
   const Params &P = c_P;
   // packet p draws from generator segment seg_base + p
   const unsigned long long seg_base = P.seg_dev ? *P.seg_dev : P.seg0;
@@ -2352,8 +2369,9 @@ namespace {
 
 // One transport pass, ready to launch: kernel, parameters, launch shape and
 // the sizes of its tallies.
+typedef void (*TransportKernel)(Params, const Params *, const uint32_t *);
 struct PassLaunch {
-  void (*kern)(Params) = nullptr;
+  TransportKernel kern = nullptr;
   Params P;
   size_t shm = 0;
   unsigned blocks = 0;
@@ -2549,7 +2567,7 @@ int prepare_pass(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int32_t 
   P.limb_tab = ctx->limb_tab;
   P.delmu = ctx->ntab > 0 ? 1.0 / (double)ctx->ntab : 0.0;
 
-  void (*&kern)(Params) = L.kern;
+  TransportKernel &kern = L.kern;
   size_t rec_bytes = 0;
   const bool fp32 = d.precision == HC_FP32;
   const bool pk = P.ipeel != 0;
@@ -2760,32 +2778,15 @@ int queue_temperature(hc_ctx *ctx, const TParams &T, bool apply) {
   return rc;
 }
 
-}  // namespace
-
-extern "C" {
-
-int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int32_t seed2,
-                     int32_t gen, uint64_t seg0, hc_tallies *out, hc_trace *trace) {
-  if (!ctx || !out) return HC_E_ARG;
-  PassLaunch L;
-  int rc = prepare_pass(ctx, np, ipeel, seed1, seed2, gen, seg0, trace != nullptr, L);
-  if (!rc) rc = zero_tallies(ctx, L, np, trace != nullptr);
-  if (rc) return rc;
+// The scalars, the longest packet and the arrays out points at from the last
+// pass of the context (L as prepare_pass laid it out).
+int download_pass(hc_ctx *ctx, const PassLaunch &L, int64_t np, hc_tallies *out,
+                  hc_trace *trace) {
   const hc_dims &d = ctx->dims;
   const size_t nc = L.nc, nsed = L.nsed, npeelsed = L.npeelsed, nimg = L.nimg;
   const bool cube = L.cube;
-  hipStream_t s = ctx->stream;
-  HCK(hipEventRecord(ctx->ev0, s));
-  if (np > 0) hipLaunchKernelGGL(L.kern, dim3(L.blocks), dim3(L.block), L.shm, s, L.P);
-  HCK(hipGetLastError());
-  HCK(hipEventRecord(ctx->ev1, s));
-  HCK(hipStreamSynchronize(s));
-  float ms = 0.f;
-  HCK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-  out->kernel_ms = ms;
+  int rc = download_scalars(ctx, out);
 
-  // download
-  rc = download_scalars(ctx, out);
   if (rc) return rc;
   unsigned long long counters[2] = {0, 0};
   HC_D2H(counters, ctx->counter, sizeof counters);
@@ -2833,6 +2834,29 @@ int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int3
   return HC_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int hc_run_iteration(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int32_t seed2,
+                     int32_t gen, uint64_t seg0, hc_tallies *out, hc_trace *trace) {
+  if (!ctx || !out) return HC_E_ARG;
+  PassLaunch L;
+  int rc = prepare_pass(ctx, np, ipeel, seed1, seed2, gen, seg0, trace != nullptr, L);
+  if (!rc) rc = zero_tallies(ctx, L, np, trace != nullptr);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  HCK(hipEventRecord(ctx->ev0, s));
+  if (np > 0) hipLaunchKernelGGL(L.kern, dim3(L.blocks), dim3(L.block), L.shm, s, L.P,
+                       (const Params *)nullptr, (const uint32_t *)nullptr);
+  HCK(hipGetLastError());
+  HCK(hipEventRecord(ctx->ev1, s));
+  HCK(hipStreamSynchronize(s));
+  float ms = 0.f;
+  HCK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  out->kernel_ms = ms;
+  return download_pass(ctx, L, np, out, trace);
+}
 int hc_temperature(hc_ctx *ctx, const double *massarr, double ltot, double nphot, double *tdust2,
                    int64_t *nchanged) {
   if (!ctx) return HC_E_ARG;
@@ -2877,7 +2901,8 @@ int hc_run_lucy(hc_ctx *ctx, int32_t niter, int64_t np, int32_t seed1, int32_t s
   HCK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   int crc = zero_tallies(ctx, L, np, false);
   if (!crc && np > 0) {
-    hipLaunchKernelGGL(L.kern, dim3(L.blocks), dim3(L.block), L.shm, s, L.P);
+    hipLaunchKernelGGL(L.kern, dim3(L.blocks), dim3(L.block), L.shm, s, L.P,
+                       (const Params *)nullptr, (const uint32_t *)nullptr);
     if (hipGetLastError() != hipSuccess) crc = fail(ctx, HC_E_HIP, "hc_run_lucy: pass launch");
   }
   if (!crc) crc = queue_temperature(ctx, T, true);
@@ -2913,6 +2938,214 @@ int hc_run_lucy(hc_ctx *ctx, int32_t niter, int64_t np, int32_t seed1, int32_t s
   HC_D2H(counters, ctx->counter, sizeof counters);
   if (nchanged) *nchanged = (int64_t)counters[0];
   out->max_packet_work = (double)counters[1];
+  return HC_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Contexts whose stream is the batch's while its graph is captured (their
+// zeroing and temperature kernels then join the capture); restored on exit.
+struct StreamSwap {
+  std::vector<hc_ctx *> ctx;
+  std::vector<hipStream_t> saved;
+  void swap_to(hc_ctx *c, hipStream_t s) {
+    ctx.push_back(c);
+    saved.push_back(c->stream);
+    c->stream = s;
+  }
+  ~StreamSwap() {
+    for (size_t i = 0; i < ctx.size(); ++i) ctx[i]->stream = saved[i];
+  }
+};
+
+struct DeviceBuffer {
+  void *p = nullptr;
+  ~DeviceBuffer() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int hc_run_batch(int32_t nmodels, hc_ctx *const *ctxs, int32_t niter, const int64_t *np,
+                 int32_t seed1, int32_t seed2, int32_t gen, const uint64_t *seg0,
+                 const double *const *massarr, const double *ltot, int32_t oversubscribe,
+                 hc_tallies *out, int64_t *nchanged, double *device_ms) {
+  if (nmodels < 1 || !ctxs || !np || !seg0 || !ltot || !out) return HC_E_ARG;
+  for (int m = 0; m < nmodels; ++m) {
+    if (!ctxs[m]) return HC_E_ARG;
+    for (int k = 0; k < m; ++k)
+      if (ctxs[k] == ctxs[m]) return fail(ctxs[0], HC_E_ARG, "hc_run_batch: a context twice");
+  }
+  hc_ctx *ctx = ctxs[0];  // errors are reported here
+  if (niter < 1) return fail(ctx, HC_E_ARG, "hc_run_batch: niter must be >= 1");
+  if (oversubscribe < 0 || oversubscribe > 64)
+    return fail(ctx, HC_E_ARG, "hc_run_batch: oversubscribe must be in 0..64");
+  const int over = oversubscribe > 0 ? oversubscribe : 4;
+  const int n = nmodels;
+  std::vector<PassLaunch> L(n);
+  std::vector<TParams> T(n);
+  for (int m = 0; m < n; ++m) {
+    hc_ctx *c = ctxs[m];
+    if (c->device != ctx->device)
+      return fail(ctx, HC_E_ARG, "hc_run_batch: contexts on different devices");
+    int rc = prepare_pass(c, np[m], 0, seed1, seed2, gen, seg0[m], false, L[m]);
+    if (!rc) rc = prepare_temperature(c, massarr ? massarr[m] : nullptr, ltot[m],
+                                      (double)np[m], T[m]);
+    if (!rc) rc = dalloc(c, &c->segdev, 1);
+    if (rc) return fail(ctx, rc, "hc_run_batch: model " + std::to_string(m) + ": " + c->err);
+    L[m].P.seg_dev = c->segdev;
+    const unsigned long long sh = seg0[m];
+    {
+      hc_ctx *ctx = c;  // HC_H2D reports on this context
+      HC_H2D(c->segdev, &sh, sizeof sh);
+    }
+  }
+  HCK(hipSetDevice(ctx->device));
+  // one launch per kernel variant (precision, species count) over its models
+  std::vector<TransportKernel> kern;
+  std::vector<std::vector<int>> members;
+  for (int m = 0; m < n; ++m) {
+    if (np[m] <= 0) continue;
+    size_t g = 0;
+    while (g < kern.size() && kern[g] != L[m].kern) ++g;
+    if (g == kern.size()) {
+      kern.push_back(L[m].kern);
+      members.emplace_back();
+    }
+    members[g].push_back(m);
+  }
+  int dev_cus = 0;
+  HCK(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  struct Group {
+    size_t shm = 0;
+    int block = 0;
+    size_t table_off = 0, map_off = 0;
+    std::vector<uint32_t> map;
+  };
+  std::vector<Group> grp(kern.size());
+  size_t bytes = 0;
+  for (size_t g = 0; g < kern.size(); ++g) {
+    Group &G = grp[g];
+    G.block = L[members[g][0]].block;
+    long long npk = 0;
+    for (int m : members[g]) {
+      G.shm = std::max(G.shm, L[m].shm);
+      npk += np[m];
+    }
+    if (G.shm > 64 * 1024)
+      HCK(hipFuncSetAttribute(reinterpret_cast<const void *>(kern[g]),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)G.shm));
+    int bpm = 0;
+    HCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpm, kern[g], G.block, G.shm));
+    const long long resident = (long long)std::max(dev_cus, 1) * std::max(bpm, 1);
+    // workgroups per model in proportion to its packets, `over` times the
+    // workgroups the device holds at once: a workgroup runs its model's
+    // packets until that model has none left, and the dispatcher then starts
+    // the next ones, which join the models still running (or end at once)
+    const double target = (double)over * (double)resident;
+    std::vector<std::pair<double, uint32_t>> order;
+    for (size_t k = 0; k < members[g].size(); ++k) {
+      const long long p = np[members[g][k]];
+      const long long cap = (p + G.block - 1) / G.block;
+      const long long w = std::max(1LL, std::min(cap, (long long)std::llround(
+                                                          target * (double)p / (double)npk)));
+      // each model's workgroups spread evenly over the dispatch order
+      for (long long j = 0; j < w; ++j) order.push_back({((double)j + 0.5) / (double)w, (uint32_t)k});
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [](const std::pair<double, uint32_t> &a, const std::pair<double, uint32_t> &b) {
+                       return a.first < b.first;
+                     });
+    G.map.resize(order.size());
+    for (size_t i = 0; i < order.size(); ++i) G.map[i] = order[i].second;
+    bytes = (bytes + 255) / 256 * 256;
+    G.table_off = bytes;
+    bytes += members[g].size() * sizeof(Params);
+    bytes = (bytes + 255) / 256 * 256;
+    G.map_off = bytes;
+    bytes += G.map.size() * sizeof(uint32_t);
+  }
+  DeviceBuffer buf;
+  hipStream_t s = ctx->stream;
+  if (bytes > 0) {
+    HCK(hipMalloc(&buf.p, bytes));
+    std::vector<unsigned char> host(bytes, 0);
+    for (size_t g = 0; g < kern.size(); ++g) {
+      Params *t = reinterpret_cast<Params *>(host.data() + grp[g].table_off);
+      for (size_t k = 0; k < members[g].size(); ++k) t[k] = L[members[g][k]].P;
+      std::memcpy(host.data() + grp[g].map_off, grp[g].map.data(),
+                  grp[g].map.size() * sizeof(uint32_t));
+    }
+    HC_H2D(buf.p, host.data(), bytes);
+  }
+  for (int m = 0; m < n; ++m) HCK(hipStreamSynchronize(ctxs[m]->stream));
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipError_t e = hipSuccess;
+  {
+    StreamSwap swap;
+    for (int m = 0; m < n; ++m)
+      if (ctxs[m] != ctx) swap.swap_to(ctxs[m], s);
+    // one Lucy iteration of every model as a graph: the tallies zeroed, the
+    // passes, the temperature updates applied on the device, the next
+    // segment bases
+    HCK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int crc = 0;
+    for (int m = 0; m < n && !crc; ++m) crc = zero_tallies(ctxs[m], L[m], np[m], false);
+    unsigned char *base = static_cast<unsigned char *>(buf.p);
+    for (size_t g = 0; g < kern.size() && !crc; ++g) {
+      const Group &G = grp[g];
+      hipLaunchKernelGGL(kern[g], dim3((unsigned)G.map.size()), dim3(G.block), G.shm, s,
+                         L[members[g][0]].P, reinterpret_cast<const Params *>(base + G.table_off),
+                         reinterpret_cast<const uint32_t *>(base + G.map_off));
+      if (hipGetLastError() != hipSuccess) crc = fail(ctx, HC_E_HIP, "hc_run_batch: pass launch");
+    }
+    for (int m = 0; m < n && !crc; ++m) crc = queue_temperature(ctxs[m], T[m], true);
+    for (int m = 0; m < n && !crc; ++m) {
+      hipLaunchKernelGGL(advance_segment, dim3(1), dim3(64), 0, s, ctxs[m]->segdev,
+                         (unsigned long long)std::max<int64_t>(np[m], 0));
+      if (hipGetLastError() != hipSuccess) crc = fail(ctx, HC_E_HIP, "hc_run_batch: launch");
+    }
+    const hipError_t ce = hipStreamEndCapture(s, &graph);
+    if (crc || ce != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      (void)hipGetLastError();
+      if (crc && ctxs[0]->err.empty()) ctx->err = "hc_run_batch: capture";
+      return crc ? crc : fail(ctx, HC_E_HIP, std::string("hc_run_batch: capture: ") +
+                                                 hipGetErrorString(ce));
+    }
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev0, s);
+    for (int i = 0; i < niter && e == hipSuccess; ++i) e = hipGraphLaunch(exec, s);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev1, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess)
+    return fail(ctx, HC_E_HIP, std::string("hc_run_batch: replay: ") + hipGetErrorString(e));
+  float ms = 0.f;
+  HCK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  if (device_ms) *device_ms = ms;
+  for (int m = 0; m < n; ++m) {
+    hc_ctx *c = ctxs[m];
+    out[m].kernel_ms = ms;
+    const int rc = download_pass(c, L[m], np[m], &out[m], nullptr);
+    if (rc) return fail(ctx, rc, "hc_run_batch: model " + std::to_string(m) + ": " + c->err);
+    if (nchanged) {
+      unsigned long long nch = 0;
+      hipError_t de = hipMemcpyAsync(&nch, c->counter, sizeof nch, hipMemcpyDeviceToHost,
+                                     c->stream);
+      if (de == hipSuccess) de = hipStreamSynchronize(c->stream);
+      if (de != hipSuccess) return fail(ctx, HC_E_HIP, "hc_run_batch: download");
+      nchanged[m] = (int64_t)nch;
+    }
+  }
   return HC_OK;
 }
 

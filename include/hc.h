@@ -199,6 +199,37 @@ int hc_run_lucy(hc_ctx *ctx, int32_t niter, int64_t np, int32_t seed1, int32_t s
                 int32_t gen, uint64_t seg0, const double *massarr, double ltot,
                 hc_tallies *out, int64_t *nchanged);
 
+/* niter Lucy iterations of nmodels independent models (a parameter grid)
+ * with the passes of all models in one launch per kernel variant (FP64 or
+ * FP32, up to 4 or up to 8 active species): hc_run_lucy for many contexts at
+ * once.  No counterpart in the reference, which runs one model per process;
+ * a grid driver calls it in place of a loop of hc_run_lucy calls.
+ *   ctxs[m]        distinct contexts on one device, grid, dust and sources set;
+ *   np[m], seg0[m] packets and first draw segment of model m (iteration i
+ *                  draws from seg0[m] + i*np[m] + p, as hc_run_lucy);
+ *   massarr[m]     as in hc_temperature (massarr itself may be NULL when
+ *                  every context already has it);
+ *   ltot[m]        the model's luminosity (hc_temperature);
+ *   oversubscribe  workgroups launched per workgroup slot of the device,
+ *                  0 = 4: each model gets workgroups in proportion to its
+ *                  packets; a workgroup runs its model's packets until none
+ *                  are left, and later workgroups join the models still
+ *                  running, so that the models' longest packets overlap.
+ * One iteration of all models (tally zeroing, the passes, every model's
+ * temperature update applied on the device) is captured from ctxs[0]'s
+ * stream as a HIP graph and replayed niter times.  Each model's results are
+ * those of hc_run_lucy on its own context (up to the order of the
+ * floating-point tally sums).  out[m] receives model m's last pass like
+ * hc_run_iteration (scalars, max_packet_work, and the arrays its non-NULL
+ * pointers name), kernel_ms the device time of the whole batch; nchanged[m]
+ * (nullable) the last update's changed (cell, species); device_ms
+ * (nullable) the device time of all iterations.  Calls that overlap on one
+ * device must use disjoint contexts. */
+int hc_run_batch(int32_t nmodels, hc_ctx *const *ctxs, int32_t niter, const int64_t *np,
+                 int32_t seed1, int32_t seed2, int32_t gen, const uint64_t *seg0,
+                 const double *const *massarr, const double *ltot, int32_t oversubscribe,
+                 hc_tallies *out, int64_t *nchanged, double *device_ms);
+
 /* Launch-shape knobs of the transport kernel, set by the autotuner
  * (hochunk3d_amd/autotune.py); no counterpart in the reference, whose packet
  * loop is serial.  Names:

@@ -61,3 +61,72 @@ def test_concurrent_batch_matches_sequential():
         for k in ("ncross", "nscat_ev", "nabs_ev"):
             assert abs(a["scalars"][-1][k] - b["scalars"][-1][k]) <= 1e-5 * a["scalars"][-1][k], k
         assert_temperatures_close(b["tdust"], a["tdust"], False)
+
+
+def test_fused_batch_matches_models_alone():
+    """hc_run_batch: the passes of FP64 and FP32 models in one launch per
+    kernel variant give each model's own results: the first iteration, from
+    the same temperatures, has the same histories (event counts, fluxes and
+    SED up to the tally sum order) as the model alone; the second starts
+    from temperatures equal up to that order."""
+    from hochunk3d_amd._lib import HC_FP32, HC_FP64, Transport, run_batch_lucy
+    models = parameter_grid(5, seed=11, packet_scale=2e-3, max_cells=150_000)
+    built = [build_model(gm.cfg) for gm in models]
+    precs = [HC_FP64, HC_FP32, HC_FP64, HC_FP32, HC_FP64]
+    nps = [gm.packets for gm in models]
+    alone = []
+    for m, p, n in zip(built, precs, nps):
+        t = Transport(m, precision=p)
+        try:
+            s, _, sed = t.run_sed(n, seg0=0)
+            t.temperature(n, apply=True)
+            s2, _ = t.run_scalars(n, seg0=n)
+            t.temperature(n, apply=True)
+            alone.append((s, sed, s2, t.get_temperature()))
+        finally:
+            t.close()
+    trs = [Transport(m, precision=p) for m, p in zip(built, precs)]
+    try:
+        res1, ms1 = run_batch_lucy(trs, 1, nps, sed=True)
+        res2, ms2 = run_batch_lucy(trs, 1, nps, seg0=nps)
+        temps = [t.get_temperature() for t in trs]
+    finally:
+        for t in trs:
+            t.close()
+    assert ms1 > 0 and ms2 > 0
+    for (s, sed, s2, temp), (f, _, fsed), (f2, _, _), ft in zip(alone, res1, res2, temps):
+        for k in ("ncross", "nscat_ev", "nabs_ev", "tot", "n_err_nowall", "n_err_dust", "n_warn",
+                  "max_packet_work"):
+            assert f[k] == s[k], k
+        np.testing.assert_allclose(f["flux"], s["flux"], rtol=1e-12)
+        np.testing.assert_allclose(fsed, sed, rtol=1e-9, atol=1e-12 * np.abs(sed).max())
+        for k in ("ncross", "nscat_ev", "nabs_ev"):
+            assert abs(f2[k] - s2[k]) <= 1e-5 * max(s2[k], 1.0), k
+        assert_temperatures_close(ft, temp, False)
+
+
+def test_fused_batch_graph_replays():
+    """Two iterations replayed from the batch's graph equal two batch calls of
+    one iteration each (the segment bases advance on the device)."""
+    from hochunk3d_amd._lib import Transport, run_batch_lucy
+    models = parameter_grid(3, seed=5, packet_scale=1e-3, max_cells=100_000)
+    built = [build_model(gm.cfg) for gm in models]
+    nps = [gm.packets for gm in models]
+    out = []
+    for split in (False, True):
+        trs = [Transport(m) for m in built]
+        try:
+            if split:
+                run_batch_lucy(trs, 1, nps)
+                res, _ = run_batch_lucy(trs, 1, nps, seg0=nps)
+            else:
+                res, _ = run_batch_lucy(trs, 2, nps, oversubscribe=2)
+            out.append((res, [t.get_temperature() for t in trs]))
+        finally:
+            for t in trs:
+                t.close()
+    (ra, ta), (rb, tb) = out
+    for a, b, x, y in zip(ra, rb, ta, tb):
+        for k in ("ncross", "nscat_ev", "nabs_ev"):
+            assert abs(a[0][k] - b[0][k]) <= 1e-5 * max(b[0][k], 1.0), k
+        assert_temperatures_close(x, y, False)
