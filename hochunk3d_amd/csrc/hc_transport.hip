@@ -83,6 +83,9 @@ struct Params {
   int opac_off[kMaxSpecies];  // table of active species s: offset in doubles
   int emis_base;    // LDS byte offset of lTint[nT] and nuint[nnu]
   int peel_base;    // LDS byte offset of the peel-off queues
+  int pool_base;    // LDS byte offset of the workgroup's event pool (pooled passes)
+  int pool_min;     // requests of one kind a walking wavefront waits for before it runs them
+  int pool_walk;    // below this many walking lanes a wavefront runs any pooled request
   // disk-accretion launch (initpacc.f90, emit_gasacc.f90)
   const double *costarr;  // cos of the theta walls (gridset.f90:580-599)
   double rddustmin, rmaxdmax, rmax, massc, mdot;
@@ -354,7 +357,9 @@ __device__ __forceinline__ void lucy_add(LucyAcc<R, NA> &la, const Pkt<R, NA> &k
 // The cell walk to optical depth tau (tauint reconstruction), split into
 // its entry (the packet on a radial wall steps into the cell its direction
 // points to) and single crossings, so the event loop can interleave lanes.
-enum { ST_WALK = 4, ST_DONE = 6, ST_ABORT = 7, ST_SCAT = 8, ST_ABS = 9, ST_END = 10 };
+enum { ST_WALK = 4, ST_DONE = 6, ST_ABORT = 7, ST_SCAT = 8, ST_ABS = 9, ST_END = 10,
+       ST_RESTART = 11, ST_SCAT_WAIT = 12, ST_ABS_WAIT = 13, ST_SCAT_READY = 14,
+       ST_ABS_READY = 15 };
 
 template <class R, int NA>
 __device__ __forceinline__ int walk_begin(const WallView &W, Pkt<R, NA> &k) {
@@ -1451,6 +1456,7 @@ struct Lane {
   long long pid;
   int iscat, iint, pir, pit, pip;
   uint32_t work;  // events + crossings of the packet so far
+  uint32_t qpos;  // pool position of the packet's pending scattering or absorption
   double tdust;   // T of the absorbing species in the cell (prefetched for re-emission)
   bool first;
 };
@@ -1552,11 +1558,18 @@ __device__ __forceinline__ void flush_peels(const WallView &W, const PeelRec<R, 
 #define HC_BLOCK_LUCY 256  // lanes per workgroup of the passes without peel-off
 #endif
 constexpr int kBlockPeel = 512;  // at most eight wavefront peel queues per workgroup
+// Event pool of the pooled Lucy passes (512-lane workgroups, one per compute
+// unit): a request record of kPoolFields doubles per lane.
+constexpr int kPoolLanes = 512;
+constexpr int kPoolFields = 12;
+constexpr uint32_t kPoolTagMask = 0x7fffffu;  // queue entry: (position & mask) << 9 | lane
+static_assert(kPoolLanes == 512, "queue entries hold 9-bit lane numbers");
 constexpr int kBlockLucy = HC_BLOCK_LUCY;
 
-template <class R, int NA, bool TRACE, bool PEEL>
-__global__ __launch_bounds__(PEEL ? kBlockPeel : kBlockLucy, HC_WAVES_PER_EU) void transport(
+template <class R, int NA, bool TRACE, bool PEEL, bool POOL>
+__global__ __launch_bounds__((PEEL || POOL) ? kBlockPeel : kBlockLucy, HC_WAVES_PER_EU) void transport(
     Params, const Params *, const uint32_t * /* read through kernel_params() */) {
+  static_assert(!(PEEL && POOL), "the event pool shares the LDS with no peel-off queues");
   const Params &P = c_P;
   // packet p draws from generator segment seg_base + p
   const unsigned long long seg_base = P.seg_dev ? *P.seg_dev : P.seg0;
@@ -1611,6 +1624,21 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : kBlockLucy, HC_WAVES_PER_EU) vo
     queue = reinterpret_cast<PeelRec<R, NA> *>(lds + P.peel_base + 64) + (size_t)wave * kQ;
     if ((threadIdx.x & 63) == 0) qcount[wave] = 0;
   }
+  // the workgroup's event pool (pooled passes): request records (kPoolFields
+  // doubles per lane, field-major), two queues of tagged lane numbers
+  // (scattering, absorption), the completed position of each lane's request,
+  // and the queues' head and tail counters
+  double *prec = nullptr;
+  uint32_t *pq = nullptr, *pdone = nullptr, *pctr = nullptr;
+  if constexpr (POOL) {
+    prec = reinterpret_cast<double *>(lds + P.pool_base);
+    pq = reinterpret_cast<uint32_t *>(prec + kPoolFields * kPoolLanes);
+    pdone = pq + 2 * kPoolLanes;
+    pctr = pdone + kPoolLanes;
+    for (int i = threadIdx.x; i < 2 * kPoolLanes; i += blockDim.x) pq[i] = 0xffffffffu;
+    for (int i = threadIdx.x; i < kPoolLanes; i += blockDim.x) pdone[i] = 0xffffffffu;
+    if (threadIdx.x < 4) pctr[threadIdx.x] = 0u;
+  }
   __syncthreads();
   auto push = [&](const Pkt<R, NA> &k, int sp, bool direct, long long pid,
                   const R *nrm) __attribute__((always_inline)) {
@@ -1630,6 +1658,7 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : kBlockLucy, HC_WAVES_PER_EU) vo
 #pragma unroll
   for (int s2 = 0; s2 < NA; ++s2) dm[s2] = 0.0;
   int st = ST_END, status = -1, slot = 0;  // status < 0: no packet yet
+  bool reload = false;  // the restarting walk enters a new cell (launch, star)
   const int ev_min = P.regen_lanes;
   const int ev_run = min(P.regen_run_lanes, ev_min);
 
@@ -1675,7 +1704,7 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : kBlockLucy, HC_WAVES_PER_EU) vo
     for (int s2 = 0; s2 < NA; ++s2)
       if (s2 == slot) alb_slot = k.alb[s2];
     if (ran(L.rng) <= (double)alb_slot) return (int)ST_SCAT;
-    L.tdust = P.tdust[cell * (size_t)P.nact + slot];  // used when the absorption runs
+    if constexpr (!POOL) L.tdust = P.tdust[cell * (size_t)P.nact + slot];  // used when the absorption runs
     return (int)ST_ABS;
   };
   // the walk after an event: new optical depth, entry into the cell
@@ -1693,9 +1722,321 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : kBlockLucy, HC_WAVES_PER_EU) vo
     return s;
   };
 
+
+  // ---- workgroup event pool (pooled passes) ----
+  // A lane whose walk reaches a scattering or an absorption hands the part of
+  // the event that only needs the packet's direction, Stokes vector,
+  // frequency and generator state to the workgroup: it writes a request
+  // record into its LDS slot and queues its lane number.  A wavefront takes
+  // up to 64 queued requests of one kind at once and runs them on its own
+  // lanes, so the expensive event code (the Stokes rejection loop, the
+  // re-emission frequency) runs for many packets together although each
+  // wavefront only has a few of them.  The owner lane picks up the result
+  // and finishes the event itself.  A packet's draws stay in their order:
+  // the owner draws nothing while its request is out.
+  const int tid = (int)threadIdx.x;
+  const int wlane = (int)__lane_id();
+  const unsigned long long below = wlane == 0 ? 0ull : (~0ull >> (64 - wlane));
+  auto rec = [&](int f, int lane_no) -> double & { return prec[f * kPoolLanes + lane_no]; };
+  auto put_rng = [&](int lane_no, const Mrg &g) __attribute__((always_inline)) {
+    rec(11, lane_no) = __builtin_bit_cast(double, ((unsigned long long)(uint32_t)g.s2 << 32) |
+                                                      (uint32_t)g.s1);
+  };
+  auto get_rng = [&](int lane_no) __attribute__((always_inline)) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, rec(11, lane_no));
+    return Mrg{(int32_t)(uint32_t)(b & 0xffffffffull), (int32_t)(uint32_t)(b >> 32)};
+  };
+  // requests of kind q (0 scattering, 1 absorption) queued and not yet
+  // taken: the head is read before the tail, and both only grow, so the
+  // difference is a count of requests that exist (at most one per lane)
+  auto pool_avail = [&](int q) __attribute__((always_inline)) {
+    const uint32_t h = __hip_atomic_load(&pctr[q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    const uint32_t t = __hip_atomic_load(&pctr[2 + q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    const uint32_t a = t - h;
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)(a > (uint32_t)kPoolLanes ? 0u : a));
+  };
+  // the wavefront's lanes that reached a scattering or absorption: the
+  // owner's part of the event, the request record, the queue entry
+  auto pool_enqueue_kind = [&](const int q) __attribute__((always_inline)) {
+    Pkt<R, NA> &k = L.k;
+    {
+      const int want = q == 0 ? (int)ST_SCAT : (int)ST_ABS;
+      const unsigned long long m = __ballot(st == want);
+      if (m == 0ull) return;
+      if (st == want) {
+        if (q == 0) {
+          k.iflag = 1;
+          ++L.iscat;
+          ++L.iint;
+          L.work++;
+          acc.nscat_ev++;
+          if constexpr (TRACE) tr.nscat++;
+          rec(0, tid) = (double)k.d.cost;
+          rec(1, tid) = (double)k.d.sint;
+          rec(2, tid) = (double)k.d.cosp;
+          rec(3, tid) = (double)k.d.sinp;
+          rec(4, tid) = (double)k.d.phi;
+          rec(5, tid) = (double)k.s.I;
+          rec(6, tid) = (double)k.s.Q;
+          rec(7, tid) = (double)k.s.U;
+          rec(8, tid) = (double)k.s.V;
+          rec(9, tid) = (double)k.nub;
+          rec(10, tid) = (double)slot;
+        } else {
+          const int idust2 = P.act[slot] - 1;
+          const size_t cell = cell_of(k.ir, k.it, k.ip);
+          L.work++;
+          lucy_open(L.la, cell);
+          L.la.nabs_any += 1;
+          // every slot adds (0 or 1): an indexed add would leave the lane's
+          // state in scratch memory
+#pragma unroll
+          for (int s2 = 0; s2 < NA; ++s2) L.la.nabs[s2] += s2 == slot ? 1.0 : 0.0;
+          k.i_orig = (idust2 == 0 || idust2 == 1 || idust2 == 4 || idust2 == 5) ? 2 : 3;
+          k.s = Stokes4<R>{R(1.0), R(0.0), R(0.0), R(0.0)};
+          ++L.iint;
+          acc.nabs_ev++;
+          if constexpr (TRACE) tr.nabs++;
+          rec(0, tid) = (double)(cell * (size_t)P.nact + (size_t)slot);  // the tdust word
+          rec(1, tid) = (double)idust2;
+        }
+        put_rng(tid, L.rng);
+      }
+      const int leader = __ffsll((long long)m) - 1;
+      uint32_t base = 0;
+      if (wlane == leader) base = atomicAdd(&pctr[2 + q], (uint32_t)__popcll(m));
+      base = (uint32_t)__shfl((int)base, leader);
+      if (st == want) {
+        const uint32_t pos = base + (uint32_t)__popcll(m & below);
+        // the record is complete before the entry that hands it out
+        __hip_atomic_store(&pq[q * kPoolLanes + (pos & (kPoolLanes - 1))],
+                           ((pos & kPoolTagMask) << 9) | (uint32_t)tid, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+        L.qpos = pos;
+        st = q == 0 ? (int)ST_SCAT_WAIT : (int)ST_ABS_WAIT;
+      }
+    }
+  };
+  auto pool_enqueue = [&]() __attribute__((always_inline)) {
+    pool_enqueue_kind(0);
+    pool_enqueue_kind(1);
+  };
+  // take up to 64 requests of kind q (at least pool_min unless eager) and
+  // run them on this wavefront's lanes
+  auto pool_consume = [&](int q, bool eager) __attribute__((always_inline)) {
+    uint32_t h = 0, n = 0;
+    if (wlane == 0) {
+      // claim [h, h + n) with n at most the requests reserved so far (the
+      // tail read after the head), so that every claimed entry is written
+      h = __hip_atomic_load(&pctr[q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      for (;;) {
+        const uint32_t t =
+            __hip_atomic_load(&pctr[2 + q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint32_t a = t - h;
+        if (a == 0u || a > (uint32_t)kPoolLanes || (!eager && a < (uint32_t)P.pool_min)) {
+          n = 0;
+          break;
+        }
+        n = a < 64u ? a : 64u;
+        if (__hip_atomic_compare_exchange_strong(&pctr[q], &h, h + n, __ATOMIC_ACQUIRE,
+                                                 __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP))
+          break;
+      }
+    }
+    n = (uint32_t)__shfl((int)n, 0);
+    if (n == 0u) return false;
+    h = (uint32_t)__shfl((int)h, 0);
+    if ((uint32_t)wlane < n) {
+      const uint32_t pos = h + (uint32_t)wlane;
+      // the producer reserved this position before writing its entry
+      uint32_t e;
+      do {
+        e = __hip_atomic_load(&pq[q * kPoolLanes + (pos & (kPoolLanes - 1))], __ATOMIC_ACQUIRE,
+                              __HIP_MEMORY_SCOPE_WORKGROUP);
+      } while ((e >> 9) != (pos & kPoolTagMask));
+      const int o = (int)(e & (uint32_t)(kPoolLanes - 1));
+      Mrg g = get_rng(o);
+      if (q == 0) {
+        Dir<R> d{R(rec(0, o)), R(rec(1, o)), R(rec(2, o)), R(rec(3, o)), R(rec(4, o))};
+        Stokes4<R> sv{R(rec(5, o)), R(rec(6, o)), R(rec(7, o)), R(rec(8, o))};
+        const HG<R> hg = hg_of<R>((int)rec(10, o), R(rec(9, o)));
+        stokes<R>(d, sv, hg, g, acc.htot, acc.hit);
+        rec(0, o) = (double)d.cost;
+        rec(1, o) = (double)d.sint;
+        rec(2, o) = (double)d.cosp;
+        rec(3, o) = (double)d.sinp;
+        rec(4, o) = (double)d.phi;
+        rec(5, o) = (double)sv.I;
+        rec(6, o) = (double)sv.Q;
+        rec(7, o) = (double)sv.U;
+        rec(8, o) = (double)sv.V;
+      } else {
+        const double tdu = P.tdust[(size_t)rec(0, o)];
+        const int idust2 = (int)rec(1, o);
+        // the draws in their order (direction, direction, frequency)
+        const R u1 = R(ran(g));
+        const R u2 = R(ran(g));
+        const LdfState<R> ldf = ldf_begin<R>(tdu, idust2, g);
+        Dir<R> d;
+        d.cost = R(1.0) - R(2.0) * u1;
+        d.sint = m_sqrt(R(1.0) - m_pow(d.cost, R(2.0)));
+        d.phi = R(kR2P) * u2;
+        d.cosp = m_cos(d.phi);
+        d.sinp = m_sin(d.phi);
+        rec(9, o) = (double)ldf_end<R>(ldf);
+        rec(0, o) = (double)d.cost;
+        rec(1, o) = (double)d.sint;
+        rec(2, o) = (double)d.cosp;
+        rec(3, o) = (double)d.sinp;
+        rec(4, o) = (double)d.phi;
+      }
+      put_rng(o, g);
+      __hip_atomic_store(&pdone[o], pos, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    return true;
+  };
+  // lanes whose request has been run
+  auto pool_poll = [&]() __attribute__((always_inline)) {
+    if (st == ST_SCAT_WAIT || st == ST_ABS_WAIT) {
+      if (__hip_atomic_load(&pdone[tid], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) ==
+          L.qpos)
+        st = st == ST_SCAT_WAIT ? (int)ST_SCAT_READY : (int)ST_ABS_READY;
+    }
+  };
+  // the owner's part after the pooled event
+  auto pool_pickup = [&]() __attribute__((always_inline)) {
+    Pkt<R, NA> &k = L.k;
+    if (st == ST_SCAT_READY || st == ST_ABS_READY) {
+      const bool sc = st == ST_SCAT_READY;
+      k.d = Dir<R>{R(rec(0, tid)), R(rec(1, tid)), R(rec(2, tid)), R(rec(3, tid)), R(rec(4, tid))};
+      L.rng = get_rng(tid);
+      if (sc) {
+        k.s = Stokes4<R>{R(rec(5, tid)), R(rec(6, tid)), R(rec(7, tid)), R(rec(8, tid))};
+        set_walk_dir(k);
+      } else {
+        k.nub = R(rec(9, tid));
+        set_walk_dir(k);
+        k.iflag = 0;
+        set_opacity(k, false);
+      }
+      L.pir = k.ir;
+      L.pit = k.it;
+      L.pip = k.ip;
+      st = ST_RESTART;
+      reload = false;
+    }
+  };
+  // star re-emission (the packet hit the stellar surface)
+  auto ev_star = [&]() __attribute__((always_inline)) {
+    Pkt<R, NA> &k = L.k;
+    L.work++;
+    star_reemit(k, L.rng, acc, nrm);
+    if constexpr (TRACE) tr.nstar++;
+    push(k, 0, true, L.pid, nrm);
+    st = ST_RESTART;
+    reload = true;
+  };
+  // the end of a packet and the launch of the next
+  auto ev_end = [&]() __attribute__((always_inline)) {
+    Pkt<R, NA> &k = L.k;
+    if (status >= 0) {  // the packet ends (0 escaped, 1 killed, 2 lost)
+      acc.maxwork = max(acc.maxwork, L.work);
+      lucy_flush(L.la);
+      if (status == 0) bin_exit(k, L.xpold, L.ypold, L.zpold, L.iscat, acc);
+      if constexpr (TRACE) {
+        hc_trace &o = P.trace[L.pid];
+        o.x = k.x;
+        o.y = k.y;
+        o.z = k.z;
+        o.cost = k.d.cost;
+        o.sint = k.d.sint;
+        o.phi = k.d.phi;
+        o.I = k.s.I;
+        o.Q = k.s.Q;
+        o.U = k.s.U;
+        o.V = k.s.V;
+        o.nub = (double)k.nub;
+        o.path = tr.path;
+        o.ncross = tr.ncross;
+        o.nscat = tr.nscat;
+        o.nabs = tr.nabs;
+        o.nstar = tr.nstar;
+        o.ntrial = 0;
+        o.cellhash = tr.hash;
+        o.status = status;
+        o.ir = k.ir;
+        o.it = k.it;
+        o.ip = k.ip;
+        o.pad = 0;
+      }
+    }
+    // take the next packet and launch it: the lanes of the wavefront that
+    // need one share one atomic on the packet counter (a block of packet
+    // numbers), each lane taking its rank among them
+    const unsigned long long need = __ballot(1);
+    const int leader = __ffsll((long long)need) - 1;
+    const int lane = (int)__lane_id();
+    unsigned long long base = 0;
+    if (lane == leader)
+      base = atomicAdd(P.next_packet + kCtrNext, (unsigned long long)__popcll(need));
+    const unsigned blo = __shfl((unsigned)(base & 0xffffffffull), leader);
+    const unsigned bhi = __shfl((unsigned)(base >> 32), leader);
+    const unsigned long long p = (((unsigned long long)bhi << 32) | blo) +
+                                 (unsigned long long)__popcll(need & ((1ull << lane) - 1ull));
+    if (p >= (unsigned long long)P.np) {
+      st = ST_DONE;
+    } else {
+      L.pid = (long long)p;
+      L.rng = mrg_segment(P.g1, P.g2, seg_base + p, P.pow1, P.pow2);
+      L.la.cell = -1;
+      L.iscat = L.iint = L.pir = L.pit = L.pip = 0;
+      L.work = 0;
+      L.first = true;
+      status = -1;
+      if constexpr (TRACE) tr = Trace{0.0, 0.0, 0, 0, 0, 0, 0, 14695981039346656037ull};
+      const long long iphot = (long long)p + 1;
+      if (iphot < P.npacc) {  // newdisktherm.F90:209
+        const int sl = launch_acc(W, k, L.rng);
+        if constexpr (TRACE) hash_cell(tr, k.ir, k.it, k.ip);
+        push(k, sl, false, L.pid, nrm);
+      } else {
+        launch_star(k, L.rng, iphot < P.npacc + P.npaccstar, nrm);
+        if constexpr (TRACE) hash_cell(tr, k.ir, k.it, k.ip);
+        push(k, 0, true, L.pid, nrm);
+      }
+      st = ST_RESTART;
+      reload = true;
+    }
+  };
+
   for (;;) {
     // ---------------- event phase ----------------
-    {
+    if constexpr (POOL) {
+      pool_enqueue();
+      const unsigned long long walkers0 = __ballot(st == ST_WALK);
+      const bool eager = __popcll(walkers0) < P.pool_walk;
+      bool ran_pool = pool_consume(0, eager);
+      ran_pool = pool_consume(1, eager) || ran_pool;
+      pool_poll();
+      pool_pickup();
+      const unsigned long long walkers = __ballot(st == ST_WALK);
+      const int n_r = __popcll(__ballot(st == ST_RESTART));
+      const int n_t = __popcll(__ballot(st == ST_STAR));
+      const int n_e = __popcll(__ballot(st == ST_END));
+      const bool all = walkers == 0ull;
+      const bool trig = n_r >= ev_min || n_t >= ev_min || n_e >= ev_min;
+      const int lim = all ? 1 : (trig ? ev_run : ev_min);
+      if (n_t > 0 && n_t >= lim && st == ST_STAR) ev_star();
+      if (n_e > 0 && n_e >= lim && st == ST_END) ev_end();
+      const bool any_ev = __ballot(st == ST_RESTART) != 0ull;
+      if (st == ST_RESTART) st = start_walk(reload);
+      // a wavefront whose lanes all wait for requests that other wavefronts
+      // are running has nothing to do until they finish
+      if (!ran_pool && !any_ev && all &&
+          __ballot(st == ST_WALK || st == ST_SCAT || st == ST_ABS || st == ST_END ||
+                   st == ST_STAR) == 0ull)
+        __builtin_amdgcn_s_sleep(2);
+    } else {
       const unsigned long long walkers = __ballot(st == ST_WALK);
       const int n_s = __popcll(__ballot(st == ST_SCAT));
       const int n_a = __popcll(__ballot(st == ST_ABS));
@@ -1720,7 +2061,8 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : kBlockLucy, HC_WAVES_PER_EU) vo
         L.pir = k.ir;
         L.pit = k.it;
         L.pip = k.ip;
-        st = start_walk(false);
+        st = ST_RESTART;
+        reload = false;
       }
       if (n_a > 0 && n_a >= lim && st == ST_ABS) {
         const int idust2 = P.act[slot] - 1;
@@ -1754,84 +2096,15 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : kBlockLucy, HC_WAVES_PER_EU) vo
         L.pir = k.ir;
         L.pit = k.it;
         L.pip = k.ip;
-        st = start_walk(false);
+        st = ST_RESTART;
+        reload = false;
       }
-      if (n_t > 0 && n_t >= lim && st == ST_STAR) {
-        L.work++;
-        star_reemit(k, L.rng, acc, nrm);
-        if constexpr (TRACE) tr.nstar++;
-        push(k, 0, true, L.pid, nrm);
-        st = start_walk(true);
-      }
-      if (n_e > 0 && n_e >= lim && st == ST_END) {
-        if (status >= 0) {  // the packet ends (0 escaped, 1 killed, 2 lost)
-          acc.maxwork = max(acc.maxwork, L.work);
-          lucy_flush(L.la);
-          if (status == 0) bin_exit(k, L.xpold, L.ypold, L.zpold, L.iscat, acc);
-          if constexpr (TRACE) {
-            hc_trace &o = P.trace[L.pid];
-            o.x = k.x;
-            o.y = k.y;
-            o.z = k.z;
-            o.cost = k.d.cost;
-            o.sint = k.d.sint;
-            o.phi = k.d.phi;
-            o.I = k.s.I;
-            o.Q = k.s.Q;
-            o.U = k.s.U;
-            o.V = k.s.V;
-            o.nub = (double)k.nub;
-            o.path = tr.path;
-            o.ncross = tr.ncross;
-            o.nscat = tr.nscat;
-            o.nabs = tr.nabs;
-            o.nstar = tr.nstar;
-            o.ntrial = 0;
-            o.cellhash = tr.hash;
-            o.status = status;
-            o.ir = k.ir;
-            o.it = k.it;
-            o.ip = k.ip;
-            o.pad = 0;
-          }
-        }
-        // take the next packet and launch it: the lanes of the wavefront that
-        // need one share one atomic on the packet counter (a block of packet
-        // numbers), each lane taking its rank among them
-        const unsigned long long need = __ballot(1);
-        const int leader = __ffsll((long long)need) - 1;
-        const int lane = (int)__lane_id();
-        unsigned long long base = 0;
-        if (lane == leader)
-          base = atomicAdd(P.next_packet + kCtrNext, (unsigned long long)__popcll(need));
-        const unsigned blo = __shfl((unsigned)(base & 0xffffffffull), leader);
-        const unsigned bhi = __shfl((unsigned)(base >> 32), leader);
-        const unsigned long long p = (((unsigned long long)bhi << 32) | blo) +
-                                     (unsigned long long)__popcll(need & ((1ull << lane) - 1ull));
-        if (p >= (unsigned long long)P.np) {
-          st = ST_DONE;
-        } else {
-          L.pid = (long long)p;
-          L.rng = mrg_segment(P.g1, P.g2, seg_base + p, P.pow1, P.pow2);
-          L.la.cell = -1;
-          L.iscat = L.iint = L.pir = L.pit = L.pip = 0;
-          L.work = 0;
-          L.first = true;
-          status = -1;
-          if constexpr (TRACE) tr = Trace{0.0, 0.0, 0, 0, 0, 0, 0, 14695981039346656037ull};
-          const long long iphot = (long long)p + 1;
-          if (iphot < P.npacc) {  // newdisktherm.F90:209
-            const int sl = launch_acc(W, k, L.rng);
-            if constexpr (TRACE) hash_cell(tr, k.ir, k.it, k.ip);
-            push(k, sl, false, L.pid, nrm);
-          } else {
-            launch_star(k, L.rng, iphot < P.npacc + P.npaccstar, nrm);
-            if constexpr (TRACE) hash_cell(tr, k.ir, k.it, k.ip);
-            push(k, 0, true, L.pid, nrm);
-          }
-          st = start_walk(true);
-        }
-      }
+      if (n_t > 0 && n_t >= lim && st == ST_STAR) ev_star();
+      if (n_e > 0 && n_e >= lim && st == ST_END) ev_end();
+      // every lane whose event ran starts its next walk here, the event
+      // kinds together: the draw of the new optical depth and the first
+      // walls of the walk are the same code for all of them
+      if (st == ST_RESTART) st = start_walk(reload);
     }
     if constexpr (PEEL) {
       const int n = qcount[wave];
@@ -1843,6 +2116,23 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : kBlockLucy, HC_WAVES_PER_EU) vo
     }
     if (__ballot(st != ST_DONE) == 0ull) break;
     // ---------------- walk phase ----------------
+    if constexpr (POOL) {
+      for (;;) {
+        pool_enqueue();
+        pool_poll();
+        if (__ballot(st == ST_WALK) == 0ull) break;
+        if (__popcll(__ballot(st == ST_SCAT_READY || st == ST_ABS_READY)) >= ev_min ||
+            __popcll(__ballot(st == ST_END)) >= ev_min || __popcll(__ballot(st == ST_STAR)) >= ev_min ||
+            pool_avail(0) >= (uint32_t)P.pool_min || pool_avail(1) >= (uint32_t)P.pool_min)
+          break;
+        if (st == ST_WALK) {
+          int s = walk_step<R, NA, TRACE>(W, L.k, tau, tsum, dm, L.la, acc, tr);
+          if (++L.work > P.max_work) s = ST_ABORT;
+          st = after_step(s);
+        }
+      }
+      continue;
+    }
     for (;;) {
       if (__ballot(st == ST_WALK) == 0ull) break;
       if (__popcll(__ballot(st == ST_SCAT)) >= ev_min || __popcll(__ballot(st == ST_ABS)) >= ev_min ||
@@ -1999,6 +2289,9 @@ struct hc_ctx {
   int peel_queue = 0;  // records per wavefront peel queue, 0 = 96 or what fits
   int peel_waves = 0;  // wavefront queues per peel workgroup: 4 or 8 (0 = 4)
   size_t lds_per_block = 64 * 1024;  // the device's limit (hc_create)
+  int pool = 0;        // pooled scattering/absorption in Lucy passes: switched off
+  int pool_min = 48;   // requests a walking wavefront takes at once
+  int pool_walk = 16;  // walking lanes below which a wavefront takes any request
   uint32_t max_work = 1u << 24;
 };
 
@@ -2571,24 +2864,6 @@ int prepare_pass(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int32_t 
   size_t rec_bytes = 0;
   const bool fp32 = d.precision == HC_FP32;
   const bool pk = P.ipeel != 0;
-#define HC_PICK(R_, NA_)                                                                   \
-  kern = trace ? (pk ? transport<R_, NA_, true, true> : transport<R_, NA_, true, false>)  \
-               : (pk ? transport<R_, NA_, false, true> : transport<R_, NA_, false, false>); \
-  rec_bytes = sizeof(PeelRec<R_, NA_>)
-  if (ctx->nact <= 4) {
-    if (fp32) {
-      HC_PICK(float, 4);
-    } else {
-      HC_PICK(double, 4);
-    }
-  } else {
-    if (fp32) {
-      HC_PICK(float, 8);
-    } else {
-      HC_PICK(double, 8);
-    }
-  }
-#undef HC_PICK
   // dynamic LDS: the wall arrays, then (peel-off passes) four wavefront
   // queues of peel records; two workgroups share a compute unit
   const size_t walls =
@@ -2607,8 +2882,40 @@ int prepare_pass(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int32_t 
   const size_t emis = ((size_t)(d.nT + d.nnu) * sizeof(double) + 63) / 64 * 64;
   P.peel_base = (int)(walls + opac + emis);
   size_t shm = walls + opac + emis;
+  // Lucy passes pool their scattering and absorption events over a 512-lane
+  // workgroup when the pool fits beside the tables
+  const size_t pool_bytes = (size_t)kPoolFields * kPoolLanes * sizeof(double) +
+                            (size_t)3 * kPoolLanes * sizeof(uint32_t) + 4 * sizeof(uint32_t);
+  // (switched off: hc_set_option refuses "pool" 1, and no pooled kernel
+  // is instantiated)
+  const bool pool = false;
+  (void)pool_bytes;
+  P.pool_base = (int)shm;
+  P.pool_min = ctx->pool_min;
+  P.pool_walk = ctx->pool_walk;
+#define HC_PICK(R_, NA_)                                                                    \
+  kern = trace ? (pk ? transport<R_, NA_, true, true, false>                                 \
+                     : transport<R_, NA_, true, false, false>)                               \
+               : (pk ? transport<R_, NA_, false, true, false>                                \
+                     : transport<R_, NA_, false, false, false>);                             \
+  rec_bytes = sizeof(PeelRec<R_, NA_>)
+  if (ctx->nact <= 4) {
+    if (fp32) {
+      HC_PICK(float, 4);
+    } else {
+      HC_PICK(double, 4);
+    }
+  } else {
+    if (fp32) {
+      HC_PICK(float, 8);
+    } else {
+      HC_PICK(double, 8);
+    }
+  }
+#undef HC_PICK
   P.peel_q = 0;
-  int block = pk ? 256 : kBlockLucy;
+  int block = pk ? 256 : (pool ? kPoolLanes : kBlockLucy);
+  if (pool) shm += pool_bytes;
   if (pk) {
     // one queue per wavefront, four wavefronts per workgroup; eight on
     // request share the workgroup's copy of the walls and tables (two
@@ -3244,6 +3551,19 @@ int hc_set_option(hc_ctx *ctx, const char *name, int64_t value) {
   } else if (n == "blocks_per_cu") {
     if (value < 0 || value > 64) return fail(ctx, HC_E_ARG, "blocks_per_cu must be in 0..64");
     ctx->blocks_per_cu = (int)value;
+  } else if (n == "pool") {
+    if (value < 0 || value > 1) return fail(ctx, HC_E_ARG, "pool must be 0 or 1");
+    if (value == 1)
+      return fail(ctx, HC_E_UNSUPPORTED,
+                  "pool: the workgroup event pool is switched off (its first GPU run did not "
+                  "finish and no cause has been found)");
+    ctx->pool = 0;
+  } else if (n == "pool_min") {
+    if (value < 1 || value > 64) return fail(ctx, HC_E_ARG, "pool_min must be in 1..64");
+    ctx->pool_min = (int)value;
+  } else if (n == "pool_walk") {
+    if (value < 0 || value > 64) return fail(ctx, HC_E_ARG, "pool_walk must be in 0..64");
+    ctx->pool_walk = (int)value;
   } else {
     return fail(ctx, HC_E_ARG, "hc_set_option: unknown option " + n);
   }
