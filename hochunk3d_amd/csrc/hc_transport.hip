@@ -3558,12 +3558,6 @@ int hc_set_option(hc_ctx *ctx, const char *name, int64_t value) {
                   "pool: the workgroup event pool is switched off (its first GPU run did not "
                   "finish and no cause has been found)");
     ctx->pool = 0;
-  } else if (n == "pool_min") {
-    if (value < 1 || value > 64) return fail(ctx, HC_E_ARG, "pool_min must be in 1..64");
-    ctx->pool_min = (int)value;
-  } else if (n == "pool_walk") {
-    if (value < 0 || value > 64) return fail(ctx, HC_E_ARG, "pool_walk must be in 0..64");
-    ctx->pool_walk = (int)value;
   } else {
     return fail(ctx, HC_E_ARG, "hc_set_option: unknown option " + n);
   }

@@ -130,6 +130,24 @@ module hc_iface
       integer(c_int64_t), intent(out) :: nchanged
     end function hc_run_lucy
 
+    ! niter Lucy iterations of nmodels contexts with all passes in one launch
+    ! per kernel variant (a parameter grid); ctxs, np, seg0, ltot, out and
+    ! nchanged have nmodels entries; massarr holds nmodels addresses (c_loc of
+    ! each model's massarr, or c_null_ptr once a context has it)
+    integer(c_int) function hc_run_batch(nmodels, ctxs, niter, np, seed1, seed2, gen, seg0, &
+                                         massarr, ltot, oversubscribe, out, nchanged, &
+                                         device_ms) bind(C, name='hc_run_batch')
+      import :: c_int, c_ptr, c_double, c_int64_t, c_int32_t, hc_tallies
+      integer(c_int32_t), value :: nmodels, niter, seed1, seed2, gen, oversubscribe
+      type(c_ptr), intent(in) :: ctxs(*)
+      integer(c_int64_t), intent(in) :: np(*), seg0(*)
+      type(c_ptr), intent(in) :: massarr(*)
+      real(c_double), intent(in) :: ltot(*)
+      type(hc_tallies), intent(inout) :: out(*)
+      integer(c_int64_t), intent(out) :: nchanged(*)
+      real(c_double), intent(out) :: device_ms
+    end function hc_run_batch
+
     integer(c_int) function hc_set_option(ctx, name, value) bind(C, name='hc_set_option')
       import :: c_int, c_ptr, c_char, c_int64_t
       type(c_ptr), value :: ctx

@@ -248,6 +248,8 @@ int hc_run_batch(int32_t nmodels, hc_ctx *const *ctxs, int32_t niter, const int6
  *                   and tables in the LDS, else the pass is refused;
  *   "blocks_per_cu" 0..64: workgroups per compute unit, 0 = as many as fit
  *                   (default 0);
+ *   "pool"          0 only: the workgroup event pool of the Lucy passes is
+ *                   switched off, and 1 is refused (HC_E_UNSUPPORTED);
  *   "max_work"      1..2^31-1: events + cell crossings after which a packet
  *                   is dropped and counted in n_warn (default 2^24, above the
  *                   longest packets of the benchmark models: ~4e5 on the 8M-
