@@ -25,7 +25,7 @@ _ip = C.POINTER(C.c_int32)
 # Every symbol include/hc.h declares.
 EXPORTS = (
     "hc_create", "hc_set_grid", "hc_set_temperature", "hc_get_temperature", "hc_set_dust",
-    "hc_set_sources", "hc_set_atmosphere", "hc_set_limb_table",
+    "hc_set_sources", "hc_set_atmosphere", "hc_set_limb_table", "hc_set_bw",
     "hc_run_iteration", "hc_temperature", "hc_run_lucy", "hc_run_batch", "hc_set_option",
     "hc_dropped_packets", "hc_destroy",
     "hc_last_error", "hc_build_density", "hc_build_density_noalloc", "hc_build_emissivity",
@@ -119,6 +119,7 @@ def load() -> C.CDLL:
     lib.hc_set_sources.argtypes = [C.c_void_p, C.POINTER(HcSources)]
     lib.hc_set_atmosphere.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, _dp, _dp]
     lib.hc_set_limb_table.argtypes = [C.c_void_p, C.c_int32, _dp, _dp]
+    lib.hc_set_bw.argtypes = [C.c_void_p, _dp, _dp]
     lib.hc_run_iteration.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
                                      C.c_int32, C.c_uint64, C.POINTER(HcTallies), C.c_void_p]
     lib.hc_temperature.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, _dp,
@@ -265,6 +266,17 @@ class Transport:
         a = [_f64(v) for v in (atm.atmosnu, atm.hnuint, atm.lognu, atm.loghnu)]
         self._check(self.lib.hc_set_atmosphere(self.h, a[0].size, *(_ptr(v) for v in a)),
                     "hc_set_atmosphere")
+
+    def set_bw(self, on: bool = True):
+        """Bjorkman-Wood re-emission (hc_set_bw) with the model's kapdBint
+        (built on first use) and massarr; on=False returns to Lucy's."""
+        if not on:
+            self._check(self.lib.hc_set_bw(self.h, None, None), "hc_set_bw")
+            return
+        from .dust import ensure_kapdB
+        kdb = _f64(ensure_kapdB(self.model.dust))
+        mass = _f64(self.model.massarr)
+        self._check(self.lib.hc_set_bw(self.h, _ptr(kdb), _ptr(mass)), "hc_set_bw")
 
     def set_option(self, name: str, value: int):
         """Launch-shape knob: regen_lanes, regen_run_lanes, peel_queue,

@@ -101,6 +101,14 @@ struct Params {
   int limb, ntab;
   const double *limb_tab;
   double delmu;
+  // Bjorkman-Wood re-emission (ilucy = 0): each absorption brings its
+  // cell's temperature up to date in tdust_w (and the walk's mask densm_w)
+  // and re-emits from kapdBint; serial: one lane runs the packets in order
+  int bw, serial;
+  const double *kapdBint, *kappap, *mass4;
+  const uint16_t *cdfguide_bw;
+  double *tdust_w, *densm_w;
+  double tsub_ev, ltot, nphot;
 };
 
 // Launch parameters are the transport kernel's argument: every wave reads
@@ -807,15 +815,17 @@ struct LdfState {
   const double *row;
   double a0, a1, e0, e1;
   int g0a, g1a, g0b, g1b;
+  bool guided;
 };
 
 template <class R>
-__device__ __forceinline__ LdfState<R> ldf_begin(double T0d, int id, Mrg &rng) {
+__device__ __forceinline__ LdfState<R> ldf_begin(double T0d, int id, Mrg &rng,
+                                                 const double *table, const uint16_t *guide) {
   const Params &P = c_P;
   const int nT = P.nT, nnu = P.nnu;
   // lTint from LDS (staged by the transport kernel)
   const double *lT = reinterpret_cast<const double *>(hc_lds + P.emis_base);
-  const double *kb = P.kapBint + (size_t)nT * nnu * id;
+  const double *kb = table + (size_t)nT * nnu * id;
   LdfState<R> L;
   L.T0 = R(T0d);
   const R lT0 = m_log(L.T0);
@@ -836,8 +846,9 @@ __device__ __forceinline__ LdfState<R> ldf_begin(double T0d, int id, Mrg &rng) {
   L.e1 = pe[1];
   const int bq = clampi((int)(L.xi * R(kCdfGuide)), 0, kCdfGuide - 1);
   L.g0a = L.g1a = L.g0b = L.g1b = 0;
-  if (P.cdfguide) {
-    const uint16_t *g0 = P.cdfguide + ((size_t)id * nT + (size_t)(j - 1)) * (kCdfGuide + 1);
+  L.guided = guide != nullptr;
+  if (guide) {
+    const uint16_t *g0 = guide + ((size_t)id * nT + (size_t)(j - 1)) * (kCdfGuide + 1);
     const uint16_t *g1 = g0 + (kCdfGuide + 1);
     L.g0a = g0[bq];
     L.g1a = g1[bq];
@@ -873,7 +884,7 @@ __device__ __forceinline__ R ldf_end(const LdfState<R> &L) {
   // a short bracket's interior nodes pb+1 .. pb+4, read with its ends
   int pb = 0;
   R q1 = R(0.0), q2 = R(0.0), q3 = R(0.0), q4 = R(0.0);
-  if (P.cdfguide) {
+  if (L.guided) {
     // The interpolated CDF lies between rows j and j+1, so the node counts
     // of both rows at the probability bin of xi bound the bracket; the
     // rows are strictly increasing (checked on the host), so the bracket is
@@ -939,7 +950,68 @@ __device__ __forceinline__ R ldf_end(const LdfState<R> &L) {
 
 template <class R>
 __device__ __forceinline__ R lucydustfreq(double T0d, int id, Mrg &rng) {
-  return ldf_end<R>(ldf_begin<R>(T0d, id, rng));
+  const Params &P = c_P;
+  return ldf_end<R>(ldf_begin<R>(T0d, id, rng, P.kapBint, P.cdfguide));
+}
+
+// Planck-mean opacity at T (dust.f90:114-142) from the kappap table
+__device__ __forceinline__ double kapp_at(const double *kappap, const double *lTint, int nT,
+                                          double t, int id) {
+  const double *kp = kappap + (size_t)nT * id;
+  const double lt = hcm_log(t);
+  const int k = (int)((double)(nT - 1) * (lt - lTint[0]) / (lTint[nT - 1] - lTint[0])) + 1;
+  if (k < 1 || k > nT - 1) return k < 1 ? kp[0] : kp[nT - 1];
+  const double f = (lt - lTint[k - 1]) / (lTint[k] - lTint[k - 1]);
+  return (1.0 - f) * kp[k - 1] + f * kp[k];
+}
+
+// dusttemp.f90:6-58: the temperature at which na absorbed packets balance
+// the emission of a cell-species of (rho V)^(1/4) = m4, by fixed-point
+// iteration from T0 to 1e-5 (capped at 1000 sweeps where the reference
+// would spin), clamped to [0.1, 1600] K
+__device__ __forceinline__ double dusttemp_fp(double na, double T0, double ltot, double nphot,
+                                              double kappav, const double *kappap,
+                                              const double *lTint, int nT, double m4, int id) {
+  double Td = (T0 != 0.0) ? T0 : 0.1 / 11605.0;
+  double Tprev = 0.5 * Td;
+  int iter = 0;
+  while (fabs(Td - Tprev) > 1.e-5 * Td && iter < 1000) {
+    Tprev = Td;
+    Td = hcm_pow((na * ltot) / (4.0 * kSigT * nphot * kappav * kapp_at(kappap, lTint, nT, Td, id)),
+                 0.25) /
+         11605.0;
+    if (m4 > 0.0)
+      Td = Td / m4;
+    else
+      Td = 0.1 / 11605.0;
+    if (Td > 1600.0 / 11605.0) Td = 1600.0 / 11605.0;
+    Td = fmax(Td, 0.1 / 11605.0);
+    ++iter;
+  }
+  return Td;
+}
+
+// Bjorkman-Wood branch of emit_common.f90:57-66 for an absorption of
+// species id in tdust word w: the absorption is counted (propagate.f90:123),
+// the cell-species temperature is brought up to date with its count so far
+// (dusttemp), the walk's sublimation mask follows it, and the packet
+// re-emits at the mean of the old and new temperatures (3/4 new) on the
+// first absorption, at the new one after.  Packets running side by side
+// take their counts in the order of the atomic; the stored temperature is
+// the highest computed (it grows with the count), in serial mode the last.
+__device__ __forceinline__ double bw_temperature(size_t w, int id) {
+  const Params &P = c_P;
+  const double na = atomicAdd(P.nabs + w, 1.0) + 1.0;
+  const double told = __hip_atomic_load(P.tdust_w + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const double tnew = dusttemp_fp(na, told, P.ltot, P.nphot, P.kappav[id], P.kappap, P.lTint,
+                                  P.nT, P.mass4[w], id);
+  if (P.serial)
+    P.tdust_w[w] = tnew;
+  else
+    atomicMax(reinterpret_cast<unsigned long long *>(P.tdust_w + w),
+              (unsigned long long)__double_as_longlong(tnew));
+  if (tnew >= P.tsub_ev) P.densm_w[w] = -P.dens[w];
+  return na < 2.0 ? 0.25 * told + 0.75 * tnew : tnew;
 }
 
 // opacset for all active species: kapd = kappa * R*(cm) * kappa_V and the
@@ -1193,14 +1265,25 @@ __device__ __forceinline__ int launch_acc(const WallView &W, Pkt<R, NA> &k, Mrg 
   int slot = 0;
   if (idust2 >= 0) {  // initpacc.f90:95-115, then the Lucy re-emission
     slot = P.slot_of[idust2];
+    const size_t w = cell * (size_t)P.nact + slot;
     atomicAdd(&P.dbig[cell], 1);
-    atomic_add(&P.nabs[cell * (size_t)P.nact + slot], 1.0);
+    double tem;
+    if (P.bw) {
+      tem = bw_temperature(w, idust2);
+    } else {
+      atomic_add(&P.nabs[w], 1.0);
+      tem = P.tdust[w];
+    }
     k.d.cost = R(1.0) - R(2.0) * R(ran(rng));
     k.d.sint = m_sqrt(R(1.0) - m_pow(k.d.cost, R(2.0)));
     k.d.phi = R(kR2P) * R(ran(rng));
     k.d.cosp = m_cos(k.d.phi);
     k.d.sinp = m_sin(k.d.phi);
-    k.nub = lucydustfreq<R>(P.tdust[cell * (size_t)P.nact + slot], idust2, rng);
+    if (P.bw && tem >= P.tsub_ev)  // dustfreq: the Planck function at sublimation
+      k.nub = bbfreq<R>(rng, R(tem));
+    else
+      k.nub = ldf_end<R>(ldf_begin<R>(tem, idust2, rng, P.bw ? P.kapdBint : P.kapBint,
+                                      P.bw ? P.cdfguide_bw : P.cdfguide));
   } else {  // emit_gasacc.f90:22-42
     const double t0 = 3.0 * kGn * (kMsol * P.massc) * P.mdot /
                       (8.0 * kPi * kSigT * hcm_pow(rtot * P.rstar * kRsol, 3.0));
@@ -1329,7 +1412,6 @@ __device__ __forceinline__ void launch_star(Pkt<R, NA> &k, Mrg &rng, bool hotspo
 // Star re-emission of a packet that reached the stellar surface.
 template <class R, int NA>
 __device__ __forceinline__ void star_reemit(Pkt<R, NA> &k, Mrg &rng, Acc &acc, R *nrm) {
-  const Params &P = c_P;
   acc.aflux++;
   const R cosb = clamp1(R(k.z / sqrt(k.rsq)));
   const R sinb = m_sqrt(R(1.0) - cosb * cosb);
@@ -1657,7 +1739,9 @@ __global__ __launch_bounds__((PEEL || POOL) ? kBlockPeel : kBlockLucy, HC_WAVES_
   double dm[NA];  // masked densities of the packet's cell (walk_step prefetches the next)
 #pragma unroll
   for (int s2 = 0; s2 < NA; ++s2) dm[s2] = 0.0;
-  int st = ST_END, status = -1, slot = 0;  // status < 0: no packet yet
+  // status < 0: no packet yet; in serial mode only the grid's first lane runs
+  int st = (P.serial && (blockIdx.x != 0 || threadIdx.x != 0)) ? (int)ST_DONE : (int)ST_END;
+  int status = -1, slot = 0;
   bool reload = false;  // the restarting walk enters a new cell (launch, star)
   const int ev_min = P.regen_lanes;
   const int ev_run = min(P.regen_run_lanes, ev_min);
@@ -1876,7 +1960,7 @@ __global__ __launch_bounds__((PEEL || POOL) ? kBlockPeel : kBlockLucy, HC_WAVES_
         // the draws in their order (direction, direction, frequency)
         const R u1 = R(ran(g));
         const R u2 = R(ran(g));
-        const LdfState<R> ldf = ldf_begin<R>(tdu, idust2, g);
+        const LdfState<R> ldf = ldf_begin<R>(tdu, idust2, g, P.kapBint, P.cdfguide);
         Dir<R> d;
         d.cost = R(1.0) - R(2.0) * u1;
         d.sint = m_sqrt(R(1.0) - m_pow(d.cost, R(2.0)));
@@ -2070,22 +2154,31 @@ __global__ __launch_bounds__((PEEL || POOL) ? kBlockPeel : kBlockLucy, HC_WAVES_
         L.work++;
         lucy_open(L.la, cell);
         L.la.nabs_any += 1;
+        double tem = L.tdust;
+        if (P.bw) {
+          tem = bw_temperature(cell * (size_t)P.nact + slot, idust2);
+        } else {
 #pragma unroll
-        for (int s2 = 0; s2 < NA; ++s2)
-          if (s2 == slot) L.la.nabs[s2] += 1.0;
+          for (int s2 = 0; s2 < NA; ++s2)
+            if (s2 == slot) L.la.nabs[s2] += 1.0;
+        }
         k.i_orig = (idust2 == 0 || idust2 == 1 || idust2 == 4 || idust2 == 5) ? 2 : 3;
         k.s = Stokes4<R>{R(1.0), R(0.0), R(0.0), R(0.0)};
         // the draws in their order (direction, direction, frequency); the
         // frequency's table loads fly while the direction is computed
         const R u1 = R(ran(L.rng));
         const R u2 = R(ran(L.rng));
-        const LdfState<R> ldf = ldf_begin<R>(L.tdust, idust2, L.rng);
+        const bool planck = P.bw && tem >= P.tsub_ev;  // dustfreq at sublimation
+        LdfState<R> ldf;
+        if (!planck)
+          ldf = ldf_begin<R>(tem, idust2, L.rng, P.bw ? P.kapdBint : P.kapBint,
+                             P.bw ? P.cdfguide_bw : P.cdfguide);
         k.d.cost = R(1.0) - R(2.0) * u1;
         k.d.sint = m_sqrt(R(1.0) - m_pow(k.d.cost, R(2.0)));
         k.d.phi = R(kR2P) * u2;
         k.d.cosp = m_cos(k.d.phi);
         k.d.sinp = m_sin(k.d.phi);
-        k.nub = ldf_end<R>(ldf);
+        k.nub = planck ? bbfreq<R>(L.rng, R(tem)) : ldf_end<R>(ldf);
         set_walk_dir(k);
         k.iflag = 0;
         set_opacity(k, false);
@@ -2097,7 +2190,7 @@ __global__ __launch_bounds__((PEEL || POOL) ? kBlockPeel : kBlockLucy, HC_WAVES_
         L.pit = k.it;
         L.pip = k.ip;
         st = ST_RESTART;
-        reload = false;
+        reload = P.bw != 0;  // the cell's mask may have changed with its temperature
       }
       if (n_t > 0 && n_t >= lim && st == ST_STAR) ev_star();
       if (n_e > 0 && n_e >= lim && st == ST_END) ev_end();
@@ -2290,6 +2383,10 @@ struct hc_ctx {
   int peel_waves = 0;  // wavefront queues per peel workgroup: 4 or 8 (0 = 4)
   size_t lds_per_block = 64 * 1024;  // the device's limit (hc_create)
   int pool = 0;        // pooled scattering/absorption in Lucy passes: switched off
+  int bw = 0;          // Bjorkman-Wood re-emission (hc_set_bw)
+  int serial = 0;      // one lane runs the packets in order (hc_set_option "serial")
+  double *kapdBint = nullptr;
+  uint16_t *cdfguide_bw = nullptr;
   int pool_min = 48;   // requests a walking wavefront takes at once
   int pool_walk = 16;  // walking lanes below which a wavefront takes any request
   uint32_t max_work = 1u << 24;
@@ -2344,6 +2441,39 @@ int64_t powmod_host(int64_t a, uint64_t e, int64_t m) {
     e >>= 1;
   }
   return r;
+}
+
+// The guide of a normalised cumulative emissivity table (kapBint or
+// kapdBint, (nT,nnu,ndg)): per species and temperature row, the number of
+// nodes <= b/kCdfGuide.  Built only when every row increases strictly (then
+// the bracket of an interpolated CDF is unique); *out is null otherwise.
+int build_cdf_guide(hc_ctx *ctx, const double *table, uint16_t **out) {
+  const hc_dims &d = ctx->dims;
+  const int nT = d.nT, nnu = d.nnu;
+  std::vector<uint16_t> cg((size_t)d.ndg * nT * (kCdfGuide + 1));
+  bool strict = nnu < 65536;
+  for (int id = 0; id < d.ndg && strict; ++id)
+    for (int j = 0; j < nT && strict; ++j) {
+      const double *kb = table + (size_t)nT * nnu * id + j;
+      for (int i = 1; i < nnu; ++i)
+        if (!(kb[(size_t)nT * i] > kb[(size_t)nT * (i - 1)])) {
+          strict = false;
+          break;
+        }
+      uint16_t *g = cg.data() + ((size_t)id * nT + j) * (kCdfGuide + 1);
+      int cnt = 0;
+      for (int b = 0; b <= kCdfGuide; ++b) {
+        const double v = (double)b / (double)kCdfGuide;
+        while (cnt < nnu && kb[(size_t)nT * cnt] <= v) ++cnt;
+        g[b] = (uint16_t)cnt;
+      }
+    }
+  if (*out) {
+    (void)hipFree(*out);
+    *out = nullptr;
+  }
+  if (strict) return upload(ctx, out, cg.data(), cg.size());
+  return HC_OK;
 }
 
 // Cell-major, active-species-inner copy of a (nrg,ntg,npg,ndg) array.
@@ -2572,38 +2702,8 @@ int hc_set_dust(hc_ctx *ctx, const int32_t *nlambda, const double *nudust, const
   }
   rc = upload(ctx, &ctx->guide, guide.data(), guide.size());
   if (rc) return rc;
-  // emissivity-CDF guide: per species and temperature row, the number of
-  // nodes with kapBint <= b/kCdfGuide.  Used only when every row increases
-  // strictly (then the bracket of an interpolated CDF is unique).
-  {
-    const int nT = d.nT, nnu = d.nnu;
-    std::vector<uint16_t> cg((size_t)d.ndg * nT * (kCdfGuide + 1));
-    bool strict = nnu < 65536;
-    for (int id = 0; id < d.ndg && strict; ++id)
-      for (int j = 0; j < nT && strict; ++j) {
-        const double *kb = kapBint + (size_t)nT * nnu * id + j;
-        for (int i = 1; i < nnu; ++i)
-          if (!(kb[(size_t)nT * i] > kb[(size_t)nT * (i - 1)])) {
-            strict = false;
-            break;
-          }
-        uint16_t *g = cg.data() + ((size_t)id * nT + j) * (kCdfGuide + 1);
-        int cnt = 0;
-        for (int b = 0; b <= kCdfGuide; ++b) {
-          const double v = (double)b / (double)kCdfGuide;
-          while (cnt < nnu && kb[(size_t)nT * cnt] <= v) ++cnt;
-          g[b] = (uint16_t)cnt;
-        }
-      }
-    if (ctx->cdfguide) {
-      (void)hipFree(ctx->cdfguide);
-      ctx->cdfguide = nullptr;
-    }
-    if (strict) {
-      rc = upload(ctx, &ctx->cdfguide, cg.data(), cg.size());
-      if (rc) return rc;
-    }
-  }
+  rc = build_cdf_guide(ctx, kapBint, &ctx->cdfguide);
+  if (rc) return rc;
   ctx->have_dust = true;
   return HC_OK;
 }
@@ -2859,6 +2959,19 @@ int prepare_pass(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int32_t 
   P.ntab = ctx->ntab;
   P.limb_tab = ctx->limb_tab;
   P.delmu = ctx->ntab > 0 ? 1.0 / (double)ctx->ntab : 0.0;
+  if (ctx->bw && !ctx->mass4)
+    return fail(ctx, HC_E_STATE, "transport pass: Bjorkman-Wood re-emission needs hc_set_bw");
+  P.bw = ctx->bw;
+  P.serial = ctx->serial;
+  P.kapdBint = ctx->kapdBint;
+  P.cdfguide_bw = ctx->cdfguide_bw;
+  P.kappap = ctx->kappap;
+  P.mass4 = ctx->mass4;
+  P.tdust_w = ctx->tdust;
+  P.densm_w = ctx->densm;
+  P.tsub_ev = ctx->tsub / 11605.0;
+  P.ltot = S.ltot;
+  P.nphot = (double)np;
 
   TransportKernel &kern = L.kern;
   size_t rec_bytes = 0;
@@ -2947,6 +3060,7 @@ int prepare_pass(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int32_t 
                                             : std::max(bpm, 1);
   long long blocks = (long long)std::max(dev_cus, 1) * per_cu;
   blocks = std::min<long long>(blocks, std::max<long long>((np + block - 1) / block, 1));
+  if (ctx->serial) blocks = 1;
   L.shm = shm;
   L.blocks = (unsigned)blocks;
   L.block = block;
@@ -3517,6 +3631,34 @@ int hc_set_limb_table(hc_ctx *ctx, int32_t ntab, const double *xmu, const double
   return HC_OK;
 }
 
+int hc_set_bw(hc_ctx *ctx, const double *kapdBint, const double *massarr) {
+  if (!ctx) return HC_E_ARG;
+  if (!kapdBint) {
+    ctx->bw = 0;
+    return HC_OK;
+  }
+  if (!massarr) return fail(ctx, HC_E_ARG, "hc_set_bw: massarr required with kapdBint");
+  if (!ctx->have_grid || !ctx->have_dust)
+    return fail(ctx, HC_E_STATE, "hc_set_bw: grid and dust must be set first");
+  const hc_dims &d = ctx->dims;
+  HCK(hipSetDevice(ctx->device));
+  const size_t n = (size_t)d.nT * d.nnu * d.ndg;
+  for (int id = 0; id < d.ndg; ++id) {
+    const double *last = kapdBint + (size_t)d.nT * d.nnu * id + (size_t)d.nT * (d.nnu - 1);
+    for (int j = 0; j < d.nT; ++j)
+      if (!(last[j] == 1.0)) return fail(ctx, HC_E_ARG, "hc_set_bw: kapdBint rows must end at 1");
+  }
+  int rc = upload(ctx, &ctx->kapdBint, kapdBint, n);
+  if (!rc) rc = build_cdf_guide(ctx, kapdBint, &ctx->cdfguide_bw);
+  if (rc) return rc;
+  std::vector<double> tmp;
+  to_device_layout(ctx, massarr, tmp);
+  rc = upload(ctx, &ctx->mass4, tmp.data(), tmp.size());
+  if (rc) return rc;
+  ctx->bw = 1;
+  return HC_OK;
+}
+
 int hc_dropped_packets(hc_ctx *ctx, int64_t *pids, int32_t cap, int64_t *ndropped) {
   if (!ctx || !ndropped || cap < 0 || (cap > 0 && !pids)) return HC_E_ARG;
   if (!ctx->counter) return fail(ctx, HC_E_STATE, "hc_dropped_packets: no pass has run");
@@ -3551,6 +3693,9 @@ int hc_set_option(hc_ctx *ctx, const char *name, int64_t value) {
   } else if (n == "blocks_per_cu") {
     if (value < 0 || value > 64) return fail(ctx, HC_E_ARG, "blocks_per_cu must be in 0..64");
     ctx->blocks_per_cu = (int)value;
+  } else if (n == "serial") {
+    if (value < 0 || value > 1) return fail(ctx, HC_E_ARG, "serial must be 0 or 1");
+    ctx->serial = (int)value;
   } else if (n == "pool") {
     if (value < 0 || value > 1) return fail(ctx, HC_E_ARG, "pool must be 0 or 1");
     if (value == 1)
@@ -3576,7 +3721,7 @@ int hc_destroy(hc_ctx *ctx) {
                   ctx->killed, ctx->dbig,    ctx->sed,     ctx->peel,    ctx->img_i,  ctx->img_q,
                   ctx->img_u,  ctx->img_v,   ctx->si_init, ctx->scal,    ctx->mass4,  ctx->counter,
                   ctx->trace,  ctx->guide,   ctx->tdust2,  ctx->cdfguide, ctx->segdev,
-                  ctx->atm,    ctx->atm_guide, ctx->limb_tab};
+                  ctx->atm,    ctx->atm_guide, ctx->limb_tab, ctx->kapdBint, ctx->cdfguide_bw};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);

@@ -10,7 +10,9 @@ Host-side setup that produces the arrays the transport pass reads:
   frequency grids of the emissivity tables.
 * ``setemiss`` builds the normalised cumulative emissivity table
   ``kapBint(nT, nnu, ndg)`` and the Planck-mean opacity ``kappap(nT, ndg)``
-  (dust.f90:394-474).  The reference integrates each frequency bin with an
+  (dust.f90:394-474); ``kapdB_table`` builds the same call's
+  ``kapdBint(nT, nnu, ndg)``, the cumulative of kappa_abs dB_nu/dT that the
+  Bjorkman-Wood re-emission samples (dustfreq, dust.f90:158-311).  The reference integrates each frequency bin with an
   adaptive Simpson rule to 1e-5; here each bin uses 8-point Gauss-Legendre
   quadrature, which is exact to far below that tolerance for this smooth
   integrand.  The tables are boundary inputs: the GPU path and the CPU oracle
@@ -19,6 +21,7 @@ Host-side setup that produces the arrays the transport pass reads:
 from __future__ import annotations
 
 from dataclasses import dataclass
+from typing import Optional
 
 import numpy as np
 
@@ -73,6 +76,7 @@ class DustTables:
     nuint: np.ndarray          # (nnu,)
     kapBint: np.ndarray        # (nT, nnu, ndg) F-order
     kappap: np.ndarray         # (nT, ndg) F-order
+    kapdBint: Optional[np.ndarray] = None  # (nT, nnu, ndg): see ensure_kapdB
 
 
 def _species_arrays(table: np.ndarray):
@@ -109,6 +113,74 @@ def _planck_norm(nu: np.ndarray, T: np.ndarray) -> np.ndarray:
     with np.errstate(divide="ignore", invalid="ignore"):
         b = 0.1539897 * x ** 3 / np.expm1(x)
     return np.where(x == 0.0, 0.0, b)
+
+
+def _dplanck_norm(nu: np.ndarray, T: np.ndarray) -> np.ndarray:
+    """Temperature derivative of the normalised Planck function,
+    (dB_nu/dT)/(dB/dT) (kT/h) = (15/pi^4) x^4 e^x / (4 (e^x - 1)^2),
+    mathsubs.f90:26-49."""
+    x = nu / T
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        em = np.expm1(x)
+        b = 0.1539897 * x ** 4 * np.exp(x) / (4.0 * em * em)
+    return np.where((x == 0.0) | ~np.isfinite(b), 0.0, b)
+
+
+def _cumulative_table(nud, kd, ad, lTint, nuint, weight, nquad: int = 8):
+    """setemiss's cumulative bin integrals of kappa_abs * weight(nu, T) on
+    the (nT, nnu) grid (frequencies in units of T), unnormalised, with the
+    reference's cut of bins beyond nu0/T = 50 (dust.f90:425-436)."""
+    xg, wg = np.polynomial.legendre.leggauss(nquad)
+    fq = 0.5 * (xg + 1.0)
+    wq = 0.5 * wg
+    T = np.exp(lTint)[:, None]
+    nu1 = nuint[None, :] * T
+    nu0 = np.concatenate([np.zeros((len(lTint), 1)), nu1[:, :-1]], axis=1)
+    dnu = nu1 - nu0
+    dx = dnu / T
+    k1, a1 = opacset_np(nu1, nud, kd, ad)
+    kap1 = (1.0 - a1) * k1
+    k0, a0 = opacset_np(nu0, nud, kd, ad)
+    kap0 = (1.0 - a0) * k0
+    acc = np.zeros_like(nu1)
+    for f, w in zip(fq, wq):
+        nu = nu0 + f * dnu
+        acc += w * ((1.0 - f) * kap0 + f * kap1) * weight(nu, T)
+    acc = np.where(nu0 / T < 50.0, acc, 0.0)
+    return np.cumsum(dx * acc, axis=1)
+
+
+def kapdB_species(nud, kd, ad, lTint, nuint, nquad: int = 8) -> np.ndarray:
+    """kapdBint (nT, nnu) of one species: the cumulative of kappa_abs
+    dB_nu/dT (setemiss, dust.f90:426-462), normalised to 1 at the last
+    node."""
+    cum = _cumulative_table(nud, kd, ad, lTint, nuint, _dplanck_norm, nquad)
+    out = cum / cum[:, -1:]
+    out[:, -1] = 1.0
+    return out
+
+
+def kapdB_table(dust: "DustTables") -> np.ndarray:
+    """kapdBint (nT, nnu, ndg) for the species of `dust` (species with the
+    same opacity columns share one computation)."""
+    nT, nnu = dust.lTint.size, dust.nuint.size
+    out = np.zeros((nT, nnu, NDG), order="F")
+    cache = {}
+    for id_ in range(NDG):
+        nl = int(dust.nlambda[id_])
+        cols = (dust.nudust[:nl, id_], dust.kappad[:nl, id_], dust.adust[:nl, id_])
+        key = b"".join(np.ascontiguousarray(c).tobytes() for c in cols)
+        if key not in cache:
+            cache[key] = kapdB_species(*cols, dust.lTint, dust.nuint)
+        out[:, :, id_] = cache[key]
+    return out
+
+
+def ensure_kapdB(dust: "DustTables") -> np.ndarray:
+    """dust.kapdBint, built on first use (Bjorkman-Wood re-emission only)."""
+    if dust.kapdBint is None:
+        dust.kapdBint = kapdB_table(dust)
+    return dust.kapdBint
 
 
 def setemiss_species(nud, kd, ad, lTint, nuint, nquad: int = 8, device=None):

@@ -148,6 +148,13 @@ module hc_iface
       real(c_double), intent(out) :: device_ms
     end function hc_run_batch
 
+    ! Bjorkman-Wood re-emission (CLUCY=NO): kapdBint (nT,nnu,ndg) from setemiss
+    ! and massarr (nrg,ntg,npg,ndg); c_null_ptr for both returns to Lucy's
+    integer(c_int) function hc_set_bw(ctx, kapdBint, massarr) bind(C, name='hc_set_bw')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, kapdBint, massarr
+    end function hc_set_bw
+
     integer(c_int) function hc_set_option(ctx, name, value) bind(C, name='hc_set_option')
       import :: c_int, c_ptr, c_char, c_int64_t
       type(c_ptr), value :: ctx

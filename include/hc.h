@@ -166,6 +166,24 @@ int hc_set_atmosphere(hc_ctx *ctx, int32_t nhnu, const double *atmosnu, const do
  * The reference's tabl_mod, which sets ntab, is not in its source tree. */
 int hc_set_limb_table(hc_ctx *ctx, int32_t ntab, const double *xmu, const double *prob);
 
+/* Bjorkman-Wood re-emission (CLUCY = NO, ilucy = 0) instead of Lucy's:
+ * kapdBint (nT,nnu,ndg) is setemiss's normalised cumulative of
+ * kappa_abs dB_nu/dT (dust.f90:426-462, rows ending at 1) and massarr
+ * (nrg,ntg,npg,ndg) = (rho V)^(1/4) (gridset).  In the passes that follow,
+ * each absorption counts itself in nabs and brings its cell-species
+ * temperature up to date with the count so far (dusttemp with
+ * nphot = np, emit_common.f90:57-66), in the device temperatures (which
+ * hc_get_temperature returns) and in the walk's sublimation mask; the
+ * packet re-emits at the mean of the old and new temperatures (3/4 new) on
+ * the cell's first absorption, at the new one after, with dustfreq
+ * (dust.f90:158-311: the Planck function at or above Tsub).  The reference
+ * processes packets one after the other; here packets in flight take their
+ * counts in the order of the device's atomics and the stored temperature is
+ * the highest computed, so results equal the reference's for the serial
+ * order only with hc_set_option(ctx, "serial", 1).  kapdBint = NULL returns
+ * to Lucy re-emission. */
+int hc_set_bw(hc_ctx *ctx, const double *kapdBint, const double *massarr);
+
 /* One transport pass (the body of disk(), newdisktherm.F90:194-547): np
  * packets, peel-off when ipeel=1.  Packet p (0-based) draws from segment
  * seg0+p of L'Ecuyer-Cote generator gen seeded with (seed1,seed2)
@@ -248,6 +266,9 @@ int hc_run_batch(int32_t nmodels, hc_ctx *const *ctxs, int32_t niter, const int6
  *                   and tables in the LDS, else the pass is refused;
  *   "blocks_per_cu" 0..64: workgroups per compute unit, 0 = as many as fit
  *                   (default 0);
+ *   "serial"        0 or 1: one lane of the device runs the pass's packets
+ *                   in their order (the reference's serial loop; for
+ *                   Bjorkman-Wood parity, default 0);
  *   "pool"          0 only: the workgroup event pool of the Lucy passes is
  *                   switched off, and 1 is refused (HC_E_UNSUPPORTED);
  *   "max_work"      1..2^31-1: events + cell crossings after which a packet

@@ -353,12 +353,13 @@ double hco_bbfreq(hco_rng *r, double teff) {  /* random.f90:49-77 */
   return -HCO_LOG(z1 * z2 * z3 * z4) * teff / (double)i;
 }
 
-/* lucydustfreq.f90:19-157, including its guard chain.  The row index j is
- * clamped to [1,nT-1] where the reference would index out of bounds. */
-double hco_lucydustfreq(const hco_dust *d, hco_rng *r, double T0, int id) {
+/* lucydustfreq.f90:19-157, including its guard chain, on the normalised
+ * cumulative table tab (nT,nnu,ndg).  The row index j is clamped to
+ * [1,nT-1] where the reference would index out of bounds. */
+static double cdf_freq(const hco_dust *d, const double *tab, hco_rng *r, double T0, int id) {
   const int nT = d->nT, nnu = d->nnu;
   const double *lTint = d->lTint, *nuint = d->nuint;
-  const double *kb = d->kapBint + (size_t)nT * nnu * id;
+  const double *kb = tab + (size_t)nT * nnu * id;
   double lT0, xi, f, P0, P1, P;
   int j, i0, i1, i;
 #define KB(jj, ii) kb[IDX2((jj) - 1, (ii) - 1, nT)]
@@ -419,6 +420,20 @@ double hco_lucydustfreq(const hco_dust *d, hco_rng *r, double T0, int id) {
     f = HCO_LOG(xi / P0) / HCO_LOG(P1 / P0);
 #undef KB
   return T0 * nuint[i0 - 1] * HCO_POW(nuint[i1 - 1] / nuint[i0 - 1], f);
+}
+
+double hco_lucydustfreq(const hco_dust *d, hco_rng *r, double T0, int id) {
+  return cdf_freq(d, d->kapBint, r, T0, id);
+}
+
+/* dustfreq (dust.f90:158-311), the Bjorkman-Wood re-emission frequency:
+ * at or above the sublimation temperature tsub_k (K) the Planck function
+ * (bbfreq), else lucydustfreq's sampling of the kappa_abs dB/dT table
+ * kapdBint.  The reference stops where lucydustfreq only warns (P1 <= 0,
+ * xi <= 0, P1 < P0, j out of bounds); here both take lucydustfreq's path. */
+double hco_dustfreq(const hco_dust *d, hco_rng *r, double T0, int id, double tsub_k) {
+  if (T0 >= tsub_k / 11605.0) return hco_bbfreq(r, T0);
+  return cdf_freq(d, d->kapdBint, r, T0, id);
 }
 
 double hco_kapp(const hco_dust *d, double T, int id) {  /* dust.f90:114-142 */
@@ -905,6 +920,7 @@ typedef struct {
   size_t ncell;
   int nact;               /* species with any density */
   int act[HCO_NDG];
+  double nphot;           /* packets of the pass (dusttemp's nphot) */
 } ctx_t;
 
 static inline size_t cell_index(const hco_grid *g, int ir, int it, int ip) {
@@ -1401,6 +1417,30 @@ static int locate_cost(const hco_grid *g, double cosb) {  /* nr.f:131-153 on cos
   return jl;
 }
 
+/* emit_common.f90:57-70: the temperature a thermal packet re-emits at, for
+ * an absorption already counted in nabs[k].  Lucy branch: the cell's
+ * temperature.  Bjorkman-Wood branch (ilucy = 0): the cell's temperature is
+ * first brought up to date with its absorptions so far, in place (the pass
+ * reads it from then on, the walk's sublimation mask included), and the
+ * packet re-emits at the mean of the old and new temperatures (3/4 new) on
+ * the cell's first absorption, at the new one after. */
+static double reemit_temperature(ctx_t *c, size_t k, int id) {
+  const hco_grid *g = c->g;
+  if (!c->s->bw) return g->tdust[k];
+  double *td = (double *)g->tdust;
+  const double na = c->t->nabs[k], told = td[k];
+  const double tnew =
+      hco_dusttemp(c->d, na, told, c->s->ltot, c->nphot, g->massarr[k], id);
+  td[k] = tnew;
+  return na < 2.0 ? 0.25 * told + 0.75 * tnew : tnew;
+}
+
+/* reemit.f90:30-34: lucydustfreq, or dustfreq in the Bjorkman-Wood branch */
+static double reemit_freq(ctx_t *c, double tem, int id) {
+  return c->s->bw ? hco_dustfreq(c->d, &c->rng, tem, id, c->g->tsub)
+                  : hco_lucydustfreq(c->d, &c->rng, tem, id);
+}
+
 static void set_kapd_launch(ctx_t *c, pkt_t *P);
 static void bin_init(ctx_t *c, double nub, int i_orig_init);
 static void peel_all(ctx_t *c, const pkt_t *P, double sip, int idust2);
@@ -1457,13 +1497,14 @@ static void launch_acc(ctx_t *c, pkt_t *P) {
   if (idust2 >= 0) {  /* initpacc.f90:95-115 (the thermal disk species are not VSG/PAH) */
     if (t->dbig) t->dbig[cell] += 1;
     if (t->nabs) t->nabs[cell + c->ncell * idust2] += 1.0;
-    /* emit_common.f90:57-70 -> reemit.f90:15-33 at the cell's temperature */
+    /* emit_common.f90:57-70 -> reemit.f90:15-33 */
+    const double tem = reemit_temperature(c, cell + c->ncell * idust2, idust2);
     P->cost = 1.0 - 2.0 * hco_ran(&c->rng);
     P->sint = sqrt(1.0 - HCO_POW(P->cost, 2.0));
     P->phi = R2P * hco_ran(&c->rng);
     P->cosp = HCO_COS(P->phi);
     P->sinp = HCO_SIN(P->phi);
-    P->nub = hco_lucydustfreq(d, &c->rng, g->tdust[cell + c->ncell * idust2], idust2);
+    P->nub = reemit_freq(c, tem, idust2);
   } else {
     /* emit_gasacc.f90:22-42: Pringle (1981) surface temperature */
     t0 = 3.0 * GN * (MSOL * s->massc) * s->mdot /
@@ -1859,7 +1900,8 @@ static int propagate(ctx_t *c, pkt_t *P) {
         if (t->dbig) t->dbig[cell] += 1;
         P->i_orig = (idust2 == 1 || idust2 == 2 || idust2 == 5 || idust2 == 6) ? 2 : 3;
         if (t->nabs) t->nabs[k] += 1.0;
-        /* emit_common.f90:57-70 (Lucy branch) -> reemit.f90:15-33 */
+        /* emit_common.f90:57-70 -> reemit.f90:15-33 */
+        const double tem = reemit_temperature(c, k, idust2 - 1);
         P->sip = 1.0;
         P->sqp = P->sup = P->svp = 0.0;
         P->cost = 1.0 - 2.0 * hco_ran(&c->rng);
@@ -1867,7 +1909,7 @@ static int propagate(ctx_t *c, pkt_t *P) {
         P->phi = R2P * hco_ran(&c->rng);
         P->cosp = HCO_COS(P->phi);
         P->sinp = HCO_SIN(P->phi);
-        P->nub = hco_lucydustfreq(c->d, &c->rng, g->tdust[k], idust2 - 1);
+        P->nub = reemit_freq(c, tem, idust2 - 1);
         P->uy = P->sint * P->sinp;
         P->ux = P->sint * P->cosp;
         P->uz = P->cost;
@@ -1910,6 +1952,8 @@ int hco_run_pass(const hco_grid *g, const hco_dust *d, const hco_setup *s, hco_t
   c.rmincgs = s->rstar * RSOL;
   c.tsub_ev = g->tsub / 11605.0;
   c.ncell = (size_t)g->nrg * g->ntg * g->npg;
+  c.nphot = (double)np;
+  if (s->bw && (!d->kapdBint || !t->nabs || !g->massarr)) return -1;
   /* source split, newdisktherm.F90:143-176 (no external illumination) */
   ns = (int64_t)((double)np * (1.0 - s->accfrac));
   rn = (double)(float)np * (1.0 - s->accfrac);

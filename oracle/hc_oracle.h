@@ -76,6 +76,8 @@ typedef struct {
   const double *nuint;          /* [nnu] */
   const double *kapBint;        /* (nT,nnu,ndg) normalised cumulative */
   const double *kappap;         /* (nT,ndg) */
+  const double *kapdBint;       /* (nT,nnu,ndg) normalised cumulative of kappa_abs dB/dT
+                                 * (Bjorkman-Wood re-emission), or NULL */
 } hco_dust;
 
 typedef struct {
@@ -115,6 +117,11 @@ typedef struct {
   int ntab;
   const double *xmu, *prob;                        /* [ntab] */
   double delmu;
+  /* 1: Bjorkman-Wood re-emission (ilucy = 0, emit_common.f90:57-70, dustfreq):
+   * each absorption updates grid->tdust of its cell in place from the
+   * absorption count so far (dusttemp with nphot = np), and needs
+   * dust->kapdBint, tallies->nabs and grid->massarr; 0: Lucy (lucydustfreq) */
+  int bw;
 } hco_setup;
 
 typedef struct {
@@ -174,6 +181,7 @@ int hco_walk_ray(const hco_grid *g, double x, double y, double z, double ux, dou
                  int *ir, int *it, int *ip, int on_wall);
 double hco_bbfreq(hco_rng *r, double teff);
 double hco_lucydustfreq(const hco_dust *d, hco_rng *r, double T0, int id);
+double hco_dustfreq(const hco_dust *d, hco_rng *r, double T0, int id, double tsub_k);
 double hco_kapp(const hco_dust *d, double T, int id);
 double hco_dusttemp(const hco_dust *d, double na, double T0, double ltot, double nphot,
                     double mass4, int id);

@@ -34,7 +34,7 @@ class Dust(C.Structure):
                 ("nudust", _dp), ("kappad", _dp), ("adust", _dp), ("gdust", _dp),
                 ("pldust", _dp), ("kappav", C.c_double * NDG), ("nT", C.c_int),
                 ("nnu", C.c_int), ("lTint", _dp), ("nuint", _dp), ("kapBint", _dp),
-                ("kappap", _dp)]
+                ("kappap", _dp), ("kapdBint", _dp)]
 
 
 class Setup(C.Structure):
@@ -53,7 +53,8 @@ class Setup(C.Structure):
         ("massc", C.c_double), ("mdot", C.c_double),
         ("fmass", C.c_double * NDG), ("z1", C.c_double * NDG), ("bflare", C.c_double * NDG),
         ("nhnu", C.c_int), ("atmosnu", _dp), ("hnuint", _dp), ("lognu", _dp), ("loghnu", _dp),
-        ("ntab", C.c_int), ("xmu", _dp), ("prob", _dp), ("delmu", C.c_double)]
+        ("ntab", C.c_int), ("xmu", _dp), ("prob", _dp), ("delmu", C.c_double),
+        ("bw", C.c_int)]
 
 
 class Tallies(C.Structure):
@@ -139,6 +140,9 @@ def load(variant: str = "libm") -> C.CDLL:
         lib.hco_bbfreq.restype = C.c_double
         lib.hco_lucydustfreq.argtypes = [C.POINTER(Dust), C.POINTER(Rng), C.c_double, C.c_int]
         lib.hco_lucydustfreq.restype = C.c_double
+        lib.hco_dustfreq.argtypes = [C.POINTER(Dust), C.POINTER(Rng), C.c_double, C.c_int,
+                                     C.c_double]
+        lib.hco_dustfreq.restype = C.c_double
         lib.hco_kapp.argtypes = [C.POINTER(Dust), C.c_double, C.c_int]
         lib.hco_kapp.restype = C.c_double
         lib.hco_dusttemp.argtypes = [C.POINTER(Dust)] + [C.c_double] * 5 + [C.c_int]
@@ -176,13 +180,18 @@ def _p(a, ct=C.c_double):
 class Oracle:
     """The oracle bound to a model (hochunk3d_amd.model.Model)."""
 
-    def __init__(self, model, variant: str = "dm"):
+    def __init__(self, model, variant: str = "dm", bw: bool = False):
+        """bw: Bjorkman-Wood re-emission (ilucy = 0): each absorption updates
+        the oracle's copy of the cell temperature (self.tdust) in place and
+        re-emits with dustfreq on the model's kapdBint table."""
         self.lib = load(variant)
         m = self.model = model
         f = lambda a: np.asfortranarray(np.asarray(a, np.float64))  # noqa: E731
         self._arrs = {k: f(getattr(m, k)) for k in (
             "r2arr", "tan2arr", "thetarr", "aarr", "barr", "phiarr", "densarr", "tdust",
             "massarr", "u", "aperture2", "coste", "sinte", "cospe", "sinpe")}
+        # a Bjorkman-Wood pass writes the temperatures: the oracle's own copy
+        self._arrs["tdust"] = np.array(self._arrs["tdust"], order="F", copy=True)
         d = m.dust
         self._darrs = {k: f(getattr(d, k)) for k in (
             "nudust", "kappad", "adust", "gdust", "pldust", "lTint", "nuint", "kapBint",
@@ -201,6 +210,10 @@ class Oracle:
         dd.nT, dd.nnu = d.lTint.size, d.nuint.size
         dd.lTint, dd.nuint, dd.kapBint, dd.kappap = (_p(D["lTint"]), _p(D["nuint"]),
                                                      _p(D["kapBint"]), _p(D["kappap"]))
+        if bw:
+            from hochunk3d_amd.dust import ensure_kapdB
+            self._darrs["kapdBint"] = f(ensure_kapdB(d))
+            dd.kapdBint = _p(self._darrs["kapdBint"])
         self.dust = dd
         c = m.cfg
         s = Setup()
@@ -235,7 +248,13 @@ class Oracle:
             s.ntab = self._arrs["xmu"].size
             s.xmu, s.prob = _p(self._arrs["xmu"]), _p(self._arrs["prob"])
             s.delmu = 1.0 / float(s.ntab)
+        s.bw = int(bool(bw))
         self.setup = s
+
+    @property
+    def tdust(self) -> np.ndarray:
+        """The oracle's cell temperatures (updated by Bjorkman-Wood passes)."""
+        return self._arrs["tdust"]
 
     def set_temperature(self, tdust):
         self._arrs["tdust"][...] = tdust

@@ -19,7 +19,7 @@ program golden_driver
   real*8 :: xran, nu, Tk
   logical(1) :: onw
   character(len=80) :: dname(ndg)
-  real*8, external :: lucydustfreq, KapP, dusttemp
+  real*8, external :: lucydustfreq, KapP, dusttemp, dustfreq
   integer, external :: ifix_dummy
 
   call set_all_seeds(12345678, 87654321)
@@ -106,6 +106,12 @@ program golden_driver
   do i = 1, nT
     write(*,'(A,1X,I0,1X,2(ES26.17E3,1X))') 'kappap', i, kappap(i,1), lTint(i)
   end do
+  ! the Bjorkman-Wood table of the same setemiss call (kappa_abs dB/dT)
+  do i = 1, nT, 37
+    do jj = 1, nnu, 31
+      write(*,'(A,1X,I0,1X,I0,1X,ES26.17E3)') 'kapdBint', i, jj, kapdBint(i,jj,1)
+    end do
+  end do
   do j = 1, 80
     Tk = 0.12d0 * 1.1d0**j
     nu = KapP(Tk/11605.d0, 1)
@@ -127,6 +133,25 @@ program golden_driver
     Tk = 0.2d0 * 1.045d0**j
     nu = lucydustfreq(Tk/11605.d0, 1)
     write(*,'(A,1X,2(ES26.17E3,1X))') 'lucy', Tk/11605.d0, nu
+  end do
+
+  ! ---- 8b. dustfreq (Bjorkman-Wood re-emission) on the same formula table
+  !      in kapdBint, below and above the sublimation temperature (bbfreq)
+  do id = 1, ndg
+    do jj = 1, nnu
+      do i = 1, nT
+        kapdBint(i,jj,id) = (dble(jj-1)/1024.d0)**2 * (dble(i)/1000.d0) + &
+             (dble(jj-1)/1024.d0) * (1.d0 - dble(i)/1000.d0)
+      end do
+    end do
+  end do
+  Tsub = 1600.d0
+  call set_current_generator(7)
+  call reinitialize_current_generator(-1)
+  do j = 1, 200
+    Tk = 0.2d0 * 1.046d0**j
+    nu = dustfreq(Tk/11605.d0, 1)
+    write(*,'(A,1X,2(ES26.17E3,1X))') 'dustfreq', Tk/11605.d0, nu
   end do
 
   ! ---- 9. dusttemp on a formula Planck-mean table

@@ -126,6 +126,16 @@ def test_setemiss_tables(dust):
     np.testing.assert_allclose(dust.kappap[:, 0], kp[:, 1], rtol=2e-5)
 
 
+def test_setemiss_bw_table(dust):
+    """The Bjorkman-Wood table kapdBint (kappa_abs dB/dT) of the same setemiss
+    call, to the reference's QSIMP tolerance."""
+    from hochunk3d_amd.dust import kapdB_table
+    kdb = kapdB_table(dust)
+    ref = G["kapdBint"]
+    got = kdb[ref[:, 0].astype(int) - 1, ref[:, 1].astype(int) - 1, 0]
+    np.testing.assert_allclose(got, ref[:, 2], rtol=2e-5, atol=1e-12)
+
+
 def test_kapp_on_reference_table(lib, dust):
     kp = np.asfortranarray(np.tile(G["kappap"][:, 1:2], (1, 8)))
     d, keep = _dust_struct(dust, kappap=kp)
@@ -145,6 +155,22 @@ def test_lucydustfreq_on_formula_table(lib, dust):
     r = _rng(lib, 2)
     for T, nu in G["lucy"]:
         assert lib.hco_lucydustfreq(C.byref(d), C.byref(r), T, 0) == nu, T
+
+
+def test_dustfreq_on_formula_table(lib, dust):
+    """dustfreq (Bjorkman-Wood re-emission) on the formula table in
+    kapdBint, below and above the 1600 K sublimation temperature (where it
+    draws bbfreq): the reference's draws and frequencies bit for bit."""
+    d, keep = _dust_struct(dust)
+    kb = _formula_kapB(dust.lTint.size, dust.nuint.size)
+    keep["kdb"] = kb
+    d.kapdBint = kb.ctypes.data_as(C.POINTER(C.c_double))
+    r = _rng(lib, 7)
+    hot = 0
+    for T, nu in G["dustfreq"]:
+        assert lib.hco_dustfreq(C.byref(d), C.byref(r), T, 0, 1600.0) == nu, T
+        hot += T >= 1600.0 / 11605.0
+    assert 1 <= hot < len(G["dustfreq"])
 
 
 def test_dusttemp_on_formula_table(lib, dust):
