@@ -26,6 +26,7 @@ _ip = C.POINTER(C.c_int32)
 EXPORTS = (
     "hc_create", "hc_set_grid", "hc_set_temperature", "hc_get_temperature", "hc_set_dust",
     "hc_set_sources", "hc_set_atmosphere", "hc_set_limb_table", "hc_set_bw",
+    "hc_set_filters", "hc_get_filter_images",
     "hc_run_iteration", "hc_temperature", "hc_run_lucy", "hc_run_batch", "hc_set_option",
     "hc_dropped_packets", "hc_destroy",
     "hc_last_error", "hc_build_density", "hc_build_density_noalloc", "hc_build_emissivity",
@@ -120,6 +121,8 @@ def load() -> C.CDLL:
     lib.hc_set_atmosphere.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, _dp, _dp]
     lib.hc_set_limb_table.argtypes = [C.c_void_p, C.c_int32, _dp, _dp]
     lib.hc_set_bw.argtypes = [C.c_void_p, _dp, _dp]
+    lib.hc_set_filters.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _ip, _dp, _dp]
+    lib.hc_get_filter_images.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
     lib.hc_run_iteration.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
                                      C.c_int32, C.c_uint64, C.POINTER(HcTallies), C.c_void_p]
     lib.hc_temperature.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, _dp,
@@ -182,6 +185,9 @@ class PassResult:
     scalars: dict
     trace: Optional[np.ndarray]
     kernel_ms: float
+    # (image_b_i, q, u, v), each (npeel, nx, nx, nfilt): the filter images of
+    # a peeled pass when filters are set
+    filt: Optional[tuple] = None
 
 
 class HcError(RuntimeError):
@@ -212,6 +218,7 @@ class Transport:
             raise HcError(f"hc_create failed ({rc})")
         self.h = h
         self._keep = []
+        self._filters = None
         self._check(self.lib.hc_set_grid(
             h, _ptr(_f64(m.r2arr)), _ptr(_f64(m.tan2arr)), _ptr(_f64(m.thetarr)),
             _ptr(_f64(m.aarr)), _ptr(_f64(m.barr)), _ptr(_f64(m.phiarr)),
@@ -277,6 +284,32 @@ class Transport:
         kdb = _f64(ensure_kapdB(self.model.dust))
         mass = _f64(self.model.massarr)
         self._check(self.lib.hc_set_bw(self.h, _ptr(kdb), _ptr(mass)), "hc_set_bw")
+
+    def set_filters(self, filters):
+        """Filter curves (hochunk3d_amd.filters.Filters) for the filter
+        images of peeled passes (hc_set_filters); None switches them off."""
+        if filters is None:
+            self._filters = None
+            self._check(self.lib.hc_set_filters(self.h, 0, 0, None, None, None),
+                        "hc_set_filters")
+            return
+        nw = np.ascontiguousarray(filters.nwfilt, np.int32)
+        fw, fp = _f64(filters.filtwave), _f64(filters.filtphi)
+        self._check(self.lib.hc_set_filters(self.h, filters.nfilt, fw.shape[0],
+                                            _ptr(nw, C.c_int32), _ptr(fw), _ptr(fp)),
+                    "hc_set_filters")
+        self._filters = filters
+
+    def filter_images(self):
+        """(image_b_i, q, u, v) of the last peeled pass, (npeel, nx, nx, nfilt)."""
+        if self._filters is None:
+            raise HcError("filter_images: no filters set")
+        cfg = self.model.cfg
+        shp = (max(cfg.npeel, 1), cfg.nximg, cfg.nximg, self._filters.nfilt)
+        out = [np.zeros(shp, order="F") for _ in range(4)]
+        self._check(self.lib.hc_get_filter_images(self.h, *(_ptr(a) for a in out)),
+                    "hc_get_filter_images")
+        return tuple(out)
 
     def set_option(self, name: str, value: int):
         """Launch-shape knob: regen_lanes, regen_run_lanes, peel_queue,
@@ -368,8 +401,9 @@ class Transport:
         sed = np.moveaxis(sed, -1, 0)
         if peel is not None:
             peel = np.moveaxis(peel, -1, 0)
+        filt = self.filter_images() if (ipeel and self._filters is not None) else None
         return PassResult(dtauabs, nabs, jmean, killed, dbig, sed, peel, img_i, img_q, img_u,
-                          img_v, si_init, scal, tr, t.kernel_ms)
+                          img_v, si_init, scal, tr, t.kernel_ms, filt)
 
     def temperature(self, np_: int, apply: bool = False):
         """Lucy temperature update from the last pass.  apply=False returns the

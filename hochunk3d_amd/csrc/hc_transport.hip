@@ -109,6 +109,12 @@ struct Params {
   const uint16_t *cdfguide_bw;
   double *tdust_w, *densm_w;
   double tsub_ev, ltot, nphot;
+  // filter-convolved images (peeled passes): filt_mod's curves and the
+  // (npeel,nx,nx,nfilt) I, Q, U, V accumulators, one block after the other
+  int nfilt, nwfiltmax;
+  const int32_t *nwfilt;
+  const double *filtwave, *filtphi;
+  double *imgb;
 };
 
 // Launch parameters are the transport kernel's argument: every wave reads
@@ -769,6 +775,37 @@ __device__ __forceinline__ void peel_tally(const PeelRec<R, NA> &k, const PeelWa
           atomic_add(&P.peel[q + 8 * n], 1.0);
         }
       }
+    }
+  }
+  // peeloff_3d.f90:179-211: every filter whose node range holds the ray's
+  // frequency strictly weights it by the response at the last node below
+  // it (the reference scans up to it; the tables increase strictly, checked
+  // on the host, so the bisection finds the same node)
+  if (P.nfilt > 0 && ix <= P.nxhst && iy <= P.nxhst && ix > 0 && iy > 0) {
+    const double wave = 1.2398 / (double)k.nub;
+    const double freq = 2.9979e14 / wave;
+    const size_t nx = (size_t)P.nxhst;
+    const size_t nb = (size_t)P.npeel * nx * nx * (size_t)P.nfilt;
+    for (int f = 0; f < P.nfilt; ++f) {
+      const double *fw = P.filtwave + (size_t)P.nwfiltmax * f;
+      const int nf = P.nwfilt[f] - 1;
+      if (!(freq > fw[0] && freq < fw[nf - 1])) continue;
+      int lo = 0, hi = nf - 1;  // fw[lo] < freq <= fw[hi]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (freq > fw[mid])
+          lo = mid;
+        else
+          hi = mid;
+      }
+      const double wt = P.filtphi[(size_t)P.nwfiltmax * f + lo];
+      const size_t q = (size_t)peelid +
+                       (size_t)P.npeel * ((size_t)(ix - 1) +
+                                          nx * ((size_t)(iy - 1) + nx * (size_t)f));
+      atomic_add(&P.imgb[q], (double)s.I * wt);
+      atomic_add(&P.imgb[q + nb], (double)s.Q * wt);
+      atomic_add(&P.imgb[q + 2 * nb], (double)s.U * wt);
+      atomic_add(&P.imgb[q + 3 * nb], (double)s.V * wt);
     }
   }
   if (P.imcube && ix <= P.nxhst && iy <= P.nxhst && ix > 0 && iy > 0) {
@@ -2387,6 +2424,12 @@ struct hc_ctx {
   int serial = 0;      // one lane runs the packets in order (hc_set_option "serial")
   double *kapdBint = nullptr;
   uint16_t *cdfguide_bw = nullptr;
+  // filter curves (hc_set_filters) and the filter images of the last pass
+  int nfilt = 0, nwfiltmax = 0;
+  int32_t *nwfilt = nullptr;
+  double *filtwave = nullptr, *filtphi = nullptr, *imgb = nullptr;
+  size_t imgb_words = 0;
+  bool imgb_valid = false;
   int pool_min = 48;   // requests a walking wavefront takes at once
   int pool_walk = 16;  // walking lanes below which a wavefront takes any request
   uint32_t max_work = 1u << 24;
@@ -2972,6 +3015,22 @@ int prepare_pass(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int32_t 
   P.tsub_ev = ctx->tsub / 11605.0;
   P.ltot = S.ltot;
   P.nphot = (double)np;
+  if (P.ipeel && ctx->nfilt > 0) {
+    const size_t words = (size_t)4 * std::max(d.npeel, 1) * d.nxhst * d.nxhst * ctx->nfilt;
+    if (ctx->imgb && ctx->imgb_words < words) {
+      HCK(hipFree(ctx->imgb));
+      ctx->imgb = nullptr;
+    }
+    int rc2 = dalloc(ctx, &ctx->imgb, words);
+    if (rc2) return rc2;
+    ctx->imgb_words = words;
+    P.nfilt = ctx->nfilt;
+    P.nwfiltmax = ctx->nwfiltmax;
+    P.nwfilt = ctx->nwfilt;
+    P.filtwave = ctx->filtwave;
+    P.filtphi = ctx->filtphi;
+    P.imgb = ctx->imgb;
+  }
 
   TransportKernel &kern = L.kern;
   size_t rec_bytes = 0;
@@ -3092,6 +3151,12 @@ int zero_tallies(hc_ctx *ctx, const PassLaunch &L, int64_t np, bool trace) {
   z(ctx->scal, S_COUNT * sizeof(double));
   z(ctx->counter, kCounters * sizeof(unsigned long long));
   if (trace) z(ctx->trace, (size_t)np * sizeof(hc_trace));
+  // only a peeled pass with filters refills the filter images; the
+  // images of the last such pass stay readable across Lucy passes
+  if (L.P.nfilt > 0) {
+    z(ctx->imgb, ctx->imgb_words * sizeof(double));
+    ctx->imgb_valid = true;
+  }
   if (L.cube) {
     const size_t w = d.imcube == 2 ? sizeof(float) : sizeof(double);
     z(ctx->img_i, L.nimg * kNo * w);
@@ -3659,6 +3724,53 @@ int hc_set_bw(hc_ctx *ctx, const double *kapdBint, const double *massarr) {
   return HC_OK;
 }
 
+int hc_set_filters(hc_ctx *ctx, int32_t nfilt, int32_t nwfiltmax, const int32_t *nwfilt,
+                   const double *filtwave, const double *filtphi) {
+  if (!ctx) return HC_E_ARG;
+  if (nfilt == 0) {
+    ctx->nfilt = 0;
+    ctx->imgb_valid = false;
+    return HC_OK;
+  }
+  if (nfilt < 0 || nwfiltmax < 2 || !nwfilt || !filtwave || !filtphi)
+    return fail(ctx, HC_E_ARG, "hc_set_filters: bad arguments");
+  for (int f = 0; f < nfilt; ++f) {
+    const int nf = nwfilt[f] - 1;
+    if (nf < 2 || nf > nwfiltmax) return fail(ctx, HC_E_ARG, "hc_set_filters: nwfilt out of range");
+    const double *fw = filtwave + (size_t)nwfiltmax * f;
+    for (int i = 1; i < nf; ++i)
+      if (!(fw[i] > fw[i - 1]))
+        return fail(ctx, HC_E_ARG, "hc_set_filters: filter frequencies must increase strictly");
+  }
+  HCK(hipSetDevice(ctx->device));
+  const size_t n = (size_t)nwfiltmax * nfilt;
+  for (void **p : {(void **)&ctx->nwfilt, (void **)&ctx->filtwave, (void **)&ctx->filtphi})
+    if (*p) {
+      HCK(hipFree(*p));
+      *p = nullptr;
+    }
+  int rc = upload(ctx, &ctx->nwfilt, nwfilt, (size_t)nfilt);
+  if (!rc) rc = upload(ctx, &ctx->filtwave, filtwave, n);
+  if (!rc) rc = upload(ctx, &ctx->filtphi, filtphi, n);
+  if (rc) return rc;
+  ctx->nfilt = nfilt;
+  ctx->nwfiltmax = nwfiltmax;
+  ctx->imgb_valid = false;
+  return HC_OK;
+}
+
+int hc_get_filter_images(hc_ctx *ctx, double *image_b_i, double *image_b_q, double *image_b_u,
+                         double *image_b_v) {
+  if (!ctx) return HC_E_ARG;
+  if (!ctx->imgb_valid || !ctx->imgb)
+    return fail(ctx, HC_E_STATE, "hc_get_filter_images: no peeled pass with filters yet");
+  const size_t nb = ctx->imgb_words / 4;
+  double *out[4] = {image_b_i, image_b_q, image_b_u, image_b_v};
+  for (int k = 0; k < 4; ++k)
+    if (out[k]) HC_D2H(out[k], ctx->imgb + nb * k, nb * sizeof(double));
+  return HC_OK;
+}
+
 int hc_dropped_packets(hc_ctx *ctx, int64_t *pids, int32_t cap, int64_t *ndropped) {
   if (!ctx || !ndropped || cap < 0 || (cap > 0 && !pids)) return HC_E_ARG;
   if (!ctx->counter) return fail(ctx, HC_E_STATE, "hc_dropped_packets: no pass has run");
@@ -3721,7 +3833,8 @@ int hc_destroy(hc_ctx *ctx) {
                   ctx->killed, ctx->dbig,    ctx->sed,     ctx->peel,    ctx->img_i,  ctx->img_q,
                   ctx->img_u,  ctx->img_v,   ctx->si_init, ctx->scal,    ctx->mass4,  ctx->counter,
                   ctx->trace,  ctx->guide,   ctx->tdust2,  ctx->cdfguide, ctx->segdev,
-                  ctx->atm,    ctx->atm_guide, ctx->limb_tab, ctx->kapdBint, ctx->cdfguide_bw};
+                  ctx->atm,    ctx->atm_guide, ctx->limb_tab, ctx->kapdBint, ctx->cdfguide_bw,
+                  ctx->nwfilt, ctx->filtwave, ctx->filtphi, ctx->imgb};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);

@@ -148,6 +148,24 @@ module hc_iface
       real(c_double), intent(out) :: device_ms
     end function hc_run_batch
 
+    ! filter curves of filt_mod (after filt) for the filter images
+    integer(c_int) function hc_set_filters(ctx, nfilt, nwfiltmax, nwfilt, filtwave, filtphi) &
+        bind(C, name='hc_set_filters')
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nfilt, nwfiltmax
+      integer(c_int32_t), intent(in) :: nwfilt(*)
+      real(c_double), intent(in) :: filtwave(*), filtphi(*)
+    end function hc_set_filters
+
+    ! image_b_{i,q,u,v} (npeel,nxhst,nxhst,nfilt) of the last peeled pass
+    integer(c_int) function hc_get_filter_images(ctx, bi, bq, bu, bv) &
+        bind(C, name='hc_get_filter_images')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: bi(*), bq(*), bu(*), bv(*)
+    end function hc_get_filter_images
+
     ! Bjorkman-Wood re-emission (CLUCY=NO): kapdBint (nT,nnu,ndg) from setemiss
     ! and massarr (nrg,ntg,npg,ndg); c_null_ptr for both returns to Lucy's
     integer(c_int) function hc_set_bw(ctx, kapdBint, massarr) bind(C, name='hc_set_bw')

@@ -14,6 +14,10 @@ and formats of the reference's ASCII build (output.F90, flux.f90).
 * `write_image_cube` has no counterpart in the ASCII build, which never
   writes the IMCUBE cube (only output_FITS.F90 does, through cfitsio): it
   writes the normalised cube as one Fortran unformatted file.
+* `write_filter_images` writes the filter-convolved peel images
+  e_<filter>_<theta>_<phi>_{I,Q,U,V}_img.dat (output.F90:380-411), after
+  `diskflux` has divided Q, U, V by I and put I in intensity units
+  (flux.f90:87-102).
 * `write_run_outputs` writes what output() writes for one pass.
 
 Formatted output follows the Fortran edit descriptors (Fw.d, ESw.d, Iw) and
@@ -155,6 +159,10 @@ class Fluxes:
     image_m_q: Optional[np.ndarray] = None
     image_m_u: Optional[np.ndarray] = None
     image_m_v: Optional[np.ndarray] = None
+    image_b_i: Optional[np.ndarray] = None
+    image_b_q: Optional[np.ndarray] = None
+    image_b_u: Optional[np.ndarray] = None
+    image_b_v: Optional[np.ndarray] = None
 
 
 def nunorm_of(nfreq: int, numin: float, numax: float = NUMAX) -> float:
@@ -193,12 +201,13 @@ def _sed_error(si, x, x2, nums, first: bool):
 
 def diskflux(model: Model, np_: float, sed: np.ndarray, peel: Optional[np.ndarray] = None,
              si_init: Optional[np.ndarray] = None, cube=None,
-             numax: float = NUMAX) -> Fluxes:
+             numax: float = NUMAX, filt=None) -> Fluxes:
     """flux.f90:23-121 on one pass's tallies.
 
     sed: (9, nfreq, nmu, nph, nap, no) I,Q,U,V,I^2,Q^2,U^2,V^2,N sums;
     peel: (9, nfreq, npeel, nap, no) likewise; si_init: (nfreq, 5);
-    cube: (img_i (npeel,nx,nx,nfreq,no), img_q, img_u, img_v (npeel,nx,nx,nfreq)).
+    cube: (img_i (npeel,nx,nx,nfreq,no), img_q, img_u, img_v (npeel,nx,nx,nfreq));
+    filt: the filter images (image_b_i, q, u, v), each (npeel,nx,nx,nfilt).
     np_ is the pass's packet count (rnp)."""
     sed = np.asarray(sed)
     nfreq, nmu, nph = sed.shape[1:4]
@@ -227,12 +236,22 @@ def diskflux(model: Model, np_: float, sed: np.ndarray, peel: Optional[np.ndarra
             out.ti2, out.tq2, out.tu2, out.tv2 = (perr(ti, ti2), perr(tq, tq2), perr(tu, tu2),
                                                   perr(tv, tv2))
             out.ti, out.tq, out.tu, out.tv = ti / q, tq / q, tu / q, tv / q       # :80-83
+        def image_norm_intensity(nx):                                        # :37
+            return (rnp / ltot * 4.0 * PI / 1.0e17
+                    * (2.0 * model.rmaxi / model.autors / float(nx)) ** 2
+                    * PC2CM ** 2 / 206265.0 ** 2)
+
+        if filt is not None:
+            bi, bq, bu, bv = (np.asarray(a).astype(_F4) for a in filt)
+            pos = bi > _F4(0.0)
+            out.image_b_q = np.where(pos, bq / bi, _F4(0.0)).astype(_F4)     # :87-98
+            out.image_b_u = np.where(pos, bu / bi, _F4(0.0)).astype(_F4)
+            out.image_b_v = np.where(pos, bv / bi, _F4(0.0)).astype(_F4)
+            out.image_b_i = bi / _F4(image_norm_intensity(bi.shape[1]))      # :101
         if cube is not None:
             mi, mq, mu_, mv = (np.asarray(a).astype(_F4) for a in cube)
             nx = mi.shape[1]
-            # :37, intensity units of the images
-            inorm = (rnp / ltot * 4.0 * PI / 1.0e17 * (2.0 * model.rmaxi / model.autors / float(nx)) ** 2
-                   * PC2CM ** 2 / 206265.0 ** 2)
+            inorm = image_norm_intensity(nx)
             i1 = mi[..., 0]
             pos = i1 > _F4(0.0)
             mq = np.where(pos, mq / i1, _F4(0.0)).astype(_F4)                # :107-115
@@ -460,26 +479,61 @@ def write_image_cube(path: str, fl: Fluxes) -> None:
             _record(fh, a.astype("<f4"))
 
 
+def write_filter_images(dirpath: str, fl: Fluxes, names: Sequence[str],
+                        thete: Sequence[float], phie: Sequence[float],
+                        partial: bool = False) -> list:
+    """e_<filter>_<theta>_<phi>_{I,Q,U,V}_img<suffix>.dat for each observer
+    and filter (output.F90:380-411): one row per iy, the nx values of ix in
+    '(nx(ES11.4,1X))'; thete/phie are the observers' angles in radians."""
+    if fl.image_b_i is None:
+        raise ValueError("no filter images (run a peeled pass with filters set)")
+    npeel, nx, _, nfilt = fl.image_b_i.shape
+    if len(names) != nfilt:
+        raise ValueError("one name per filter is needed")
+    if len(thete) < npeel or len(phie) < npeel:
+        raise ValueError("one (theta, phi) per observer is needed")
+    psuf = "_partial" if partial else ""
+    paths = []
+    for pk in range(npeel):
+        for ift in range(nfilt):
+            beg = f"e_{names[ift].strip()}_{angle_label(thete[pk])}_{angle_label(phie[pk])}"
+            for tag, a in (("I", fl.image_b_i), ("Q", fl.image_b_q), ("U", fl.image_b_u),
+                           ("V", fl.image_b_v)):
+                path = os.path.join(dirpath, f"{beg}_{tag}_img{psuf}.dat")
+                with open(path, "w") as fh:
+                    for iy in range(nx):
+                        fh.write("".join(fmt_es(a[pk, ix, iy, ift], 11, 4) + " "
+                                         for ix in range(nx)) + "\n")
+                paths.append(path)
+    return paths
+
+
 def write_run_outputs(dirpath: str, model: Model, res, np_: float, tdust: np.ndarray,
                       tdust2: np.ndarray, dsg: Optional[np.ndarray] = None,
-                      partial: bool = False) -> list:
+                      partial: bool = False,
+                      filter_names: Optional[Sequence[str]] = None) -> list:
     """What output() writes (output.F90:167-503) for one pass `res` (a
     PassResult): flux*.dat, peel_flux*.dat when the pass peeled, nabs.unf,
     dsgratio.unf (dsg/(dbig+dsg); dsg counts the small-grain events, which
     this transport path does not have, so it defaults to zeros), tmidplane*.dat,
-    tave*.dat; and imcube.unf when the pass filled the cube.  tdust2 is the
+    tave*.dat; imcube.unf when the pass filled the cube; and the filter
+    images when the pass tallied them and filter_names names the filters.  tdust2 is the
     temperature after the update, tdust the one the pass ran with
     (initarr.f90:82 copies tdust2 into tdust for the next pass)."""
     os.makedirs(dirpath, exist_ok=True)
     cube = None
     if getattr(res, "img_i", None) is not None:
         cube = (res.img_i, res.img_q, res.img_u, res.img_v)
-    fl = diskflux(model, np_, res.sed, res.peel, res.si_init, cube)
+    filt = getattr(res, "filt", None) if filter_names is not None else None
+    fl = diskflux(model, np_, res.sed, res.peel, res.si_init, cube, filt=filt)
     paths = write_flux(dirpath, fl, model.numin)
     if fl.ti is not None:
         thete = np.arctan2(model.sinte, model.coste)
         phie = np.arctan2(model.sinpe, model.cospe)
         paths += write_peel_flux(dirpath, fl, thete, phie, model.numin, partial=partial)
+        if fl.image_b_i is not None:
+            paths += write_filter_images(dirpath, fl, filter_names, thete, phie,
+                                         partial=partial)
     p = os.path.join(dirpath, "nabs.unf")
     write_grid_unf(p, model, res.nabs)
     paths.append(p)

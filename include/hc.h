@@ -184,6 +184,24 @@ int hc_set_limb_table(hc_ctx *ctx, int32_t ntab, const double *xmu, const double
  * to Lucy re-emission. */
 int hc_set_bw(hc_ctx *ctx, const double *kapdBint, const double *massarr);
 
+/* Filter curves for the filter-convolved images of peeled passes
+ * (IMFILT, peeloff_3d.f90:182-211): filt_mod's arrays after filt()
+ * (filt_nu.f90): nwfilt[nfilt] line counts of the filter files (nodes + 1),
+ * filtwave (nwfiltmax,nfilt) frequencies in Hz increasing strictly per
+ * filter, filtphi (nwfiltmax,nfilt) responses normalised over frequency.
+ * nfilt = 0 switches the images off (the default). */
+int hc_set_filters(hc_ctx *ctx, int32_t nfilt, int32_t nwfiltmax, const int32_t *nwfilt,
+                   const double *filtwave, const double *filtphi);
+
+/* The filter images of the last peeled pass, image_b_{i,q,u,v}
+ * (npeel,nxhst,nxhst,nfilt), FP64 (the reference's are real*4): a ray at
+ * frequency nu adds its Stokes vector times the response at the last node
+ * below nu of every filter whose node range holds nu.  Lucy passes leave
+ * them as they are; hc_set_filters discards them (HC_E_STATE until the next
+ * peeled pass).  Any pointer may be NULL. */
+int hc_get_filter_images(hc_ctx *ctx, double *image_b_i, double *image_b_q, double *image_b_u,
+                         double *image_b_v);
+
 /* One transport pass (the body of disk(), newdisktherm.F90:194-547): np
  * packets, peel-off when ipeel=1.  Packet p (0-based) draws from segment
  * seg0+p of L'Ecuyer-Cote generator gen seeded with (seed1,seed2)

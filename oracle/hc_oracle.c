@@ -1171,6 +1171,32 @@ static void peeloff(ctx_t *c, const pkt_t *P0, double sip, int idust2, int peeli
   ximage = yp1 * cospe - xp1 * sinpe;
   ix = (int)((ximage + s->rmaxi) / (2.0 * s->rmaxi / (double)s->nxhst)) + 1;
   iy = (int)((yimage + s->rmaxi) / (2.0 * s->rmaxi / (double)s->nxhst)) + 1;
+  /* peeloff_3d.f90:179-211: the filter-convolved images.  For every filter
+   * whose node range holds the ray's frequency strictly, the weight is the
+   * response at the last node below it (the reference's upward scan) */
+  if (s->nfilt > 0 && t->image_b_i && ix <= s->nxhst && iy <= s->nxhst && ix > 0 && iy > 0) {
+    const double wave = 1.2398 / P0->nub;
+    const double freq = 2.9979e14 / wave;
+    int ifilt;
+    for (ifilt = 0; ifilt < s->nfilt; ifilt++) {
+      const double *fw = s->filtwave + (size_t)s->nwfiltmax * ifilt;
+      const int nf = s->nwfilt[ifilt] - 1;
+      if (freq > fw[0] && freq < fw[nf - 1]) {
+        int iw = 1;
+        while (freq > fw[iw - 1]) iw++;
+        iw = iw - 1;
+        const double wt = s->filtphi[(size_t)s->nwfiltmax * ifilt + (size_t)(iw - 1)];
+        const size_t q = (size_t)peelid +
+                         (size_t)s->npeel * ((size_t)(ix - 1) +
+                                             (size_t)s->nxhst * ((size_t)(iy - 1) +
+                                                                 (size_t)s->nxhst * ifilt));
+        t->image_b_i[q] += sip * wt;
+        t->image_b_q[q] += sqp * wt;
+        t->image_b_u[q] += sup * wt;
+        t->image_b_v[q] += svp * wt;
+      }
+    }
+  }
   rho2 = ximage * ximage + yimage * yimage;
   inub = clampi((int)((double)s->nfreq * HCO_LOG(P0->nub / s->numin) / s->lnurat) + 1, 1, s->nfreq);
   if (P0->iflag == 1)

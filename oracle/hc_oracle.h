@@ -122,6 +122,12 @@ typedef struct {
    * absorption count so far (dusttemp with nphot = np), and needs
    * dust->kapdBint, tallies->nabs and grid->massarr; 0: Lucy (lucydustfreq) */
   int bw;
+  /* filter curves of filt_mod (filt_nu.f90): nfilt filters, nwfilt[nfilt]
+   * line counts (nodes + 1), filtwave/filtphi (nwfiltmax,nfilt) with the
+   * frequencies (Hz) increasing; used by peeled passes when nfilt > 0 */
+  int nfilt, nwfiltmax;
+  const int32_t *nwfilt;
+  const double *filtwave, *filtphi;
 } hco_setup;
 
 typedef struct {
@@ -139,6 +145,9 @@ typedef struct {
   double tot, flux, sflux, nscat, scount, abflux, aflux, sumsub, sumall;
   double hit, htot;
   int64_t n_err_nowall, n_err_dust, n_warn;
+  /* filter-convolved images (npeel,nx,nx,nfilt) of I,Q,U,V (peeloff_3d.f90:
+   * 182-211), or NULL */
+  double *image_b_i, *image_b_q, *image_b_u, *image_b_v;
 } hco_tallies;
 
 /* Per-packet record for the deterministic-trace comparison. */

@@ -54,7 +54,8 @@ class Setup(C.Structure):
         ("fmass", C.c_double * NDG), ("z1", C.c_double * NDG), ("bflare", C.c_double * NDG),
         ("nhnu", C.c_int), ("atmosnu", _dp), ("hnuint", _dp), ("lognu", _dp), ("loghnu", _dp),
         ("ntab", C.c_int), ("xmu", _dp), ("prob", _dp), ("delmu", C.c_double),
-        ("bw", C.c_int)]
+        ("bw", C.c_int), ("nfilt", C.c_int), ("nwfiltmax", C.c_int), ("nwfilt", _ip),
+        ("filtwave", _dp), ("filtphi", _dp)]
 
 
 class Tallies(C.Structure):
@@ -63,7 +64,8 @@ class Tallies(C.Structure):
                 ("img_q", _dp), ("img_u", _dp), ("img_v", _dp), ("si_init", _dp)] + [
         (n, C.c_double) for n in ("tot", "flux", "sflux", "nscat", "scount", "abflux", "aflux",
                                   "sumsub", "sumall", "hit", "htot")] + [
-        ("n_err_nowall", C.c_int64), ("n_err_dust", C.c_int64), ("n_warn", C.c_int64)]
+        ("n_err_nowall", C.c_int64), ("n_err_dust", C.c_int64), ("n_warn", C.c_int64),
+        ("image_b_i", _dp), ("image_b_q", _dp), ("image_b_u", _dp), ("image_b_v", _dp)]
 
 
 class Rng(C.Structure):
@@ -251,6 +253,22 @@ class Oracle:
         s.bw = int(bool(bw))
         self.setup = s
 
+    def set_filters(self, filters):
+        """Filter curves (hochunk3d_amd.filters.Filters) for the
+        filter-convolved images of peeled passes; None removes them."""
+        s = self.setup
+        if filters is None:
+            s.nfilt = 0
+            self._filters = None
+            return
+        self._farrs = (np.ascontiguousarray(filters.nwfilt, np.int32),
+                       np.asfortranarray(filters.filtwave, np.float64),
+                       np.asfortranarray(filters.filtphi, np.float64))
+        s.nfilt, s.nwfiltmax = filters.nfilt, filters.filtwave.shape[0]
+        s.nwfilt = _p(self._farrs[0], C.c_int32)
+        s.filtwave, s.filtphi = _p(self._farrs[1]), _p(self._farrs[2])
+        self._filters = filters
+
     @property
     def tdust(self) -> np.ndarray:
         """The oracle's cell temperatures (updated by Bjorkman-Wood passes)."""
@@ -283,6 +301,11 @@ class Oracle:
         t.dtauabs, t.jmean, t.peel = _p(out["dtauabs"]), _p(out["jmean"]), _p(out["peel"])
         t.img_i, t.img_q, t.img_u, t.img_v = (_p(out[k]) for k in ("img_i", "img_q", "img_u",
                                                                      "img_v"))
+        if getattr(self, "_filters", None) is not None:
+            nb = (max(c.npeel, 1), c.nximg, c.nximg, self._filters.nfilt)
+            for k in ("image_b_i", "image_b_q", "image_b_u", "image_b_v"):
+                out[k] = np.zeros(nb, order="F")
+                setattr(t, k, _p(out[k]))
         out["_struct"] = t
         return out
 
@@ -344,6 +367,11 @@ class Oracle:
         t.sed, t.peel, t.si_init = _p(out["sed"]), _p(out["peel"]), _p(out["si_init"])
         t.img_i, t.img_q, t.img_u, t.img_v = (_p(out[k]) for k in ("img_i", "img_q", "img_u",
                                                                      "img_v"))
+        if ipeel and getattr(self, "_filters", None) is not None:
+            nb = (max(c.npeel, 1), c.nximg, c.nximg, self._filters.nfilt)
+            for k in ("image_b_i", "image_b_q", "image_b_u", "image_b_v"):
+                out[k] = np.zeros(nb, order="F")
+                setattr(t, k, _p(out[k]))
         tr = np.zeros(p_end - p_begin, TRACE_DTYPE) if trace else None
         rc = self.lib.hco_run_pass(C.byref(self.grid), C.byref(self.dust), C.byref(self.setup),
                                    C.byref(t), int(np_), int(p_begin), int(p_end), int(ipeel),

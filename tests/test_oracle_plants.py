@@ -11,6 +11,7 @@ import subprocess
 import pytest
 
 import test_atmosphere as at
+import test_filters as fi
 import test_oracle_closed_forms as cf
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -56,6 +57,16 @@ PLANTS = [
      "(s->prob[n] - s->prob[n - 1]) * 2.0 * s->delmu;", ["limb"]),
     ("limb peel weight 2 (mu + mu^2)", "? 12.0 / 5.0 * P->sip", "? 2.0 * P->sip",
      ["limbpeel"]),
+    ("filter weight from the node above",
+     "iw = iw - 1;\n        const double wt = s->filtphi[(size_t)s->nwfiltmax * ifilt + (size_t)(iw - 1)];",
+     "const double wt = s->filtphi[(size_t)s->nwfiltmax * ifilt + (size_t)(iw - 1)];",
+     ["filter"]),
+    ("filter range closed at the top", "if (freq > fw[0] && freq < fw[nf - 1]) {",
+     "if (freq > fw[0] && freq <= fw[nf - 1]) {", ["filteredge"]),
+    ("filter range closed at the bottom", "if (freq > fw[0] && freq < fw[nf - 1]) {",
+     "if (freq >= fw[0] && freq < fw[nf - 1]) {", ["filteredge"]),
+    ("filter Q unweighted", "t->image_b_q[q] += sqp * wt;", "t->image_b_q[q] += sqp;",
+     ["filter"]),
 ]
 
 
@@ -78,6 +89,8 @@ def _checks():
                   at.test_atmosfreq_fills_brackets_with_their_probability_and_log_law],
         "limb": [at.test_limb_darkening_cosine_law],
         "limbpeel": [at.test_limb_darkened_disk_flux_and_peel_weight],
+        "filter": [fi.test_filter_image_pixel_weight_and_stokes],
+        "filteredge": [fi.test_filter_image_outside_every_node_range_is_empty],
     }
 
 
@@ -92,7 +105,7 @@ def test_planted_mistake_is_caught(tmp_path, name, orig, planted, checks):
                                        "-lm"], check=True)
     table = _checks()
     old = cf.ORACLE
-    cf.ORACLE = at.ORACLE = str(so)
+    cf.ORACLE = at.ORACLE = fi.ORACLE = str(so)
     caught = []
     try:
         for key in checks:
@@ -102,5 +115,5 @@ def test_planted_mistake_is_caught(tmp_path, name, orig, planted, checks):
                 except AssertionError:
                     caught.append(key)
     finally:
-        cf.ORACLE = at.ORACLE = old
+        cf.ORACLE = at.ORACLE = fi.ORACLE = old
     assert caught, f"planted mistake '{name}' passed every check {checks}"
