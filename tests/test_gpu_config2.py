@@ -1,0 +1,71 @@
+"""The full 200x199x200-cell fractal model (the Lucy benchmark's model) on the
+GPU, by properties that hold at any size: no packet lost on a wall, dropped
+at the work bound or warned about; every packet escapes with weight 1 or is
+killed; a Lucy temperature update moves the cells; a peeled pass puts the
+peeled SED's energy into the image cube per observer.  The packet-exact
+comparison with the oracle runs on reduced grids of the same construction
+(test_gpu_parity.py)."""
+import time
+
+import numpy as np
+import pytest
+
+from hochunk3d_amd import build_model, config2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def model():
+    t0 = time.time()
+    # the image spans the whole grid (rmax 5000 AU) so that it holds every peeled ray
+    # the model's initial temperatures, as the benchmark's first iteration has
+    # them (a synthetic warm profile leaves the inner midplane unmasked and
+    # one packet walks ~7e6 steps there)
+    m = build_model(config2(rmaxi_au=5000.0), device=0)
+    print(f"config2 {m.nrg - 1}x{m.ntg - 1}x{m.npg - 1} cells built in {time.time() - t0:.1f}s",
+          flush=True)
+    return m
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_full_model_lucy_pass_bookkeeping(model, precision):
+    from hochunk3d_amd._lib import HC_FP32, HC_FP64, Transport
+    g = Transport(model, precision=HC_FP64 if precision == "fp64" else HC_FP32)
+    try:
+        n = 2_000_000
+        s, ms = g.run_scalars(n)
+        print(f"{precision} lucy pass {n} packets: {ms:.0f} ms, "
+              f"longest {s['max_packet_work']:.0f}", flush=True)
+        assert s["n_err_nowall"] == 0 and s["n_err_dust"] == 0 and s["n_warn"] == 0
+        assert g.dropped_packets()[0] == 0
+        tol = 1e-9 if precision == "fp64" else 1e-5
+        assert abs(s["sumall"] + s["abflux"] - n) <= tol * n
+        assert s["nscat"] > n and s["tot"] > 0
+        t0 = g.get_temperature()
+        g.temperature(n, apply=True)
+        t1 = g.get_temperature()
+        live = model.densarr[:-1, :-1, :-1, :] > 0
+        moved = (t1[:-1, :-1, :-1, :] != t0[:-1, :-1, :-1, :]) & live
+        assert moved.sum() > 0.5 * live.sum()
+        assert np.isfinite(t1).all() and (t1[:-1, :-1, :-1, :][live] > 0).all()
+    finally:
+        g.close()
+
+
+def test_full_model_peeled_pass_cube_holds_the_peeled_sed(model):
+    from hochunk3d_amd._lib import Transport
+    g = Transport(model)
+    try:
+        npl = 20_000
+        r = g.run(npl, ipeel=1, seg0=77)
+        print(f"peeled pass {npl} packets: {r.kernel_ms:.0f} ms", flush=True)
+    finally:
+        g.close()
+    sc = r.scalars
+    assert sc["n_err_nowall"] == 0 and sc["n_warn"] == 0
+    big = r.peel.shape[3] - 1
+    tot = r.img_i[..., 0].sum(axis=(1, 2, 3))
+    sed = r.peel[0, :, :, big, 0].sum(axis=0)
+    assert np.all(sed > 0)
+    np.testing.assert_allclose(tot, sed, rtol=1e-9)
