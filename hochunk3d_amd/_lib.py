@@ -26,7 +26,7 @@ _ip = C.POINTER(C.c_int32)
 EXPORTS = (
     "hc_create", "hc_set_grid", "hc_set_temperature", "hc_get_temperature", "hc_set_dust",
     "hc_set_sources", "hc_set_atmosphere", "hc_set_limb_table", "hc_set_bw",
-    "hc_set_filters", "hc_get_filter_images",
+    "hc_set_filters", "hc_get_filter_images", "hc_set_diffusion",
     "hc_run_iteration", "hc_temperature", "hc_run_lucy", "hc_run_batch", "hc_set_option",
     "hc_dropped_packets", "hc_destroy",
     "hc_last_error", "hc_build_density", "hc_build_density_noalloc", "hc_build_emissivity",
@@ -123,6 +123,8 @@ def load() -> C.CDLL:
     lib.hc_set_bw.argtypes = [C.c_void_p, _dp, _dp]
     lib.hc_set_filters.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _ip, _dp, _dp]
     lib.hc_get_filter_images.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+    lib.hc_set_diffusion.argtypes = [C.c_void_p, C.POINTER(C.c_int8), C.POINTER(C.c_int8), _dp,
+                                     _dp]
     lib.hc_run_iteration.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
                                      C.c_int32, C.c_uint64, C.POINTER(HcTallies), C.c_void_p]
     lib.hc_temperature.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, _dp,
@@ -231,6 +233,8 @@ class Transport:
             _ptr(_f64(d.kappav)), _ptr(_f64(d.lTint)), _ptr(_f64(d.nuint)),
             _ptr(_f64(d.kapBint)), _ptr(_f64(d.kappap))), "hc_set_dust")
         self.set_sources()
+        if getattr(m, "diffus", None) is not None:  # DIFFUSION=YES
+            self.set_diffusion()
 
     def set_sources(self, **over):
         m = self.model
@@ -299,6 +303,24 @@ class Transport:
                                             _ptr(nw, C.c_int32), _ptr(fw), _ptr(fp)),
                     "hc_set_filters")
         self._filters = filters
+
+    def set_diffusion(self, on: bool = True):
+        """The diffusion approximation with the model's layer (model.diffus,
+        model.diffdir from hochunk3d_amd.diffusion) and Rosseland table
+        (hc_set_diffusion); on=False switches it off."""
+        if not on:
+            self._check(self.lib.hc_set_diffusion(self.h, None, None, None, None),
+                        "hc_set_diffusion")
+            return
+        from .dust import ensure_kappar
+        m = self.model
+        if m.diffus is None:
+            raise HcError("set_diffusion: the model has no diffusion layer")
+        dif = np.asfortranarray(m.diffus, np.int8)
+        ddir = np.asfortranarray(m.diffdir, np.int8)
+        self._check(self.lib.hc_set_diffusion(
+            self.h, _ptr(dif, C.c_int8), _ptr(ddir, C.c_int8), _ptr(_f64(m.rarr)),
+            _ptr(_f64(ensure_kappar(m.dust)))), "hc_set_diffusion")
 
     def filter_images(self):
         """(image_b_i, q, u, v) of the last peeled pass, (npeel, nx, nx, nfilt)."""

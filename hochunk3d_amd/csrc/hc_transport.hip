@@ -115,6 +115,13 @@ struct Params {
   const int32_t *nwfilt;
   const double *filtwave, *filtphi;
   double *imgb;
+  // diffusion approximation (gridset.f90:666-789, emit_common.f90:80-387):
+  // the layer mask and direction per cell (int8, cell_of layout; null when
+  // off), the radial walls, sin of the theta walls, the Rosseland table
+  // kappar[id][nT], and the pass's stellar packet count
+  const int8_t *diffus, *diffdir;
+  const double *rarr, *sintarr, *kappar;
+  double nstar;
 };
 
 // Launch parameters are the transport kernel's argument: every wave reads
@@ -1249,6 +1256,272 @@ __device__ __forceinline__ void acc_position(Mrg &rng, double &x, double &y, dou
   }
 }
 
+// ---------------------------------------------------------------------------
+// Diffusion approximation (emit_common.f90:80-387, Lucy mode): a packet
+// absorbed in (or launched into) a cell of the optically thick layer that
+// gridset marks (gridset.f90:666-789) leaves from the layer's surface.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool dif_at(int ir, int it, int ip) {
+  const Params &P = c_P;
+  if (ir < 1 || ir > P.nrg || it < 1 || it > P.ntg) return false;
+  return P.diffus[cell_of(ir, it, ip)] != 0;
+}
+__device__ __forceinline__ int dir_at(int ir, int it, int ip) {
+  const Params &P = c_P;
+  if (ir < 1 || ir > P.nrg || it < 1 || it > P.ntg) return 0;
+  return P.diffdir[cell_of(ir, it, ip)];
+}
+__device__ __forceinline__ double pow4(double x) {  // x**4 as (x*x)*(x*x)
+  x = x * x;
+  return x * x;
+}
+// chiR (dust.f90:83-111): the Rosseland table at T, linear in ln T
+__device__ __forceinline__ double chir(double T, int id) {
+  const Params &P = c_P;
+  const int nT = P.nT;
+  const double *kr = P.kappar + (size_t)nT * id;
+  const double lT = hcm_log(T);
+  const int k = (int)((double)(nT - 1) * (lT - P.lTint[0]) / (P.lTint[nT - 1] - P.lTint[0])) + 1;
+  if (k < 1 || k > nT - 1) return k < 1 ? kr[0] : kr[nT - 1];
+  const double f = (lT - P.lTint[k - 1]) / (P.lTint[k] - P.lTint[k - 1]);
+  return (1.0 - f) * kr[k - 1] + f * kr[k];
+}
+// isotrp.f90:1-36: a direction about s with density ~ |u.s| (rejection in
+// the cube, reversed when against s)
+__device__ __forceinline__ void isotrp(Mrg &rng, double sux, double suy, double suz, double &ux,
+                                       double &uy, double &uz) {
+  double x = 0.0, y = 0.0, z = 0.0, r2 = 2.0, mur = 0.0;
+  while (r2 * r2 > fabs(mur)) {
+    x = ran(rng);
+    x = x + x - 1.0;
+    y = ran(rng);
+    y = y + y - 1.0;
+    z = ran(rng);
+    z = z + z - 1.0;
+    mur = x * sux + y * suy + z * suz;
+    r2 = x * x + y * y + z * z;
+  }
+  double r = sqrt(r2);
+  if (mur < 0.0) r = -r;
+  ux = x / r;
+  uy = y / r;
+  uz = z / r;
+}
+// The sublimation mask follows a temperature the layer raised.
+__device__ __forceinline__ void dif_mask(size_t w, double t) {
+  const Params &P = c_P;
+  if (t >= P.tsub_ev) P.densm_w[w] = -P.dens[w];
+}
+// The surface cell's T^4 gains c, the temperature capped at sublimation;
+// T4 returns the uncapped sum, which starts the column below.  Packets in
+// flight apply the same step to the cell one after the other
+// (compare-and-swap), so after n hits the temperature is the same n-fold
+// step in any order.
+__device__ __forceinline__ double dif_surface(size_t w, double c, double tsube, double &T4) {
+  const Params &P = c_P;
+  double *p = P.tdust_w + w;
+  if (P.serial) {
+    T4 = pow4(*p) + c;
+    const double t = fmin(tsube, hcm_pow(T4, 0.25));
+    *p = t;
+    dif_mask(w, t);
+    return t;
+  }
+  unsigned long long *pi = reinterpret_cast<unsigned long long *>(p);
+  unsigned long long old = __hip_atomic_load(pi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (;;) {
+    T4 = pow4(__longlong_as_double((long long)old)) + c;
+    const double t = fmin(tsube, hcm_pow(T4, 0.25));
+    const unsigned long long prev =
+        atomicCAS(pi, old, (unsigned long long)__double_as_longlong(t));
+    if (prev == old) {
+      dif_mask(w, t);
+      return t;
+    }
+    old = prev;
+  }
+}
+// A cell behind the surface takes max(T, tnew).
+__device__ __forceinline__ void dif_raise(size_t w, double tnew) {
+  const Params &P = c_P;
+  if (P.serial) {
+    P.tdust_w[w] = fmax(tnew, P.tdust_w[w]);
+  } else {
+    atomicMax(reinterpret_cast<unsigned long long *>(P.tdust_w + w),
+              (unsigned long long)__double_as_longlong(tnew));
+  }
+  dif_mask(w, tnew);
+}
+
+// emit_common.f90:80-387 for a packet in a layer cell (the oracle's
+// emit_diffusion states each step): the packet moves to the layer's surface
+// (radially inwards onto the inner wall, or in theta to 1.00001 cos of the
+// surface cone), leaves about the surface normal with density ~ cos
+// (isotrp), the surface cell of the two disk species gains
+// 4 pi T*^4 / (nstar dA) in T^4 and the cells behind it are raised to the
+// column's T^4 sum; the emitting species is drawn among the surface cell's
+// disk species with weights w8 (per species id), and the packet's energy
+// from the Planck function at the grey-atmosphere temperature.  Packet
+// position, direction (k.ux.. = the isotrp vector) and cell are updated,
+// the Stokes vector is kept.  Returns the emitting species id (0-based), or
+// -1 (an unknown direction, or a surface cell without disk dust).
+template <class R, int NA>
+__device__ __forceinline__ int emit_diffusion(Pkt<R, NA> &k, Mrg &rng, const double (&w8)[2]) {
+  const Params &P = c_P;
+  const double tsube = 1600.0 / 11605.0;
+  const double *ra = P.rarr;
+  int ir = k.ir, it = k.it;
+  const int ip = k.ip;
+  const int dd = dir_at(ir, it, ip);
+  double sux, suy, suz, ux, uy, uz, dA, nacc = 0.0;
+  double T4s = P.tstar * P.tstar;
+  T4s = T4s * T4s;
+  const double rtot = sqrt(k.rsq);
+  const int sl0 = P.slot_of[0], sl1 = P.slot_of[1];
+  auto setdir = [&]() __attribute__((always_inline)) {
+    k.ux = ux;
+    k.uy = uy;
+    k.uz = uz;
+    const double cost = uz;
+    const double sint = sqrt(1.0 - cost * cost);
+    const double cosp = ux / sint, sinp = uy / sint;
+    k.d.cost = R(cost);
+    k.d.sint = R(sint);
+    k.d.cosp = R(cosp);
+    k.d.sinp = R(sinp);
+    k.d.phi = R(hcm_atan2(sinp, cosp));
+  };
+  // the column sums of one species behind the surface cell (ir, it): radial
+  // (dir -1) outwards, or in theta away from the surface (dir -2 / +2)
+  auto column = [&](int id, int sl, int dir, double T4) __attribute__((always_inline)) {
+    const double cc = 3.0 * kPi * T4s / (P.nstar * dA * 1.0);
+    const double kvr = P.kappav[id];
+    double T4d = T4;
+    nacc = 0.0;
+    if (dir == -1) {
+      for (int i = ir; dir_at(i, it, ip) == -1 && dif_at(ir, it, ip); ++i)
+        nacc = nacc + P.nabs[cell_of(i, it, ip) * (size_t)P.nact + sl];
+      for (int i = ir + 1; dir_at(i, it, ip) == -1 && dif_at(ir, it, ip); ++i) {
+        const size_t km = cell_of(i - 1, it, ip) * (size_t)P.nact + sl;
+        T4d = T4d + cc * nacc * kvr * kRsol * P.rstar * P.dens[km] * chir(P.tdust_w[km], id) *
+                        (ra[i - 1] - ra[i - 2]);
+        dif_raise(cell_of(i, it, ip) * (size_t)P.nact + sl, fmin(tsube, hcm_pow(T4d, 0.25)));
+        nacc = nacc - P.nabs[km];
+      }
+    } else if (dir == -2) {
+      for (int i = it; dir_at(ir, i, ip) == -2 && dif_at(ir, it, ip); ++i)
+        nacc = nacc + P.nabs[cell_of(ir, i, ip) * (size_t)P.nact + sl];
+      for (int i = it + 1; dir_at(ir, i, ip) == -2 && dif_at(ir, it, ip); ++i) {
+        const size_t km = cell_of(ir, i - 1, ip) * (size_t)P.nact + sl;
+        T4d = T4d + cc * nacc * kvr * kRsol * P.rstar * P.dens[km] * chir(P.tdust_w[km], id) *
+                        0.5 * (ra[ir - 1] + ra[ir]) * (P.thetarr[i - 1] - P.thetarr[i - 2]);
+        dif_raise(cell_of(ir, i, ip) * (size_t)P.nact + sl, fmin(tsube, hcm_pow(T4d, 0.25)));
+        nacc = nacc - P.nabs[km];
+      }
+    } else {
+      const double cc2 = 3.0 * kPi * T4s / (P.nstar * dA);
+      for (int i = it; dir_at(ir, i, ip) == 2 && dif_at(ir, it, ip); --i)
+        nacc = nacc + P.nabs[cell_of(ir, i, ip) * (size_t)P.nact + sl];
+      for (int i = it - 1; dir_at(ir, i, ip) == 2 && dif_at(ir, it, ip); --i) {
+        const size_t kp = cell_of(ir, i + 1, ip) * (size_t)P.nact + sl;
+        T4d = T4d + cc2 * nacc * kvr * kRsol * P.rstar * P.dens[kp] * chir(P.tdust_w[kp], id) *
+                        0.5 * (ra[ir - 1] + ra[ir]) * (P.thetarr[i + 1] - P.thetarr[i]);
+        dif_raise(cell_of(ir, i, ip) * (size_t)P.nact + sl, fmin(tsube, hcm_pow(T4d, 0.25)));
+        nacc = nacc - P.nabs[kp];
+      }
+    }
+  };
+  // the emitting species and the packet's energy (emit_common.f90:185-205)
+  auto finish = [&](double t0, double t1, double dA_) __attribute__((always_inline)) {
+    const size_t wc = cell_of(ir, it, ip) * (size_t)P.nact;
+    const double xran = ran(rng);
+    const double d0 = w8[0] * P.dens[wc + sl0], d1 = w8[1] * P.dens[wc + sl1];
+    const double opac = d0 + d1;
+    int id2 = -1;
+    if (opac != 0.0) {
+      if (xran <= d0 / opac + (double)1.e-10f)
+        id2 = 0;
+      else if (xran <= d0 / opac + d1 / opac + (double)1.e-10f)
+        id2 = 1;
+    }
+    if (id2 < 0) return -1;
+    const double T4 = pow4(id2 == 0 ? t0 : t1);
+    const double T4eff = 4.0 * kPi * T4s * nacc / (dA_ * P.nstar * 1.0);
+    const double mu = ux * sux + uy * suy + uz * suz;
+    const double T = fmin(hcm_pow(T4 + 0.75 * mu * T4eff, 0.25), tsube);
+    k.nub = bbfreq<R>(rng, R(T));
+    return id2;
+  };
+  if (dd == -1) {
+    while (dir_at(ir, it, ip) == -1 && dif_at(ir, it, ip)) --ir;
+    ++ir;
+    const double rnew = ra[ir - 1];
+    k.on_wall = true;
+    k.x = k.x * rnew / rtot;
+    k.y = k.y * rnew / rtot;
+    k.z = k.z * rnew / rtot;
+    k.rsq = rnew * rnew;
+    sux = -k.x / rnew;
+    suy = -k.y / rnew;
+    suz = -k.z / rnew;
+    isotrp(rng, sux, suy, suz, ux, uy, uz);
+    setdir();
+    dA = 2.0 * kPi * (ra[ir - 1] * ra[ir - 1]) * (P.costarr[it - 1] - P.costarr[it]);
+    const double c = 4.0 * kPi * T4s / (P.nstar * dA * 1.0);
+    const size_t wc = cell_of(ir, it, ip) * (size_t)P.nact;
+    double T4;
+    const double t0 = dif_surface(wc + sl0, c, tsube, T4);
+    column(0, sl0, -1, T4);
+    const double t1 = dif_surface(wc + sl1, c, tsube, T4);
+    column(1, sl1, -1, T4);
+    const int id2 = finish(t0, t1, dA);
+    k.ir = ir - 1;
+    return id2;
+  }
+  if (dd == -2 || dd == 2) {
+    if (dd == -2) {
+      while (dif_at(ir, it, ip)) --it;
+      ++it;
+    } else {
+      while (dif_at(ir, it, ip)) ++it;
+    }
+    const double cosb = k.z / rtot;
+    const double sinb = sqrt(1.0 - cosb * cosb);
+    const double cosbnew = 1.00001 * P.costarr[it - 1];
+    const double sinbnew = sqrt(1.0 - cosbnew * cosbnew);
+    k.x = k.x * sinbnew / sinb;
+    k.y = k.y * sinbnew / sinb;
+    k.z = rtot * cosbnew;
+    const double mu = P.costarr[it - 1], sth = P.sintarr[it - 1];
+    if (mu > 0.0) {
+      sux = -(mu * k.x) / sth / rtot;
+      suy = -(mu * k.y) / sth / rtot;
+      suz = sth;
+    } else {
+      sux = mu * k.x / sth / rtot;
+      suy = mu * k.y / sth / rtot;
+      suz = -sth;
+    }
+    isotrp(rng, sux, suy, suz, ux, uy, uz);
+    setdir();
+    if (P.costarr[it - 1] < 0.0) --it;
+    dA = kPi * sth * (ra[ir] * ra[ir] - ra[ir - 1] * ra[ir - 1]);
+    const double c = 4.0 * kPi * T4s / (P.nstar * dA * 1.0);
+    const size_t wc = cell_of(ir, it, ip) * (size_t)P.nact;
+    const int dir = dir_at(ir, it, ip) == -2 ? -2 : 2;
+    double T4;
+    const double t0 = dif_surface(wc + sl0, c, tsube, T4);
+    column(0, sl0, dir, T4);
+    const double t1 = dif_surface(wc + sl1, c, tsube, T4);
+    column(1, sl1, dir, T4);
+    const int id2 = finish(t0, t1, dA);
+    it = dir_at(ir, it, ip) == -2 ? it - 1 : it + 1;
+    k.it = it;
+    return id2;
+  }
+  return -1;
+}
+
 // Disk-accretion launch: initpacc.f90:3-122 without the diffusion layer,
 // then emit_common.f90:25-78 (Lucy re-emission at the cell's temperature) or
 // emit_gasacc.f90:1-44 (Pringle 1981 gas emission where the cell has no
@@ -1256,8 +1529,10 @@ __device__ __forceinline__ void acc_position(Mrg &rng, double &x, double &y, dou
 // the emitting species with the kapd the previous packet left (select_dust
 // reads the module global); packets here are independent, so the thermal
 // disk species are weighted by kappa_V(id) rho(id), which is that choice
-// whenever the species share one dust table.  Returns the emitting slot
-// (0 for gas emission).
+// whenever the species share one dust table.  A packet born in a cell of
+// the diffusion layer leaves from the layer's surface (emit_diffusion).
+// Returns the emitting slot (0 for gas emission), or -1 for a layer cell
+// without disk dust (where the reference stops).
 template <class R, int NA>
 __device__ __forceinline__ int launch_acc(const WallView &W, Pkt<R, NA> &k, Mrg &rng) {
   const Params &P = c_P;
@@ -1300,6 +1575,25 @@ __device__ __forceinline__ int launch_acc(const WallView &W, Pkt<R, NA> &k, Mrg 
     }
   }
   int slot = 0;
+  k.on_wall = false;
+  if (P.diffus != nullptr && P.diffus[cell] != 0) {
+    // initpacc.f90:95-112: the absorption shared by the two disk species in
+    // proportion to kappa_V rho (kapd restated as above), then the diffusion
+    // branch of emit_common from the layer's surface
+    if (idust2 >= 0) atomicAdd(&P.dbig[cell], 1);
+    const double op2 = P.kappav[0] * rho[0] + P.kappav[1] * rho[1];
+    if (op2 == 0.0) return -1;
+    atomic_add(&P.nabs[cell * (size_t)P.nact + P.slot_of[0]], P.kappav[0] * rho[0] / op2);
+    atomic_add(&P.nabs[cell * (size_t)P.nact + P.slot_of[1]], P.kappav[1] * rho[1] / op2);
+    const double w8[2] = {P.kappav[0], P.kappav[1]};
+    const int id2 = emit_diffusion(k, rng, w8);
+    if (id2 < 0) return -1;
+    bin_init(k.nub, 4);  // newdisktherm.F90:219-250
+    set_opacity(k, true);
+    k.iflag = 0;
+    k.i_orig = 2;
+    return P.slot_of[id2];
+  }
   if (idust2 >= 0) {  // initpacc.f90:95-115, then the Lucy re-emission
     slot = P.slot_of[idust2];
     const size_t w = cell * (size_t)P.nact + slot;
@@ -1337,7 +1631,6 @@ __device__ __forceinline__ int launch_acc(const WallView &W, Pkt<R, NA> &k, Mrg 
   set_opacity(k, true);
   k.iflag = 0;
   k.i_orig = 2;
-  k.on_wall = false;
   return slot;
 }
 
@@ -2116,17 +2409,23 @@ __global__ __launch_bounds__((PEEL || POOL) ? kBlockPeel : kBlockLucy, HC_WAVES_
       status = -1;
       if constexpr (TRACE) tr = Trace{0.0, 0.0, 0, 0, 0, 0, 0, 14695981039346656037ull};
       const long long iphot = (long long)p + 1;
+      st = ST_RESTART;
+      reload = true;
       if (iphot < P.npacc) {  // newdisktherm.F90:209
         const int sl = launch_acc(W, k, L.rng);
-        if constexpr (TRACE) hash_cell(tr, k.ir, k.it, k.ip);
-        push(k, sl, false, L.pid, nrm);
+        if (sl < 0) {
+          acc.err_dust++;
+          status = 2;
+          st = ST_END;
+        } else {
+          if constexpr (TRACE) hash_cell(tr, k.ir, k.it, k.ip);
+          push(k, sl, false, L.pid, nrm);
+        }
       } else {
         launch_star(k, L.rng, iphot < P.npacc + P.npaccstar, nrm);
         if constexpr (TRACE) hash_cell(tr, k.ir, k.it, k.ip);
         push(k, 0, true, L.pid, nrm);
       }
-      st = ST_RESTART;
-      reload = true;
     }
   };
 
@@ -2191,43 +2490,67 @@ __global__ __launch_bounds__((PEEL || POOL) ? kBlockPeel : kBlockLucy, HC_WAVES_
         L.work++;
         lucy_open(L.la, cell);
         L.la.nabs_any += 1;
-        double tem = L.tdust;
-        if (P.bw) {
-          tem = bw_temperature(cell * (size_t)P.nact + slot, idust2);
-        } else {
-#pragma unroll
-          for (int s2 = 0; s2 < NA; ++s2)
-            if (s2 == slot) L.la.nabs[s2] += 1.0;
-        }
         k.i_orig = (idust2 == 0 || idust2 == 1 || idust2 == 4 || idust2 == 5) ? 2 : 3;
-        k.s = Stokes4<R>{R(1.0), R(0.0), R(0.0), R(0.0)};
-        // the draws in their order (direction, direction, frequency); the
-        // frequency's table loads fly while the direction is computed
-        const R u1 = R(ran(L.rng));
-        const R u2 = R(ran(L.rng));
-        const bool planck = P.bw && tem >= P.tsub_ev;  // dustfreq at sublimation
-        LdfState<R> ldf;
-        if (!planck)
-          ldf = ldf_begin<R>(tem, idust2, L.rng, P.bw ? P.kapdBint : P.kapBint,
-                             P.bw ? P.cdfguide_bw : P.cdfguide);
-        k.d.cost = R(1.0) - R(2.0) * u1;
-        k.d.sint = m_sqrt(R(1.0) - m_pow(k.d.cost, R(2.0)));
-        k.d.phi = R(kR2P) * u2;
-        k.d.cosp = m_cos(k.d.phi);
-        k.d.sinp = m_sin(k.d.phi);
-        k.nub = planck ? bbfreq<R>(L.rng, R(tem)) : ldf_end<R>(ldf);
-        set_walk_dir(k);
-        k.iflag = 0;
-        set_opacity(k, false);
-        push(k, slot, false, L.pid, nrm);
-        ++L.iint;
-        acc.nabs_ev++;
-        if constexpr (TRACE) tr.nabs++;
-        L.pir = k.ir;
-        L.pit = k.it;
-        L.pip = k.ip;
-        st = ST_RESTART;
-        reload = P.bw != 0;  // the cell's mask may have changed with its temperature
+        const bool layer = P.diffus != nullptr && P.diffus[cell] != 0;
+        bool ok = true;
+        if (layer) {
+          // propagate.f90:123-142: no count in the layer; the packet leaves
+          // from its surface (Stokes vector kept)
+          double w8[2] = {0.0, 0.0};
+#pragma unroll
+          for (int s2 = 0; s2 < NA; ++s2) {
+            if (s2 == P.slot_of[0]) w8[0] = k.kapd[s2];
+            if (s2 == P.slot_of[1]) w8[1] = k.kapd[s2];
+          }
+          const int id2 = emit_diffusion(k, L.rng, w8);
+          ok = id2 >= 0;
+          if (ok) slot = P.slot_of[id2];
+        } else {
+          double tem = L.tdust;
+          if (P.bw) {
+            tem = bw_temperature(cell * (size_t)P.nact + slot, idust2);
+          } else {
+#pragma unroll
+            for (int s2 = 0; s2 < NA; ++s2)
+              if (s2 == slot) L.la.nabs[s2] += 1.0;
+          }
+          k.s = Stokes4<R>{R(1.0), R(0.0), R(0.0), R(0.0)};
+          // the draws in their order (direction, direction, frequency); the
+          // frequency's table loads fly while the direction is computed
+          const R u1 = R(ran(L.rng));
+          const R u2 = R(ran(L.rng));
+          const bool planck = P.bw && tem >= P.tsub_ev;  // dustfreq at sublimation
+          LdfState<R> ldf;
+          if (!planck)
+            ldf = ldf_begin<R>(tem, idust2, L.rng, P.bw ? P.kapdBint : P.kapBint,
+                               P.bw ? P.cdfguide_bw : P.cdfguide);
+          k.d.cost = R(1.0) - R(2.0) * u1;
+          k.d.sint = m_sqrt(R(1.0) - m_pow(k.d.cost, R(2.0)));
+          k.d.phi = R(kR2P) * u2;
+          k.d.cosp = m_cos(k.d.phi);
+          k.d.sinp = m_sin(k.d.phi);
+          k.nub = planck ? bbfreq<R>(L.rng, R(tem)) : ldf_end<R>(ldf);
+        }
+        if (ok) {
+          set_walk_dir(k);
+          k.iflag = 0;
+          set_opacity(k, false);
+          push(k, slot, false, L.pid, nrm);
+          ++L.iint;
+          acc.nabs_ev++;
+          if constexpr (TRACE) tr.nabs++;
+          L.pir = k.ir;
+          L.pit = k.it;
+          L.pip = k.ip;
+          st = ST_RESTART;
+          // the cell's mask may have changed with its temperature, or the
+          // packet left the layer for another cell
+          reload = P.bw != 0 || layer;
+        } else {
+          acc.err_dust++;
+          status = 2;
+          st = ST_END;
+        }
       }
       if (n_t > 0 && n_t >= lim && st == ST_STAR) ev_star();
       if (n_e > 0 && n_e >= lim && st == ST_END) ev_end();
@@ -2354,6 +2677,15 @@ __global__ __launch_bounds__(256) void lucy_temperature(TParams T) {
   if ((threadIdx.x & 63) == 0 && changed) atomicAdd(T.nchanged, changed);
 }
 
+// The temperatures of the diffusion layer's cells back to 0.1 K (every
+// species), as initarr does before each iteration.
+__global__ void reset_layer(const int8_t *diffus, double *tdust, size_t ncellw, int nact) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t n = ncellw * (size_t)nact;
+  for (; i < n; i += (size_t)gridDim.x * blockDim.x)
+    if (diffus[i / (size_t)nact]) tdust[i] = 0.1 / 11605.0;
+}
+
 // Masked density: zero where the species is at or above Tsub.
 __global__ void mask_density(const double *dens, const double *tdust, double tsub_ev,
                              double *densm, size_t n) {
@@ -2430,6 +2762,9 @@ struct hc_ctx {
   double *filtwave = nullptr, *filtphi = nullptr, *imgb = nullptr;
   size_t imgb_words = 0;
   bool imgb_valid = false;
+  // diffusion layer (hc_set_diffusion)
+  int8_t *diffus = nullptr, *diffdir = nullptr;
+  double *rarr = nullptr, *sintarr = nullptr, *kappar = nullptr;
   int pool_min = 48;   // requests a walking wavefront takes at once
   int pool_walk = 16;  // walking lanes below which a wavefront takes any request
   uint32_t max_work = 1u << 24;
@@ -2657,6 +2992,10 @@ int hc_set_grid(hc_ctx *ctx, const double *r2arr, const double *tan2arr, const d
       ct[i] = tan2arr[i] == 0.0 ? (i == 0 ? 1.0 : -1.0)
                                 : (tan2arr[i] == -1.0 ? 0.0 : std::cos(thetarr[i]));
     if (!rc) rc = upload(ctx, &ctx->costarr, ct.data(), (size_t)d.ntg);
+    // sintarr likewise: 0 at the poles, 1 at a midplane wall
+    for (int i = 0; i < d.ntg; ++i)
+      ct[i] = tan2arr[i] == 0.0 ? 0.0 : (tan2arr[i] == -1.0 ? 1.0 : std::sin(thetarr[i]));
+    if (!rc) rc = upload(ctx, &ctx->sintarr, ct.data(), (size_t)d.ntg);
   }
   if (rc) return rc;
   std::vector<double> tmp;
@@ -2936,6 +3275,8 @@ int prepare_pass(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int32_t 
   if (rn - (double)ns > 0.5) ns = ns + 1;
   P.npacc = (long long)((double)np * S.accfrac);
   P.npaccstar = (long long)(S.accfrac2 * (double)np);
+  if (ns + P.npacc != np) ns = ns + np - (ns + P.npacc);
+  P.nstar = (double)ns;
   P.np = np;
   P.nfreq = d.nfreq;
   P.nmu = d.nmu;
@@ -3031,6 +3372,16 @@ int prepare_pass(hc_ctx *ctx, int64_t np, int32_t ipeel, int32_t seed1, int32_t 
     P.filtphi = ctx->filtphi;
     P.imgb = ctx->imgb;
   }
+  if (ctx->diffus) {
+    if (ctx->bw)
+      return fail(ctx, HC_E_UNSUPPORTED,
+                  "transport pass: the diffusion approximation runs with Lucy re-emission only");
+    P.diffus = ctx->diffus;
+    P.diffdir = ctx->diffdir;
+    P.rarr = ctx->rarr;
+    P.kappar = ctx->kappar;
+  }
+  P.sintarr = ctx->sintarr;
 
   TransportKernel &kern = L.kern;
   size_t rec_bytes = 0;
@@ -3259,6 +3610,12 @@ int queue_temperature(hc_ctx *ctx, const TParams &T, bool apply) {
     hipLaunchKernelGGL(copy_words, dim3(std::max(std::min(blocks, 2048u), 1u)), dim3(256), 0,
                        ctx->stream, (const double *)ctx->tdust2, ctx->tdust, nc);
     HCK(hipGetLastError());
+    if (ctx->diffus) {  // initarr.f90:78-84: the layer restarts at 0.1 K
+      hipLaunchKernelGGL(reset_layer, dim3(std::max(std::min(blocks, 2048u), 1u)), dim3(256), 0,
+                         ctx->stream, (const int8_t *)ctx->diffus, ctx->tdust, ctx->ncellw,
+                         ctx->nact);
+      HCK(hipGetLastError());
+    }
     rc = queue_mask(ctx);
   }
   return rc;
@@ -3771,6 +4128,48 @@ int hc_get_filter_images(hc_ctx *ctx, double *image_b_i, double *image_b_q, doub
   return HC_OK;
 }
 
+int hc_set_diffusion(hc_ctx *ctx, const int8_t *diffus, const int8_t *diffdir, const double *rarr,
+                     const double *kappar) {
+  if (!ctx) return HC_E_ARG;
+  for (void **p : {(void **)&ctx->diffus, (void **)&ctx->diffdir, (void **)&ctx->rarr,
+                   (void **)&ctx->kappar})
+    if (*p) {
+      HCK(hipFree(*p));
+      *p = nullptr;
+    }
+  if (!diffus) return HC_OK;
+  if (!diffdir || !rarr || !kappar) return fail(ctx, HC_E_ARG, "hc_set_diffusion: null array");
+  if (!ctx->have_grid || !ctx->have_dust)
+    return fail(ctx, HC_E_STATE, "hc_set_diffusion: set the grid and the dust first");
+  if (ctx->slot[0] < 0 || ctx->slot[1] < 0)
+    return fail(ctx, HC_E_ARG, "hc_set_diffusion: disk species 1 and 2 must both hold dust");
+  const hc_dims &d = ctx->dims;
+  for (size_t c = 0; c < ctx->ncellw; ++c) {
+    if (!diffus[c]) continue;
+    if (diffdir[c] != -1 && diffdir[c] != -2 && diffdir[c] != 2)
+      return fail(ctx, HC_E_ARG,
+                  "hc_set_diffusion: a layer cell's direction must be -1, -2 or 2");
+    // a layer cell lies inside the grid with a cell on each side of it
+    const int ir = (int)(c % (size_t)d.nrg) + 1;
+    const int it = (int)((c / (size_t)d.nrg) % (size_t)d.ntg) + 1;
+    const int ip = (int)(c / ((size_t)d.nrg * d.ntg)) + 1;
+    if (ir < 2 || ir > d.nrg - 1 || it < 2 || it > d.ntg - 2 || ip > d.npg - 1)
+      return fail(ctx, HC_E_ARG,
+                  "hc_set_diffusion: layer cells must have 2 <= ir < nrg and 2 <= it <= ntg-2");
+  }
+  HCK(hipSetDevice(ctx->device));
+  int rc = 0;
+  for (auto q : {std::make_pair(&ctx->diffus, diffus), std::make_pair(&ctx->diffdir, diffdir)}) {
+    if (rc) break;
+    HCK(hipMalloc((void **)q.first, ctx->ncellw));
+    HCK(hipMemcpy(*q.first, q.second, ctx->ncellw, hipMemcpyHostToDevice));
+  }
+  // kappar (nT, ndg) column-major is kappar[id][nT] already
+  if (!rc) rc = upload(ctx, &ctx->rarr, rarr, (size_t)d.nrg);
+  if (!rc) rc = upload(ctx, &ctx->kappar, kappar, (size_t)d.nT * d.ndg);
+  return rc;
+}
+
 int hc_dropped_packets(hc_ctx *ctx, int64_t *pids, int32_t cap, int64_t *ndropped) {
   if (!ctx || !ndropped || cap < 0 || (cap > 0 && !pids)) return HC_E_ARG;
   if (!ctx->counter) return fail(ctx, HC_E_STATE, "hc_dropped_packets: no pass has run");
@@ -3834,7 +4233,8 @@ int hc_destroy(hc_ctx *ctx) {
                   ctx->img_u,  ctx->img_v,   ctx->si_init, ctx->scal,    ctx->mass4,  ctx->counter,
                   ctx->trace,  ctx->guide,   ctx->tdust2,  ctx->cdfguide, ctx->segdev,
                   ctx->atm,    ctx->atm_guide, ctx->limb_tab, ctx->kapdBint, ctx->cdfguide_bw,
-                  ctx->nwfilt, ctx->filtwave, ctx->filtphi, ctx->imgb};
+                  ctx->nwfilt, ctx->filtwave, ctx->filtphi, ctx->imgb,
+                  ctx->diffus, ctx->diffdir, ctx->rarr, ctx->sintarr, ctx->kappar};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);

@@ -12,7 +12,9 @@ Host-side setup that produces the arrays the transport pass reads:
   ``kapBint(nT, nnu, ndg)`` and the Planck-mean opacity ``kappap(nT, ndg)``
   (dust.f90:394-474); ``kapdB_table`` builds the same call's
   ``kapdBint(nT, nnu, ndg)``, the cumulative of kappa_abs dB_nu/dT that the
-  Bjorkman-Wood re-emission samples (dustfreq, dust.f90:158-311).  The reference integrates each frequency bin with an
+  Bjorkman-Wood re-emission samples (dustfreq, dust.f90:158-311), and
+  ``kappar_species`` its Rosseland mean ``kappar(nT, ndg)`` (the diffusion
+  approximation's chiR).  The reference integrates each frequency bin with an
   adaptive Simpson rule to 1e-5; here each bin uses 8-point Gauss-Legendre
   quadrature, which is exact to far below that tolerance for this smooth
   integrand.  The tables are boundary inputs: the GPU path and the CPU oracle
@@ -77,6 +79,7 @@ class DustTables:
     kapBint: np.ndarray        # (nT, nnu, ndg) F-order
     kappap: np.ndarray         # (nT, ndg) F-order
     kapdBint: Optional[np.ndarray] = None  # (nT, nnu, ndg): see ensure_kapdB
+    kappar: Optional[np.ndarray] = None    # (nT, ndg) Rosseland mean: see ensure_kappar
 
 
 def _species_arrays(table: np.ndarray):
@@ -181,6 +184,48 @@ def ensure_kapdB(dust: "DustTables") -> np.ndarray:
     if dust.kapdBint is None:
         dust.kapdBint = kapdB_table(dust)
     return dust.kapdBint
+
+
+def kappar_species(nud, kd, ad, lTint, nuint, nquad: int = 8) -> np.ndarray:
+    """kappar (nT,) of one species, the Rosseland mean in units of kappa_V
+    (setemiss, dust.f90:414-456): per frequency bin the integral of
+    (dB_nu/dT) / chi_nu with the extinction chi interpolated linearly across
+    the bin (chidB, dust.f90:515-527), bins beyond nu0/T = 50 cut, the bins
+    summed, and the sum inverted."""
+    xg, wg = np.polynomial.legendre.leggauss(nquad)
+    fq = 0.5 * (xg + 1.0)
+    wq = 0.5 * wg
+    T = np.exp(lTint)[:, None]
+    nu1 = nuint[None, :] * T
+    nu0 = np.concatenate([np.zeros((len(lTint), 1)), nu1[:, :-1]], axis=1)
+    dnu = nu1 - nu0
+    dx = dnu / T
+    c1, _ = opacset_np(nu1, nud, kd, ad)
+    c0, _ = opacset_np(nu0, nud, kd, ad)
+    acc = np.zeros_like(nu1)
+    for f, w in zip(fq, wq):
+        nu = nu0 + f * dnu
+        acc += w * _dplanck_norm(nu, T) / ((1.0 - f) * c0 + f * c1)
+    acc = np.where(nu0 / T < 50.0, acc, 0.0)
+    return 1.0 / np.sum(dx * acc, axis=1)
+
+
+def ensure_kappar(dust: "DustTables") -> np.ndarray:
+    """dust.kappar (nT, ndg), built on first use (the diffusion
+    approximation's chiR only)."""
+    if dust.kappar is None:
+        nT = dust.lTint.size
+        out = np.zeros((nT, NDG), order="F")
+        cache = {}
+        for id_ in range(NDG):
+            nl = int(dust.nlambda[id_])
+            cols = (dust.nudust[:nl, id_], dust.kappad[:nl, id_], dust.adust[:nl, id_])
+            key = b"".join(np.ascontiguousarray(c).tobytes() for c in cols)
+            if key not in cache:
+                cache[key] = kappar_species(*cols, dust.lTint, dust.nuint)
+            out[:, id_] = cache[key]
+        dust.kappar = out
+    return dust.kappar
 
 
 def setemiss_species(nud, kd, ad, lTint, nuint, nquad: int = 8, device=None):

@@ -166,6 +166,16 @@ module hc_iface
       real(c_double), intent(out) :: bi(*), bq(*), bu(*), bv(*)
     end function hc_get_filter_images
 
+    ! the diffusion approximation (DIFFUSION=YES): gridset's diffus as 0/1 and
+    ! diffdir, int8 (nrg,ntg,npg), rarr(nrg), setemiss's kappar(nT,ndg)
+    integer(c_int) function hc_set_diffusion(ctx, diffus, diffdir, rarr, kappar) &
+        bind(C, name='hc_set_diffusion')
+      import :: c_int, c_ptr, c_int8_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int8_t), intent(in) :: diffus(*), diffdir(*)
+      real(c_double), intent(in) :: rarr(*), kappar(*)
+    end function hc_set_diffusion
+
     ! Bjorkman-Wood re-emission (CLUCY=NO): kapdBint (nT,nnu,ndg) from setemiss
     ! and massarr (nrg,ntg,npg,ndg); c_null_ptr for both returns to Lucy's
     integer(c_int) function hc_set_bw(ctx, kapdBint, massarr) bind(C, name='hc_set_bw')

@@ -100,6 +100,10 @@ class ModelConfig:
     # DENSINPUT=YES: a densarr.unf whose walls and disk species 1..2 replace
     # the computed ones (gridsethydro.f90:346-362)
     densinput: Optional[str] = None
+    # DIFFUSION=YES: re-emit packets absorbed deep in the optically thick
+    # disk from the surface of that layer (gridset.f90:666-789,
+    # emit_common.f90:80-387; hochunk3d_amd.diffusion)
+    diffusion: bool = False
 
 
 @dataclass
@@ -146,6 +150,9 @@ class Model:
     # stellar spectrum table (atmosinit) and limb-darkening table (xmu, prob)
     atmosphere: Optional[object] = None
     limb_tab: Optional[tuple] = None
+    # diffusion layer (nrg,ntg,npg) int8 mask and direction, or None
+    diffus: Optional[np.ndarray] = None
+    diffdir: Optional[np.ndarray] = None
 
     @property
     def ncells(self) -> int:
@@ -461,6 +468,9 @@ def build_model(cfg: ModelConfig, device: Optional[int] = None) -> Model:
                  tuple(fm8), tuple(z18), tuple(b8))
     model.atmosphere = atm
     model.limb_tab = ltab
+    if cfg.diffusion:
+        from .diffusion import diffusion_mask
+        model.diffus, model.diffdir = diffusion_mask(model)
     return model
 
 

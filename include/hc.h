@@ -202,6 +202,26 @@ int hc_set_filters(hc_ctx *ctx, int32_t nfilt, int32_t nwfiltmax, const int32_t 
 int hc_get_filter_images(hc_ctx *ctx, double *image_b_i, double *image_b_q, double *image_b_u,
                          double *image_b_v);
 
+/* The diffusion approximation (DIFFUSION=YES; gridset.f90:666-789,
+ * emit_common.f90:80-387, propagate.f90:123-142, initpacc.f90:103-112,
+ * initarr.f90:78-84), Lucy re-emission only.  diffus/diffdir
+ * (nrg,ntg,npg) int8: gridset's layer mask and its direction (-1 radially
+ * inwards, -2 up, +2 down); rarr[nrg] the radial walls; kappar (nT,ndg)
+ * setemiss's Rosseland-mean table (chiR).  A packet absorbed in a layer
+ * cell is not counted in nabs and leaves from the layer's surface at the
+ * grey-atmosphere temperature of the surface cell, which every such packet
+ * heats (4 pi T*^4 / (nstar dA) in T^4) together with the cells behind it;
+ * an accretion packet born in the layer shares its count between the two
+ * disk species.  Packets in flight update these temperatures with atomics
+ * (the surface by the same step per hit in any order, the cells behind by
+ * maximum); hc_set_option(ctx, "serial", 1) runs the reference's order.  The
+ * device temperature update (hc_temperature with apply, hc_run_lucy) resets
+ * the layer to 0.1 K as initarr does; a caller applying tdust2 itself does
+ * the same.  diffus = NULL switches the approximation off.  Needs dust in
+ * both disk species (ids 1 and 2). */
+int hc_set_diffusion(hc_ctx *ctx, const int8_t *diffus, const int8_t *diffdir, const double *rarr,
+                     const double *kappar);
+
 /* One transport pass (the body of disk(), newdisktherm.F90:194-547): np
  * packets, peel-off when ipeel=1.  Packet p (0-based) draws from segment
  * seg0+p of L'Ecuyer-Cote generator gen seeded with (seed1,seed2)

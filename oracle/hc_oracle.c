@@ -436,15 +436,22 @@ double hco_dustfreq(const hco_dust *d, hco_rng *r, double T0, int id, double tsu
   return cdf_freq(d, d->kapdBint, r, T0, id);
 }
 
-double hco_kapp(const hco_dust *d, double T, int id) {  /* dust.f90:114-142 */
+/* A per-species (nT) table at temperature T: linear in ln T between the
+ * nodes of lTint, the end values outside (KapP, dust.f90:114-142, and chiR,
+ * dust.f90:83-111, read kappap and kappar this way). */
+static double lT_interp(const hco_dust *d, const double *tab, double T, int id) {
   const int nT = d->nT;
-  const double *kp = d->kappap + (size_t)nT * id;
+  const double *kp = tab + (size_t)nT * id;
   double lT = HCO_LOG(T), f;
   int k = (int)((double)(nT - 1) * (lT - d->lTint[0]) / (d->lTint[nT - 1] - d->lTint[0])) + 1;
   if (k < 1 || k > nT - 1) return k < 1 ? kp[0] : kp[nT - 1];
   f = (lT - d->lTint[k - 1]) / (d->lTint[k] - d->lTint[k - 1]);
   return (1.0 - f) * kp[k - 1] + f * kp[k];
 }
+
+double hco_kapp(const hco_dust *d, double T, int id) { return lT_interp(d, d->kappap, T, id); }
+
+double hco_chir(const hco_dust *d, double T, int id) { return lT_interp(d, d->kappar, T, id); }
 
 /* dusttemp.f90:6-58 (mass4 = massarr = (rho V)^(1/4)); the fixed-point
  * loop is capped at 1000 sweeps where the reference would spin. */
@@ -921,6 +928,7 @@ typedef struct {
   int nact;               /* species with any density */
   int act[HCO_NDG];
   double nphot;           /* packets of the pass (dusttemp's nphot) */
+  double nstar;           /* stellar packets of the pass (emit_common's nstar) */
 } ctx_t;
 
 static inline size_t cell_index(const hco_grid *g, int ir, int it, int ip) {
@@ -1471,6 +1479,261 @@ static void set_kapd_launch(ctx_t *c, pkt_t *P);
 static void bin_init(ctx_t *c, double nub, int i_orig_init);
 static void peel_all(ctx_t *c, const pkt_t *P, double sip, int idust2);
 
+/* isotrp.f90:1-36: a direction about the unit vector s with density
+ * proportional to |u.s|: points drawn in the cube [-1,1]^3 until
+ * |p|^4 <= |p.s| (inside the unit ball, |p|^3 <= cos of its angle to s),
+ * reversed when they point against s, normalised. */
+void hco_isotrp(hco_rng *r, double sux, double suy, double suz, double *u) {
+  double x = 0.0, y = 0.0, z = 0.0, r2 = 2.0, mur = 0.0, rr;
+  while (r2 * r2 > fabs(mur)) {
+    x = hco_ran(r);
+    x = x + x - 1.0;
+    y = hco_ran(r);
+    y = y + y - 1.0;
+    z = hco_ran(r);
+    z = z + z - 1.0;
+    mur = x * sux + y * suy + z * suz;
+    r2 = x * x + y * y + z * z;
+  }
+  rr = sqrt(r2);
+  if (mur < 0.0) rr = -rr;
+  u[0] = x / rr;
+  u[1] = y / rr;
+  u[2] = z / rr;
+}
+
+/* The diffusion mask and direction of a cell; cells off the grid's theta
+ * range count as outside the layer. */
+static inline int dif_at(const hco_grid *g, int ir, int it, int ip) {
+  if (ir < 1 || ir > g->nrg || it < 1 || it > g->ntg) return 0;
+  return g->diffus[cell_index(g, ir, it, ip)] != 0;
+}
+static inline int dir_at(const hco_grid *g, int ir, int it, int ip) {
+  if (ir < 1 || ir > g->nrg || it < 1 || it > g->ntg) return 0;
+  return g->diffdir[cell_index(g, ir, it, ip)];
+}
+static double sint_wall(const hco_grid *g, int it) {  /* gridset.f90:581-599 */
+  if (g->tan2arr[it] == 0.0) return 0.0;
+  if (g->tan2arr[it] == -1.0) return 1.0;
+  return sin(g->thetarr[it]);
+}
+static inline double pow4(double x) {  /* x**4 as (x*x)*(x*x) */
+  x = x * x;
+  return x * x;
+}
+
+/* select_dust(is_thermal_disk) (select_dust.f90) with per-species weights w
+ * (the packet's kapd); 0 when the cell holds no thermal disk dust. */
+static int select_thermal_disk(ctx_t *c, size_t cell, const double *w) {
+  const double *dens = c->g->densarr;
+  double xran = hco_ran(&c->rng), opac = 0.0, frac = 0.0;
+  int id;
+  for (id = 0; id < HCO_NDG; id++)
+    if (IS_THERMAL_DISK[id]) opac = opac + w[id] * dens[cell + c->ncell * id];
+  if (opac == 0.0) return 0;
+  for (id = 0; id < HCO_NDG; id++)
+    if (IS_THERMAL_DISK[id]) {
+      frac = frac + w[id] * dens[cell + c->ncell * id] / opac;
+      if (xran <= frac + (double)1.e-10f) return id + 1;
+    }
+  return 0;
+}
+
+static void set_direction(pkt_t *P, const double *u) {  /* emit_common.f90:116-121 */
+  P->ux = u[0];
+  P->uy = u[1];
+  P->uz = u[2];
+  P->cost = P->uz;
+  P->sint = sqrt(1.0 - P->cost * P->cost);
+  P->cosp = P->ux / P->sint;
+  P->sinp = P->uy / P->sint;
+  P->phi = HCO_ATAN2(P->sinp, P->cosp);
+}
+
+/* emit_common.f90:80-387, the diffusion branch (Lucy mode, ilucy = 1) for a
+ * packet absorbed in (or launched into) a cell of the diffusion layer:
+ *  - radial direction (diffdir -1): inwards to the layer's inner radial
+ *    wall; the packet is moved radially onto it (on_wall) and leaves in a
+ *    direction about the inward normal with density ~ cos (isotrp);
+ *  - theta direction (diffdir -2 above the midplane, +2 below): along the
+ *    theta cells to the layer's surface cone, the packet moved in theta to
+ *    cos(theta) = 1.00001 cos of that wall (just outside the layer), and
+ *    leaving about the cone's outward normal;
+ *  - the surface cell of the two disk species gains
+ *    4 pi T*^4 / (nstar dA) in T^4 (dA the surface's ring area), capped at
+ *    the sublimation temperature, and the cells behind it along the
+ *    direction take max(T, T_surface^4 + the column sum of
+ *    3 pi T*^4 / (nstar dA) nacc kappa_V R* rho chiR(T) ds) (nacc the absorbed
+ *    counts still behind, which only accretion photons add);
+ *  - the emitting species is chosen among the disk species of the surface
+ *    cell with the weights w (the packet's kapd), and the packet's energy
+ *    from the Planck function (bbfreq) at the grey-atmosphere temperature
+ *    (T_surface^4 + 3/4 mu T_eff^4)^(1/4), mu = the cosine to the normal.
+ * The reference's loops over the layer test diffus of the surface cell,
+ * which every branch reaches, so they run on diffdir alone; that is kept.
+ * Returns the emitting species (1-based) or 0 on an unknown direction or
+ * a surface cell without disk dust (where the reference stops). */
+static int emit_diffusion(ctx_t *c, pkt_t *P, const double *w) {
+  const hco_grid *g = c->g;
+  const hco_dust *d = c->d;
+  const hco_setup *s = c->s;
+  const double tsube = 1600.0 / 11605.0;
+  double *td = (double *)g->tdust;
+  const double *nabs = c->t->nabs;
+  const double *ra = g->rarr;
+  int ir = P->ir, it = P->it, ip = P->ip, id, i, idust2;
+  double sux, suy, suz, u[3], dA, T4s, T4d, T4, T4eff, nacc = 0.0, mu, T;
+  const int dd = dir_at(g, ir, it, ip);
+  T4s = s->tstar * s->tstar;
+  T4s = T4s * T4s;
+  if (dd == -1) {
+    double rnew;
+    while (dir_at(g, ir, it, ip) == -1 && dif_at(g, ir, it, ip)) ir = ir - 1;
+    ir = ir + 1;
+    rnew = ra[ir - 1];
+    P->on_wall = 1;
+    P->xp = P->xp * rnew / P->rtot;
+    P->yp = P->yp * rnew / P->rtot;
+    P->zp = P->zp * rnew / P->rtot;
+    P->rtot = rnew;
+    P->rsq = rnew * rnew;
+    sux = -P->xp / P->rtot;
+    suy = -P->yp / P->rtot;
+    suz = -P->zp / P->rtot;
+    hco_isotrp(&c->rng, sux, suy, suz, u);
+    set_direction(P, u);
+    dA = 2.0 * PI * (ra[ir - 1] * ra[ir - 1]) * (cost_wall(g, it - 1) - cost_wall(g, it));
+    for (id = 0; id < 2; id++) {
+      const size_t k0 = cell_index(g, ir, it, ip) + c->ncell * id;
+      T4 = pow4(td[k0]) + 4.0 * PI * T4s / (c->nstar * dA * 1.0);
+      td[k0] = fmin(tsube, HCO_POW(T4, 0.25));
+      nacc = 0.0;
+      i = ir;
+      while (dir_at(g, i, it, ip) == -1 && dif_at(g, ir, it, ip)) {
+        nacc = nacc + nabs[cell_index(g, i, it, ip) + c->ncell * id];
+        i = i + 1;
+      }
+      T4d = T4;
+      i = ir + 1;
+      while (dir_at(g, i, it, ip) == -1 && dif_at(g, ir, it, ip)) {
+        const size_t km = cell_index(g, i - 1, it, ip) + c->ncell * id;
+        const size_t ki = cell_index(g, i, it, ip) + c->ncell * id;
+        double told, tnew;
+        T4d = T4d + (3.0 * PI * T4s / (c->nstar * dA * 1.0)) * nacc * d->kappav[id] * RSOL *
+                        s->rstar * g->densarr[km] * hco_chir(d, td[km], id) * (ra[i - 1] - ra[i - 2]);
+        told = td[ki];
+        tnew = fmin(tsube, HCO_POW(T4d, 0.25));
+        td[ki] = fmax(tnew, told);
+        nacc = nacc - nabs[km];
+        i = i + 1;
+      }
+    }
+    idust2 = select_thermal_disk(c, cell_index(g, ir, it, ip), w);
+    if (idust2 == 0) return 0;
+    T4 = pow4(td[cell_index(g, ir, it, ip) + c->ncell * (idust2 - 1)]);
+    T4eff = 4.0 * PI * T4s * nacc / (1.0 * dA * c->nstar);
+    mu = P->ux * sux + P->uy * suy + P->uz * suz;
+    T = fmin(HCO_POW(T4 + 0.75 * mu * T4eff, 0.25), tsube);
+    P->nub = hco_bbfreq(&c->rng, T);
+    P->ir = ir - 1;
+    return idust2;
+  }
+  if (dd == -2 || dd == 2) {
+    double cosb, sinb, cosbnew, sinbnew, sth;
+    if (dd == -2) {
+      while (dif_at(g, ir, it, ip)) it = it - 1;
+      it = it + 1;
+    } else {
+      while (dif_at(g, ir, it, ip)) it = it + 1;
+    }
+    cosb = P->zp / P->rtot;
+    sinb = sqrt(1.0 - cosb * cosb);
+    cosbnew = 1.00001 * cost_wall(g, it - 1);
+    sinbnew = sqrt(1.0 - cosbnew * cosbnew);
+    P->xp = P->xp * sinbnew / sinb;
+    P->yp = P->yp * sinbnew / sinb;
+    P->zp = P->rtot * cosbnew;
+    mu = cost_wall(g, it - 1);
+    sth = sint_wall(g, it - 1);
+    if (mu > 0.0) {
+      sux = -(mu * P->xp) / sth / P->rtot;
+      suy = -(mu * P->yp) / sth / P->rtot;
+      suz = sth;
+    } else {
+      sux = mu * P->xp / sth / P->rtot;
+      suy = mu * P->yp / sth / P->rtot;
+      suz = -sth;
+    }
+    hco_isotrp(&c->rng, sux, suy, suz, u);
+    set_direction(P, u);
+    if (cost_wall(g, it - 1) < 0.0) it = it - 1;
+    dA = PI * sth * (ra[ir] * ra[ir] - ra[ir - 1] * ra[ir - 1]);
+    for (id = 0; id < 2; id++) {
+      const size_t k0 = cell_index(g, ir, it, ip) + c->ncell * id;
+      T4 = pow4(td[k0]) + 4.0 * PI * T4s / (c->nstar * dA * 1.0);
+      td[k0] = fmin(tsube, HCO_POW(T4, 0.25));
+      T4d = T4;
+      if (dir_at(g, ir, it, ip) == -2) {
+        nacc = 0.0;
+        i = it;
+        while (dir_at(g, ir, i, ip) == -2 && dif_at(g, ir, it, ip)) {
+          nacc = nacc + nabs[cell_index(g, ir, i, ip) + c->ncell * id];
+          i = i + 1;
+        }
+        i = it + 1;
+        while (dir_at(g, ir, i, ip) == -2 && dif_at(g, ir, it, ip)) {
+          const size_t km = cell_index(g, ir, i - 1, ip) + c->ncell * id;
+          const size_t ki = cell_index(g, ir, i, ip) + c->ncell * id;
+          double told, tnew;
+          T4d = T4d + (3.0 * PI * T4s / (c->nstar * dA * 1.0)) * nacc * d->kappav[id] * RSOL *
+                          s->rstar * g->densarr[km] * hco_chir(d, td[km], id) * 0.5 *
+                          (ra[ir - 1] + ra[ir]) * (g->thetarr[i - 1] - g->thetarr[i - 2]);
+          told = td[ki];
+          tnew = fmin(tsube, HCO_POW(T4d, 0.25));
+          td[ki] = fmax(told, tnew);
+          nacc = nacc - nabs[km];
+          i = i + 1;
+        }
+      } else {
+        nacc = 0.0;
+        i = it;
+        while (dir_at(g, ir, i, ip) == 2 && dif_at(g, ir, it, ip)) {
+          nacc = nacc + nabs[cell_index(g, ir, i, ip) + c->ncell * id];
+          i = i - 1;
+        }
+        i = it - 1;
+        while (dir_at(g, ir, i, ip) == 2 && dif_at(g, ir, it, ip)) {
+          const size_t kp = cell_index(g, ir, i + 1, ip) + c->ncell * id;
+          const size_t ki = cell_index(g, ir, i, ip) + c->ncell * id;
+          double told, tnew;
+          T4d = T4d + (3.0 * PI * T4s / (c->nstar * dA)) * nacc * d->kappav[id] * RSOL *
+                          s->rstar * g->densarr[kp] * hco_chir(d, td[kp], id) * 0.5 *
+                          (ra[ir - 1] + ra[ir]) * (g->thetarr[i + 1] - g->thetarr[i]);
+          told = td[ki];
+          tnew = fmin(tsube, HCO_POW(T4d, 0.25));
+          td[ki] = fmax(told, tnew);
+          nacc = nacc - nabs[kp];
+          i = i - 1;
+        }
+      }
+    }
+    idust2 = select_thermal_disk(c, cell_index(g, ir, it, ip), w);
+    if (idust2 == 0) return 0;
+    T4 = pow4(td[cell_index(g, ir, it, ip) + c->ncell * (idust2 - 1)]);
+    T4eff = 4.0 * PI * T4s * nacc / (dA * c->nstar * 1.0);
+    mu = P->ux * sux + P->uy * suy + P->uz * suz;
+    T = fmin(HCO_POW(T4 + 0.75 * mu * T4eff, 0.25), tsube);
+    P->nub = hco_bbfreq(&c->rng, T);
+    if (dir_at(g, ir, it, ip) == -2)
+      it = it - 1;
+    else
+      it = it + 1;
+    P->it = it;
+    return idust2;
+  }
+  return 0;
+}
+
 /* Disk-accretion launch: initpacc.f90:3-122 without the diffusion layer,
  * emit_common.f90:25-78 (Lucy branch) or emit_gasacc.f90:1-44, then
  * newdisktherm.F90:219-250.  The reference picks the emitting species with
@@ -1478,7 +1741,7 @@ static void peel_all(ctx_t *c, const pkt_t *P, double sip, int idust2);
  * a packet-independent restatement weights the thermal disk species by
  * kappa_V(id) * rho(id), which is that choice whenever the species share one
  * dust table (kapd(id) = kappa(nu) R* kappa_V(id) for any nu). */
-static void launch_acc(ctx_t *c, pkt_t *P) {
+static int launch_acc(ctx_t *c, pkt_t *P) {
   const hco_setup *s = c->s;
   const hco_grid *g = c->g;
   const hco_dust *d = c->d;
@@ -1520,8 +1783,21 @@ static void launch_acc(ctx_t *c, pkt_t *P) {
       }
     }
   }
-  if (idust2 >= 0) {  /* initpacc.f90:95-115 (the thermal disk species are not VSG/PAH) */
-    if (t->dbig) t->dbig[cell] += 1;
+  P->on_wall = 0;
+  if (idust2 >= 0 && t->dbig) t->dbig[cell] += 1;  /* initpacc.f90:95-101 */
+  if (g->diffus && g->diffus[cell]) {
+    /* initpacc.f90:103-112: the absorption shared by the two disk species in
+     * proportion to kappa rho (kapd restated as kappa_V, as above), then
+     * emit_common's diffusion branch from the layer's surface */
+    double w[HCO_NDG] = {0};
+    const double op2 = d->kappav[0] * g->densarr[cell] + d->kappav[1] * g->densarr[cell + c->ncell];
+    if (op2 == 0.0) return -1;
+    for (id = 0; id < 2; id++) {
+      w[id] = d->kappav[id];
+      if (t->nabs) t->nabs[cell + c->ncell * id] += d->kappav[id] * g->densarr[cell + c->ncell * id] / op2;
+    }
+    if (emit_diffusion(c, P, w) == 0) return -1;
+  } else if (idust2 >= 0) {  /* initpacc.f90:113-115 (the thermal disk species are not VSG/PAH) */
     if (t->nabs) t->nabs[cell + c->ncell * idust2] += 1.0;
     /* emit_common.f90:57-70 -> reemit.f90:15-33 */
     const double tem = reemit_temperature(c, cell + c->ncell * idust2, idust2);
@@ -1543,17 +1819,19 @@ static void launch_acc(ctx_t *c, pkt_t *P) {
     P->cosp = HCO_COS(P->phi);
     P->sinp = HCO_SIN(P->phi);
   }
-  P->uy = P->sint * P->sinp;
-  P->ux = P->sint * P->cosp;
-  P->uz = P->cost;
+  if (!(g->diffus && g->diffus[cell])) {  /* emit_common.f90:74-76 */
+    P->uy = P->sint * P->sinp;
+    P->ux = P->sint * P->cosp;
+    P->uz = P->cost;
+  }
   /* newdisktherm.F90:219-250 */
   hco_opacset(d, P->nub, &P->op);
   bin_init(c, P->nub, 4);
   set_kapd_launch(c, P);
   P->iflag = 0;
   P->i_orig = 2;
-  P->on_wall = 0;
   if (c->ipeel) peel_all(c, P, P->sip, idust2 < 0 ? 0 : idust2);
+  return 0;
 }
 
 /* Star / hot-spot launch: initpspot.f90:3-175 (or initp_ps.f90:3-79 for the
@@ -1925,20 +2203,30 @@ static int propagate(ctx_t *c, pkt_t *P) {
         size_t k = cell + c->ncell * (size_t)(idust2 - 1);
         if (t->dbig) t->dbig[cell] += 1;
         P->i_orig = (idust2 == 1 || idust2 == 2 || idust2 == 5 || idust2 == 6) ? 2 : 3;
-        if (t->nabs) t->nabs[k] += 1.0;
-        /* emit_common.f90:57-70 -> reemit.f90:15-33 */
-        const double tem = reemit_temperature(c, k, idust2 - 1);
-        P->sip = 1.0;
-        P->sqp = P->sup = P->svp = 0.0;
-        P->cost = 1.0 - 2.0 * hco_ran(&c->rng);
-        P->sint = sqrt(1.0 - HCO_POW(P->cost, 2.0));
-        P->phi = R2P * hco_ran(&c->rng);
-        P->cosp = HCO_COS(P->phi);
-        P->sinp = HCO_SIN(P->phi);
-        P->nub = reemit_freq(c, tem, idust2 - 1);
-        P->uy = P->sint * P->sinp;
-        P->ux = P->sint * P->cosp;
-        P->uz = P->cost;
+        if (g->diffus && g->diffus[cell]) {
+          /* propagate.f90:123-142: no count in the layer; emit_common's
+           * diffusion branch moves the packet to the layer's surface */
+          idust2 = emit_diffusion(c, P, P->kapd);
+          if (idust2 == 0) {
+            t->n_err_dust++;
+            return HCO_ERROR;
+          }
+        } else {
+          if (t->nabs) t->nabs[k] += 1.0;
+          /* emit_common.f90:57-70 -> reemit.f90:15-33 */
+          const double tem = reemit_temperature(c, k, idust2 - 1);
+          P->sip = 1.0;
+          P->sqp = P->sup = P->svp = 0.0;
+          P->cost = 1.0 - 2.0 * hco_ran(&c->rng);
+          P->sint = sqrt(1.0 - HCO_POW(P->cost, 2.0));
+          P->phi = R2P * hco_ran(&c->rng);
+          P->cosp = HCO_COS(P->phi);
+          P->sinp = HCO_SIN(P->phi);
+          P->nub = reemit_freq(c, tem, idust2 - 1);
+          P->uy = P->sint * P->sinp;
+          P->ux = P->sint * P->cosp;
+          P->uz = P->cost;
+        }
         P->iflag = 0;
         hco_opacset(c->d, P->nub, &P->op);
         set_kapd_reemit(c, P);
@@ -1980,6 +2268,7 @@ int hco_run_pass(const hco_grid *g, const hco_dust *d, const hco_setup *s, hco_t
   c.ncell = (size_t)g->nrg * g->ntg * g->npg;
   c.nphot = (double)np;
   if (s->bw && (!d->kapdBint || !t->nabs || !g->massarr)) return -1;
+  if (g->diffus && (s->bw || !g->diffdir || !g->rarr || !d->kappar || !t->nabs)) return -1;
   /* source split, newdisktherm.F90:143-176 (no external illumination) */
   ns = (int64_t)((double)np * (1.0 - s->accfrac));
   rn = (double)(float)np * (1.0 - s->accfrac);
@@ -1987,6 +2276,7 @@ int hco_run_pass(const hco_grid *g, const hco_dust *d, const hco_setup *s, hco_t
   npacc = (int64_t)((double)np * s->accfrac);
   npaccstar = (int64_t)(s->accfrac2 * (double)np);
   if (ns + npacc != np) ns = ns + np - (ns + npacc);
+  c.nstar = (double)ns;
   for (p = p_begin; p < p_end; p++) {
     pkt_t P;
     int64_t iphot = p + 1;
@@ -1998,10 +2288,15 @@ int hco_run_pass(const hco_grid *g, const hco_dust *d, const hco_setup *s, hco_t
       memset(c.tr, 0, sizeof *c.tr);
       c.tr->cellhash = 14695981039346656037ull;
     }
-    if (iphot < npacc)  /* newdisktherm.F90:209 */
-      launch_acc(&c, &P);
-    else
+    if (iphot < npacc) { /* newdisktherm.F90:209 */
+      if (launch_acc(&c, &P)) {  /* a layer cell without disk dust */
+        t->n_err_dust++;
+        finish_trace(&c, &P, HCO_ERROR);
+        continue;
+      }
+    } else {
       launch_star(&c, &P, iphot < npacc + npaccstar ? 1 : 0);
+    }
     hash_cell(c.tr, P.ir, P.it, P.ip);
     status = propagate(&c, &P);
     finish_trace(&c, &P, status);
@@ -2083,6 +2378,40 @@ void hco_peel_ray(const hco_grid *g, const hco_setup *s, hco_tallies *t, const d
   P.iflag = iflag;
   P.i_orig = i_orig;
   peeloff(&c, &P, stokes[0], 0, peelid);
+}
+
+/* Leaf entry of the diffusion branch (emit_diffusion) for the closed-form
+ * tests: a packet at xyz in cell (1-based) with walk opacities kapd leaves
+ * from the layer's surface; the grid's tdust is updated in place, the
+ * tallies' nabs read.  out: x, y, z, ux, uy, uz, nub; cell and on_wall
+ * updated; returns the emitting species (1-based) or 0. */
+int hco_emit_diffusion(const hco_grid *g, const hco_dust *d, const hco_setup *s, hco_tallies *t,
+                       hco_rng *r, double nstar, const double *xyz, int *cell, int *on_wall,
+                       const double *kapd, double *out) {
+  ctx_t c;
+  pkt_t P;
+  const double u0[3] = {0.0, 0.0, 1.0};
+  static const double alb0[HCO_NDG] = {0};
+  int id2;
+  leaf_ctx(&c, g, s, t);
+  c.d = d;
+  c.nstar = nstar;
+  c.rng = *r;
+  leaf_packet(&P, xyz, u0, cell, *on_wall, kapd, alb0);
+  id2 = emit_diffusion(&c, &P, P.kapd);
+  *r = c.rng;
+  out[0] = P.xp;
+  out[1] = P.yp;
+  out[2] = P.zp;
+  out[3] = P.ux;
+  out[4] = P.uy;
+  out[5] = P.uz;
+  out[6] = P.nub;
+  cell[0] = P.ir;
+  cell[1] = P.it;
+  cell[2] = P.ip;
+  *on_wall = P.on_wall;
+  return id2;
 }
 
 /* Elementary-function probe for the accuracy tests: fn 0 log, 1 exp, 2 sin,

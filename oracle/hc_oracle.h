@@ -34,6 +34,10 @@
  *   purely scattering shell;
  *   tests/test_oracle_plants.py plants a mistake in each of these and
  *   requires the closed-form tests to fail;
+ *   the diffusion approximation: isotrp, chiR and ierfc bit-exact against
+ *   the reference's routines; the re-emission surface, the packet's move
+ *   onto it and the column temperatures of emit_common by the geometry and
+ *   the closed forms of its T^4 sums;
  *   the star launch: the surface point uniform on the star or within the
  *   spot caps, the spot flag of stellar packets, and the emission direction
  *   (its cosine with the normal equals the sqrt of its draw, its local
@@ -64,6 +68,11 @@ typedef struct {
   const double *tdust;          /* (nrg,ntg,npg,ndg) eV */
   const double *massarr;        /* (nrg,ntg,npg,ndg) (rho*V)^(1/4), for tfinal */
   double tsub;                  /* K */
+  /* diffusion approximation (gridset.f90:666-789, emit_common.f90:80-387):
+   * rarr [nrg] walls, diffus/diffdir (nrg,ntg,npg) the mask and direction
+   * of gridset; diffus = NULL switches the approximation off */
+  const double *rarr;
+  const int8_t *diffus, *diffdir;
 } hco_grid;
 
 typedef struct {
@@ -78,6 +87,7 @@ typedef struct {
   const double *kappap;         /* (nT,ndg) */
   const double *kapdBint;       /* (nT,nnu,ndg) normalised cumulative of kappa_abs dB/dT
                                  * (Bjorkman-Wood re-emission), or NULL */
+  const double *kappar;         /* (nT,ndg) Rosseland mean / kappa_V (chiR), or NULL */
 } hco_dust;
 
 typedef struct {
@@ -186,6 +196,11 @@ int hco_find_next_wall(const hco_walls *w, int on_wall, double *t, int *face);
 int hco_locate(const double *xx, int n, double x);
 /* initpacc.f90:27-52: the emission point of a disk-accretion packet. */
 void hco_acc_position(const hco_setup *s, hco_rng *r, double *xyz);
+void hco_isotrp(hco_rng *r, double sux, double suy, double suz, double *u);
+double hco_chir(const hco_dust *d, double T, int id);
+int hco_emit_diffusion(const hco_grid *g, const hco_dust *d, const hco_setup *s, hco_tallies *t,
+                       hco_rng *r, double nstar, const double *xyz, int *cell, int *on_wall,
+                       const double *kapd, double *out);
 int hco_walk_ray(const hco_grid *g, double x, double y, double z, double ux, double uy, double uz,
                  int *ir, int *it, int *ip, int on_wall);
 double hco_bbfreq(hco_rng *r, double teff);

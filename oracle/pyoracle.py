@@ -26,7 +26,8 @@ class Grid(C.Structure):
     _fields_ = [("nrg", C.c_int), ("ntg", C.c_int), ("npg", C.c_int),
                 ("r2arr", _dp), ("tan2arr", _dp), ("thetarr", _dp), ("aarr", _dp),
                 ("barr", _dp), ("phiarr", _dp), ("densarr", _dp), ("tdust", _dp),
-                ("massarr", _dp), ("tsub", C.c_double)]
+                ("massarr", _dp), ("tsub", C.c_double), ("rarr", _dp),
+                ("diffus", C.POINTER(C.c_int8)), ("diffdir", C.POINTER(C.c_int8))]
 
 
 class Dust(C.Structure):
@@ -34,7 +35,7 @@ class Dust(C.Structure):
                 ("nudust", _dp), ("kappad", _dp), ("adust", _dp), ("gdust", _dp),
                 ("pldust", _dp), ("kappav", C.c_double * NDG), ("nT", C.c_int),
                 ("nnu", C.c_int), ("lTint", _dp), ("nuint", _dp), ("kapBint", _dp),
-                ("kappap", _dp), ("kapdBint", _dp)]
+                ("kappap", _dp), ("kapdBint", _dp), ("kappar", _dp)]
 
 
 class Setup(C.Structure):
@@ -147,6 +148,13 @@ def load(variant: str = "libm") -> C.CDLL:
         lib.hco_dustfreq.restype = C.c_double
         lib.hco_kapp.argtypes = [C.POINTER(Dust), C.c_double, C.c_int]
         lib.hco_kapp.restype = C.c_double
+        lib.hco_chir.argtypes = [C.POINTER(Dust), C.c_double, C.c_int]
+        lib.hco_chir.restype = C.c_double
+        lib.hco_isotrp.argtypes = [C.POINTER(Rng), C.c_double, C.c_double, C.c_double, _dp]
+        lib.hco_emit_diffusion.argtypes = [C.POINTER(Grid), C.POINTER(Dust), C.POINTER(Setup),
+                                           C.POINTER(Tallies), C.POINTER(Rng), C.c_double, _dp,
+                                           _ip, _ip, _dp, _dp]
+        lib.hco_emit_diffusion.restype = C.c_int
         lib.hco_dusttemp.argtypes = [C.POINTER(Dust)] + [C.c_double] * 5 + [C.c_int]
         lib.hco_dusttemp.restype = C.c_double
         lib.hco_opacset.argtypes = [C.POINTER(Dust), C.c_double, C.POINTER(Opac)]
@@ -216,6 +224,18 @@ class Oracle:
             from hochunk3d_amd.dust import ensure_kapdB
             self._darrs["kapdBint"] = f(ensure_kapdB(d))
             dd.kapdBint = _p(self._darrs["kapdBint"])
+        if getattr(m, "diffus", None) is not None:
+            # the diffusion approximation: the model's mask, its walls and
+            # the Rosseland table of chiR
+            from hochunk3d_amd.dust import ensure_kappar
+            A["rarr"] = f(m.rarr)
+            A["diffus"] = np.asfortranarray(m.diffus, np.int8)
+            A["diffdir"] = np.asfortranarray(m.diffdir, np.int8)
+            self.grid.rarr = _p(A["rarr"])
+            self.grid.diffus = _p(A["diffus"], C.c_int8)
+            self.grid.diffdir = _p(A["diffdir"], C.c_int8)
+            self._darrs["kappar"] = f(ensure_kappar(d))
+            dd.kappar = _p(self._darrs["kappar"])
         self.dust = dd
         c = m.cfg
         s = Setup()
@@ -337,6 +357,26 @@ class Oracle:
                               _p(x), _p(cl, C.c_int32), int(on_wall), _p(kd), _p(sv), _p(d),
                               float(nub), int(iflag), int(i_orig), int(peelid))
         return out
+
+    def emit_diffusion(self, xyz, cell, kapd, nstar: float, rng, nabs=None, on_wall=False):
+        """hco_emit_diffusion: the packet at xyz in cell (1-based) leaves the
+        diffusion layer; updates self.tdust in place.  Returns (species
+        1-based, x, y, z, ux, uy, uz, nub, cell, on_wall)."""
+        m = self.model
+        nab = (np.zeros((m.nrg, m.ntg, m.npg, NDG), order="F") if nabs is None
+               else np.asfortranarray(nabs, np.float64))
+        t = Tallies()
+        t.nabs = _p(nab)
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        cl = np.ascontiguousarray(cell, dtype=np.int32)
+        kd = np.ascontiguousarray(kapd, dtype=np.float64)
+        ow = C.c_int(int(on_wall))
+        out = np.zeros(7)
+        id2 = self.lib.hco_emit_diffusion(C.byref(self.grid), C.byref(self.dust),
+                                          C.byref(self.setup), C.byref(t), C.byref(rng),
+                                          float(nstar), _p(x), _p(cl, C.c_int32), C.byref(ow),
+                                          _p(kd), _p(out))
+        return (id2, *out[:7], tuple(int(v) for v in cl), bool(ow.value))
 
     def run(self, np_: int, p_begin: int = 0, p_end: Optional[int] = None, ipeel: int = 0,
             seed1: int = 12345678, seed2: int = 87654321, gen: int = 1, seg0: int = 0,

@@ -19,7 +19,8 @@ program golden_driver
   real*8 :: xran, nu, Tk
   logical(1) :: onw
   character(len=80) :: dname(ndg)
-  real*8, external :: lucydustfreq, KapP, dusttemp, dustfreq
+  real*8, external :: lucydustfreq, KapP, dusttemp, dustfreq, chiR, ierfc
+  real*8 :: sux, suy, suz
   integer, external :: ifix_dummy
 
   call set_all_seeds(12345678, 87654321)
@@ -116,6 +117,32 @@ program golden_driver
     Tk = 0.12d0 * 1.1d0**j
     nu = KapP(Tk/11605.d0, 1)
     write(*,'(A,1X,2(ES26.17E3,1X))') 'kapp', Tk/11605.d0, nu
+  end do
+  ! the Rosseland-mean table of the same setemiss call, and chiR on it
+  ! (inside the table and beyond both of its ends)
+  do i = 1, nT
+    write(*,'(A,1X,I0,1X,ES26.17E3)') 'kappar', i, kappar(i,1)
+  end do
+  do j = 1, 110
+    Tk = 0.06d0 * 1.1d0**j
+    nu = chiR(Tk/11605.d0, 1)
+    write(*,'(A,1X,2(ES26.17E3,1X))') 'chir', Tk/11605.d0, nu
+  end do
+
+  ! ---- 7b. ierfc (mathsubs.f90) and isotrp on generator 8
+  do j = 1, 60
+    xs = 1.9d0 * dble(j) / 60.d0
+    nu = ierfc(xs)
+    write(*,'(A,1X,2(ES26.17E3,1X))') 'ierfc', xs, nu
+  end do
+  call set_current_generator(8)
+  call reinitialize_current_generator(-1)
+  do j = 1, 300
+    suz = cos(0.37d0 * dble(j))
+    sux = sqrt(1.d0 - suz*suz) * cos(1.3d0 * dble(j))
+    suy = sqrt(1.d0 - suz*suz) * sin(1.3d0 * dble(j))
+    call isotrp(sux, suy, suz, ux, uy, uz)
+    write(*,'(A,1X,6(ES26.17E3,1X))') 'isotrp', sux, suy, suz, ux, uy, uz
   end do
 
   ! ---- 8. lucydustfreq on a formula table (exact in IEEE arithmetic)

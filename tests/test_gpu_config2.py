@@ -19,9 +19,9 @@ pytestmark = pytest.mark.gpu
 def model():
     t0 = time.time()
     # the image spans the whole grid (rmax 5000 AU) so that it holds every peeled ray
-    # the model's initial temperatures, as the benchmark's first iteration has
-    # them (a synthetic warm profile leaves the inner midplane unmasked and
-    # one packet walks ~7e6 steps there)
+    # from the initial temperatures, as the benchmark starts (a synthetic warm
+    # profile leaves the inner midplane unmasked and one packet walks ~7e6
+    # steps there)
     m = build_model(config2(rmaxi_au=5000.0), device=0)
     print(f"config2 {m.nrg - 1}x{m.ntg - 1}x{m.npg - 1} cells built in {time.time() - t0:.1f}s",
           flush=True)
@@ -34,6 +34,12 @@ def test_full_model_lucy_pass_bookkeeping(model, precision):
     g = Transport(model, precision=HC_FP64 if precision == "fp64" else HC_FP32)
     try:
         n = 2_000_000
+        # three Lucy iterations from the initial 0.1 K bring the temperatures
+        # to a realistic state (at 0.1 K re-emitted packets leave at radio
+        # wavelengths and the pass is trivial)
+        for i in range(3):
+            g.run_scalars(n, seg0=(i + 1) * n)
+            g.temperature(n, apply=True)
         s, ms = g.run_scalars(n)
         print(f"{precision} lucy pass {n} packets: {ms:.0f} ms, "
               f"longest {s['max_packet_work']:.0f}", flush=True)
@@ -41,13 +47,14 @@ def test_full_model_lucy_pass_bookkeeping(model, precision):
         assert g.dropped_packets()[0] == 0
         tol = 1e-9 if precision == "fp64" else 1e-5
         assert abs(s["sumall"] + s["abflux"] - n) <= tol * n
-        assert s["nscat"] > n and s["tot"] > 0
+        print(f"interacting packets {s['tot']:.0f}, scatterings {s['nscat']:.0f}", flush=True)
+        assert s["tot"] > 0.01 * n and s["nscat"] > 0.01 * n
         t0 = g.get_temperature()
         g.temperature(n, apply=True)
         t1 = g.get_temperature()
         live = model.densarr[:-1, :-1, :-1, :] > 0
         moved = (t1[:-1, :-1, :-1, :] != t0[:-1, :-1, :-1, :]) & live
-        assert moved.sum() > 0.5 * live.sum()
+        assert moved.sum() > 0.3 * live.sum()
         assert np.isfinite(t1).all() and (t1[:-1, :-1, :-1, :][live] > 0).all()
     finally:
         g.close()

@@ -3,9 +3,10 @@
 tests/golden/ref_leaves.npz holds input/output vectors that
 oracle/ref/golden_driver.f90 wrote after compiling the reference's sources
 (ecuyer_cote_mod, random, radface, phiface, closest_wall, nr.f locate, dust.f90
-opacset/setemiss/KapP, lucydustfreq, dusttemp, select_dust, dustmat,
-findangle3) in the build container.  Everything but setemiss (adaptive Simpson
-in the reference, Gauss-Legendre here) must match bit for bit.
+opacset/setemiss/KapP/chiR, lucydustfreq, dusttemp, select_dust, dustmat,
+findangle3, mathsubs ierfc, isotrp) in the build container.  Everything but
+setemiss (adaptive Simpson in the reference, Gauss-Legendre here) must match
+bit for bit.
 """
 import ctypes as C
 import os
@@ -86,7 +87,7 @@ def test_dustinit_arrays(dust):
     np.testing.assert_array_equal(dust.gdust[i, 0], d[:, 4])
 
 
-def _dust_struct(dust, kapBint=None, kappap=None):
+def _dust_struct(dust, kapBint=None, kappap=None, kappar=None):
     keep = {}
 
     def p(name, arr):
@@ -105,6 +106,8 @@ def _dust_struct(dust, kapBint=None, kappap=None):
     d.lTint, d.nuint = p("lT", dust.lTint), p("nu", dust.nuint)
     d.kapBint = p("kb", dust.kapBint if kapBint is None else kapBint)
     d.kappap = p("kp", dust.kappap if kappap is None else kappap)
+    if kappar is not None:
+        d.kappar = p("kr", kappar)
     return d, keep
 
 
@@ -214,3 +217,39 @@ def test_findangle3(lib):
         out = [C.c_double() for _ in range(4)]
         lib.hco_findangle3(*[float(v) for v in row[:10]], *[C.byref(o) for o in out])
         assert [o.value for o in out] == list(row[10:14])
+
+
+def test_setemiss_rosseland_table(dust):
+    """kappar (the Rosseland mean of setemiss, the diffusion layer's chiR)
+    to the reference's QSIMP tolerance."""
+    from hochunk3d_amd.dust import ensure_kappar
+    ref = G["kappar"]
+    got = ensure_kappar(dust)[ref[:, 0].astype(int) - 1, 0]
+    np.testing.assert_allclose(got, ref[:, 1], rtol=2e-5)
+
+
+def test_chir_on_reference_table(lib, dust):
+    """chiR (dust.f90:83-111) on the reference's own kappar, inside the table
+    and beyond both ends: bit for bit, the oracle's and the host's."""
+    from hochunk3d_amd.diffusion import chir
+    kr = np.asfortranarray(np.tile(G["kappar"][:, 1:2], (1, 8)))
+    d, keep = _dust_struct(dust, kappar=kr)
+    for T, val in G["chir"]:
+        assert lib.hco_chir(C.byref(d), T, 0) == val, T
+        assert chir(kr, dust.lTint, T, 0) == val, T
+
+
+def test_ierfc(lib):
+    from hochunk3d_amd.diffusion import ierfc
+    for x, val in G["ierfc"]:
+        assert ierfc(x) == val, x
+
+
+def test_isotrp(lib):
+    """isotrp (isotrp.f90) on generator 8: the reference's draws and
+    directions bit for bit."""
+    r = _rng(lib, 8)
+    u = np.zeros(3)
+    for sux, suy, suz, ux, uy, uz in G["isotrp"]:
+        lib.hco_isotrp(C.byref(r), sux, suy, suz, u.ctypes.data_as(C.POINTER(C.c_double)))
+        assert (u[0], u[1], u[2]) == (ux, uy, uz)

@@ -12,6 +12,7 @@ import pytest
 
 import test_atmosphere as at
 import test_filters as fi
+import test_oracle_diffusion as dif
 import test_oracle_closed_forms as cf
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -63,6 +64,16 @@ PLANTS = [
      ["filter"]),
     ("filter range closed at the top", "if (freq > fw[0] && freq < fw[nf - 1]) {",
      "if (freq > fw[0] && freq <= fw[nf - 1]) {", ["filteredge"]),
+    ("isotrp without the square", "while (r2 * r2 > fabs(mur)) {",
+     "while (r2 > fabs(mur)) {", ["isotrp"]),
+    ("radial surface ring area without 2",
+     "dA = 2.0 * PI * (ra[ir - 1] * ra[ir - 1])", "dA = PI * (ra[ir - 1] * ra[ir - 1])",
+     ["difradial"]),
+    ("theta surface move onto the cone itself", "cosbnew = 1.00001 * cost_wall(g, it - 1);",
+     "cosbnew = cost_wall(g, it - 1);", ["diftheta"]),
+    ("column density of the next cell",
+     "const size_t km = cell_index(g, i - 1, it, ip) + c->ncell * id;",
+     "const size_t km = cell_index(g, i, it, ip) + c->ncell * id;", ["difcolumn"]),
     ("filter range closed at the bottom", "if (freq > fw[0] && freq < fw[nf - 1]) {",
      "if (freq >= fw[0] && freq < fw[nf - 1]) {", ["filteredge"]),
     ("filter Q unweighted", "t->image_b_q[q] += sqp * wt;", "t->image_b_q[q] += sqp;",
@@ -89,6 +100,11 @@ def _checks():
                   at.test_atmosfreq_fills_brackets_with_their_probability_and_log_law],
         "limb": [at.test_limb_darkening_cosine_law],
         "limbpeel": [at.test_limb_darkened_disk_flux_and_peel_weight],
+        "isotrp": [dif.test_isotrp_is_lambertian_about_the_normal],
+        "difradial": [lambda: dif.test_radial_surface_move_direction_and_temperatures(
+            dif.radial_model())],
+        "diftheta": [lambda: dif.test_theta_surfaces_up_and_down(dif.thick_model())],
+        "difcolumn": [lambda: dif.test_column_sums_with_absorbed_counts(dif.radial_model())],
         "filter": [fi.test_filter_image_pixel_weight_and_stokes],
         "filteredge": [fi.test_filter_image_outside_every_node_range_is_empty],
     }
@@ -105,7 +121,7 @@ def test_planted_mistake_is_caught(tmp_path, name, orig, planted, checks):
                                        "-lm"], check=True)
     table = _checks()
     old = cf.ORACLE
-    cf.ORACLE = at.ORACLE = fi.ORACLE = str(so)
+    cf.ORACLE = at.ORACLE = fi.ORACLE = dif.ORACLE = str(so)
     caught = []
     try:
         for key in checks:
@@ -115,5 +131,5 @@ def test_planted_mistake_is_caught(tmp_path, name, orig, planted, checks):
                 except AssertionError:
                     caught.append(key)
     finally:
-        cf.ORACLE = at.ORACLE = fi.ORACLE = old
+        cf.ORACLE = at.ORACLE = fi.ORACLE = dif.ORACLE = old
     assert caught, f"planted mistake '{name}' passed every check {checks}"
