@@ -264,7 +264,7 @@ def main():
     ap.add_argument("--warmup-np", type=float, default=0,
                     help="packets per warm-up iteration (default a quarter of --np)")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 path")
-    ap.add_argument("--peel-np", type=float, default=500_000,
+    ap.add_argument("--peel-np", type=float, default=2_000_000,
                     help="packets of the final peeled pass timed after the Lucy steps (0: skip)")
     ap.add_argument("--autotune-np", type=float, default=-1,
                     help="packets per autotuning probe pass (-1: the step's packet count; "
@@ -350,21 +350,34 @@ def main():
     if args.peel_np > 0:
         pnp = int(args.peel_np)
         if anp > 0:
-            r = tune_transport(tr, max(pnp // 5, 1000), ipeel=1)
+            r = tune_transport(tr, max(pnp // 20, 1000), ipeel=1)
             tuned["fp64_peeled"] = {"options": r.options, "gain_vs_default": round(r.gain, 3),
                                     "probes": len(r.trials)}
             log(f"autotune peeled pass: {r.options}, x{r.gain:.3f} vs default")
+        # a quarter-size pass first: the difference of the two passes' times
+        # is the rate of the bulk, which a 1e8-packet final pass runs at (a
+        # pass also carries its longest packet's serial walk, ~1 s)
+        q = max(pnp // 4, 1)
         t0 = time.perf_counter()
-        s, ms = tr.run_scalars(pnp, ipeel=1, seg0=seg + npk)
+        tr.run_scalars(q, ipeel=1, seg0=seg + npk)
+        wallq = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        s, ms = tr.run_scalars(pnp, ipeel=1, seg0=seg + npk + q)
         wallp = time.perf_counter() - t0
+        marginal = (pnp - q) / (wallp - wallq) if wallp > wallq else None
         peel = {"packets": pnp, "value": round(pnp / wallp, 2), "unit": "packets/s",
                 "kernel_ms": round(ms, 3),
+                "bulk_packets_per_s": round(marginal, 2) if marginal else None,
+                "final_pass_1e8_packets_s": round(1e8 / marginal, 1) if marginal else None,
                 "peel_rays_per_packet": round(s["npeelray"] / pnp, 1),
                 "peel_crossings_per_packet": round(s["npeelcross"] / pnp, 1),
-                "observers": cfg.npeel, "image_pixels": f"{cfg.nximg}x{cfg.nximg}"}
+                "observers": cfg.npeel, "image_pixels": f"{cfg.nximg}x{cfg.nximg}",
+                "note": f"bulk rate from passes of {q} and {pnp} packets "
+                        f"({wallq:.2f} s and {wallp:.2f} s)"}
         log(f"peeled pass: {pnp} packets in {wallp * 1e3:.0f} ms ({pnp / wallp:.3e} packets/s), "
             f"{s['npeelray'] / pnp:.0f} peel rays and {s['npeelcross'] / pnp:.0f} peel-ray "
-            f"crossings per packet")
+            f"crossings per packet; bulk {marginal or 0:.3e} packets/s, a 1e8-packet final "
+            f"pass ~{1e8 / marginal if marginal else 0:.0f} s")
     tr.close()
 
     fp32 = None

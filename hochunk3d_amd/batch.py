@@ -164,3 +164,65 @@ def run_batch(models: List[GridModel], iterations: int = 3, streams: int = 4,
             tr.close()
     return BatchResult(sum(p["packets"] for p in per), wall, setup,
                        sum(p["kernel_ms"] for p in per), per)
+
+
+def measure_grid(mode: str, nmodels: int = 256, iterations: int = 4, first: int = 0,
+                 log=print) -> dict:
+    """The parameter grid at its drawn packet counts, iteration by iteration
+    from the initial temperatures, on the GPU: mode "fp32fused"/"fp64fused"
+    runs every model's passes in one launch per iteration (hc_run_batch),
+    "fp64plain" one model after the other on the plain FP64 path (a pass and
+    its temperature update each).  Models are built with device densities.
+    Returns the packets/s of each iteration and of all of them."""
+    from ._lib import HC_FP32, HC_FP64, Transport, run_batch_lucy
+    grid = parameter_grid(nmodels + first)[first:]
+    pk = [g.packets for g in grid]
+    ncell = sum(g.cfg.nr * g.cfg.ntheta * g.cfg.nphi for g in grid)
+    log(f"{mode}: {len(grid)} models, {ncell / 1e6:.1f}M cells, {sum(pk) / 1e9:.3f}e9 packets "
+        f"per iteration")
+    t0 = time.perf_counter()
+    prec = HC_FP32 if mode.startswith("fp32") else HC_FP64
+    trs = []
+    try:
+        for i, g in enumerate(grid):
+            trs.append(Transport(build_model(g.cfg, device=0), precision=prec))
+            if i % 64 == 63:
+                log(f"built and uploaded {i + 1} models in {time.perf_counter() - t0:.0f} s")
+        rates, total = [], 0.0
+        for it in range(iterations):
+            t1 = time.perf_counter()
+            if mode.endswith("fused"):
+                run_batch_lucy(trs, 1, pk, seg0=[it * p for p in pk])
+            else:
+                for i, tr in enumerate(trs):
+                    tr.run_scalars(pk[i], seg0=it * pk[i])
+                    tr.temperature(pk[i], apply=True)
+                    if i % 32 == 31:
+                        log(f"  iteration {it}: {i + 1} models in "
+                            f"{time.perf_counter() - t1:.0f} s")
+            w = time.perf_counter() - t1
+            total += w
+            rates.append(sum(pk) / w)
+            log(f"iteration {it}: {sum(pk) / w:.4e} packets/s ({w:.1f} s)")
+    finally:
+        for tr in trs:
+            tr.close()
+    out = {"mode": mode, "models": len(grid), "cells": ncell, "packets_per_iteration": sum(pk),
+           "iteration_packets_per_s": [round(r, 1) for r in rates],
+           "packets_per_s": round(iterations * sum(pk) / total, 1)}
+    log(f"{mode}: {out['packets_per_s']:.4e} packets/s over {iterations} iterations")
+    return out
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="the parameter grid at its drawn packet counts")
+    ap.add_argument("mode", choices=["fp32fused", "fp64fused", "fp64plain"])
+    ap.add_argument("--models", type=int, default=256)
+    ap.add_argument("--iterations", type=int, default=4)
+    ap.add_argument("--first", type=int, default=0)
+    a = ap.parse_args()
+    print(json.dumps(measure_grid(a.mode, a.models, a.iterations, a.first,
+                                  log=lambda m: print(m, flush=True))))
