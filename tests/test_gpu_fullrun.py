@@ -99,7 +99,7 @@ def _z(xa, xb, keep):
     ma, mb = xa.mean(axis=0), xb.mean(axis=0)
     va, vb = xa.var(axis=0, ddof=1), xb.var(axis=0, ddof=1)
     ok = keep & (va + vb > 0)
-    return ((ma - mb) / np.sqrt((va + vb) / K))[ok]
+    return (ma - mb)[ok] / np.sqrt((va + vb)[ok] / K)
 
 
 def _check(name, zb, zf, zg):
