@@ -710,12 +710,22 @@ __device__ __forceinline__ int peel_step(const WallView &W, const PeelRec<R, NA>
   double dt;
   int face;
   if (S.x == 0.0 && S.y == 0.0) S.ip = axis_phi_cell(S.ux, S.uy, S.ip);
+  // the cell's densities are loaded before its walls are computed, so that
+  // the load's latency overlaps the roots (a walking ray is inside the grid);
+  // held across the wavefront's steps instead, they cost more spills than
+  // the longer overlap gains
+  double dm[NA];
+  {
+    const double *dp = P.densm + cell_of(S.ir, S.it, S.ip) * (size_t)P.nact;
+#pragma unroll
+    for (int s = 0; s < NA; ++s) dm[s] = (s < P.nact) ? dp[s] : 0.0;
+  }
   if (S.ncross > max_cross ||
       !next_wall(W, S.ux, S.uy, S.uz, S.x, S.y, S.z, S.ir, S.it, S.ip, S.on_wall, dt, face)) {
     S.err = 1;
     return kPeelIdle;
   }
-  S.tsum = S.tsum + rec_opac(k, P.densm + cell_of(S.ir, S.it, S.ip) * (size_t)P.nact) * dt;
+  S.tsum = S.tsum + rec_opac(k, dm) * dt;
   if (S.tsum > kTauMax) {
     S.tau = S.tsum;
     return kPeelIdle;
