@@ -167,13 +167,15 @@ def run_batch(models: List[GridModel], iterations: int = 3, streams: int = 4,
 
 
 def measure_grid(mode: str, nmodels: int = 256, iterations: int = 4, first: int = 0,
-                 log=print) -> dict:
+                 log=print, oversubscribe: int = 0) -> dict:
     """The parameter grid at its drawn packet counts, iteration by iteration
     from the initial temperatures, on the GPU: mode "fp32fused"/"fp64fused"
     runs every model's passes in one launch per iteration (hc_run_batch),
     "fp64plain" one model after the other on the plain FP64 path (a pass and
     its temperature update each).  Models are built with device densities.
-    Returns the packets/s of each iteration and of all of them."""
+    oversubscribe: the fused launch's workgroups per resident workgroup (0:
+    the library's default).  Returns the packets/s of each iteration and of
+    all of them."""
     from ._lib import HC_FP32, HC_FP64, Transport, run_batch_lucy
     grid = parameter_grid(nmodels + first)[first:]
     pk = [g.packets for g in grid]
@@ -192,7 +194,7 @@ def measure_grid(mode: str, nmodels: int = 256, iterations: int = 4, first: int 
         for it in range(iterations):
             t1 = time.perf_counter()
             if mode.endswith("fused"):
-                run_batch_lucy(trs, 1, pk, seg0=[it * p for p in pk])
+                run_batch_lucy(trs, 1, pk, seg0=[it * p for p in pk], oversubscribe=oversubscribe)
             else:
                 for i, tr in enumerate(trs):
                     tr.run_scalars(pk[i], seg0=it * pk[i])
@@ -208,6 +210,7 @@ def measure_grid(mode: str, nmodels: int = 256, iterations: int = 4, first: int 
         for tr in trs:
             tr.close()
     out = {"mode": mode, "models": len(grid), "cells": ncell, "packets_per_iteration": sum(pk),
+           "oversubscribe": oversubscribe,
            "iteration_packets_per_s": [round(r, 1) for r in rates],
            "packets_per_s": round(iterations * sum(pk) / total, 1)}
     log(f"{mode}: {out['packets_per_s']:.4e} packets/s over {iterations} iterations")
@@ -223,6 +226,8 @@ if __name__ == "__main__":
     ap.add_argument("--models", type=int, default=256)
     ap.add_argument("--iterations", type=int, default=4)
     ap.add_argument("--first", type=int, default=0)
+    ap.add_argument("--oversubscribe", type=int, default=0)
     a = ap.parse_args()
     print(json.dumps(measure_grid(a.mode, a.models, a.iterations, a.first,
-                                  log=lambda m: print(m, flush=True))))
+                                  log=lambda m: print(m, flush=True),
+                                  oversubscribe=a.oversubscribe)))

@@ -267,7 +267,7 @@ int hc_run_lucy(hc_ctx *ctx, int32_t niter, int64_t np, int32_t seed1, int32_t s
  *                  every context already has it);
  *   ltot[m]        the model's luminosity (hc_temperature);
  *   oversubscribe  workgroups launched per workgroup slot of the device,
- *                  0 = 4: each model gets workgroups in proportion to its
+ *                  0 = 64: each model gets workgroups in proportion to its
  *                  packets; a workgroup runs its model's packets until none
  *                  are left, and later workgroups join the models still
  *                  running, so that the models' longest packets overlap.
