@@ -181,6 +181,17 @@ def _zerod(xmu: np.ndarray, rx: np.ndarray) -> np.ndarray:
     return 0.5 * (lo + hi)
 
 
+def ulrich_density(rhoe0, rc, r, cosa):
+    """Rotating infall (Ulrich 1976) at radius r and |cos theta| cosa
+    (densenv.f90:36-45): rhoe0 (r/rc)^-1.5 / ((mu/mu0 + 2 mu0^2 rc/r)
+    sqrt(1 + mu/mu0)) on the streamline mu0 of the point (the envelope is
+    taken symmetric about the midplane)."""
+    rx = r / rc
+    mu0 = np.maximum(_zerod(cosa, rx), 1e-12)
+    factor = 1.0 / ((cosa / mu0) + 2.0 * (mu0 ** 2) / rx)
+    return rhoe0 * rx ** (-1.5) * factor / np.sqrt(1.0 + cosa / mu0)
+
+
 def theta_walls(ntg: int) -> np.ndarray:
     """Power-law theta walls symmetric about the midplane (gridset.f90:529-559)."""
     ncell = ntg - 1
@@ -283,11 +294,7 @@ def _densities_host(cfg, rarr, thetarr, phiarr, rmincgs, autors, rmind, rmine, r
     # Ulrich envelope (envset.f90, densenv.f90) with a polynomial cavity
     rc, rhoe0, c1e = _envelope_params(cfg, rmincgs, autors, rmax)
     cosa = np.abs(cost)
-    rx = R / rc
-    xmu = cosa
-    mu0 = np.maximum(_zerod(xmu, rx), 1e-12)
-    factor = 1.0 / ((xmu / mu0) + 2.0 * (mu0 ** 2) / rx)
-    env = rhoe0 * rx ** (-1.5) * factor / np.sqrt(1.0 + xmu / mu0)
+    env = ulrich_density(rhoe0, rc, R, cosa)
     env = np.where(R > rmine, np.maximum(env, cfg.rhoamb), 0.0)
     env = np.where(R < rmax, env, 0.0)
     # polynomial cavity z > c1e rp^ex1 (envset.f90:25-33, z01 = 0)

@@ -90,3 +90,11 @@ def test_argument_errors_are_reported_before_any_device_call():
     assert frac(64, 5, ok) == E_ARG          # 64^6 = 2^36 points
     assert frac(4, 9, ok) == E_ARG           # levels <= 8
     assert frac(4, 2, even) == E_ARG
+
+
+def test_filter_and_diffusion_entry_points_reject_a_null_context():
+    lib = _lib.load()
+    E_ARG = -1
+    assert lib.hc_set_filters(None, 0, 0, None, None, None) == E_ARG
+    assert lib.hc_get_filter_images(None, None, None, None, None) == E_ARG
+    assert lib.hc_set_diffusion(None, None, None, None, None) == E_ARG

@@ -133,3 +133,16 @@ def test_parallel_layer_surface_steps_and_reset(thick):
             assert np.all(t2[..., :2][np.repeat(lay[..., None], 2, axis=3)] == 0.1 / 11605.0)
         finally:
             g.close()
+
+
+def test_diffusion_with_bjorkman_wood_is_refused(thick):
+    from hochunk3d_amd._lib import HcError, Transport
+    g = Transport(thick)
+    try:
+        g.set_bw(True)
+        with pytest.raises(HcError):
+            g.run(100)
+        g.set_bw(False)
+        assert g.run(100).scalars["n_err_dust"] == 0
+    finally:
+        g.close()
