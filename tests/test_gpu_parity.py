@@ -266,3 +266,30 @@ def test_atmosphere_and_limb_launch_matches_oracle(cplanckst, limb, spots):
     cp = c.run(400, ipeel=1, trace=True, seg0=70_000)
     compare_traces(rp.trace, cp["trace"])
     rel_close(rp.peel, cp["peel"], rtol=1e-9, atol=1e-12)
+
+
+def test_config1_at_its_full_binning_matches_oracle():
+    """The plain model at its benchmark binning (200 frequencies, 10
+    observers, 101^2-pixel images, 64x64x32 cells): a Lucy pass and a peeled
+    pass packet by packet, the image cube and the peeled SED to 1e-9."""
+    from hochunk3d_amd._lib import Transport
+    from oracle.pyoracle import Oracle
+    m = build_model(config1())
+    assert (m.cfg.nfreq, m.cfg.nximg, m.cfg.npeel) == (200, 101, 10)
+    m.tdust[...] = synthetic_temperature(m)
+    g = Transport(m, device=0)
+    try:
+        rg = g.run(10000, ipeel=0, trace=True, seg0=31)
+        pg = g.run(800, ipeel=1, trace=True, seg0=41)
+    finally:
+        g.close()
+    c = Oracle(m)
+    rc = c.run(10000, ipeel=0, trace=True, seg0=31)
+    compare_traces(rg.trace, rc["trace"])
+    _check_tallies(rg, rc)
+    pc = c.run(800, ipeel=1, trace=True, seg0=41)
+    compare_traces(pg.trace, pc["trace"])
+    _check_tallies(pg, pc)
+    rel_close(pg.peel, pc["peel"], rtol=1e-9, atol=1e-12)
+    for k in ("img_i", "img_q", "img_u", "img_v"):
+        rel_close(getattr(pg, k), pc[k], rtol=1e-9, atol=1e-12)
