@@ -78,10 +78,10 @@ def pass_bytes(s: dict, fp32: bool = False) -> float:
 # fractal disk + envelope model at 2.388e7 packets (3 x cells), after five
 # warm-up iterations: rocprofv3 FETCH_SIZE x 2 plus WRITE_SIZE x 1 (KiB per
 # dispatch; the factors are what the two counters read for a 256 MiB device
-# copy).  72.9 kB read and 53.0 kB written per packet, 6.4 x the algorithmic
+# copy).  75.1 kB read and 53.3 kB written per packet, 6.5 x the algorithmic
 # density bytes: the reads are 64-128 B lines for 32 B of density, the
 # writes evictions of the per-cell tally lines the atomics dirty.
 CONFIG2_STEP_PACKETS = 23_880_000
 CONFIG2_TRAFFIC_PROFILE = "profiles/transport_fp64_config2_pmc.md"
-CONFIG2_STEP_HBM_READ = 1.74109e12
-CONFIG2_STEP_HBM_WRITE = 1.26454e12
+CONFIG2_STEP_HBM_READ = 1.79390e12
+CONFIG2_STEP_HBM_WRITE = 1.27173e12
