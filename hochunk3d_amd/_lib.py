@@ -335,7 +335,7 @@ class Transport:
 
     def set_option(self, name: str, value: int):
         """Launch-shape knob: regen_lanes, regen_run_lanes, peel_queue,
-        peel_waves, blocks_per_cu or max_work (include/hc.h documents each)."""
+        blocks_per_cu or max_work (include/hc.h documents each)."""
         self._check(self.lib.hc_set_option(self.h, name.encode(), int(value)), "hc_set_option")
 
     def dropped_packets(self):

@@ -6,8 +6,7 @@ its lanes' packets, not what the packets do:
 - regen_run_lanes: waiters that make an event kind run once the walk has
   stopped;
 - blocks_per_cu: resident workgroups per compute unit;
-- peel_queue: depth of the LDS peel-off queues (peeled passes only);
-- peel_waves: wavefront queues per peel-off workgroup (peeled passes only).
+- peel_queue: depth of the LDS peel-off queues (peeled passes only).
 
 Every candidate is timed on the same packets (the same generator segments)
 from the same temperatures, so the probes do the same work and only the
@@ -23,8 +22,7 @@ from typing import Callable, Dict, List, Optional, Sequence
 # the library's defaults (include/hc.h): regen 8/8 on the FP64 path, 12/8 on
 # the FP32 path, the fastest of 8/8, 12/8 and 16/16 on full passes of the
 # 200x199x200-cell model
-DEFAULTS = {"regen_lanes": 8, "regen_run_lanes": 8, "blocks_per_cu": 0, "peel_queue": 0,
-            "peel_waves": 0}
+DEFAULTS = {"regen_lanes": 8, "regen_run_lanes": 8, "blocks_per_cu": 0, "peel_queue": 0}
 DEFAULTS_FP32 = dict(DEFAULTS, regen_lanes=12)
 
 # regen_lanes 24 and 32 can win on the short probes, whose time is mostly
@@ -36,7 +34,6 @@ LUCY_SPACE: Dict[str, Sequence[int]] = {
     "blocks_per_cu": (1, 0),
 }
 PEEL_SPACE: Dict[str, Sequence[int]] = {
-    "peel_waves": (8, 4),
     "peel_queue": (65, 80, 96, 128),
     "regen_lanes": (8, 16, 32),
 }

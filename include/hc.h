@@ -298,10 +298,6 @@ int hc_run_batch(int32_t nmodels, hc_ctx *const *ctxs, int32_t niter, const int6
  *   "peel_queue"    0 or 65..1024: peel-off records per wavefront queue in
  *                   LDS (0 = 96, or fewer when the LDS left by the grid's
  *                   walls and dust tables holds fewer);
- *   "peel_waves"    0, 4 or 8: wavefronts (each with its peel queue) per
- *                   workgroup of a peel-off pass, 0 = 4 (default 0); 8 needs
- *                   eight queues of at least 65 records beside the walls
- *                   and tables in the LDS, else the pass is refused;
  *   "blocks_per_cu" 0..64: workgroups per compute unit, 0 = as many as fit
  *                   (default 0);
  *   "serial"        0 or 1: one lane of the device runs the pass's packets

@@ -75,17 +75,17 @@ def test_peeled_pass_matches_oracle(pair):
         rel_close(getattr(rg, k), rc[k], rtol=1e-9, atol=1e-12)
 
 
-@pytest.mark.parametrize("waves", [4, 8])
-def test_peeled_pass_either_workgroup_shape_matches_oracle(pair, waves):
-    """Peel-off workgroups of four or eight wavefront queues (peel_waves)
-    give the same packets; only the order of the tally sums differs."""
+@pytest.mark.parametrize("depth", [65, 128])
+def test_peeled_pass_any_queue_depth_matches_oracle(pair, depth):
+    """Peel-off queues of the shallowest and a deep size (peel_queue) give
+    the same packets; only the order of the tally sums differs."""
     g, c = pair
     n = 1500
-    g.set_option("peel_waves", waves)
+    g.set_option("peel_queue", depth)
     try:
         rg = g.run(n, ipeel=1, trace=True, seg0=777)
     finally:
-        g.set_option("peel_waves", 0)
+        g.set_option("peel_queue", 0)
     rc = c.run(n, ipeel=1, trace=True, seg0=777)
     compare_traces(rg.trace, rc["trace"])
     _check_tallies(rg, rc)
