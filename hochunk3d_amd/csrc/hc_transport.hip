@@ -301,15 +301,16 @@ __device__ __forceinline__ int axis_phi_cell(double ux, double uy, int ip) {
   return axis_phi_cell_call(c_P.phiarr, c_P.npg, ux, uy, ip);
 }
 
+// The cell across wall `face`: 1/2 the inner/outer sphere, 3/4 the theta
+// walls, 5 the lower phi plane, anything else the upper one; phi is periodic.
+// Written as selects: a switch over the three references compiles to a
+// pointer chosen at run time, which puts the cell indices in scratch memory,
+// and every scratch access then waits on the in-order memory counter that
+// the density loads and the tally atomics also count on.
 __device__ __forceinline__ void face_step(int face, int &ir, int &it, int &ip) {
-  switch (face) {
-    case 1: ir -= 1; break;
-    case 2: ir += 1; break;
-    case 3: it -= 1; break;
-    case 4: it += 1; break;
-    case 5: ip -= 1; break;
-    default: ip += 1; break;
-  }
+  ir += face == 1 ? -1 : (face == 2 ? 1 : 0);
+  it += face == 3 ? -1 : (face == 4 ? 1 : 0);
+  ip += face == 5 ? -1 : ((face < 1 || face > 5) ? 1 : 0);
   if (ip == 0) ip = c_P.npg - 1;
   if (ip == c_P.npg) ip = 1;
 }
@@ -459,20 +460,23 @@ __device__ __forceinline__ int walk_step(const WallView &W, Pkt<R, NA> &k, doubl
     if constexpr (TRACE) tr.path += ds;
     return ST_INTERACT;
   };
-  const double opac = cell_opac(k, dm);
   // After an interaction (the packet is off the walls) the next one is often
   // closer than any wall of the cell: then the walls' roots are not needed,
   // and the point is the one the full step below reaches.
   if (!k.on_wall) {
-    const double ds = (tau - tsum) / opac;  // no shortcut for opac = 0 (inf or NaN)
+    const double ds = (tau - tsum) / cell_opac(k, dm);  // opac = 0: inf or NaN, no shortcut
     if (ds < wall_clearance(W, k)) return interact(ds);
   }
+  // A packet on a wall has just crossed it: its densities were loaded at the
+  // crossing, and the flush atomics issued after that load share its in-order
+  // counter.  The roots come first, so that wait overlaps them.
   double dt;
   int face;
   if (!next_wall(W, k.ux, k.uy, k.uz, k.x, k.y, k.z, k.ir, k.it, k.ip, k.on_wall, dt, face)) {
     acc.err_wall++;
     return ST_ERROR;
   }
+  const double opac = cell_opac(k, dm);
   const double dtau = opac * dt;
   if (tsum + dtau >= tau) {
     double ds = (tau - tsum) / opac;
@@ -1888,10 +1892,6 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-#ifndef HC_WAVES_PER_EU
-#define HC_WAVES_PER_EU 1  // occupancy floor; below it the compiler would spill
-#endif
-
 // Peel records per wavefront queue (Params::peel_q, sized on the host to the
 // LDS left after the wall arrays).  An iteration adds at most one record per
 // lane, so a queue flushed whenever it holds more than peel_q - 64 records
@@ -1995,7 +1995,8 @@ static_assert(kPoolLanes == 512, "queue entries hold 9-bit lane numbers");
 constexpr int kBlockLucy = HC_BLOCK_LUCY;
 
 template <class R, int NA, bool TRACE, bool PEEL, bool POOL>
-__global__ __launch_bounds__(PEEL ? kBlockPeel : POOL ? kPoolLanes : kBlockLucy, HC_WAVES_PER_EU) void transport(
+__global__ __launch_bounds__(PEEL ? kBlockPeel : POOL ? kPoolLanes : kBlockLucy,
+                             (PEEL || POOL || TRACE || NA > 4) ? 1 : 2) void transport(
     Params, const Params *, const uint32_t * /* read through kernel_params() */) {
   static_assert(!(PEEL && POOL), "the event pool shares the LDS with no peel-off queues");
   const Params &P = c_P;
