@@ -189,10 +189,23 @@ struct Pkt {
 };
 
 struct Acc {  // per-lane counters, flushed once per lane
-  uint32_t tot, flux, sflux, nscat, scount, abflux, aflux, hit, htot, err_wall, err_dust;
+  uint32_t nscat, hit, htot, err_wall, err_dust;
   uint32_t ncross, nscat_ev, nabs_ev, npeelray, npeelcross, warn, maxwork;
+};
+
+// Counters of the once-per-packet events (launch, exit), one set per
+// wavefront in LDS rather than in every lane's registers (registers the walk
+// needs: with them the FP64 kernel spilled to scratch, and a scratch reload
+// waits on the same in-order counter as the density loads and the tally
+// atomics).  Lanes add with LDS atomics; each wavefront flushes its own set.
+struct WaveCold {
+  unsigned int tot, flux, sflux, scount, abflux, aflux;
   double sumsub, sumall;
 };
+constexpr int kMaxWaves = 8;  // wavefronts of the largest workgroup (512 lanes)
+__shared__ WaveCold hc_cold[kMaxWaves];
+__device__ __forceinline__ WaveCold &cold() { return hc_cold[threadIdx.x >> 6]; }
+__device__ __forceinline__ void cold_inc(unsigned int &c) { atomicAdd(&c, 1u); }
 
 // Lucy tally of the cell the packet is in: segments accumulate here and go to
 // the device arrays with one atomic per species when the packet leaves the
@@ -203,7 +216,7 @@ struct LucyAcc {
   long long cell;
   double tau[NA];
   double len;
-  double nabs[NA];  // absorptions per species (nabs)
+  uint32_t nabs[NA];  // absorptions per species (nabs)
   int nabs_any;     // absorptions of any species (dbig)
   uint32_t ntally;  // per-cell tally atomics issued by this lane (all its packets)
 };
@@ -338,8 +351,8 @@ __device__ __forceinline__ void lucy_flush(LucyAcc<R, NA> &la) {
     double *na = P.nabs + (size_t)la.cell * (size_t)P.nact;
 #pragma unroll
     for (int s = 0; s < NA; ++s)
-      if (s < P.nact && la.nabs[s] != 0.0) {
-        atomic_add(&na[s], la.nabs[s]);
+      if (s < P.nact && la.nabs[s] != 0u) {
+        atomic_add(&na[s], (double)la.nabs[s]);
         la.ntally++;
       }
     atomicAdd(&P.dbig[la.cell], la.nabs_any);
@@ -356,7 +369,7 @@ __device__ __forceinline__ void lucy_open(LucyAcc<R, NA> &la, size_t cell) {
 #pragma unroll
     for (int s = 0; s < NA; ++s) {
       la.tau[s] = 0.0;
-      la.nabs[s] = 0.0;
+      la.nabs[s] = 0u;
     }
     la.len = 0.0;
     la.nabs_any = 0;
@@ -1756,7 +1769,7 @@ __device__ __forceinline__ void launch_star(Pkt<R, NA> &k, Mrg &rng, bool hotspo
 // Star re-emission of a packet that reached the stellar surface.
 template <class R, int NA>
 __device__ __forceinline__ void star_reemit(Pkt<R, NA> &k, Mrg &rng, Acc &acc, R *nrm) {
-  acc.aflux++;
+  cold_inc(cold().aflux);
   const R cosb = clamp1(R(k.z / sqrt(k.rsq)));
   const R sinb = m_sqrt(R(1.0) - cosb * cosb);
   R lp = m_atan2(R(k.y), R(k.x));
@@ -1818,9 +1831,9 @@ __device__ __forceinline__ void bin_exit(Pkt<R, NA> &k, double xpold, double ypo
   const double y = zpold * P.sini[it - 1] - ypold * ut * sp - xpold * ut * cp;
   const double rho2 = (x * x + y * y);
   if (rho2 < P.aperture2[P.nap - 1] * 1.0001) {
-    acc.flux++;
+    cold_inc(cold().flux);
     if (k.iflag == 1) {
-      acc.sflux++;
+      cold_inc(cold().sflux);
       acc.nscat += (uint32_t)iscat;
     }
   }
@@ -1856,9 +1869,9 @@ __device__ __forceinline__ void bin_exit(Pkt<R, NA> &k, double xpold, double ypo
       }
     }
   }
-  if (counted) acc.sumsub += (double)k.s.I;
-  acc.sumall += (double)k.s.I;
-  if (k.iflag == 1) acc.scount++;
+  if (counted) atomic_add(&cold().sumsub, (double)k.s.I);
+  atomic_add(&cold().sumall, (double)k.s.I);
+  if (k.iflag == 1) cold_inc(cold().scount);
 }
 
 // Optical depth to the next event, -ln(xi) (propagate.f90:33).  In FP32 a
@@ -2078,6 +2091,7 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : POOL ? kPoolLanes : kBlockLucy,
     }
   };
   Acc acc{};
+  if ((threadIdx.x & 63) == 0) cold() = WaveCold{};  // this wavefront's set, before its adds
   Lane<R, NA> L;
   L.la.ntally = 0;
   Trace tr{0.0, 0.0, 0, 0, 0, 0, 0, 14695981039346656037ull};
@@ -2117,7 +2131,7 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : POOL ? kPoolLanes : kBlockLucy,
       if (k.ir != L.pir || k.it != L.pit || k.ip != L.pip) L.iint = 0;
       if (L.iint > P.iintmax) {
         if (P.killed) atomicAdd(&P.killed[cell], 1);
-        acc.abflux++;
+        cold_inc(cold().abflux);
         status = 1;
         return (int)ST_END;
       }
@@ -2128,7 +2142,7 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : POOL ? kPoolLanes : kBlockLucy,
       status = 2;
       return (int)ST_END;
     }
-    if (L.first) acc.tot++;
+    if (L.first) cold_inc(cold().tot);
     L.first = false;
     R alb_slot = R(0.0);  // k.alb[slot] without a dynamically indexed register array
 #pragma unroll
@@ -2222,7 +2236,7 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : POOL ? kPoolLanes : kBlockLucy,
           // every slot adds (0 or 1): an indexed add would leave the lane's
           // state in scratch memory
 #pragma unroll
-          for (int s2 = 0; s2 < NA; ++s2) L.la.nabs[s2] += s2 == slot ? 1.0 : 0.0;
+          for (int s2 = 0; s2 < NA; ++s2) L.la.nabs[s2] += s2 == slot ? 1u : 0u;
           k.i_orig = (idust2 == 0 || idust2 == 1 || idust2 == 4 || idust2 == 5) ? 2 : 3;
           k.s = Stokes4<R>{R(1.0), R(0.0), R(0.0), R(0.0)};
           ++L.iint;
@@ -2529,7 +2543,7 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : POOL ? kPoolLanes : kBlockLucy,
           } else {
 #pragma unroll
             for (int s2 = 0; s2 < NA; ++s2)
-              if (s2 == slot) L.la.nabs[s2] += 1.0;
+              if (s2 == slot) L.la.nabs[s2] += 1u;
           }
           k.s = Stokes4<R>{R(1.0), R(0.0), R(0.0), R(0.0)};
           // the draws in their order (direction, direction, frequency); the
@@ -2615,17 +2629,22 @@ __global__ __launch_bounds__(PEEL ? kBlockPeel : POOL ? kPoolLanes : kBlockLucy,
       }
     }
   }
-  const double v[S_COUNT] = {(double)acc.tot,    (double)acc.flux,     (double)acc.sflux,
-                             (double)acc.nscat,  (double)acc.scount,   (double)acc.abflux,
-                             (double)acc.aflux,  acc.sumsub,           acc.sumall,
+  // (the wavefront's once-per-packet counters are added from its LDS set below)
+  const double v[S_COUNT] = {0.0,                0.0,                  0.0,
+                             (double)acc.nscat,  0.0,                  0.0,
+                             0.0,                0.0,                  0.0,
                              (double)acc.hit,    (double)acc.htot,     (double)acc.err_wall,
                              (double)acc.err_dust, (double)acc.warn,
                              (double)acc.ncross, (double)acc.nscat_ev, (double)acc.nabs_ev,
                              (double)acc.npeelray, (double)acc.npeelcross,
                              (double)L.la.ntally};
+  const WaveCold &wc = cold();
+  const double vc[S_COUNT] = {(double)wc.tot, (double)wc.flux, (double)wc.sflux, 0.0,
+                              (double)wc.scount, (double)wc.abflux, (double)wc.aflux,
+                              wc.sumsub, wc.sumall};
 #pragma unroll
   for (int i = 0; i < S_COUNT; ++i) {
-    const double w = wave_sum(v[i]);
+    const double w = wave_sum(v[i]) + vc[i];
     if ((threadIdx.x & 63) == 0 && w != 0.0) atomic_add(&P.scal[i], w);
   }
   unsigned mw = acc.maxwork;
