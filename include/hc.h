@@ -296,7 +296,7 @@ int hc_run_batch(int32_t nmodels, hc_ctx *const *ctxs, int32_t niter, const int6
  *                   with at least this many waiting lanes runs (capped at
  *                   regen_lanes; default 8);
  *   "peel_queue"    0 or 65..1024: peel-off records per wavefront queue in
- *                   LDS (0 = 96, or fewer when the LDS left by the grid's
+ *                   LDS (0 = 128, or fewer when the LDS left by the grid's
  *                   walls and dust tables holds fewer);
  *   "blocks_per_cu" 0..64: workgroups per compute unit, 0 = as many as fit
  *                   (default 0);
