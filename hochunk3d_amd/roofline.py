@@ -83,5 +83,5 @@ def pass_bytes(s: dict, fp32: bool = False) -> float:
 # writes evictions of the per-cell tally lines the atomics dirty.
 CONFIG2_STEP_PACKETS = 23_880_000
 CONFIG2_TRAFFIC_PROFILE = "profiles/transport_fp64_config2_pmc.md"
-CONFIG2_STEP_HBM_READ = 1.79020e12
-CONFIG2_STEP_HBM_WRITE = 1.23579e12
+CONFIG2_STEP_HBM_READ = 1.72086e12
+CONFIG2_STEP_HBM_WRITE = 1.15943e12
